@@ -1,0 +1,353 @@
+"""CPU: the oracle's float front end against closed forms, FP64 restatements,
+the reference's own host code (golden vectors / oracle/_ref) and the
+invariants the mathematics fixes."""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cujpegxl_amd.synth import synthetic_nv12
+
+
+def test_numeric_primitives_error_bounds(jo):
+    L = jo.lib()
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(0.0037, 1.2, 4000), rng.uniform(1e-3, 50, 2000)])
+    for x in xs.astype(np.float32):
+        x = float(x)
+        assert abs(L.jo_cbrt(x) - np.cbrt(x)) <= 2.5e-7 * np.cbrt(x)
+        assert abs(L.jo_log2(x) - np.log2(x)) <= 2e-7 * max(1.0, abs(np.log2(x)))
+    for x in rng.uniform(-20, 20, 4000).astype(np.float32):
+        assert abs(L.jo_exp2(float(x)) - 2.0 ** float(x)) <= 3e-7 * 2.0 ** float(x)
+    for x in rng.uniform(0.05, 1.0, 4000).astype(np.float32):
+        want = float(x) ** 2.4
+        assert abs(L.jo_pow24(float(x)) - want) <= 1.5e-6 * want
+    assert L.jo_cbrt(0.0) == 0.0 and L.jo_cbrt(8.0) == pytest.approx(2.0, rel=1e-7)
+
+
+def test_f16_conversion_matches_ieee_rne(jo):
+    L = jo.lib()
+    rng = np.random.default_rng(2)
+    vals = np.concatenate([
+        rng.normal(0, 1e-5, 3000), rng.normal(0, 1e-3, 3000), rng.normal(0, 1, 3000),
+        np.array([0.0, -0.0, 6.1035156e-05, 5.9604645e-08, 2.98e-8, 65504.0, 65519.0, 1e-9]),
+    ]).astype(np.float32)
+    for v in vals:
+        h = L.jo_f32_to_f16(float(v))
+        assert h == int(np.float32(v).astype(np.float16).view(np.uint16)), v
+        assert L.jo_f16_to_f32(h) == float(np.float16(v))
+
+
+def _nv12(w, h, seed):
+    luma, chroma = synthetic_nv12(w, h, seed)
+    return np.ascontiguousarray(luma), np.ascontiguousarray(chroma)
+
+
+def test_xyb_against_fp64_and_flat_tile(jo):
+    luma, chroma = _nv12(128, 96, 7)
+    got = jo.nv12_to_xyb(luma, chroma, 128, 96)
+    want = jo.nv12_to_xyb_f64(luma, chroma, 128, 96)
+    # 1e-5 relative on the magnitude scale of each channel's values.
+    assert np.max(np.abs(got - want)) <= 1e-5 * np.max(np.abs(want))
+    # xyb_conformance_test.py:60-64 flat tile: the chroma upsample is exact.
+    luma = np.full((64, 64), 140, np.uint8)
+    chroma = np.empty((32, 64), np.uint8)
+    chroma[:, 0::2], chroma[:, 1::2] = 100, 170
+    got = jo.nv12_to_xyb(luma, chroma, 64, 64)
+    assert np.ptp(got.reshape(3, -1), axis=1).max() == 0.0
+    want = jo.nv12_to_xyb_f64(luma, chroma, 64, 64)
+    assert np.max(np.abs(got - want)) < 3e-7
+    # Grey (Cb = Cr = 128) maps to X = 0 (opsin rows 0 and 1 are equal for grey).
+    chroma[:] = 128
+    assert np.abs(jo.nv12_to_xyb(luma, chroma, 64, 64)[0]).max() < 1e-7
+
+
+def test_chroma_upsample_is_the_exact_bilinear(jo):
+    """tests/reference.py:23-46: centred bilinear 4:2:0 -> 4:4:4, each tap
+    clamped to the edge independently, evaluated here in float64."""
+    rng = np.random.default_rng(6)
+    w, h = 22, 14
+    chroma = rng.integers(0, 256, (h // 2, w), dtype=np.uint8)
+    cb = np.zeros((h, w), np.float32)
+    cr = np.zeros((h, w), np.float32)
+    jo.lib().jo_chroma_upsample(jo.ptr(chroma), w, w, h, jo.ptr(cb), jo.ptr(cr))
+
+    def upsample(plane):
+        ch, cw = plane.shape
+        xs = (np.arange(w) + 0.5) / 2.0 - 0.5
+        ys = (np.arange(h) + 0.5) / 2.0 - 0.5
+        x0, y0 = np.floor(xs), np.floor(ys)
+        fx, fy = xs - x0, ys - y0
+        x0i, x1i = np.clip(x0.astype(int), 0, cw - 1), np.clip(x0.astype(int) + 1, 0, cw - 1)
+        y0i, y1i = np.clip(y0.astype(int), 0, ch - 1), np.clip(y0.astype(int) + 1, 0, ch - 1)
+        p = plane.astype(np.float64)
+        top = p[y0i][:, x0i] * (1 - fx) + p[y0i][:, x1i] * fx
+        bot = p[y1i][:, x0i] * (1 - fx) + p[y1i][:, x1i] * fx
+        return top * (1 - fy)[:, None] + bot * fy[:, None]
+
+    np.testing.assert_array_equal(cb, upsample(chroma[:, 0::2]))  # exact in float
+    np.testing.assert_array_equal(cr, upsample(chroma[:, 1::2]))
+
+
+def test_gaborish_against_bruteforce(jo):
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-0.1, 0.9, (3, 40, 56)).astype(np.float32)
+    got = jo.gaborish(x)
+    k = [-0.09495815671340026, -0.041031725066768575, 0.013710004822696948,
+         0.006510206083837737, -0.0014789063378272242]
+    wgt = np.zeros((5, 5))
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            a, b = sorted((abs(dx), abs(dy)))
+            wgt[dy + 2, dx + 2] = {(0, 0): 1, (0, 1): k[0], (0, 2): k[2], (1, 1): k[1],
+                                   (1, 2): k[3], (2, 2): k[4]}[(a, b)]
+    wgt /= wgt.sum()
+    pad = np.pad(x.astype(np.float64), ((0, 0), (2, 2), (2, 2)), mode="edge")
+    want = np.zeros_like(pad[:, 2:-2, 2:-2])
+    for dy in range(5):
+        for dx in range(5):
+            want += wgt[dy, dx] * pad[:, dy : dy + 40, dx : dx + 56]
+    assert np.max(np.abs(got - want)) < 2e-6
+    flat = np.full((3, 16, 16), 0.25, np.float32)
+    assert np.max(np.abs(jo.gaborish(flat) - 0.25)) < 1e-6  # weights sum to one
+
+
+def _aq_block_f64(xyb, bx, by, w, h, d):
+    """FP64 restatement of one block's AQ integer from frontend_fused.cu:54-204
+    (the tile-local erosion included), for the blocks of the tile at (0, 0)."""
+    import math
+
+    def ratio(v, inv):
+        v = max(v, 0.0)
+        mul, mul2, lg = 226.77216153508914, 1 / 73.377132366608819, 0.693147181
+        ret = mul2 * 18.6580932135 * lg
+        num = ret * 3 * mul * v * v + 1e-2
+        den = lg * mul * v ** 3 + 7.7825991679894591 * lg + 1e-2
+        return num / den if inv else den / num
+
+    ty = np.pad(xyb[1].astype(np.float64), 1, mode="edge")
+    pre = np.zeros((8, 8))
+    for cy in range(8):
+        for cx in range(8):
+            s = 0.0
+            for dy in range(4):
+                for dx in range(4):
+                    lx, ly = cx * 4 + dx + 1, cy * 4 + dy + 1
+                    yv = ty[ly, lx]
+                    base = 0.25 * (ty[ly + 1, lx] + ty[ly - 1, lx] + ty[ly, lx - 1] + ty[ly, lx + 1])
+                    diff = min((ratio(yv + 0.019, False) * (yv - base)) ** 2, 0.2)
+                    s += 0.25 * math.sqrt(diff * math.sqrt(211.66567973503678e8) + 27.505837037000106)
+            pre[cy, cx] = 0.25 * s
+    efm = (2 - d) * 0.5 if d < 2 else 0.0
+    km = np.array([0.125, 0.10 - efm * 0.10, 0.09 - efm * 0.09, 0.06 - efm * 0.06])
+    km *= 0.29959705784054957 / km.sum()
+    er = 0.0
+    for cy in (2 * by, 2 * by + 1):
+        for cx in (2 * bx, 2 * bx + 1):
+            nb = sorted(pre[min(max(cy + a, 0), 7), min(max(cx + b, 0), 7)]
+                        for a in (-1, 0, 1) for b in (-1, 0, 1))
+            er += float(np.dot(km, nb[:4]))
+    yv = xyb[1, by * 8 : by * 8 + 8, bx * 8 : bx * 8 + 8].astype(np.float64)
+    xv = xyb[0, by * 8 : by * 8 + 8, bx * 8 : bx * 8 + 8].astype(np.float64)
+    bv = xyb[2, by * 8 : by * 8 + 8, bx * 8 : bx * 8 + 8].astype(np.float64)
+    hf = np.minimum(0.0206, np.abs(np.diff(yv, axis=1))).sum() + \
+        np.minimum(0.0206, np.abs(np.diff(yv, axis=0))).sum()
+    gam = sum(ratio(a + 0.16 - b, True) + ratio(a + 0.16 + b, True)
+              for a, b in zip(yv.ravel(), xv.ravel())) * 0.5 / 64
+    pye = yv + 0.084381641171960495 + np.abs(xv)
+    lim = 0.027121074570634722
+    blue = np.where(bv > pye, np.minimum(bv - pye, lim), 0).sum()
+    if blue >= 32 * lim:
+        blue = 64 * lim - blue
+    blue = min(blue, 15.398788439047934 * lim)
+    v1 = max(er * 0.80061762862741759, 1e-3)
+    mask = (-0.7647 + 9.4708735624378946 / (v1 * v1 + 0.25 * 3.7179635626140772)
+            + 17.35036561631863 / (v1 + 302.59587815579727)
+            + 6.7943250517376494 / (v1 * v1 + 3.7179635626140772))
+    ov = mask + hf * -0.38 + 0.42 + 0.1005613337192697 * math.log2(gam) + blue * 0.14207000358439159
+    scale = 0.725 / d
+    damp = 1.0 if d < 2 else max(0.0, 1 - (d - 2) / 12)
+    q = math.exp(ov) * scale * damp + (1 - damp) * 0.48 * scale
+    return q
+
+
+@pytest.mark.parametrize("d", [1.0, 3.0])
+def test_adaptive_quant_against_fp64(jo, d):
+    luma, chroma = _nv12(32, 32, 9)
+    xyb = jo.nv12_to_xyb(luma, chroma, 32, 32)
+    qf = jo.adaptive_quant(xyb, d)
+    inv = jo.calibrate(d).inv_global_scale
+    for by in range(4):
+        for bx in range(4):
+            want = _aq_block_f64(xyb, bx, by, 32, 32, d) * inv + 0.5
+            got = int(qf[by * 4 + bx])
+            if abs(want - round(want)) > 1e-3:  # not at a rounding tie
+                assert got == min(256, max(1, int(want))), (bx, by, want)
+
+
+def test_adaptive_quant_flat_and_range(jo):
+    luma = np.full((64, 96), 100, np.uint8)
+    chroma = np.full((32, 96), 128, np.uint8)
+    xyb = jo.nv12_to_xyb(luma, chroma, 96, 64)
+    qf = jo.adaptive_quant(xyb, 1.0)
+    assert np.ptp(qf) == 0  # a flat frame modulates every block alike
+    luma, chroma = _nv12(256, 128, 4)
+    qf = jo.adaptive_quant(jo.nv12_to_xyb(luma, chroma, 256, 128), 1.0)
+    assert qf.min() >= 1 and qf.max() <= 256 and np.ptp(qf) > 0
+
+
+def test_selection_matches_reference(jo, golden):
+    """transform_select_gpu_test.cc:67-118 planes against the reference's host
+    decide_region; the flat plane picks all DCT32."""
+    py, px = np.mgrid[0:64, 0:96]
+    y = np.where(px < 48, 0.2 + 0.3 * ((px + py) / (96 + 64)),
+                 0.5 + 0.4 * np.where((px + py) & 1, 1.0, -1.0)).astype(np.float32)
+    np.testing.assert_array_equal(jo.select_transforms(y, 1.0), golden["sel_texture_96x64"])
+    flat = np.full((64, 64), 0.37, np.float32)
+    acs = jo.select_transforms(flat, 1.0)
+    np.testing.assert_array_equal(acs, golden["sel_flat_64"])
+    assert acs.reshape(8, 8)[::4, ::4].tolist() == [[32, 32], [32, 32]]
+    i = np.arange(88 * 72)
+    y = (0.3 + 0.2 * ((i * 7) % 13) / 13.0).astype(np.float32).reshape(72, 88)
+    np.testing.assert_array_equal(jo.select_transforms(y, 1.0), golden["sel_edge_88x72"])
+
+
+def _valid_tiling(acs, bw, bh):
+    owner = np.full(bw * bh, -1)
+    for by in range(bh):
+        for bx in range(bw):
+            s = int(acs[by * bw + bx])
+            if s == 0:
+                continue
+            assert s in (8, 16, 32)
+            k = s // 8
+            assert bx + k <= bw and by + k <= bh
+            for dy in range(k):
+                for dx in range(k):
+                    assert owner[(by + dy) * bw + bx + dx] == -1
+                    owner[(by + dy) * bw + bx + dx] = by * bw + bx
+    assert (owner >= 0).all()
+
+
+def test_selection_on_synthetic_frames(jo):
+    """Real content: valid tilings, all three sizes used, and the decisions
+    agree with the reference's host decide_region where _ref is built (its
+    float reduction order differs, so only near-ties may flip)."""
+    luma, chroma = _nv12(264, 200, 3)
+    xyb = jo.gaborish(jo.nv12_to_xyb(luma, chroma, 264, 200))
+    acs = jo.select_transforms(xyb[1], 1.0)
+    _valid_tiling(acs, 33, 25)
+    assert {8, 16, 32} <= set(np.unique(acs).tolist())
+    R = jo.ref()
+    if R is not None:
+        want = np.full(acs.size, -1, np.int8)
+        y = np.ascontiguousarray(xyb[1])
+        R.ref_select_transforms(jo.ptr(y), 264, 200, 1.0, jo.ptr(want))
+        assert np.mean(acs == want) > 0.99
+
+
+def test_dct_against_fp64(jo):
+    rng = np.random.default_rng(8)
+    x = rng.uniform(-0.2, 1.0, (3, 64, 64)).astype(np.float32)
+    for acs_side in (8, 16, 32):
+        acs = np.zeros(64, np.int8)
+        for by in range(0, 8, acs_side // 8):
+            for bx in range(0, 8, acs_side // 8):
+                acs[by * 8 + bx] = acs_side
+        got = jo.forward_dct(x, acs)
+        want = jo.forward_dct_f64(x, acs)
+        err = np.abs(got - want)
+        assert err.max() <= 1e-5 * max(1.0, np.abs(want).max()), acs_side
+    # Orthogonality: the DC of a constant block is the constant (1/N scaling).
+    c = jo.forward_dct(np.full((3, 32, 32), 0.42, np.float32), np.array(
+        [32] + [0] * 15, np.int8))
+    assert abs(c[0] - 0.42) < 1e-6 and np.abs(c[1:1024]).max() < 1e-6
+
+
+def test_cfl_maps_and_llf(jo, golden):
+    R = jo.ref()
+    sums = golden["cfl_sums"]
+    # jo_cfl_maps reduces per-region partials; feed each sum as one region.
+    for k in range(sums.shape[0]):
+        part = np.zeros((1, 3), np.float32)
+        part[0] = sums[k]
+        ytox, ytob = np.zeros(1, np.int8), np.zeros(1, np.int8)
+        jo.lib().jo_cfl_maps(jo.ptr(part), 32, 32, jo.ptr(ytox), jo.ptr(ytob))
+        if R is not None:
+            mx, mb = C.c_int(), C.c_int()
+            R.ref_cfl_maps_from_sums(*map(float, part[0]), C.byref(mx), C.byref(mb))
+            assert (ytox[0], ytob[0]) == (mx.value, mb.value)
+    assert (ytox[0], ytob[0]) == (golden["cfl_maps"][-1][0], golden["cfl_maps"][-1][1]) or R
+    # Degenerate tile maps to the base (cfl_gpu_test.cc:115-124).
+    part = np.zeros((1, 3), np.float32)
+    ytox, ytob = np.ones(1, np.int8), np.ones(1, np.int8)
+    jo.lib().jo_cfl_maps(jo.ptr(part), 32, 32, jo.ptr(ytox), jo.ptr(ytob))
+    assert (ytox[0], ytob[0]) == (0, 0)
+    for m in (2, 4):
+        for k in range(golden[f"llf{m}_in"].shape[0]):
+            llf = np.ascontiguousarray(golden[f"llf{m}_in"][k])
+            dc = np.zeros(m * m, np.float32)
+            jo.lib().jo_dc_from_llf(m, jo.ptr(llf), jo.ptr(dc))
+            want = golden[f"llf{m}_out"][k]
+            assert np.all(np.abs(dc - want) <= np.spacing(np.abs(want)) * 1.01)
+    # llf_dc_conformance_test.py:74-82: a constant block's DCs equal the constant.
+    for n in (16, 32):
+        m = n // 8
+        acs = np.zeros(m * m, np.int8)
+        acs[0] = n
+        coef = jo.forward_dct(np.full((3, n, n), 0.42, np.float32), acs)
+        llf = np.array([coef[(e // 64) * 64 + e % 64] for e in
+                        [fx * n + fy for fx in range(m) for fy in range(m)]], np.float32)
+        dc = np.zeros(m * m, np.float32)
+        jo.lib().jo_dc_from_llf(m, jo.ptr(llf), jo.ptr(dc))
+        assert np.max(np.abs(dc - 0.42)) < 2e-6
+
+
+def test_quantization_is_rint_of_coeff_over_step(jo):
+    """quant_selfconsistency_test.py:127-179: every quantized integer equals
+    rint(residual / step) in float64 with the calibrated step (ties excepted),
+    and the CfL / base-correlation residuals are the decoder's."""
+    w8, w16, w32 = jo.dequant_weights()
+    for seed, d, (w, h) in ((4, 1.0, (64, 64)), (5, 3.0, (96, 64))):
+        luma, chroma = _nv12(w, h, seed)
+        xyb = jo.gaborish(jo.nv12_to_xyb(luma, chroma, w, h))
+        acs = jo.select_transforms(xyb[1], d)
+        c16 = jo.to_f16(jo.forward_dct(xyb, acs))
+        qf = jo.adaptive_quant(jo.nv12_to_xyb(luma, chroma, w, h), d)
+        _, ytox, ytob = jo.cfl(c16, acs, w, h)
+        ac, dc, _ = jo.quantize(c16, acs, ytox, ytob, qf, w, h, d)
+        cal = jo.calibrate(d)
+        coef = c16.view(np.float16).astype(np.float64).reshape(3, -1)
+        bw = w // 8
+        ties = checked = 0
+        for blk in range(acs.size):
+            n = int(acs[blk])
+            if n == 0:
+                continue
+            bx, by = blk % bw, blk // bw
+            qgsf = float(qf[blk]) * cal.global_scale_float
+            t = (by // 8) * ((bw + 7) // 8) + bx // 8
+            yx, yb = ytox[t] / 84.0, 1.0 + ytob[t] / 84.0
+            W = {8: w8, 16: w16, 32: w32}[n]
+            m = n // 8
+            for e in range(n * n):
+                if e // n < m and e % n < m:
+                    continue
+                q = (e // 64)
+                slot = ((by + q // m) * bw + bx + q % m) * 64 + e % 64
+                yv = coef[1, slot] * qgsf / W[1, e]
+                qy = np.rint(yv)
+                yr = qy * W[1, e] / qgsf
+                xv = (coef[0, slot] - yx * yr) * qgsf / W[0, e]
+                bv = (coef[2, slot] - yb * yr) * qgsf / W[2, e]
+                for val, got in ((yv, ac.reshape(3, -1)[1, slot]), (xv, ac.reshape(3, -1)[0, slot]),
+                                 (bv, ac.reshape(3, -1)[2, slot])):
+                    if abs(abs(val - np.floor(val)) - 0.5) < 1e-4:
+                        ties += 1
+                        continue
+                    assert got == np.rint(val), (blk, e, val, got)
+                    checked += 1
+        assert checked > 1000 and ties < checked // 100
