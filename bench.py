@@ -1,0 +1,131 @@
+"""Benchmark: 4K NV12 frames/s at distance 1.0 on one MI355X.
+
+Workload: a 3840x2160 seeded synthetic NV12 frame (cujpegxl_amd/synth.py,
+seed 0) resident in HBM, encoded to a `.jxl` file on the host by the plain HIP
+VarDCT path (RD selection over DCT8/16/32, CfL, adaptive quantization,
+32-cluster prefix coding).  One step is one synchronous
+cujpegxl_encoder_encode call: NV12 in HBM -> file bytes in host memory.
+
+Prints one JSON line.  `roofline` is the dominant kernel's achieved
+algorithmic HBM bandwidth (HIP events on the encoder's stream, over the timed
+steps); `cpu_baseline` is the C oracle (a scalar restatement of the reference
+path) encoding the same frame on one host core.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+WIDTH, HEIGHT, DISTANCE, SEED = 3840, 2160, 1.0, 0
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(luma: np.ndarray, chroma: np.ndarray, budget_s: float = 12.0) -> dict:
+    import oracle
+
+    oracle.lib()
+    frames, t0 = 0, time.perf_counter()
+    while True:
+        oracle.encode_nv12(luma, chroma, WIDTH, HEIGHT, DISTANCE)
+        frames += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or frames >= 4:
+            break
+    return {"value": frames / el, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{frames} full 3840x2160 frame(s), d=1.0, seed {SEED}, single thread, "
+                      f"{el:.1f} s"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build encodes on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        return 2
+
+    import cujpegxl_amd as cj
+    from cujpegxl_amd.synth import synthetic_nv12
+
+    luma, chroma = synthetic_nv12(WIDTH, HEIGHT, SEED)
+    nv12 = np.concatenate([luma.ravel(), chroma.ravel()])
+    dev = cj.DeviceBuffer.from_array(nv12)
+    enc = cj.Encoder(WIDTH, HEIGHT, DISTANCE, stages=True)
+    lib = enc.lib
+
+    for _ in range(args.warmup):
+        data = enc.encode_device(dev.ptr, dev.ptr + WIDTH * HEIGHT, WIDTH, WIDTH)
+    enc.set_profiling(True)
+    stage_sum = {k: 0.0 for k in cj.STAGE_NAMES}
+    lib.cjx_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        data = enc.encode_device(dev.ptr, dev.ptr + WIDTH * HEIGHT, WIDTH, WIDTH)
+        for k, v in enc.timings().items():
+            stage_sum[k] += v
+    lib.cjx_device_synchronize()
+    elapsed = time.perf_counter() - t0
+    enc.set_profiling(False)
+
+    ms_per_step = 1e3 * elapsed / args.steps
+    fps = args.steps / elapsed
+    stage_ms = {k: v / args.steps for k, v in stage_sum.items()}
+    region_ms = stage_ms["region"]
+    nv12_bytes = 1.5 * WIDTH * HEIGHT
+    achieved = nv12_bytes / (region_ms * 1e-3) / 1e9
+    out = {
+        "metric": "4K NV12 frames/s at d=1.0 on 1 MI355X",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3840x2160 NV12 8-bit seeded synthetic frame, distance 1.0, plain HIP "
+                        "VarDCT (DCT8/16/32 RD selection, CfL, adaptive quant, 32-cluster prefix "
+                        "coding), NV12 resident in HBM, .jxl bytes to host",
+            "path": "plain",
+            "bytes_per_frame": len(data),
+            "bpp": round(8.0 * len(data) / (WIDTH * HEIGHT), 4),
+            "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+            "frame_alg_bytes_frac": round((nv12_bytes + len(data)) * fps / (HBM_PEAK_GBS * 1e9), 6),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "region_kernel",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 6),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(luma, chroma)
+    dev.close()
+    enc.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,293 @@
+"""cujpegxl for AMD Instinct MI355X: Python access to the C ABI.
+
+Mirrors the reference's `pycujpegxl` module (python/pycujpegxl.cc:31-90):
+`encode`, `encode_with_stats`, `query_backend`, `api_version`, `API_VERSION`,
+with the same argument meaning and errors (ValueError for a wrongly sized or
+non-contiguous array, RuntimeError for an encode or device failure).  Unlike
+the reference, encoding goes through the C ABI (libcujpegxl.so.1).  Host NV12
+is uploaded with the stage library's device allocator, so no framework is
+needed for device memory.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_DIR = PKG / "lib"
+PUBLIC_LIB = LIB_DIR / "libcujpegxl.so.1"
+STAGES_LIB = LIB_DIR / "libcujpegxl_stages.so"
+
+API_VERSION = 3
+
+OK, NOT_IMPLEMENTED, INVALID_ARGUMENT, UNSUPPORTED_RESOLUTION, NO_DEVICE, INTERNAL_ERROR, \
+    BUFFER_TOO_SMALL = range(7)
+
+STAGE_NAMES = ("region", "quantize", "histograms", "codes", "sizes", "emit", "readback")
+BUFFERS = {
+    "coeffs": 0, "acs": 1, "quant_field": 2, "cfl_partials": 3, "ytox": 4, "ytob": 5, "ac": 6,
+    "dc": 7, "nz_grid": 8, "nz_count": 9, "histograms": 10, "ac_group_bytes": 11,
+    "dc_group_bytes": 12, "code_depth": 13, "code_bits": 14,
+}
+
+
+class Config(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("distance", C.c_float),
+                ("device_ordinal", C.c_int32)]
+
+
+class Nv12Input(C.Structure):
+    _fields_ = [("luma", C.c_size_t), ("chroma", C.c_size_t), ("luma_pitch", C.c_size_t),
+                ("chroma_pitch", C.c_size_t)]
+
+
+class Output(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_size_t), ("capacity", C.c_size_t)]
+
+
+_public = None
+_stages = None
+
+
+def _bind_public(L: C.CDLL) -> C.CDLL:
+    L.cujpegxl_api_version.restype = C.c_uint32
+    L.cujpegxl_status_string.restype = C.c_char_p
+    L.cujpegxl_status_string.argtypes = [C.c_int]
+    L.cujpegxl_backend_string.restype = C.c_char_p
+    L.cujpegxl_backend_string.argtypes = [C.c_int]
+    L.cujpegxl_query_backend.argtypes = [C.c_int32, C.POINTER(C.c_int)]
+    L.cujpegxl_max_output_size.argtypes = [C.POINTER(Config), C.POINTER(C.c_size_t)]
+    L.cujpegxl_encoder_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+    L.cujpegxl_encoder_configure.argtypes = [C.c_void_p, C.POINTER(Config)]
+    L.cujpegxl_encoder_encode.argtypes = [C.c_void_p, C.POINTER(Nv12Input), C.POINTER(Output)]
+    L.cujpegxl_encoder_destroy.argtypes = [C.c_void_p]
+    L.cujpegxl_encoder_destroy.restype = None
+    return L
+
+
+def public_lib() -> C.CDLL:
+    """libcujpegxl.so.1; raises if it has not been built."""
+    global _public
+    if _public is None:
+        if not PUBLIC_LIB.exists():
+            raise RuntimeError(f"{PUBLIC_LIB} is missing: run cujpegxl_amd/build.py")
+        _public = _bind_public(C.CDLL(str(PUBLIC_LIB)))
+    return _public
+
+
+def stages_lib() -> C.CDLL:
+    """libcujpegxl_stages.so (C ABI plus the cjx_* stage harness)."""
+    global _stages
+    if _stages is None:
+        if not STAGES_LIB.exists():
+            raise RuntimeError(f"{STAGES_LIB} is missing: run cujpegxl_amd/build.py")
+        L = _bind_public(C.CDLL(str(STAGES_LIB)))
+        P = C.c_void_p
+        L.cjx_stage_buffer_bytes.restype = C.c_size_t
+        L.cjx_stage_buffer_bytes.argtypes = [P, C.c_int]
+        L.cjx_stage_fetch.argtypes = [P, C.c_int, P, C.c_size_t]
+        L.cjx_stage_encode_coefficients.argtypes = [P, P, P, P, P, P, P, C.c_uint32, C.c_uint32, P,
+                                                    C.c_size_t, C.POINTER(C.c_size_t)]
+        L.cjx_stage_encoder_create_any.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+        L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
+        L.cjx_stage_timings.argtypes = [P, C.POINTER(C.c_float), C.c_int]
+        L.cjx_device_alloc.restype = P
+        L.cjx_device_alloc.argtypes = [C.c_size_t]
+        L.cjx_device_free.argtypes = [P]
+        L.cjx_device_free.restype = None
+        L.cjx_copy_to_device.argtypes = [P, P, C.c_size_t]
+        L.cjx_copy_to_host.argtypes = [P, P, C.c_size_t]
+        L.cjx_host_table.restype = C.c_size_t
+        L.cjx_host_table.argtypes = [C.c_int, P, C.c_size_t]
+        _stages = L
+    return _stages
+
+
+def status_string(status: int) -> str:
+    return public_lib().cujpegxl_status_string(status).decode()
+
+
+def api_version() -> int:
+    return int(public_lib().cujpegxl_api_version())
+
+
+def query_backend(device_ordinal: int = 0) -> str:
+    b = C.c_int(0)
+    s = public_lib().cujpegxl_query_backend(device_ordinal, C.byref(b))
+    if s != OK:
+        raise RuntimeError(f"query_backend failed: {status_string(s)}")
+    return public_lib().cujpegxl_backend_string(b.value).decode()
+
+
+def max_output_size(width: int, height: int, distance: float = 1.0) -> int:
+    n = C.c_size_t(0)
+    s = public_lib().cujpegxl_max_output_size(C.byref(Config(width, height, distance, 0)), C.byref(n))
+    if s != OK:
+        raise ValueError(status_string(s))
+    return n.value
+
+
+class DeviceBuffer:
+    """A device allocation from the stage library (freed on close/GC)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = nbytes
+        self.ptr = stages_lib().cjx_device_alloc(max(nbytes, 1))
+        if not self.ptr:
+            raise RuntimeError("device allocation failed")
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        buf = cls(a.nbytes)
+        if stages_lib().cjx_copy_to_device(buf.ptr, a.ctypes.data, a.nbytes) != OK:
+            raise RuntimeError("host to device copy failed")
+        return buf
+
+    def to_host(self, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(self.nbytes, np.uint8)
+        if stages_lib().cjx_copy_to_host(out.ctypes.data, self.ptr, self.nbytes) != OK:
+            raise RuntimeError("device to host copy failed")
+        return out.view(dtype)
+
+    def close(self) -> None:
+        if self.ptr:
+            stages_lib().cjx_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Encoder:
+    """An encoder of the C ABI (or of the stage library with `stages=True`,
+    which also exposes intermediate buffers and per-stage timings)."""
+
+    def __init__(self, width: int, height: int, distance: float = 1.0, device_ordinal: int = 0,
+                 stages: bool = False, any_size: bool = False):
+        self.lib = stages_lib() if (stages or any_size) else public_lib()
+        self.width, self.height, self.distance = width, height, distance
+        h = C.c_void_p()
+        cfg = Config(width, height, distance, device_ordinal)
+        if any_size:
+            s = self.lib.cjx_stage_encoder_create_any(C.byref(cfg), C.byref(h))
+        else:
+            s = self.lib.cujpegxl_encoder_create(C.byref(cfg), C.byref(h))
+        if s != OK:
+            raise RuntimeError(f"encoder_create failed: {status_string(s)}")
+        self.handle = h
+        bw, bh = width // 8, height // 8
+        self.capacity = 3 * bw * bh * 384 + 48 * bw * bh + (1 << 20)
+        self.out = np.empty(self.capacity, np.uint8)
+
+    def encode_device(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int,
+                      chroma_pitch: int) -> bytes:
+        inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
+        out = Output(self.out.ctypes.data, 0, self.capacity)
+        s = self.lib.cujpegxl_encoder_encode(self.handle, C.byref(inp), C.byref(out))
+        if s != OK:
+            raise RuntimeError(f"encode failed: {status_string(s)}")
+        return self.out[: out.size].tobytes()
+
+    def encode_status(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
+                      capacity: int) -> tuple[int, int]:
+        inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
+        out = Output(self.out.ctypes.data, 0, capacity)
+        s = self.lib.cujpegxl_encoder_encode(self.handle, C.byref(inp), C.byref(out))
+        return s, out.size
+
+    def fetch(self, name: str, dtype) -> np.ndarray:
+        which = BUFFERS[name]
+        n = self.lib.cjx_stage_buffer_bytes(self.handle, which)
+        out = np.empty(n, np.uint8)
+        if self.lib.cjx_stage_fetch(self.handle, which, out.ctypes.data, n) != OK:
+            raise RuntimeError(f"fetch {name} failed")
+        return out.view(dtype)
+
+    def encode_coefficients(self, ac, dc, acs, qf, ytox, ytob, global_scale: int,
+                            quant_dc: int) -> bytes:
+        arrs = [np.ascontiguousarray(ac, np.int16), np.ascontiguousarray(dc, np.int32),
+                np.ascontiguousarray(acs, np.int8), np.ascontiguousarray(qf, np.int32),
+                None if ytox is None else np.ascontiguousarray(ytox, np.int8),
+                None if ytob is None else np.ascontiguousarray(ytob, np.int8)]
+        size = C.c_size_t(0)
+        s = self.lib.cjx_stage_encode_coefficients(
+            self.handle, *(None if a is None else a.ctypes.data for a in arrs), global_scale,
+            quant_dc, self.out.ctypes.data, self.capacity, C.byref(size))
+        if s != OK:
+            raise RuntimeError(f"encode_coefficients failed: {status_string(s)}")
+        return self.out[: size.value].tobytes()
+
+    def set_profiling(self, on: bool) -> None:
+        self.lib.cjx_stage_set_profiling(self.handle, 1 if on else 0)
+
+    def timings(self) -> dict[str, float]:
+        ms = (C.c_float * len(STAGE_NAMES))()
+        self.lib.cjx_stage_timings(self.handle, ms, len(STAGE_NAMES))
+        return dict(zip(STAGE_NAMES, (float(x) for x in ms)))
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.cujpegxl_encoder_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _planes(nv12: np.ndarray, width: int, height: int) -> np.ndarray:
+    if not isinstance(nv12, np.ndarray) or nv12.dtype != np.uint8:
+        raise ValueError("nv12 must be a uint8 numpy array")
+    if nv12.size != width * height * 3 // 2:
+        raise ValueError("nv12 array must hold width*height*3/2 bytes")
+    if not nv12.flags.c_contiguous:
+        raise ValueError("nv12 array must be C-contiguous")
+    return nv12
+
+
+def _encode(nv12, width, height, distance, device_ordinal, stats: bool):
+    a = _planes(nv12, width, height)
+    enc = Encoder(width, height, distance, device_ordinal, stages=stats)
+    dev = DeviceBuffer.from_array(a)
+    try:
+        if stats:
+            enc.set_profiling(True)
+        data = enc.encode_device(dev.ptr, dev.ptr + width * height, width, width)
+        t = enc.timings() if stats else None
+    finally:
+        dev.close()
+        enc.close()
+    return data, t
+
+
+def encode(nv12: np.ndarray, width: int, height: int, distance: float = 1.0,
+           device_ordinal: int = 0) -> bytes:
+    """Host NV12 (luma rows then interleaved CbCr rows) -> `.jxl` bytes."""
+    return _encode(nv12, width, height, distance, device_ordinal, False)[0]
+
+
+def encode_with_stats(nv12: np.ndarray, width: int, height: int, distance: float = 1.0,
+                      device_ordinal: int = 0) -> tuple[bytes, list[dict]]:
+    """As `encode`, plus stage records {name, bytes_moved, gpu_us, cpu_us} for
+    the front end, the entropy coder and the assembly (pycujpegxl.cc:56-77);
+    gpu_us is device time from HIP events, bytes_moved the stage's
+    compulsory HBM traffic."""
+    data, t = _encode(nv12, width, height, distance, device_ordinal, True)
+    px = width * height
+    fe = (t["region"] + t["quantize"]) * 1e3
+    ent = (t["histograms"] + t["codes"] + t["sizes"] + t["emit"]) * 1e3
+    asm = t["readback"] * 1e3
+    return data, [
+        {"name": "frontend", "bytes_moved": px * 3 // 2, "gpu_us": fe, "cpu_us": 0.0},
+        {"name": "entropy", "bytes_moved": len(data), "gpu_us": ent, "cpu_us": 0.0},
+        {"name": "assembly", "bytes_moved": len(data), "gpu_us": asm, "cpu_us": 0.0},
+    ]

@@ -1,0 +1,110 @@
+// Shared definitions for the gfx950 VarDCT encoder: frame geometry, the
+// per-frame parameter block, and the constant tables every kernel reads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "numeric.h"
+
+namespace cjx {
+
+// Geometry (vardct_layout.h, entropy.cu:21-22, frontend_quantize.cu CfL tiles):
+// 8x8 blocks; 32x32 selection regions (4x4 blocks); 64x64 CfL tiles (8x8
+// blocks); 256x256 AC groups (32x32 blocks); 2048x2048 DC groups (256x256).
+constexpr int kRegionBlocks = 4;
+constexpr int kTileBlocks = 8;
+constexpr int kAcGroupBlocks = 32;
+constexpr int kDcGroupBlocks = 256;
+constexpr int kHistStride = 256;  // hybrid-uint token alphabet bound
+constexpr int kNumClusters = 32;  // fixed AC context clustering
+constexpr int kNumAcContexts = 15 * (37 + 458);
+
+struct Geometry {
+    std::uint32_t width, height;
+    std::uint32_t bw, bh;        // 8x8 blocks
+    std::uint32_t rw, rh;        // 32x32 regions
+    std::uint32_t cmw, cmh;      // 64x64 CfL tiles
+    std::uint32_t xg, yg;        // AC groups
+    std::uint32_t xdg, ydg;      // DC groups
+    std::uint32_t nblocks;
+    std::uint32_t num_ac, num_dc;
+
+    static Geometry of(std::uint32_t w, std::uint32_t h) {
+        Geometry g{};
+        g.width = w;
+        g.height = h;
+        g.bw = w / 8;
+        g.bh = h / 8;
+        g.rw = (g.bw + 3) / 4;
+        g.rh = (g.bh + 3) / 4;
+        g.cmw = (g.bw + 7) / 8;
+        g.cmh = (g.bh + 7) / 8;
+        g.xg = (g.bw + 31) / 32;
+        g.yg = (g.bh + 31) / 32;
+        g.xdg = (g.bw + 255) / 256;
+        g.ydg = (g.bh + 255) / 256;
+        g.nblocks = g.bw * g.bh;
+        g.num_ac = g.xg * g.yg;
+        g.num_dc = g.xdg * g.ydg;
+        return g;
+    }
+};
+
+// Distance-derived constants (quant_calibration.h:49-88 and
+// frontend_fused.cu:214-239), computed once on the host per configuration.
+struct FrameParams {
+    std::uint32_t global_scale;
+    std::uint32_t quant_dc;
+    std::uint32_t raw_quant_field;
+    float global_scale_float;
+    float inv_global_scale;
+    float qgsf_select;  // raw_quant_field * global_scale_float
+    float km[4];        // fuzzy-erosion weights
+    float mul_pb, add_pb;
+    float dc_scale;     // global_scale_float * quant_dc
+};
+
+FrameParams make_frame_params(float distance);
+
+// Constant tables (filled once per device by init_device_tables()).
+struct DeviceTables {
+    float basis8[64];
+    float basis16[256];
+    float basis32[1024];
+    float wsel8[64];       // DCT8 Y weight sampled at each DCTn frequency
+    float wsel16[256];
+    float wsel32[1024];
+    float dequant8[3][64];
+    float dequant16[3][256];
+    float dequant32[3][1024];
+    std::uint16_t order8[64];     // natural scan order, raw index per scan position
+    std::uint16_t order16[256];
+    std::uint16_t order32[1024];
+    double llf2[4];        // T[f][o] = ortho_2(f, o) * resample_scale_2[f]
+    double llf4[16];
+};
+
+extern __constant__ DeviceTables g_tables;
+
+// Host mirror of the tables (same values), built once.
+const DeviceTables& host_tables();
+bool init_device_tables();
+
+__device__ __forceinline__ const float* basis_of(int n) {
+    return n == 32 ? g_tables.basis32 : (n == 16 ? g_tables.basis16 : g_tables.basis8);
+}
+__device__ __forceinline__ const float* wsel_of(int n) {
+    return n == 32 ? g_tables.wsel32 : (n == 16 ? g_tables.wsel16 : g_tables.wsel8);
+}
+__device__ __forceinline__ float dequant_of(int n, int c, int e) {
+    return n == 32 ? g_tables.dequant32[c][e]
+                   : (n == 16 ? g_tables.dequant16[c][e] : g_tables.dequant8[c][e]);
+}
+__device__ __forceinline__ const std::uint16_t* order_of(int n) {
+    return n == 32 ? g_tables.order32 : (n == 16 ? g_tables.order16 : g_tables.order8);
+}
+
+}  // namespace cjx

@@ -1,0 +1,216 @@
+// The encoder's device code in one translation unit (the kernels share the
+// __constant__ tables), plus the host-callable launchers.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "common.h"
+#include "kernels.h"
+#include "prefix.h"
+
+namespace cjx {
+__constant__ DeviceTables g_tables;
+}  // namespace cjx
+
+#include "frontend.inl"
+#include "entropy.inl"
+#include "assemble.inl"
+
+namespace cjx {
+
+bool init_device_tables() {
+    const DeviceTables& t{host_tables()};
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_tables), &t, sizeof(t)) == hipSuccess;
+}
+
+// One thread per (block position, channel): non-zero AC counts of a
+// first-block and the normalized grid over its covered positions, for frames
+// whose coefficients are supplied directly.
+__global__ void nz_kernel(Geometry geo, const std::int16_t* __restrict__ ac, const std::int8_t* __restrict__ acs,
+                          std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count) {
+    const std::size_t i{blockIdx.x * 256ull + threadIdx.x};
+    const std::size_t nblk{geo.nblocks};
+    if (i >= 3 * nblk) {
+        return;
+    }
+    const int c{static_cast<int>(i / nblk)};
+    const std::size_t pos{i % nblk};
+    const int n{acs[pos]};
+    if (n == 0) {
+        return;
+    }
+    const int bx{static_cast<int>(pos % geo.bw)}, by{static_cast<int>(pos / geo.bw)};
+    const int m{n / 8};
+    const std::int16_t* plane{ac + c * nblk * 64};
+    std::uint32_t cnt{0};
+    for (int e{0}; e < n * n; ++e) {
+        if (e / n < m && e % n < m) {
+            continue;
+        }
+        const int q{e >> 6};
+        cnt += plane[(static_cast<std::size_t>(by + q / m) * geo.bw + bx + q % m) * 64 + (e & 63)] != 0;
+    }
+    nz_count[c * nblk + pos] = static_cast<std::uint16_t>(cnt);
+    const std::uint32_t l2{n == 16 ? 2u : (n == 32 ? 4u : 0u)};
+    for (int dy{0}; dy < m; ++dy) {
+        for (int dx{0}; dx < m; ++dx) {
+            nz_grid[c * nblk + static_cast<std::size_t>(by + dy) * geo.bw + bx + dx] =
+                static_cast<std::uint8_t>((cnt + m * m - 1) >> l2);
+        }
+    }
+}
+
+__global__ void fill_u32_kernel(std::uint32_t* p, std::size_t n, std::uint32_t v) {
+    for (std::size_t i{blockIdx.x * 256ull + threadIdx.x}; i < n; i += gridDim.x * 256ull) {
+        p[i] = v;
+    }
+}
+
+std::size_t sizeof_constants() { return sizeof(Constants); }
+std::size_t sizeof_codeset() { return sizeof(CodeSet); }
+std::size_t sizeof_layout() { return sizeof(Layout); }
+std::size_t sizeof_dc_blobs() { return sizeof(DcBlobs); }
+std::uint32_t dc_chunks_per_group() { return (2 + 6 * 65536 + 2 * 1024 + kDcChunk - 1) / kDcChunk; }
+std::uint32_t blob_words_per_group() { return 2 * kBlobWords; }
+std::uint64_t layout_total_offset() { return offsetof(Layout, total_bytes); }
+
+bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::uint32_t dc_pre_bits,
+                   std::uint32_t dc_mid_bits) {
+    return head_bits <= 32u * (kHeadWords / 4) && ac_prefix_bits <= 32u * (kAcGlobalWords / 2) &&
+           dc_pre_bits <= 32u * 64 && dc_mid_bits <= 32u * 64;
+}
+
+void fill_constants(void* dst, const std::uint32_t* head, std::uint32_t head_bits,
+                    const std::uint32_t* dc_global, std::uint32_t dc_global_bits,
+                    const std::uint32_t* ac_prefix, std::uint32_t ac_prefix_bits,
+                    const std::uint32_t* dc_pre, std::uint32_t dc_pre_bits, const std::uint32_t* dc_mid,
+                    std::uint32_t dc_mid_bits) {
+    Constants* k{static_cast<Constants*>(dst)};
+    std::memset(k, 0, sizeof(Constants));
+    k->head_bits = head_bits;
+    k->dc_global_bits = dc_global_bits;
+    k->ac_prefix_bits = ac_prefix_bits;
+    k->dc_pre_bits = dc_pre_bits;
+    k->dc_mid_bits = dc_mid_bits;
+    std::memcpy(k->head_words, head, (head_bits + 31) / 32 * 4);
+    std::memcpy(k->dc_global_words, dc_global, (dc_global_bits + 31) / 32 * 4);
+    std::memcpy(k->ac_prefix_words, ac_prefix, (ac_prefix_bits + 31) / 32 * 4);
+    std::memcpy(k->dc_pre_words, dc_pre, (dc_pre_bits + 31) / 32 * 4);
+    std::memcpy(k->dc_mid_words, dc_mid, (dc_mid_bits + 31) / 32 * 4);
+}
+
+static unsigned int grid_for(std::size_t n, std::size_t per) {
+    return static_cast<unsigned int>((n + per - 1) / per);
+}
+
+void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
+                   std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
+                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part) {
+    region_kernel<<<geo.rw * geo.rh, 256, 0, s>>>(geo, prm, luma, luma_pitch, chroma, chroma_pitch, coeffs,
+                                                  acs, qf, cfl_part);
+}
+
+void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
+                          const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
+                          const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
+                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count) {
+    tile_quantize_kernel<<<geo.cmw * geo.cmh, 256, 0, s>>>(geo, prm, coeffs, acs, qf, cfl_part, ytox, ytob,
+                                                           ac, dc, nz_grid, nz_count);
+}
+
+void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+               std::uint8_t* nz_grid, std::uint16_t* nz_count) {
+    nz_kernel<<<grid_for(3ull * geo.nblocks, 256), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count);
+}
+
+void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v) {
+    const unsigned int g{grid_for(n, 256) < 1024 ? grid_for(n, 256) : 1024};
+    fill_u32_kernel<<<g > 0 ? g : 1, 256, 0, s>>>(p, n, v);
+}
+
+static unsigned int item_grid(const Geometry& geo) {
+    const std::size_t waves{3ull * geo.nblocks};
+    const unsigned int g{grid_for(waves, 4)};
+    return g < 4096 ? g : 4096;
+}
+
+void launch_ac_hist(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* hist) {
+    ac_hist_kernel<<<item_grid(geo) < 1024 ? item_grid(geo) : 1024, 256, 0, s>>>(geo, ac, acs, nz_grid,
+                                                                               nz_count, hist);
+}
+
+void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
+                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                    std::uint32_t* dc_hist, std::uint32_t* meta_hist, std::uint32_t* fb_count) {
+    const dim3 grid{dc_chunks_per_group(), geo.num_dc};
+    dc_hist_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, dc_hist, meta_hist, fb_count);
+}
+
+void launch_huffman(hipStream_t s, int count, const std::uint32_t* hists, CodeSet* cs) {
+    huffman_kernel<<<count, 64, 0, s>>>(hists, cs);
+}
+
+void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
+                     const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words) {
+    dc_blobs_kernel<<<geo.num_dc, 64, 0, s>>>(geo, k, cs, fb_count, blobs, blob_words);
+}
+
+void launch_ac_bits(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
+                    std::uint32_t* item_bits) {
+    ac_bits_kernel<<<item_grid(geo), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, &cs->depth[0][0], item_bits);
+}
+
+void launch_ac_group_scan(hipStream_t s, const Geometry& geo, const std::uint32_t* item_bits,
+                          std::uint32_t* item_off, std::uint32_t* group_bytes, std::uint64_t* group_bits) {
+    ac_group_scan_kernel<<<geo.num_ac, 1024, 0, s>>>(geo, item_bits, item_off, group_bytes, group_bits);
+}
+
+void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
+                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                    const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
+                    std::uint32_t* group_bytes, std::uint64_t* group_bits) {
+    const std::uint32_t cpg{dc_chunks_per_group()};
+    const dim3 grid{cpg, geo.num_dc};
+    dc_chunk_bits_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
+                                              &cs->depth[kNumClusters + geo.num_dc][0], blobs, chunk_bits, cpg);
+    dc_chunk_scan_kernel<<<geo.num_dc, 1024, 0, s>>>(geo, chunk_bits, cpg, group_bytes, group_bits);
+}
+
+void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
+                   const std::uint32_t* ac_group_bytes, const std::uint32_t* dc_group_bytes,
+                   const std::uint64_t* ac_group_bits, const std::uint64_t* dc_group_bits, const DcBlobs* blobs,
+                   Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base) {
+    layout_kernel<<<1, 64, 0, s>>>(geo, k, cs, ac_group_bytes, dc_group_bytes, ac_group_bits, dc_group_bits,
+                                   blobs, lay, ac_base, dc_base);
+}
+
+void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* lay, std::uint32_t* out_words,
+                             std::uint64_t cap_words) {
+    zero_kernel<<<1024, 256, 0, s>>>(lay, out_words, cap_words);
+    headers_kernel<<<1, 64, 0, s>>>(k, lay, out_words, cap_words);
+}
+
+void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
+                    const std::uint32_t* item_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
+                    std::uint64_t cap_words) {
+    ac_emit_kernel<<<item_grid(geo), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, &cs->depth[0][0],
+                                                  &cs->code[0][0], item_off, ac_base, out_words, cap_words);
+}
+
+void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
+                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                    const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
+                    const std::uint64_t* chunk_off, const std::uint64_t* dc_base, std::uint32_t* out_words,
+                    std::uint64_t cap_words) {
+    const std::uint32_t cpg{dc_chunks_per_group()};
+    const dim3 grid{cpg, geo.num_dc};
+    dc_emit_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
+                                        &cs->code[kNumClusters][0], &cs->depth[kNumClusters + geo.num_dc][0],
+                                        &cs->code[kNumClusters + geo.num_dc][0], blobs, blob_words, chunk_off,
+                                        cpg, dc_base, out_words, cap_words);
+}
+
+}  // namespace cjx

@@ -1,0 +1,574 @@
+// Frame encoder orchestration (frame_encoder.cu:88-352 in the reference):
+// one stream, buffers allocated once per configuration, every stage on the
+// device, and exactly two host synchronizations per frame -- one to learn the
+// file size, one after copying the file out.
+#include "encoder.h"
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "kernels.h"
+#include "prefix.h"
+
+namespace cjx {
+namespace {
+
+// Host bit writer over a growable word vector.
+struct HostBits {
+    std::vector<std::uint32_t> words = std::vector<std::uint32_t>(16, 0u);
+    std::uint32_t pos{0};
+    void write(std::uint32_t n, std::uint32_t v) {
+        while ((pos + n + 64) / 32 >= words.size()) {
+            words.resize(words.size() * 2, 0u);
+        }
+        BitSink s{words.data(), pos, static_cast<std::uint32_t>(words.size() * 32)};
+        s.write(n, v);
+        pos = s.pos;
+    }
+    void pad() { pos = (pos + 7u) & ~7u; }
+    void append(const HostBits& o) { copy_bits(*this, o.words.data(), o.pos); }
+};
+
+constexpr Distr kDimEnc[4]{{false, 1, 9}, {false, 1, 13}, {false, 1, 18}, {false, 1, 30}};
+constexpr Distr kGlobalScaleEnc[4]{{false, 1, 11}, {false, 2049, 11}, {false, 4097, 12}, {false, 8193, 16}};
+constexpr Distr kQuantDcEnc[4]{{true, 16, 0}, {false, 1, 5}, {false, 1, 8}, {false, 1, 16}};
+constexpr Distr kOrderEnc[4]{{true, 0x5F, 0}, {true, 0x13, 0}, {true, 0, 0}, {false, 0, 13}};
+
+// write_prefix_histograms (histogram_writer.cc:59-83) for constant alphabets.
+void prefix_histograms(HostBits& w, const std::uint32_t* hist, int length, int num_contexts, std::uint32_t se,
+                       std::uint32_t msb, std::uint32_t lsb, std::uint8_t* depth, std::uint16_t* bits) {
+    auto scratch{std::make_unique<HuffScratch>()};
+    w.write(1, 0);
+    if (num_contexts > 1) {
+        w.write(1, 1);
+        w.write(2, 0);
+    }
+    w.write(1, 1);
+    write_uint_config(w, se, msb, lsb);
+    write_var_len_u16(w, static_cast<std::uint32_t>(length - 1));
+    std::fill(depth, depth + length, 0);
+    std::fill(bits, bits + length, 0);
+    if (length > 1) {
+        build_and_store_huffman(hist, length, depth, bits, w, *scratch);
+    }
+}
+
+// context_map.cc:43-168: the smallest of complex, complex+MTF and simple.
+void complex_cmap(HostBits& w, const std::uint8_t* map, int n, bool mtf) {
+    std::vector<std::uint8_t> sym(map, map + n);
+    if (mtf) {
+        std::uint8_t s[256];
+        for (int i{0}; i < 256; ++i) {
+            s[i] = static_cast<std::uint8_t>(i);
+        }
+        for (int i{0}; i < n; ++i) {
+            int r{0};
+            while (s[r] != map[i]) {
+                ++r;
+            }
+            sym[i] = static_cast<std::uint8_t>(r);
+            const std::uint8_t v{s[r]};
+            for (; r > 0; --r) {
+                s[r] = s[r - 1];
+            }
+            s[0] = v;
+        }
+    }
+    std::uint32_t hist[16]{};
+    for (std::uint8_t v : sym) {
+        std::uint32_t t, nb, b;
+        hybrid_uint(2, 0, 1, v, t, nb, b);
+        ++hist[t];
+    }
+    int alpha{1};
+    for (int i{0}; i < 16; ++i) {
+        if (hist[i]) {
+            alpha = i + 1;
+        }
+    }
+    w.write(1, 0);
+    w.write(1, mtf ? 1 : 0);
+    std::uint8_t depth[16];
+    std::uint16_t bits[16];
+    prefix_histograms(w, hist, alpha, 1, 2, 0, 1, depth, bits);
+    for (std::uint8_t v : sym) {
+        std::uint32_t t, nb, b;
+        hybrid_uint(2, 0, 1, v, t, nb, b);
+        w.write(depth[t], bits[t]);
+        w.write(nb, b);
+    }
+}
+
+HostBits best_context_map(const std::uint8_t* map, int n, int nclusters) {
+    HostBits cx, cm, sp;
+    complex_cmap(cx, map, n, false);
+    complex_cmap(cm, map, n, true);
+    const HostBits* sel{&cx};
+    if (cm.pos < sel->pos) {
+        sel = &cm;
+    }
+    if (nclusters <= 8) {
+        const std::uint32_t bpe{ceil_log2(static_cast<std::uint32_t>(nclusters))};
+        sp.write(1, 1);
+        sp.write(2, bpe);
+        for (int i{0}; i < n; ++i) {
+            sp.write(bpe, map[i]);
+        }
+        if (sp.pos <= sel->pos) {
+            sel = &sp;
+        }
+    }
+    return *sel;
+}
+
+// ac_context.h:108-131 on the host (the fixed 32-cluster context map).
+int cluster_of(std::uint32_t ctx) {
+    constexpr std::uint32_t count_contexts{15u * 37u};
+    if (ctx < count_contexts) {
+        const std::uint32_t bucket{ctx / 15u};
+        const int group{static_cast<int>(ctx % 15u) < 7 ? 0 : 1};
+        return group * 4 + (bucket < 4 ? 0 : (bucket < 12 ? 1 : (bucket < 24 ? 2 : 3)));
+    }
+    const std::uint32_t rel{ctx - count_contexts};
+    const int group{static_cast<int>(rel / 458u) < 7 ? 0 : 1};
+    const std::uint32_t z{rel % 458u};
+    static const std::uint32_t edges[11]{32, 64, 96, 128, 160, 200, 240, 280, 328, 376, 424};
+    int band{11};
+    for (int i{0}; i < 11; ++i) {
+        if (z < edges[i]) {
+            band = i;
+            break;
+        }
+    }
+    return 8 + group * 12 + band;
+}
+
+const HostBits& ac_context_map_bits() {
+    static const HostBits bits{[] {
+        std::vector<std::uint8_t> map(kNumAcContexts);
+        for (int i{0}; i < kNumAcContexts; ++i) {
+            map[i] = static_cast<std::uint8_t>(cluster_of(static_cast<std::uint32_t>(i)));
+        }
+        return best_context_map(map.data(), kNumAcContexts, kNumClusters);
+    }()};
+    return bits;
+}
+
+// frame_assembly.cc:70-101 up to (not including) the data alphabet size:
+// GroupHeader, single-leaf tree with `predictor`, the data histogram's fixed
+// leading fields.
+void modular_prefix(HostBits& w, int predictor) {
+    w.write(1, 0);
+    w.write(1, 1);
+    w.write(2, 0);
+    const std::uint32_t tokens[5]{0, static_cast<std::uint32_t>(predictor), 0, 0, 0};
+    std::uint32_t th[16]{};
+    int tl{1};
+    for (std::uint32_t t : tokens) {
+        ++th[t];
+        tl = std::max(tl, static_cast<int>(t) + 1);
+    }
+    std::uint8_t td[16];
+    std::uint16_t tb[16];
+    prefix_histograms(w, th, tl, 6, 4, 2, 0, td, tb);
+    for (std::uint32_t t : tokens) {
+        w.write(td[t], tb[t]);
+    }
+    w.write(1, 0);  // data container: no LZ77
+    w.write(1, 1);  // prefix codes
+    write_uint_config(w, 4, 2, 0);
+}
+
+std::once_flag g_tables_once[64];
+bool g_tables_ok[64];
+
+}  // namespace
+
+bool device_exists(int ordinal) {
+    int count{0};
+    if (hipGetDeviceCount(&count) != hipSuccess) {
+        return false;
+    }
+    return ordinal >= 0 && ordinal < count;
+}
+
+bool device_has_matrix_cores(int ordinal) {
+    hipDeviceProp_t p{};
+    if (hipGetDeviceProperties(&p, ordinal) != hipSuccess) {
+        return false;
+    }
+    const std::string arch{p.gcnArchName};
+    return arch.rfind("gfx90a", 0) == 0 || arch.rfind("gfx94", 0) == 0 || arch.rfind("gfx95", 0) == 0;
+}
+
+std::size_t Encoder::max_output_size(std::uint32_t width, std::uint32_t height) {
+    const std::size_t bw{width / 8u}, bh{height / 8u};
+    const std::size_t nac{((bw + 31) / 32) * ((bh + 31) / 32)};
+    const std::size_t ndc{((bw + 255) / 256) * ((bh + 255) / 256)};
+    return 3 * bw * bh * 384 + 48 * bw * bh + (nac + ndc) * 2048 + 16384;
+}
+
+Encoder::~Encoder() {
+    release();
+}
+
+void Encoder::release() {
+    if (device_ >= 0) {
+        (void)hipSetDevice(device_);
+    }
+    for (void* p : allocations_) {
+        (void)hipFree(p);
+    }
+    allocations_.clear();
+    if (total_host_ != nullptr) {
+        (void)hipHostFree(total_host_);
+        total_host_ = nullptr;
+    }
+    for (hipEvent_t& e : events_) {
+        if (e != nullptr) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    }
+    if (stream_ != nullptr) {
+        (void)hipStreamDestroy(stream_);
+        stream_ = nullptr;
+    }
+}
+
+bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distance, int device) {
+    if (width == 0 || height == 0 || width % 8 || height % 8) {
+        return false;
+    }
+    const Geometry geo{Geometry::of(width, height)};
+    if (geo.num_dc > 32) {
+        return false;  // per-group tables are sized for up to 32 DC groups
+    }
+    const bool same_shape{device == device_ && width == geo_.width && height == geo_.height && stream_ != nullptr};
+    distance_ = distance;
+    prm_ = make_frame_params(distance);
+    if (same_shape) {
+        return true;
+    }
+    release();
+    device_ = device;
+    geo_ = geo;
+    if (hipSetDevice(device) != hipSuccess) {
+        return false;
+    }
+    std::call_once(g_tables_once[device & 63], [device] { g_tables_ok[device & 63] = init_device_tables(); });
+    if (!g_tables_ok[device & 63]) {
+        return false;
+    }
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) {
+        return false;
+    }
+    for (hipEvent_t& e : events_) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            return false;
+        }
+    }
+    if (hipHostMalloc(reinterpret_cast<void**>(&total_host_), sizeof(std::uint64_t)) != hipSuccess) {
+        return false;
+    }
+    return alloc();
+}
+
+bool Encoder::alloc() {
+    const std::size_t nblk{geo_.nblocks};
+    const std::size_t tiles{static_cast<std::size_t>(geo_.cmw) * geo_.cmh};
+    const std::size_t regions{static_cast<std::size_t>(geo_.rw) * geo_.rh};
+    const std::size_t nh{static_cast<std::size_t>(kNumClusters) + 2 * geo_.num_dc};
+    out_cap_ = (max_output_size(geo_.width, geo_.height) + 64 + 3) & ~static_cast<std::size_t>(3);
+    bool ok{true};
+    auto get = [&](auto*& p, std::size_t bytes) {
+        void* q{nullptr};
+        if (ok && hipMalloc(&q, bytes < 256 ? 256 : bytes) == hipSuccess) {
+            allocations_.push_back(q);
+            p = static_cast<std::remove_reference_t<decltype(p)>>(q);
+        } else {
+            ok = false;
+        }
+    };
+    get(coeffs_, 3 * nblk * 64 * sizeof(std::uint16_t));
+    get(acs_, nblk);
+    get(qf_, nblk * sizeof(std::int32_t));
+    get(cfl_part_, regions * 3 * sizeof(float));
+    get(ytox_, tiles);
+    get(ytob_, tiles);
+    get(ac_, 3 * nblk * 64 * sizeof(std::int16_t));
+    get(dc_, 3 * nblk * sizeof(std::int32_t));
+    get(nz_grid_, 3 * nblk);
+    get(nz_count_, 3 * nblk * sizeof(std::uint16_t));
+    get(hists_, nh * kHistStride * sizeof(std::uint32_t));
+    get(fb_count_, 64 * sizeof(std::uint32_t));
+    get(codes_, sizeof_codeset());
+    get(constants_, sizeof_constants());
+    get(blobs_, sizeof_dc_blobs());
+    get(blob_words_, static_cast<std::size_t>(geo_.num_dc) * blob_words_per_group() * sizeof(std::uint32_t));
+    get(item_bits_, 3 * nblk * sizeof(std::uint32_t));
+    get(item_off_, 3 * nblk * sizeof(std::uint32_t));
+    get(ac_group_bytes_, geo_.num_ac * sizeof(std::uint32_t));
+    get(ac_group_bits_, geo_.num_ac * sizeof(std::uint64_t));
+    get(chunk_bits_, static_cast<std::size_t>(geo_.num_dc) * dc_chunks_per_group() * sizeof(std::uint64_t));
+    get(dc_group_bytes_, geo_.num_dc * sizeof(std::uint32_t));
+    get(dc_group_bits_, geo_.num_dc * sizeof(std::uint64_t));
+    get(layout_, sizeof_layout());
+    get(ac_base_, geo_.num_ac * sizeof(std::uint64_t));
+    get(dc_base_, geo_.num_dc * sizeof(std::uint64_t));
+    get(out_, out_cap_);
+    const_gs_ = const_qdc_ = 0;
+    return ok;
+}
+
+// Constant bit strings of a configuration (codestream.cc, frame_assembly.cc).
+bool Encoder::build_constants(std::uint32_t global_scale, std::uint32_t quant_dc) {
+    if (global_scale == const_gs_ && quant_dc == const_qdc_) {
+        return true;
+    }
+    HostBits head;
+    head.write(8, 0xff);
+    head.write(8, 0x0a);
+    const bool small{geo_.width % 8 == 0 && geo_.height % 8 == 0 && geo_.width <= 256 && geo_.height <= 256};
+    head.write(1, small);
+    if (small) {
+        head.write(5, geo_.height / 8 - 1);
+    } else {
+        write_u32(head, kDimEnc, geo_.height);
+    }
+    head.write(3, 0);
+    if (small) {
+        head.write(5, geo_.width / 8 - 1);
+    } else {
+        write_u32(head, kDimEnc, geo_.width);
+    }
+    head.write(1, 1);  // ImageMetadata all_default
+    head.write(1, 1);  // CustomTransformData all_default
+    head.pad();
+    // FrameHeader (codestream.cc:67-85).
+    const Distr ft[4]{{true, 0, 0}, {true, 1, 0}, {true, 2, 0}, {true, 3, 0}};
+    const Distr ups[4]{{true, 1, 0}, {true, 2, 0}, {true, 4, 0}, {true, 8, 0}};
+    const Distr passes[4]{{true, 1, 0}, {true, 2, 0}, {true, 3, 0}, {false, 4, 3}};
+    const Distr blend[4]{{true, 0, 0}, {true, 1, 0}, {true, 2, 0}, {false, 3, 2}};
+    const Distr name[4]{{true, 0, 0}, {false, 0, 4}, {false, 16, 5}, {false, 48, 10}};
+    head.write(1, 0);
+    write_u32(head, ft, 0);
+    head.write(1, 0);
+    head.write(2, 0);  // flags = 0 (U64)
+    write_u32(head, ups, 1);
+    head.write(3, 2);
+    head.write(3, 2);
+    write_u32(head, passes, 1);
+    head.write(1, 0);
+    write_u32(head, blend, 0);
+    head.write(1, 1);
+    write_u32(head, name, 0);
+    head.write(1, 1);
+    head.write(2, 0);  // extensions = 0 (U64)
+
+    HostBits dcg;  // vardct_frame.cc:533-540
+    dcg.write(1, 1);
+    write_u32(dcg, kGlobalScaleEnc, global_scale);
+    write_u32(dcg, kQuantDcEnc, quant_dc);
+    dcg.write(1, 1);
+    dcg.write(1, 1);
+    dcg.write(1, 0);
+
+    HostBits acp;  // frame_assembly.cc:131-146 + histogram_writer.cc:85-104
+    acp.write(1, 1);
+    acp.write(ceil_log2(geo_.num_ac), 0);
+    write_u32(acp, kOrderEnc, 0);
+    acp.write(1, 0);
+    acp.append(ac_context_map_bits());
+    acp.write(1, 1);
+    for (int c{0}; c < kNumClusters; ++c) {
+        write_uint_config(acp, 4, 2, 0);
+    }
+
+    HostBits pre;  // DecodeVarDCTDC extra_precision + modular prefix
+    pre.write(2, 0);
+    modular_prefix(pre, 5);
+    HostBits mid;
+    modular_prefix(mid, 0);
+
+    if (!constants_fit(head.pos, acp.pos, pre.pos, mid.pos)) {
+        return false;
+    }
+    std::vector<std::uint8_t> host(sizeof_constants());
+    fill_constants(host.data(), head.words.data(), head.pos, dcg.words.data(), dcg.pos, acp.words.data(),
+                   acp.pos, pre.words.data(), pre.pos, mid.words.data(), mid.pos);
+    if (hipMemcpyAsync(constants_, host.data(), host.size(), hipMemcpyHostToDevice, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess) {
+        return false;
+    }
+    const_gs_ = global_scale;
+    const_qdc_ = quant_dc;
+    return true;
+}
+
+void Encoder::mark(Stage s) {
+    if (profiling_) {
+        (void)hipEventRecord(events_[static_cast<int>(s)], stream_);
+    }
+}
+
+bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                          std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
+    if (hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+        return false;
+    }
+    mark(Stage::kRegion);
+    launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_);
+    mark(Stage::kQuantize);
+    launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
+                         nz_count_);
+    return run_entropy(out, cap, size);
+}
+
+bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc, const std::int8_t* acs,
+                                  const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                                  std::uint32_t global_scale, std::uint32_t quant_dc, std::uint8_t* out,
+                                  std::size_t cap, std::size_t* size) {
+    if (hipSetDevice(device_) != hipSuccess || !build_constants(global_scale, quant_dc)) {
+        return false;
+    }
+    const std::size_t nblk{geo_.nblocks};
+    const std::size_t tiles{static_cast<std::size_t>(geo_.cmw) * geo_.cmh};
+    std::vector<std::int8_t> zeros(tiles, 0);
+    const bool ok{
+        hipMemcpyAsync(ac_, ac, 3 * nblk * 64 * sizeof(std::int16_t), hipMemcpyHostToDevice, stream_) == hipSuccess &&
+        hipMemcpyAsync(dc_, dc, 3 * nblk * sizeof(std::int32_t), hipMemcpyHostToDevice, stream_) == hipSuccess &&
+        hipMemcpyAsync(acs_, acs, nblk, hipMemcpyHostToDevice, stream_) == hipSuccess &&
+        hipMemcpyAsync(qf_, qf, nblk * sizeof(std::int32_t), hipMemcpyHostToDevice, stream_) == hipSuccess &&
+        hipMemcpyAsync(ytox_, ytox != nullptr ? ytox : zeros.data(), tiles, hipMemcpyHostToDevice, stream_) ==
+            hipSuccess &&
+        hipMemcpyAsync(ytob_, ytob != nullptr ? ytob : zeros.data(), tiles, hipMemcpyHostToDevice, stream_) ==
+            hipSuccess &&
+        hipStreamSynchronize(stream_) == hipSuccess};
+    if (!ok) {
+        return false;
+    }
+    mark(Stage::kRegion);
+    mark(Stage::kQuantize);
+    launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_);
+    return run_entropy(out, cap, size);
+}
+
+bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size) {
+    const std::size_t nh{static_cast<std::size_t>(kNumClusters) + 2 * geo_.num_dc};
+    auto* codes{static_cast<CodeSet*>(codes_)};
+    auto* k{static_cast<const Constants*>(constants_)};
+    auto* blobs{static_cast<DcBlobs*>(blobs_)};
+    auto* lay{static_cast<Layout*>(layout_)};
+    mark(Stage::kHistograms);
+    launch_fill_u32(stream_, hists_, nh * kHistStride, 0u);
+    launch_fill_u32(stream_, fb_count_, 64, 0u);
+    launch_fill_u32(stream_, static_cast<std::uint32_t*>(blobs_), sizeof_dc_blobs() / 4, 0u);
+    launch_ac_hist(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, hists_);
+    launch_dc_hist(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
+                   hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
+    mark(Stage::kCodes);
+    launch_huffman(stream_, static_cast<int>(nh), hists_, codes);
+    launch_dc_blobs(stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
+    mark(Stage::kSizes);
+    launch_ac_bits(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, codes, item_bits_);
+    launch_ac_group_scan(stream_, geo_, item_bits_, item_off_, ac_group_bytes_, ac_group_bits_);
+    launch_dc_bits(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
+                   dc_group_bits_);
+    launch_layout(stream_, geo_, k, codes, ac_group_bytes_, dc_group_bytes_, ac_group_bits_, dc_group_bits_, blobs,
+                  lay, ac_base_, dc_base_);
+    mark(Stage::kEmit);
+    launch_zero_and_headers(stream_, k, lay, out_, out_cap_ / 4);
+    launch_ac_emit(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, codes, item_off_, ac_base_, out_, out_cap_ / 4);
+    launch_dc_emit(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_base_,
+                   out_, out_cap_ / 4);
+    mark(Stage::kReadback);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),
+                       sizeof(std::uint64_t), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipStreamSynchronize(stream_) != hipSuccess) {
+        return false;
+    }
+    const std::uint64_t total{*total_host_};
+    *size = static_cast<std::size_t>(total);
+    if (total > out_cap_) {
+        return false;
+    }
+    if (total <= cap && out != nullptr) {
+        if (hipMemcpyAsync(out, out_, total, hipMemcpyDeviceToHost, stream_) != hipSuccess) {
+            return false;
+        }
+    }
+    mark(Stage::kCount);
+    if (hipStreamSynchronize(stream_) != hipSuccess) {
+        return false;
+    }
+    if (profiling_) {
+        for (int i{0}; i < static_cast<int>(Stage::kCount); ++i) {
+            float ms{0.0f};
+            (void)hipEventElapsedTime(&ms, events_[i], events_[i + 1]);
+            stage_ms_[i] = ms;
+        }
+    }
+    return true;
+}
+
+std::size_t Encoder::buffer_bytes(Buffer which) const {
+    const std::size_t nblk{geo_.nblocks};
+    const std::size_t tiles{static_cast<std::size_t>(geo_.cmw) * geo_.cmh};
+    switch (which) {
+        case Buffer::kCoeffs: return 3 * nblk * 64 * 2;
+        case Buffer::kAcs: return nblk;
+        case Buffer::kQuantField: return nblk * 4;
+        case Buffer::kCflPartials: return static_cast<std::size_t>(geo_.rw) * geo_.rh * 12;
+        case Buffer::kYtoX:
+        case Buffer::kYtoB: return tiles;
+        case Buffer::kAc: return 3 * nblk * 64 * 2;
+        case Buffer::kDc: return 3 * nblk * 4;
+        case Buffer::kNzGrid: return 3 * nblk;
+        case Buffer::kNzCount: return 3 * nblk * 2;
+        case Buffer::kHistograms: return (kNumClusters + 2 * geo_.num_dc) * kHistStride * 4;
+        case Buffer::kAcGroupBytes: return geo_.num_ac * 4;
+        case Buffer::kDcGroupBytes: return geo_.num_dc * 4;
+        case Buffer::kCodeDepth: return (kNumClusters + 2 * geo_.num_dc) * kHistStride;
+        case Buffer::kCodeBits: return (kNumClusters + 2 * geo_.num_dc) * kHistStride * 2;
+        default: return 0;
+    }
+}
+
+bool Encoder::fetch(Buffer which, void* host, std::size_t bytes) {
+    if (bytes > buffer_bytes(which) || hipSetDevice(device_) != hipSuccess) {
+        return false;
+    }
+    const void* src{nullptr};
+    switch (which) {
+        case Buffer::kCoeffs: src = coeffs_; break;
+        case Buffer::kAcs: src = acs_; break;
+        case Buffer::kQuantField: src = qf_; break;
+        case Buffer::kCflPartials: src = cfl_part_; break;
+        case Buffer::kYtoX: src = ytox_; break;
+        case Buffer::kYtoB: src = ytob_; break;
+        case Buffer::kAc: src = ac_; break;
+        case Buffer::kDc: src = dc_; break;
+        case Buffer::kNzGrid: src = nz_grid_; break;
+        case Buffer::kNzCount: src = nz_count_; break;
+        case Buffer::kHistograms: src = hists_; break;
+        case Buffer::kAcGroupBytes: src = ac_group_bytes_; break;
+        case Buffer::kDcGroupBytes: src = dc_group_bytes_; break;
+        case Buffer::kCodeDepth:
+        case Buffer::kCodeBits: {
+            // Rows of the code set, one per histogram.
+            const std::size_t nh{static_cast<std::size_t>(kNumClusters) + 2 * geo_.num_dc};
+            const auto* cs{static_cast<const std::uint8_t*>(codes_)};
+            const std::size_t row{static_cast<std::size_t>(which == Buffer::kCodeDepth ? kHistStride : kHistStride * 2)};
+            const std::size_t base{static_cast<std::size_t>(which == Buffer::kCodeDepth ? 0 : 96 * kHistStride)};
+            return hipMemcpy(host, cs + base, std::min(bytes, nh * row), hipMemcpyDeviceToHost) == hipSuccess;
+        }
+        default: return false;
+    }
+    return hipMemcpy(host, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+}
+
+}  // namespace cjx

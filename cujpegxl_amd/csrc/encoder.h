@@ -1,0 +1,119 @@
+// The frame encoder: owns one stream and every device buffer of a
+// configuration, and runs the whole VarDCT pipeline for a frame.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace cjx {
+
+enum class Stage : int {
+    kRegion = 0,   // XYB, gaborish, AQ, selection, DCT, CfL partials
+    kQuantize,     // CfL maps, quantization, DC, non-zero counts
+    kHistograms,   // AC cluster, DC and AcMetadata histograms
+    kCodes,        // prefix codes and code headers
+    kSizes,        // per-item bits, group scans, section layout
+    kEmit,         // zero fill, headers, AC and DC token emission
+    kReadback,     // size and bytes to the host
+    kCount
+};
+
+// Internal buffers a harness can read back after an encode.
+enum class Buffer : int {
+    kCoeffs = 0, kAcs, kQuantField, kCflPartials, kYtoX, kYtoB, kAc, kDc, kNzGrid, kNzCount,
+    kHistograms, kAcGroupBytes, kDcGroupBytes, kCodeDepth, kCodeBits, kCount
+};
+
+class Encoder {
+public:
+    Encoder() = default;
+    ~Encoder();
+    Encoder(const Encoder&) = delete;
+    Encoder& operator=(const Encoder&) = delete;
+
+    // Selects the device and (re)allocates for the frame size; false on error.
+    bool configure(std::uint32_t width, std::uint32_t height, float distance, int device);
+
+    // NV12 device planes -> `.jxl` bytes into `out` (capacity `cap`).
+    // Returns false on a device error; `*size` is the file size either way.
+    bool encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                     std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size);
+
+    // Entropy coding and assembly only, for coefficients supplied from the host
+    // (same layouts as the front end's outputs).
+    bool encode_coefficients(const std::int16_t* ac, const std::int32_t* dc, const std::int8_t* acs,
+                             const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                             std::uint32_t global_scale, std::uint32_t quant_dc, std::uint8_t* out,
+                             std::size_t cap, std::size_t* size);
+
+    bool fetch(Buffer which, void* host, std::size_t bytes);
+    std::size_t buffer_bytes(Buffer which) const;
+
+    void set_profiling(bool on) { profiling_ = on; }
+    // Milliseconds per stage of the last encode (profiling on).
+    const std::array<float, static_cast<int>(Stage::kCount)>& stage_ms() const { return stage_ms_; }
+    hipStream_t stream() const { return stream_; }
+    const Geometry& geometry() const { return geo_; }
+
+    static std::size_t max_output_size(std::uint32_t width, std::uint32_t height);
+
+private:
+    bool alloc();
+    void release();
+    bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
+    bool run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size);
+    void mark(Stage s);
+
+    int device_{-1};
+    float distance_{1.0f};
+    Geometry geo_{};
+    FrameParams prm_{};
+    hipStream_t stream_{nullptr};
+    bool profiling_{false};
+    std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
+    std::array<float, static_cast<int>(Stage::kCount)> stage_ms_{};
+    std::uint32_t const_gs_{0}, const_qdc_{0};
+
+    // Device buffers.
+    std::uint16_t* coeffs_{nullptr};
+    std::int8_t* acs_{nullptr};
+    std::int32_t* qf_{nullptr};
+    float* cfl_part_{nullptr};
+    std::int8_t* ytox_{nullptr};
+    std::int8_t* ytob_{nullptr};
+    std::int16_t* ac_{nullptr};
+    std::int32_t* dc_{nullptr};
+    std::uint8_t* nz_grid_{nullptr};
+    std::uint16_t* nz_count_{nullptr};
+    std::uint32_t* hists_{nullptr};
+    std::uint32_t* fb_count_{nullptr};
+    void* codes_{nullptr};
+    void* constants_{nullptr};
+    void* blobs_{nullptr};
+    std::uint32_t* blob_words_{nullptr};
+    std::uint32_t* item_bits_{nullptr};
+    std::uint32_t* item_off_{nullptr};
+    std::uint32_t* ac_group_bytes_{nullptr};
+    std::uint64_t* ac_group_bits_{nullptr};
+    std::uint64_t* chunk_bits_{nullptr};
+    std::uint32_t* dc_group_bytes_{nullptr};
+    std::uint64_t* dc_group_bits_{nullptr};
+    void* layout_{nullptr};
+    std::uint64_t* ac_base_{nullptr};
+    std::uint64_t* dc_base_{nullptr};
+    std::uint32_t* out_{nullptr};
+    std::size_t out_cap_{0};
+    std::uint64_t* total_host_{nullptr};  // pinned
+    std::vector<void*> allocations_{};
+};
+
+bool device_exists(int ordinal);
+bool device_has_matrix_cores(int ordinal);
+
+}  // namespace cjx

@@ -1,0 +1,620 @@
+// Entropy-coding kernels (included into device.hip).
+//
+// AC: one wave tokenizes one block-channel (an "item": first-block x channel
+// in decoder order Y, X, B).  Each 64-coefficient chunk of the block's scan
+// order is classified with one ballot; a lane's remaining-non-zero count and
+// previous-coefficient flag -- the two serial dependencies of the AC context
+// model (entropy.cu:648-681, ac_context.h:78-84) -- come from a popcount of
+// the ballot below it, so every lane forms its token in parallel.  The same
+// tokenizer drives three passes: cluster histograms, per-item bit counts, and
+// bit emission at scanned offsets.
+//
+// DC: a DcGroup section is a list of items (header blobs, DC residual tokens,
+// AcMetadata tokens), each with a known bit length once the codes exist;
+// chunked scans give every item its bit offset and the emitter ORs its bits in.
+
+namespace cjx {
+
+__constant__ std::uint8_t c_ctx_map[39]{0, 1, 2, 2, 3,  3,  4,  5,  6,  6,  6,  6,  6,
+                                        7, 8, 9, 9, 10, 11, 12, 13, 14, 14, 14, 14, 14,
+                                        7, 8, 9, 9, 10, 11, 12, 13, 14, 14, 14, 14, 14};
+__constant__ std::uint16_t c_freq_ctx[64]{
+    0,  0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13, 14, 15, 15, 16, 16, 17, 17,
+    18, 18, 19, 19, 20, 20, 21, 21, 22, 22, 23, 23, 23, 23, 24, 24, 24, 24, 25, 25, 25, 25,
+    26, 26, 26, 26, 27, 27, 27, 27, 28, 28, 28, 28, 29, 29, 29, 29, 30, 30, 30, 30};
+__constant__ std::uint16_t c_nz_ctx[64]{
+    0,   0,   31,  62,  62,  93,  93,  93,  93,  123, 123, 123, 123, 152, 152, 152,
+    152, 152, 152, 152, 152, 180, 180, 180, 180, 180, 180, 180, 180, 180, 180, 180,
+    180, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206,
+    206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206};
+
+// ac_context.h:108-131
+__device__ __forceinline__ int ac_cluster(std::uint32_t ctx) {
+    constexpr std::uint32_t count_contexts{15u * 37u};
+    if (ctx < count_contexts) {
+        const std::uint32_t bucket{ctx / 15u};
+        const int group{static_cast<int>(ctx % 15u) < 7 ? 0 : 1};
+        const int band{bucket < 4 ? 0 : (bucket < 12 ? 1 : (bucket < 24 ? 2 : 3))};
+        return group * 4 + band;
+    }
+    const std::uint32_t rel{ctx - count_contexts};
+    const int group{static_cast<int>(rel / 458u) < 7 ? 0 : 1};
+    const std::uint32_t z{rel % 458u};
+    const int band{z < 32    ? 0
+                   : z < 64  ? 1
+                   : z < 96  ? 2
+                   : z < 128 ? 3
+                   : z < 160 ? 4
+                   : z < 200 ? 5
+                   : z < 240 ? 6
+                   : z < 280 ? 7
+                   : z < 328 ? 8
+                   : z < 376 ? 9
+                   : z < 424 ? 10
+                             : 11};
+    return 8 + group * 12 + band;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+    for (int off{32}; off > 0; off >>= 1) {
+        v += __shfl_down(v, off, 64);
+    }
+    return __shfl(v, 0, 64);
+}
+
+// Exclusive prefix over the wave.
+__device__ __forceinline__ int wave_excl_scan(int v, int lane) {
+    int x{v};
+#pragma unroll
+    for (int off{1}; off < 64; off <<= 1) {
+        const int y{__shfl_up(x, off, 64)};
+        if (lane >= off) {
+            x += y;
+        }
+    }
+    return x - v;
+}
+
+// ORs `len` (<= 32) bits of `value` at absolute bit position `pos`; words at
+// or beyond `cap_words` are never touched (the layout makes that unreachable;
+// the check keeps an inconsistent layout from writing out of bounds).
+__device__ __forceinline__ void or_bits(std::uint32_t* words, std::uint64_t cap_words, std::uint64_t pos,
+                                        std::uint32_t value, std::uint32_t len) {
+    if (len == 0) {
+        return;
+    }
+    if (len < 32) {
+        value &= (1u << len) - 1u;
+    }
+    const std::uint64_t wi{pos >> 5};
+    const std::uint32_t b{static_cast<std::uint32_t>(pos & 31u)};
+    if (wi < cap_words) {
+        atomicOr(&words[wi], value << b);
+    }
+    if (b + len > 32 && wi + 1 < cap_words) {
+        atomicOr(&words[wi + 1], value >> (32 - b));
+    }
+}
+
+struct AcItem {
+    int n;             // transform side, 0 if covered
+    int c;             // XYB channel
+    int bx, by;
+    std::uint32_t nz;  // non-zero AC count
+    std::uint32_t nz_ctx;
+    int bctx;
+};
+
+__device__ __forceinline__ AcItem ac_item(const Geometry& geo, const std::int8_t* acs,
+                                          const std::uint8_t* nz_grid, const std::uint16_t* nz_count,
+                                          std::uint32_t item) {
+    AcItem it{};
+    const std::uint32_t pos{item / 3};
+    const int ci{static_cast<int>(item % 3)};
+    it.c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+    it.bx = static_cast<int>(pos % geo.bw);
+    it.by = static_cast<int>(pos / geo.bw);
+    it.n = acs[pos];
+    if (it.n == 0) {
+        return it;
+    }
+    const std::size_t nblk{geo.nblocks};
+    it.nz = nz_count[it.c * nblk + pos];
+    const int ord{it.n == 16 ? 2 : (it.n == 32 ? 3 : 0)};
+    it.bctx = c_ctx_map[(it.c < 2 ? (it.c ^ 1) : 2) * 13 + ord];
+    const bool hl{(it.bx % kAcGroupBlocks) > 0}, ht{(it.by % kAcGroupBlocks) > 0};
+    const int top{ht ? nz_grid[it.c * nblk + pos - geo.bw] : 0};
+    const int left{hl ? nz_grid[it.c * nblk + pos - 1] : 0};
+    std::uint32_t pred{!hl ? (ht ? static_cast<std::uint32_t>(top) : 32u)
+                           : (!ht ? static_cast<std::uint32_t>(left)
+                                  : static_cast<std::uint32_t>((top + left + 1) / 2))};
+    pred = pred >= 64 ? 64 : pred;
+    it.nz_ctx = (pred < 8 ? pred : 4 + pred / 2) * 15u + static_cast<std::uint32_t>(it.bctx);
+    return it;
+}
+
+// Tokenizes one item across the wave; `emit(valid, cluster, sym, nbits, extra)`
+// is called once per lane per chunk (lane 0 of the first call carries the
+// non-zero-count token).
+template <typename Emit>
+__device__ __forceinline__ void tokenize_item(const Geometry& geo, const std::int16_t* ac,
+                                              const AcItem& it, int lane, Emit&& emit) {
+    {
+        std::uint32_t sym, nb, bits;
+        hybrid420(it.nz, sym, nb, bits);
+        emit(lane == 0, ac_cluster(it.nz_ctx), sym, nb, bits);
+    }
+    const int n{it.n};
+    const int side{n / 8};
+    const std::uint32_t cb{static_cast<std::uint32_t>(side * side)};
+    const std::uint32_t size{static_cast<std::uint32_t>(n * n)};
+    const std::uint32_t l2{n == 16 ? 2u : (n == 32 ? 4u : 0u)};
+    const std::uint32_t off{15u * 37u + 458u * static_cast<std::uint32_t>(it.bctx)};
+    const std::int16_t* plane{ac + static_cast<std::size_t>(it.c) * geo.nblocks * 64};
+    const std::uint16_t* order{order_of(n)};
+    std::uint32_t seen{0};
+    std::uint32_t prev_carry{it.nz > size / 16 ? 0u : 1u};
+    const std::uint64_t below{(1ull << lane) - 1ull};
+    for (std::uint32_t base{cb}; base < size && seen < it.nz; base += 64) {
+        const std::uint32_t k{base + static_cast<std::uint32_t>(lane)};
+        int v{0};
+        if (k < size) {
+            const std::uint32_t e{order[k]};
+            const std::uint32_t q{e >> 6};
+            const std::size_t slot{(static_cast<std::size_t>(it.by + q / side) * geo.bw + it.bx +
+                                    q % side) * 64 + (e & 63u)};
+            v = plane[slot];
+        }
+        const std::uint64_t mask{__ballot(v != 0)};
+        const std::uint32_t before{seen + static_cast<std::uint32_t>(__popcll(mask & below))};
+        const std::uint32_t prev{lane == 0 ? prev_carry
+                                           : static_cast<std::uint32_t>((mask >> (lane - 1)) & 1ull)};
+        const bool valid{k < size && before < it.nz};
+        std::uint32_t cl{0}, sym{0}, nb{0}, bits{0};
+        if (valid) {
+            const std::uint32_t rem{((it.nz - before) + cb - 1) >> l2};
+            const std::uint32_t kk{k >> l2};
+            const std::uint32_t ctx{off + (c_nz_ctx[rem] + c_freq_ctx[kk]) * 2 + prev};
+            cl = static_cast<std::uint32_t>(ac_cluster(ctx));
+            hybrid420(pack_signed(v), sym, nb, bits);
+        }
+        emit(valid, static_cast<int>(cl), sym, nb, bits);
+        seen += static_cast<std::uint32_t>(__popcll(mask));
+        prev_carry = static_cast<std::uint32_t>((mask >> 63) & 1ull);
+    }
+}
+
+// Pass 1: per-cluster token histograms.  Grid-stride over items by wave;
+// each workgroup accumulates in LDS and flushes its non-zero bins.
+__global__ __launch_bounds__(256) void ac_hist_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const std::uint8_t* __restrict__ nz_grid,
+                                                      const std::uint16_t* __restrict__ nz_count,
+                                                      std::uint32_t* __restrict__ hist) {
+    __shared__ std::uint32_t h[kNumClusters * kHistStride];
+    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kHistStride; i += 256) {
+        h[i] = 0;
+    }
+    __syncthreads();
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
+    const std::uint32_t nwaves{gridDim.x * 4};
+    const std::uint32_t items{geo.nblocks * 3};
+    for (std::uint32_t item{wave}; item < items; item += nwaves) {
+        const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
+        if (it.n == 0) {
+            continue;
+        }
+        tokenize_item(geo, ac, it, lane,
+                      [&](bool valid, int cl, std::uint32_t sym, std::uint32_t, std::uint32_t) {
+                          if (valid) {
+                              atomicAdd(&h[cl * kHistStride + sym], 1u);
+                          }
+                      });
+    }
+    __syncthreads();
+    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kHistStride; i += 256) {
+        if (h[i]) {
+            atomicAdd(&hist[i], h[i]);
+        }
+    }
+}
+
+// Pass 2: bits per item under the built codes.
+__global__ __launch_bounds__(256) void ac_bits_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const std::uint8_t* __restrict__ nz_grid,
+                                                      const std::uint16_t* __restrict__ nz_count,
+                                                      const std::uint8_t* __restrict__ depth,
+                                                      std::uint32_t* __restrict__ item_bits) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
+    const std::uint32_t nwaves{gridDim.x * 4};
+    const std::uint32_t items{geo.nblocks * 3};
+    for (std::uint32_t item{wave}; item < items; item += nwaves) {
+        const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
+        int total{0};
+        if (it.n != 0) {
+            tokenize_item(geo, ac, it, lane,
+                          [&](bool valid, int cl, std::uint32_t sym, std::uint32_t nb, std::uint32_t) {
+                              if (valid) {
+                                  total += depth[cl * kHistStride + sym] + static_cast<int>(nb);
+                              }
+                          });
+        }
+        total = wave_sum_int(total);
+        if (lane == 0) {
+            item_bits[item] = static_cast<std::uint32_t>(total);
+        }
+    }
+}
+
+// Block-wide exclusive scan of one value per thread (blockDim.x multiple of 64).
+__device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std::uint64_t& total) {
+    const int tid{static_cast<int>(threadIdx.x)};
+    const int lane{tid & 63}, wv{tid >> 6}, nw{static_cast<int>(blockDim.x) >> 6};
+    std::uint64_t x{v};
+#pragma unroll
+    for (int off{1}; off < 64; off <<= 1) {
+        const std::uint64_t y{__shfl_up(x, off, 64)};
+        if (lane >= off) {
+            x += y;
+        }
+    }
+    if (lane == 63) {
+        sh[wv] = x;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        std::uint64_t run{0};
+        for (int i{0}; i < nw; ++i) {
+            const std::uint64_t t{sh[i]};
+            sh[i] = run;
+            run += t;
+        }
+        sh[nw] = run;
+    }
+    __syncthreads();
+    const std::uint64_t r{sh[wv] + x - v};
+    total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+// Per AC group: bit offsets of its items in decoder order (group raster of
+// blocks, channels Y, X, B) and the group's byte size.  One 1024-thread
+// workgroup per group, one block per thread.
+__global__ __launch_bounds__(1024) void ac_group_scan_kernel(Geometry geo,
+                                                             const std::uint32_t* __restrict__ item_bits,
+                                                             std::uint32_t* __restrict__ item_off,
+                                                             std::uint32_t* __restrict__ group_bytes,
+                                                             std::uint64_t* __restrict__ group_bits) {
+    __shared__ std::uint64_t sh[17];
+    const int g{static_cast<int>(blockIdx.x)};
+    const int gx{g % static_cast<int>(geo.xg)}, gy{g / static_cast<int>(geo.xg)};
+    const int bx0{gx * 32}, by0{gy * 32};
+    const int gbw{min(32, static_cast<int>(geo.bw) - bx0)}, gbh{min(32, static_cast<int>(geo.bh) - by0)};
+    const int t{static_cast<int>(threadIdx.x)};
+    const bool has{t < gbw * gbh};
+    std::uint32_t b3[3]{0, 0, 0};
+    std::size_t pos{0};
+    if (has) {
+        pos = static_cast<std::size_t>(by0 + t / gbw) * geo.bw + bx0 + t % gbw;
+        for (int ci{0}; ci < 3; ++ci) {
+            b3[ci] = item_bits[pos * 3 + ci];
+        }
+    }
+    std::uint64_t total{0};
+    const std::uint64_t base{block_excl_scan(static_cast<std::uint64_t>(b3[0]) + b3[1] + b3[2], sh, total)};
+    if (has) {
+        item_off[pos * 3 + 0] = static_cast<std::uint32_t>(base);
+        item_off[pos * 3 + 1] = static_cast<std::uint32_t>(base + b3[0]);
+        item_off[pos * 3 + 2] = static_cast<std::uint32_t>(base + b3[0] + b3[1]);
+    }
+    if (t == 0) {
+        group_bytes[g] = static_cast<std::uint32_t>((total + 7) / 8);
+        group_bits[g] = total;
+    }
+}
+
+// Pass 3: emission at group byte offset + item bit offset.
+__global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const std::uint8_t* __restrict__ nz_grid,
+                                                      const std::uint16_t* __restrict__ nz_count,
+                                                      const std::uint8_t* __restrict__ depth,
+                                                      const std::uint16_t* __restrict__ codes,
+                                                      const std::uint32_t* __restrict__ item_off,
+                                                      const std::uint64_t* __restrict__ ac_group_base_bits,
+                                                      std::uint32_t* __restrict__ out_words,
+                                                      std::uint64_t cap_words) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
+    const std::uint32_t nwaves{gridDim.x * 4};
+    const std::uint32_t items{geo.nblocks * 3};
+    for (std::uint32_t item{wave}; item < items; item += nwaves) {
+        const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
+        if (it.n == 0) {
+            continue;
+        }
+        const std::uint32_t g{static_cast<std::uint32_t>(it.by / kAcGroupBlocks) * geo.xg +
+                              static_cast<std::uint32_t>(it.bx / kAcGroupBlocks)};
+        std::uint64_t pos{ac_group_base_bits[g] + item_off[item]};
+        tokenize_item(geo, ac, it, lane,
+                      [&](bool valid, int cl, std::uint32_t sym, std::uint32_t nb, std::uint32_t extra) {
+                          std::uint32_t len{0}, val{0};
+                          if (valid) {
+                              const int idx{cl * kHistStride + static_cast<int>(sym)};
+                              const std::uint32_t d{depth[idx]};
+                              len = d + nb;
+                              val = static_cast<std::uint32_t>(codes[idx]) | (extra << d);
+                          }
+                          const int pre{wave_excl_scan(static_cast<int>(len), lane)};
+                          or_bits(out_words, cap_words, pos + static_cast<std::uint64_t>(pre), val, len);
+                          pos += static_cast<std::uint64_t>(wave_sum_int(static_cast<int>(len)));
+                      });
+    }
+}
+
+// ---- DC groups -------------------------------------------------------------
+
+// Item kinds of a DcGroup section, in order (vardct_frame.cc:543-554):
+//   0: blob_pre | [0,3n): DC residuals, channels Y, X, B, group raster |
+//   blob_mid | meta: YtoX (cw*ch), YtoB (cw*ch), strategy per block (first
+//   blocks only), quant field per block (first blocks only), EPF zeros (n).
+struct DcGroupGeo {
+    int bx0, by0, dgw, dgh, n, cw, ch;
+    std::uint32_t items;
+};
+
+__device__ __forceinline__ DcGroupGeo dc_group_geo(const Geometry& geo, int g) {
+    DcGroupGeo d{};
+    d.bx0 = (g % static_cast<int>(geo.xdg)) * 256;
+    d.by0 = (g / static_cast<int>(geo.xdg)) * 256;
+    d.dgw = min(256, static_cast<int>(geo.bw) - d.bx0);
+    d.dgh = min(256, static_cast<int>(geo.bh) - d.by0);
+    d.n = d.dgw * d.dgh;
+    d.cw = (d.dgw + 7) / 8;
+    d.ch = (d.dgh + 7) / 8;
+    d.items = static_cast<std::uint32_t>(2 + 6 * d.n + 2 * d.cw * d.ch);
+    return d;
+}
+
+// dc_predict.h:22-49
+__device__ __forceinline__ std::int32_t gradient_predict(bool hl, bool ht, std::int32_t wv,
+                                                         std::int32_t nv, std::int32_t nwv) {
+    const std::int32_t left{hl ? wv : (ht ? nv : 0)};
+    const std::int32_t top{ht ? nv : left};
+    const std::int32_t tl{(hl && ht) ? nwv : left};
+    const std::int32_t lo{left < top ? left : top}, hi{left < top ? top : left};
+    if (tl < lo) {
+        return hi;
+    }
+    if (tl > hi) {
+        return lo;
+    }
+    return static_cast<std::int32_t>(static_cast<std::int64_t>(left) + top - tl);
+}
+
+// Token of DC-section item k (k != 0 and != 1 + 3n).  `kind` 0 = DC code,
+// 1 = AcMetadata code, 2 = absent (a covered block's strategy/QF slot).
+__device__ __forceinline__ int dc_item_token(const Geometry& geo, const DcGroupGeo& d, std::uint32_t k,
+                                             const std::int32_t* dc, const std::int8_t* acs,
+                                             const std::int32_t* qf, const std::int8_t* ytox,
+                                             const std::int8_t* ytob, std::uint32_t& sym,
+                                             std::uint32_t& nb, std::uint32_t& bits) {
+    const std::uint32_t n{static_cast<std::uint32_t>(d.n)};
+    if (k >= 1 && k < 1 + 3 * n) {
+        const std::uint32_t i{k - 1};
+        const int ci{static_cast<int>(i / n)};
+        const int c{ci == 0 ? 1 : (ci == 1 ? 0 : 2)};
+        const int p{static_cast<int>(i % n)};
+        const int x{p % d.dgw}, y{p / d.dgw};
+        const std::int32_t* pl{dc + static_cast<std::size_t>(c) * geo.nblocks};
+        const std::size_t b{static_cast<std::size_t>(d.by0 + y) * geo.bw + d.bx0 + x};
+        const bool hl{x > 0}, ht{y > 0};
+        const std::int32_t pred{gradient_predict(hl, ht, hl ? pl[b - 1] : 0, ht ? pl[b - geo.bw] : 0,
+                                                 (hl && ht) ? pl[b - geo.bw - 1] : 0)};
+        hybrid420(pack_signed(pl[b] - pred), sym, nb, bits);
+        return 0;
+    }
+    const std::uint32_t j{k - (2 + 3 * n)};
+    const std::uint32_t cwch{static_cast<std::uint32_t>(d.cw * d.ch)};
+    std::int32_t v{0};
+    if (j < 2 * cwch) {
+        const std::uint32_t jj{j < cwch ? j : j - cwch};
+        const std::size_t t{static_cast<std::size_t>(d.by0 / 8 + jj / d.cw) * geo.cmw + d.bx0 / 8 + jj % d.cw};
+        v = j < cwch ? ytox[t] : ytob[t];
+    } else if (j < 2 * cwch + 2 * n) {
+        const std::uint32_t jj{(j - 2 * cwch) % n};
+        const std::size_t b{static_cast<std::size_t>(d.by0 + jj / d.dgw) * geo.bw + d.bx0 + jj % d.dgw};
+        const int side{acs[b]};
+        if (side == 0) {
+            return 2;
+        }
+        v = j < 2 * cwch + n ? (side == 16 ? 4 : (side == 32 ? 5 : 0)) : qf[b] - 1;
+    }
+    hybrid420(pack_signed(v), sym, nb, bits);
+    return 1;
+}
+
+constexpr int kDcChunk = 4096;  // items per workgroup in the DC passes
+
+// DC and AcMetadata token histograms per group, plus the first-block count.
+__global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::int32_t* __restrict__ dc,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const std::int32_t* __restrict__ qf,
+                                                      const std::int8_t* __restrict__ ytox,
+                                                      const std::int8_t* __restrict__ ytob,
+                                                      std::uint32_t* __restrict__ dc_hist,
+                                                      std::uint32_t* __restrict__ meta_hist,
+                                                      std::uint32_t* __restrict__ fb_count) {
+    __shared__ std::uint32_t h[2 * kHistStride];
+    __shared__ std::uint32_t cnt;
+    const int g{static_cast<int>(blockIdx.y)};
+    const DcGroupGeo d{dc_group_geo(geo, g)};
+    for (int i{static_cast<int>(threadIdx.x)}; i < 2 * kHistStride; i += 256) {
+        h[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+        cnt = 0;
+    }
+    __syncthreads();
+    const std::uint32_t k0{blockIdx.x * kDcChunk};
+    const std::uint32_t mid{1u + 3u * static_cast<std::uint32_t>(d.n)};
+    const std::uint32_t fb_lo{2u + 3u * static_cast<std::uint32_t>(d.n) + 2u * static_cast<std::uint32_t>(d.cw * d.ch)};
+    for (std::uint32_t k{k0 + threadIdx.x}; k < min(k0 + kDcChunk, d.items); k += 256) {
+        if (k == 0 || k == mid) {
+            continue;
+        }
+        std::uint32_t sym, nb, bits;
+        const int kind{dc_item_token(geo, d, k, dc, acs, qf, ytox, ytob, sym, nb, bits)};
+        if (kind == 0) {
+            atomicAdd(&h[sym], 1u);
+        } else if (kind == 1) {
+            atomicAdd(&h[kHistStride + sym], 1u);
+            if (k >= fb_lo && k < fb_lo + static_cast<std::uint32_t>(d.n)) {
+                atomicAdd(&cnt, 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i{static_cast<int>(threadIdx.x)}; i < kHistStride; i += 256) {
+        if (h[i]) {
+            atomicAdd(&dc_hist[g * kHistStride + i], h[i]);
+        }
+        if (h[kHistStride + i]) {
+            atomicAdd(&meta_hist[g * kHistStride + i], h[kHistStride + i]);
+        }
+    }
+    if (threadIdx.x == 0 && cnt) {
+        atomicAdd(&fb_count[g], cnt);
+    }
+}
+
+struct DcBlobs {
+    // Per group: blob bit lengths and word offsets into `words`.
+    std::uint32_t pre_bits[64], mid_bits[64];
+    std::uint32_t pre_off[64], mid_off[64];
+    std::uint32_t overflow;  // a header blob exceeded its slot
+};
+
+__device__ __forceinline__ std::uint32_t dc_item_bits(const Geometry& geo, const DcGroupGeo& d,
+                                                      std::uint32_t k, int g, const std::int32_t* dc,
+                                                      const std::int8_t* acs, const std::int32_t* qf,
+                                                      const std::int8_t* ytox, const std::int8_t* ytob,
+                                                      const std::uint8_t* dc_depth,
+                                                      const std::uint8_t* meta_depth, const DcBlobs* blobs) {
+    if (k == 0) {
+        return blobs->pre_bits[g];
+    }
+    if (k == 1u + 3u * static_cast<std::uint32_t>(d.n)) {
+        return blobs->mid_bits[g];
+    }
+    std::uint32_t sym, nb, bits;
+    const int kind{dc_item_token(geo, d, k, dc, acs, qf, ytox, ytob, sym, nb, bits)};
+    if (kind == 2) {
+        return 0;
+    }
+    return (kind == 0 ? dc_depth : meta_depth)[g * kHistStride + sym] + nb;
+}
+
+// Chunk bit totals of every DC group (grid: chunks x groups).
+__global__ __launch_bounds__(256) void dc_chunk_bits_kernel(
+    Geometry geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
+    const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
+    const std::int8_t* __restrict__ ytob, const std::uint8_t* __restrict__ dc_depth,
+    const std::uint8_t* __restrict__ meta_depth, const DcBlobs* __restrict__ blobs,
+    std::uint64_t* __restrict__ chunk_bits, std::uint32_t chunks_per_group) {
+    __shared__ std::uint64_t sh[5];
+    const int g{static_cast<int>(blockIdx.y)};
+    const DcGroupGeo d{dc_group_geo(geo, g)};
+    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * (kDcChunk / 256)};
+    std::uint64_t run{0};
+    for (std::uint32_t k{k0}; k < min(k0 + kDcChunk / 256, d.items); ++k) {
+        run += dc_item_bits(geo, d, k, g, dc, acs, qf, ytox, ytob, dc_depth, meta_depth, blobs);
+    }
+    std::uint64_t total{0};
+    block_excl_scan(run, sh, total);
+    if (threadIdx.x == 0) {
+        chunk_bits[static_cast<std::size_t>(g) * chunks_per_group + blockIdx.x] = total;
+    }
+}
+
+// Exclusive scan of chunk totals per group (one workgroup per group).
+__global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::uint64_t* __restrict__ chunk_bits,
+                                                             std::uint32_t chunks_per_group,
+                                                             std::uint32_t* __restrict__ group_bytes,
+                                                             std::uint64_t* __restrict__ group_bits) {
+    __shared__ std::uint64_t sh[17];
+    const int g{static_cast<int>(blockIdx.x)};
+    const DcGroupGeo d{dc_group_geo(geo, g)};
+    const std::uint32_t used{(d.items + kDcChunk - 1) / kDcChunk};
+    const std::uint32_t t{threadIdx.x};
+    std::uint64_t v{0};
+    if (t < used) {
+        v = chunk_bits[static_cast<std::size_t>(g) * chunks_per_group + t];
+    }
+    std::uint64_t total{0};
+    const std::uint64_t ex{block_excl_scan(v, sh, total)};
+    if (t < used) {
+        chunk_bits[static_cast<std::size_t>(g) * chunks_per_group + t] = ex;
+    }
+    if (t == 0) {
+        group_bytes[g] = static_cast<std::uint32_t>((total + 7) / 8);
+        group_bits[g] = total;
+    }
+}
+
+__global__ __launch_bounds__(256) void dc_emit_kernel(
+    Geometry geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
+    const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
+    const std::int8_t* __restrict__ ytob, const std::uint8_t* __restrict__ dc_depth,
+    const std::uint16_t* __restrict__ dc_codes, const std::uint8_t* __restrict__ meta_depth,
+    const std::uint16_t* __restrict__ meta_codes, const DcBlobs* __restrict__ blobs,
+    const std::uint32_t* __restrict__ blob_words, const std::uint64_t* __restrict__ chunk_off,
+    std::uint32_t chunks_per_group, const std::uint64_t* __restrict__ dc_group_base_bits,
+    std::uint32_t* __restrict__ out_words, std::uint64_t cap_words) {
+    __shared__ std::uint64_t sh[5];
+    const int g{static_cast<int>(blockIdx.y)};
+    const DcGroupGeo d{dc_group_geo(geo, g)};
+    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * (kDcChunk / 256)};
+    const std::uint32_t k1{min(k0 + kDcChunk / 256, d.items)};
+    std::uint64_t run{0};
+    for (std::uint32_t k{k0}; k < k1; ++k) {
+        run += dc_item_bits(geo, d, k, g, dc, acs, qf, ytox, ytob, dc_depth, meta_depth, blobs);
+    }
+    std::uint64_t total{0};
+    std::uint64_t pos{block_excl_scan(run, sh, total)};
+    pos += dc_group_base_bits[g] + chunk_off[static_cast<std::size_t>(g) * chunks_per_group + blockIdx.x];
+    const std::uint32_t mid{1u + 3u * static_cast<std::uint32_t>(d.n)};
+    for (std::uint32_t k{k0}; k < k1; ++k) {
+        if (k == 0 || k == mid) {
+            const std::uint32_t nbits{k == 0 ? blobs->pre_bits[g] : blobs->mid_bits[g]};
+            const std::uint32_t* src{blob_words + (k == 0 ? blobs->pre_off[g] : blobs->mid_off[g])};
+            std::uint32_t i{0};
+            for (; i + 32 <= nbits; i += 32) {
+                or_bits(out_words, cap_words, pos + i, src[i >> 5], 32);
+            }
+            if (i < nbits) {
+                or_bits(out_words, cap_words, pos + i, src[i >> 5], nbits - i);
+            }
+            pos += nbits;
+            continue;
+        }
+        std::uint32_t sym, nb, bits;
+        const int kind{dc_item_token(geo, d, k, dc, acs, qf, ytox, ytob, sym, nb, bits)};
+        if (kind == 2) {
+            continue;
+        }
+        const std::uint8_t* dep{kind == 0 ? dc_depth : meta_depth};
+        const std::uint16_t* cod{kind == 0 ? dc_codes : meta_codes};
+        const std::uint32_t dd{dep[g * kHistStride + sym]};
+        or_bits(out_words, cap_words, pos, static_cast<std::uint32_t>(cod[g * kHistStride + sym]) | (bits << dd),
+                dd + nb);
+        pos += dd + nb;
+    }
+}
+
+}  // namespace cjx

@@ -1,0 +1,755 @@
+// Front-end kernels (included into device.hip).
+//
+// region_kernel: one 256-thread workgroup per 32x32-pixel selection region.
+// The region's NV12 pixels plus a 2-pixel clamped halo are converted to XYB
+// once into LDS; everything downstream reads LDS: inverse gaborish, the
+// adaptive-quantization field, the RD transform selection over the 21 DCT8/16/32
+// candidates of the sharpened Y, the chosen DCTs of all three channels and the
+// CfL regression partials.  Only the FP16 coefficients, the transform map, the
+// quant field and three floats per region reach HBM.
+//
+// tile_quantize_kernel: one workgroup per 64x64 CfL tile: colour maps from the
+// region partials, quantization of every coefficient with the CfL residual,
+// LLF -> DC, and the per-block non-zero counts the entropy coder predicts from.
+//
+// Element layout used throughout a region: thread (wave w, lane L) at
+// iteration k owns element i = 256k + 64w + L of [0, 1024).  Element ids nest:
+// a DCT32 owns [0,1024), the DCT16 on quad q owns [256q, 256q+256), the DCT8 on
+// block j of quad q owns [256q + 64j, +64).  Position (quad k, block w) slot L
+// holds raw coefficient e = 64 q + L of its first-block (q = covered ordinal).
+
+namespace cjx {
+
+constexpr int kHalo = 2;
+constexpr int kTileW = 32 + 2 * kHalo;  // 36
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off{32}; off > 0; off >>= 1) {
+        v += __shfl_down(v, off, 64);
+    }
+    return v;
+}
+
+__device__ __forceinline__ float clamp01(float v) {
+    return fminf(fmaxf(v, 0.0f), 1.0f);
+}
+
+// xyb_impl.cuh:37-70 on the shared numeric primitives.
+__device__ __forceinline__ float srgb_to_linear(float e) {
+    return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_det((e + 0.055f) * (1.0f / 1.055f));
+}
+
+__device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float& ox, float& oy,
+                                             float& ob) {
+    constexpr float m00{0.30f}, m02{0.078f}, m01{1.0f - 0.078f - 0.30f};
+    constexpr float m10{0.23f}, m12{0.078f}, m11{1.0f - 0.078f - 0.23f};
+    constexpr float m20{0.24342268924547819f}, m21{0.20476744424496821f};
+    constexpr float m22{1.0f - 0.24342268924547819f - 0.20476744424496821f};
+    constexpr float bias{0.0037930732552754493f};
+    const float r{clamp01(srgb_to_linear(clamp01(yv + 1.5748f * cr)))};
+    const float g{clamp01(srgb_to_linear(clamp01(yv - 0.1873242729f * cb - 0.4681242729f * cr)))};
+    const float b{clamp01(srgb_to_linear(clamp01(yv + 1.8556f * cb)))};
+    float q0{fmaxf(m00 * r + m01 * g + m02 * b + bias, 0.0f)};
+    float q1{fmaxf(m10 * r + m11 * g + m12 * b + bias, 0.0f)};
+    float q2{fmaxf(m20 * r + m21 * g + m22 * b + bias, 0.0f)};
+    const float neg_cbrt_bias{-cbrt_det(bias)};
+    q0 = cbrt_det(q0) + neg_cbrt_bias;
+    q1 = cbrt_det(q1) + neg_cbrt_bias;
+    q2 = cbrt_det(q2) + neg_cbrt_bias;
+    ox = 0.5f * (q0 - q1);
+    oy = 0.5f * (q0 + q1);
+    ob = q2;
+}
+
+// Centered 4:2:0 bilinear, clamped taps (tests/reference.py:23-46): weights
+// 1/4 and 3/4; the result is exact in float.
+__device__ __forceinline__ void chroma_taps(int x, int cw, int& i0, int& i1, float& w0, float& w1) {
+    const int k{x >> 1};
+    if (x & 1) {
+        i0 = k;
+        i1 = k + 1 < cw ? k + 1 : cw - 1;
+        w0 = 0.75f;
+        w1 = 0.25f;
+    } else {
+        i0 = k > 0 ? k - 1 : 0;
+        i1 = k;
+        w0 = 0.25f;
+        w1 = 0.75f;
+    }
+}
+
+// adaptive_quant_impl.cuh:29-61
+__device__ __forceinline__ float ratio_cbrt_gamma(float v, bool invert) {
+    constexpr float k_sg_mul{226.77216153508914f};
+    constexpr float k_sg_log2{0.693147181f};
+    constexpr float k_sg_mul2{1.0f / 73.377132366608819f};
+    constexpr float k_sg_ret_mul{k_sg_mul2 * 18.6580932135f * k_sg_log2};
+    constexpr float eps{1e-2f};
+    constexpr float num_mul{k_sg_ret_mul * 3.0f * k_sg_mul};
+    constexpr float v_offset{7.7825991679894591f * k_sg_log2 + eps};
+    constexpr float den_mul{k_sg_log2 * k_sg_mul};
+    v = v < 0.0f ? 0.0f : v;
+    const float v2{v * v};
+    const float num{num_mul * v2 + eps};
+    const float den{den_mul * v * v2 + v_offset};
+    return invert ? num / den : den / num;
+}
+
+__device__ __forceinline__ float masking_sqrt(float v) {
+    constexpr float mul{211.66567973503678f * 1e8f};
+    return 0.25f * __builtin_sqrtf(v * __builtin_sqrtf(mul) + 27.505837037000106f);
+}
+
+__device__ __forceinline__ float compute_mask(float out_val) {
+    const float v1{fmaxf(out_val * 0.80061762862741759f, 1e-3f)};
+    const float v2{1.0f / (v1 + 302.59587815579727f)};
+    const float v3{1.0f / (v1 * v1 + 3.7179635626140772f)};
+    const float v4{1.0f / (v1 * v1 + 0.25f * 3.7179635626140772f)};
+    return -0.7647f + 9.4708735624378946f * v4 + 17.35036561631863f * v2 + 6.7943250517376494f * v3;
+}
+
+__device__ __forceinline__ void cswap(float& a, float& b) {
+    if (a > b) {
+        const float t{a};
+        a = b;
+        b = t;
+    }
+}
+
+// frontend_fused.cu:35-52 with adaptive_quant_impl.cuh:63-121: weighted sum
+// of the four smallest cells of the clamped 3x3 neighbourhood.
+__device__ float erosion_cell(const float* pre, int cx, int cy, const FrameParams& p) {
+    const int xm1{cx > 0 ? cx - 1 : cx}, xp1{cx + 1 < 8 ? cx + 1 : cx};
+    const int ym1{cy > 0 ? cy - 1 : cy}, yp1{cy + 1 < 8 ? cy + 1 : cy};
+    float m0{pre[cy * 8 + cx]}, m1{pre[cy * 8 + xm1]}, m2{pre[cy * 8 + xp1]}, m3{pre[ym1 * 8 + xm1]};
+    cswap(m0, m1);
+    cswap(m0, m2);
+    cswap(m0, m3);
+    cswap(m1, m2);
+    cswap(m1, m3);
+    cswap(m2, m3);
+    const float extra[5]{pre[ym1 * 8 + cx], pre[ym1 * 8 + xp1], pre[yp1 * 8 + xm1],
+                         pre[yp1 * 8 + cx], pre[yp1 * 8 + xp1]};
+#pragma unroll
+    for (int i{0}; i < 5; ++i) {
+        const float v{extra[i]};
+        if (v < m3) {
+            if (v < m0) {
+                m3 = m2;
+                m2 = m1;
+                m1 = m0;
+                m0 = v;
+            } else if (v < m1) {
+                m3 = m2;
+                m2 = m1;
+                m1 = v;
+            } else if (v < m2) {
+                m3 = m2;
+                m2 = v;
+            } else {
+                m3 = v;
+            }
+        }
+    }
+    return p.km[0] * m0 + p.km[1] * m1 + p.km[2] * m2 + p.km[3] * m3;
+}
+
+// Block (bx, by) of the region (quad-major element layout) for size n.
+__device__ __forceinline__ void element_block(int n, int i, int& bx, int& by, int& e) {
+    if (n == 8) {
+        const int b{i >> 6};
+        const int quad{b >> 2}, j{b & 3};
+        bx = 2 * (quad & 1) + (j & 1);
+        by = 2 * (quad >> 1) + (j >> 1);
+        e = i & 63;
+    } else if (n == 16) {
+        const int quad{i >> 8};
+        bx = 2 * (quad & 1);
+        by = 2 * (quad >> 1);
+        e = i & 255;
+    } else {
+        bx = 0;
+        by = 0;
+        e = i;
+    }
+}
+
+struct RegionShared {
+    float xyb[3][kTileW * kTileW];
+    float sharp[3][32 * 32];
+    float rows[1024];
+    float coefy[3][1024];  // selection coefficients of Y for n = 8, 16, 32
+    float mask[1024];
+    float cells[64];
+    float part[4][4][2];   // [wave][quad][d|r]
+    float cost8[16];       // quad-major
+    float cost16[4];
+    float cost32;
+    float cfl[4][3];
+    int acs[16];           // region-local raster
+};
+
+__global__ __launch_bounds__(256) void region_kernel(
+    Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
+    const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
+    std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
+    std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part) {
+    __shared__ RegionShared S;
+    const int tid{static_cast<int>(threadIdx.x)};
+    const int w{tid >> 6};
+    const int lane{tid & 63};
+    const int region{static_cast<int>(blockIdx.x)};
+    const int rx{region % static_cast<int>(geo.rw)};
+    const int ry{region / static_cast<int>(geo.rw)};
+    const int px0{rx * 32}, py0{ry * 32};
+    const int W{static_cast<int>(geo.width)}, H{static_cast<int>(geo.height)};
+    const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
+    const int bx0{rx * 4}, by0{ry * 4};
+
+    // 1. NV12 -> XYB over the 36x36 tile (2-pixel halo, clamped coordinates).
+    for (int i{tid}; i < kTileW * kTileW; i += 256) {
+        const int lx{i % kTileW}, ly{i / kTileW};
+        int gx{px0 + lx - kHalo}, gy{py0 + ly - kHalo};
+        gx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+        gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+        int x0, x1, y0, y1;
+        float wx0, wx1, wy0, wy1;
+        chroma_taps(gx, W / 2, x0, x1, wx0, wx1);
+        chroma_taps(gy, H / 2, y0, y1, wy0, wy1);
+        const std::uint8_t* r0{chroma + static_cast<std::size_t>(y0) * chroma_pitch};
+        const std::uint8_t* r1{chroma + static_cast<std::size_t>(y1) * chroma_pitch};
+        const float b0{static_cast<float>(r0[2 * x0]) * wx0 + static_cast<float>(r0[2 * x1]) * wx1};
+        const float b1{static_cast<float>(r1[2 * x0]) * wx0 + static_cast<float>(r1[2 * x1]) * wx1};
+        const float c0{static_cast<float>(r0[2 * x0 + 1]) * wx0 +
+                       static_cast<float>(r0[2 * x1 + 1]) * wx1};
+        const float c1{static_cast<float>(r1[2 * x0 + 1]) * wx0 +
+                       static_cast<float>(r1[2 * x1 + 1]) * wx1};
+        const float cbu{b0 * wy0 + b1 * wy1};
+        const float cru{c0 * wy0 + c1 * wy1};
+        const float yv{static_cast<float>(luma[static_cast<std::size_t>(gy) * luma_pitch + gx]) *
+                       (1.0f / 255.0f)};
+        const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
+        const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
+        pixel_to_xyb(yv, cb, cr, S.xyb[0][i], S.xyb[1][i], S.xyb[2][i]);
+    }
+    if (tid < 16) {
+        S.acs[tid] = 8;
+    }
+    __syncthreads();
+
+    // 2. Inverse gaborish (gaborish.h:16-28; tap order frontend_transform.cu:30-48).
+    {
+        constexpr float k0{-0.09495815671340026f}, k1{-0.041031725066768575f};
+        constexpr float k2{0.013710004822696948f}, k3{0.006510206083837737f};
+        constexpr float k4{-0.0014789063378272242f};
+        constexpr float nm{1.0f / (1.0f + 4.0f * (k0 + k1 + k2 + k4 + 2.0f * k3))};
+        constexpr float wc{nm}, wr{nm * k0}, wr2{nm * k2}, wd{nm * k1}, wl{nm * k3}, wd2{nm * k4};
+        for (int i{tid}; i < 3 * 1024; i += 256) {
+            const int c{i >> 10}, p{i & 1023};
+            const float* s{&S.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+#define AT(dx, dy) s[(dy) * kTileW + (dx)]
+            const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
+            const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
+            const float dg{AT(-1, -1) + AT(1, -1) + AT(-1, 1) + AT(1, 1)};
+            const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
+                           AT(-1, 2) + AT(1, 2)};
+            const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
+            S.sharp[c][p] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
+#undef AT
+        }
+        // 3a. AQ pre-erosion terms per interior pixel (frontend_fused.cu:118-137).
+        for (int p{tid}; p < 1024; p += 256) {
+            const float* s{&S.xyb[1][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+            const float yv{s[0]};
+            const float base{0.25f * (s[kTileW] + s[-kTileW] + s[-1] + s[1])};
+            float diff{ratio_cbrt_gamma(yv + 0.019f, false) * (yv - base)};
+            diff *= diff;
+            diff = fminf(diff, 0.2f);
+            S.mask[p] = masking_sqrt(diff);
+        }
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int cx{tid & 7}, cy{tid >> 3};
+        float sum{0.0f};
+        for (int dy{0}; dy < 4; ++dy) {
+            for (int dx{0}; dx < 4; ++dx) {
+                sum += S.mask[(cy * 4 + dy) * 32 + cx * 4 + dx];
+            }
+        }
+        S.cells[tid] = 0.25f * sum;
+    }
+    __syncthreads();
+
+    // 3b. Per-block modulation and the quant integer (frontend_fused.cu:117-203).
+    for (int j{0}; j < 4; ++j) {
+        const int lb{j * 4 + w};
+        const int lbx{lb & 3}, lby{lb >> 2};
+        if (bx0 + lbx >= bw || by0 + lby >= bh) {
+            continue;
+        }
+        const int fx{lane & 7}, fy{lane >> 3};
+        const int idx{(lby * 8 + fy + kHalo) * kTileW + lbx * 8 + fx + kHalo};
+        const float yv{S.xyb[1][idx]}, xv{S.xyb[0][idx]}, bv{S.xyb[2][idx]};
+        float hf{0.0f};
+        if (fx < 7) {
+            hf += fminf(0.0206f, fabsf(yv - S.xyb[1][idx + 1]));
+        }
+        if (fy < 7) {
+            hf += fminf(0.0206f, fabsf(yv - S.xyb[1][idx + kTileW]));
+        }
+        const float iny{yv + 0.16f};
+        const float gm{ratio_cbrt_gamma(iny - xv, true) + ratio_cbrt_gamma(iny + xv, true)};
+        const float pye{yv + 0.084381641171960495f + fabsf(xv)};
+        const float bl{bv > pye ? fminf(bv - pye, 0.027121074570634722f) : 0.0f};
+        const float er{lane < 4 ? erosion_cell(S.cells, 2 * lbx + (lane & 1), 2 * lby + (lane >> 1),
+                                                prm)
+                                : 0.0f};
+        const float hf_sum{wave_sum(hf)};
+        const float gamma_overall{wave_sum(gm) * (0.5f / 64.0f)};
+        float blue_sum{wave_sum(bl)};
+        const float aq{wave_sum(er)};
+        if (lane == 0) {
+            constexpr float k_limit{0.027121074570634722f};
+            if (blue_sum >= 32.0f * k_limit) {
+                blue_sum = 64.0f * k_limit - blue_sum;
+            }
+            constexpr float k_max_limit{15.398788439047934f};
+            if (blue_sum >= k_max_limit * k_limit) {
+                blue_sum = k_max_limit * k_limit;
+            }
+            float ov{compute_mask(aq)};
+            ov += hf_sum * -0.38f + 0.42f;
+            ov += 0.1005613337192697f * log2_det(gamma_overall);
+            ov += blue_sum * 0.14207000358439159f;
+            const float qval{exp_det(ov) * prm.mul_pb + prm.add_pb};
+            int qi{static_cast<int>(qval * prm.inv_global_scale + 0.5f)};
+            qi = qi < 1 ? 1 : (qi > 256 ? 256 : qi);
+            qf_out[static_cast<std::size_t>(by0 + lby) * bw + bx0 + lbx] = qi;
+        }
+    }
+
+    // 4. RD transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
+    for (int sz{0}; sz < 3; ++sz) {
+        const int n{8 << sz};
+        const float* A{basis_of(n)};
+#pragma unroll
+        for (int k{0}; k < 4; ++k) {
+            const int i{256 * k + 64 * w + lane};
+            int bx, by, e;
+            element_block(n, i, bx, by, e);
+            const int fx{e / n}, r{e % n};
+            const float* prow{&S.sharp[1][(by * 8 + r) * 32 + bx * 8]};
+            float s{0.0f};
+            for (int c{0}; c < n; ++c) {
+                s = __builtin_fmaf(A[fx * n + c], prow[c], s);
+            }
+            S.rows[i] = s;
+        }
+        __syncthreads();
+        float d[4], rt[4];
+        const float* wsel{wsel_of(n)};
+#pragma unroll
+        for (int k{0}; k < 4; ++k) {
+            const int i{256 * k + 64 * w + lane};
+            int bx, by, e;
+            element_block(n, i, bx, by, e);
+            const int fx{e / n}, fy{e % n};
+            const float* rr{&S.rows[i - e + fx * n]};
+            float s{0.0f};
+            for (int r{0}; r < n; ++r) {
+                s = __builtin_fmaf(A[fy * n + r], rr[r], s);
+            }
+            S.coefy[sz][i] = s;
+            const int m{n / 8};
+            if (fx < m && fy < m) {
+                d[k] = 0.0f;
+                rt[k] = 0.0f;
+            } else {
+                const float wq{wsel[e]};
+                const float q{__builtin_rintf(s * prm.qgsf_select / wq)};
+                const float err{q * wq / prm.qgsf_select - s};
+                d[k] = err * err;
+                rt[k] = q != 0.0f ? 1.0f + log2_det(1.0f + fabsf(q)) : 0.0f;
+            }
+        }
+        if (n == 8) {
+#pragma unroll
+            for (int k{0}; k < 4; ++k) {
+                const float ds{wave_sum(d[k])}, rs{wave_sum(rt[k])};
+                if (lane == 0) {
+                    S.cost8[4 * k + w] = 64.0f * ds + 6.0e-4f * rs;
+                }
+            }
+        } else if (n == 16) {
+#pragma unroll
+            for (int k{0}; k < 4; ++k) {
+                const float ds{wave_sum(d[k])}, rs{wave_sum(rt[k])};
+                if (lane == 0) {
+                    S.part[w][k][0] = ds;
+                    S.part[w][k][1] = rs;
+                }
+            }
+        } else {
+            float ad{d[0]}, ar{rt[0]};
+            ad += d[1];
+            ar += rt[1];
+            ad += d[2];
+            ar += rt[2];
+            ad += d[3];
+            ar += rt[3];
+            const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
+            if (lane == 0) {
+                S.part[w][0][0] = ds;
+                S.part[w][0][1] = rs;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            if (n == 16) {
+                for (int q{0}; q < 4; ++q) {
+                    const float ds{(S.part[0][q][0] + S.part[1][q][0]) +
+                                   (S.part[2][q][0] + S.part[3][q][0])};
+                    const float rs{(S.part[0][q][1] + S.part[1][q][1]) +
+                                   (S.part[2][q][1] + S.part[3][q][1])};
+                    S.cost16[q] = 256.0f * ds + 6.0e-4f * rs;
+                }
+            } else if (n == 32) {
+                const float ds{(S.part[0][0][0] + S.part[1][0][0]) + (S.part[2][0][0] + S.part[3][0][0])};
+                const float rs{(S.part[0][0][1] + S.part[1][0][1]) + (S.part[2][0][1] + S.part[3][0][1])};
+                S.cost32 = 1024.0f * ds + 6.0e-4f * rs;
+            }
+        }
+        __syncthreads();
+    }
+
+    if (tid == 0) {
+        constexpr float inf{1.0e30f}, eps{1.0e-6f};
+        const bool fit32{bx0 + 4 <= bw && by0 + 4 <= bh};
+        const float c32{fit32 ? S.cost32 : inf};
+        float split{0.0f};
+        int choice[4];
+        for (int q{0}; q < 4; ++q) {
+            const int qx{q & 1}, qy{q >> 1};
+            float sum8{0.0f};
+            bool any8{false};
+            for (int dy{0}; dy < 2; ++dy) {
+                for (int dx{0}; dx < 2; ++dx) {
+                    if (bx0 + 2 * qx + dx < bw && by0 + 2 * qy + dy < bh) {
+                        sum8 += S.cost8[q * 4 + dy * 2 + dx];
+                        any8 = true;
+                    }
+                }
+            }
+            if (!any8) {
+                choice[q] = -1;
+                continue;
+            }
+            const bool fit16{bx0 + 2 * qx + 2 <= bw && by0 + 2 * qy + 2 <= bh};
+            const float c16{fit16 ? S.cost16[q] : inf};
+            if (fit16 && c16 <= sum8 + eps) {
+                choice[q] = 16;
+                split += c16;
+            } else {
+                choice[q] = 8;
+                split += sum8;
+            }
+        }
+        if (fit32 && c32 <= split + eps) {
+            for (int b{0}; b < 16; ++b) {
+                S.acs[b] = b == 0 ? 32 : 0;
+            }
+        } else {
+            for (int q{0}; q < 4; ++q) {
+                const int ox{2 * (q & 1)}, oy{2 * (q >> 1)};
+                if (choice[q] == 16) {
+                    S.acs[oy * 4 + ox] = 16;
+                    S.acs[oy * 4 + ox + 1] = 0;
+                    S.acs[(oy + 1) * 4 + ox] = 0;
+                    S.acs[(oy + 1) * 4 + ox + 1] = 0;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (tid < 16) {
+        const int bx{bx0 + (tid & 3)}, by{by0 + (tid >> 2)};
+        if (bx < bw && by < bh) {
+            acs_out[static_cast<std::size_t>(by) * bw + bx] = static_cast<std::int8_t>(S.acs[tid]);
+        }
+    }
+
+    // 5. Chosen DCTs of X and B (Y reuses the selection coefficients), FP16 out,
+    //    CfL partials (frontend_quantize.cu:49-117 on this region).
+    int own_n[4], own_e[4], own_idx[4], own_ox[4], own_oy[4];
+    bool inside[4];
+#pragma unroll
+    for (int k{0}; k < 4; ++k) {
+        const int bx{2 * (k & 1) + (w & 1)}, by{2 * (k >> 1) + (w >> 1)};
+        inside[k] = bx0 + bx < bw && by0 + by < bh;
+        int n{8}, ox{bx}, oy{by};
+        if (S.acs[0] == 32) {
+            n = 32;
+            ox = 0;
+            oy = 0;
+        } else if (S.acs[(by & ~1) * 4 + (bx & ~1)] == 16) {
+            n = 16;
+            ox = bx & ~1;
+            oy = by & ~1;
+        }
+        const int q{(by - oy) * (n / 8) + (bx - ox)};
+        const int e{q * 64 + lane};
+        own_n[k] = n;
+        own_e[k] = e;
+        own_ox[k] = ox;
+        own_oy[k] = oy;
+        own_idx[k] = n == 32 ? e : 256 * k + 64 * w + lane;
+    }
+    float xc[4], bc[4];
+    for (int ch{0}; ch < 3; ch += 2) {
+#pragma unroll
+        for (int k{0}; k < 4; ++k) {
+            const int n{own_n[k]}, e{own_e[k]};
+            const float* A{basis_of(n)};
+            const int fx{e / n}, r{e % n};
+            const float* prow{&S.sharp[ch][(own_oy[k] * 8 + r) * 32 + own_ox[k] * 8]};
+            float s{0.0f};
+            for (int c{0}; c < n; ++c) {
+                s = __builtin_fmaf(A[fx * n + c], prow[c], s);
+            }
+            S.rows[own_idx[k]] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k{0}; k < 4; ++k) {
+            const int n{own_n[k]}, e{own_e[k]};
+            const float* A{basis_of(n)};
+            const int fx{e / n}, fy{e % n};
+            const float* rr{&S.rows[own_idx[k] - e + fx * n]};
+            float s{0.0f};
+            for (int r{0}; r < n; ++r) {
+                s = __builtin_fmaf(A[fy * n + r], rr[r], s);
+            }
+            (ch == 0 ? xc : bc)[k] = s;
+        }
+        __syncthreads();
+    }
+    const std::size_t cplane{static_cast<std::size_t>(geo.nblocks) * 64};
+    float acc[3]{0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k{0}; k < 4; ++k) {
+        const int n{own_n[k]}, e{own_e[k]};
+        const int sz{n == 32 ? 2 : (n == 16 ? 1 : 0)};
+        const _Float16 hx{static_cast<_Float16>(xc[k])};
+        const _Float16 hy{static_cast<_Float16>(S.coefy[sz][own_idx[k]])};
+        const _Float16 hb{static_cast<_Float16>(bc[k])};
+        float v[3]{0.0f, 0.0f, 0.0f};
+        if (inside[k]) {
+            const int bx{bx0 + 2 * (k & 1) + (w & 1)}, by{by0 + 2 * (k >> 1) + (w >> 1)};
+            const std::size_t slot{(static_cast<std::size_t>(by) * bw + bx) * 64 + lane};
+            coeffs[slot] = __builtin_bit_cast(std::uint16_t, hx);
+            coeffs[cplane + slot] = __builtin_bit_cast(std::uint16_t, hy);
+            coeffs[2 * cplane + slot] = __builtin_bit_cast(std::uint16_t, hb);
+            const int m{n / 8};
+            if (!(e / n < m && e % n < m)) {
+                const float xr{static_cast<float>(hx)}, yr{static_cast<float>(hy)};
+                const float br{static_cast<float>(hb)};
+                v[0] = xr * yr;
+                v[1] = yr * yr;
+                v[2] = (br - yr) * yr;
+            }
+        }
+#pragma unroll
+        for (int c{0}; c < 3; ++c) {
+            acc[c] = k == 0 ? v[c] : acc[c] + v[c];
+        }
+    }
+#pragma unroll
+    for (int c{0}; c < 3; ++c) {
+        const float s{wave_sum(acc[c])};
+        if (lane == 0) {
+            S.cfl[w][c] = s;
+        }
+    }
+    __syncthreads();
+    if (tid < 3) {
+        cfl_part[static_cast<std::size_t>(region) * 3 + tid] =
+            (S.cfl[0][tid] + S.cfl[1][tid]) + (S.cfl[2][tid] + S.cfl[3][tid]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+struct TileShared {
+    float llf[64][3][16];     // by first-block position in the tile
+    unsigned int nz[64][3];
+    int own[64];              // packed owner: local position | (n << 8)
+    float maps[2];
+};
+
+__device__ __forceinline__ int owner_in_tile(const std::int8_t* acs, int bw, int bx, int by, int& ox,
+                                             int& oy) {
+    const int rxb{bx & ~3}, ryb{by & ~3};
+    if (acs[static_cast<std::size_t>(ryb) * bw + rxb] == 32) {
+        ox = rxb;
+        oy = ryb;
+        return 32;
+    }
+    const int qx{bx & ~1}, qy{by & ~1};
+    if (acs[static_cast<std::size_t>(qy) * bw + qx] == 16) {
+        ox = qx;
+        oy = qy;
+        return 16;
+    }
+    ox = bx;
+    oy = by;
+    return 8;
+}
+
+__global__ __launch_bounds__(256) void tile_quantize_kernel(
+    Geometry geo, FrameParams prm, const std::uint16_t* __restrict__ coeffs,
+    const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
+    const float* __restrict__ cfl_part, std::int8_t* __restrict__ ytox_out,
+    std::int8_t* __restrict__ ytob_out, std::int16_t* __restrict__ ac, std::int32_t* __restrict__ dc,
+    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count) {
+    __shared__ TileShared S;
+    const int tid{static_cast<int>(threadIdx.x)};
+    const int w{tid >> 6};
+    const int lane{tid & 63};
+    const int tile{static_cast<int>(blockIdx.x)};
+    const int tx{tile % static_cast<int>(geo.cmw)}, ty{tile / static_cast<int>(geo.cmw)};
+    const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
+    const std::size_t nblk{geo.nblocks};
+    const std::size_t cplane{nblk * 64};
+
+    if (tid < 64) {
+        S.nz[tid][0] = S.nz[tid][1] = S.nz[tid][2] = 0u;
+        const int bx{tx * 8 + (tid & 7)}, by{ty * 8 + (tid >> 3)};
+        int ox{bx}, oy{by}, n{8};
+        if (bx < bw && by < bh) {
+            n = owner_in_tile(acs, bw, bx, by, ox, oy);
+        }
+        S.own[tid] = ((oy - ty * 8) * 8 + (ox - tx * 8)) | (n << 8);
+    }
+    if (tid == 0) {
+        // cfl.h:58-64 on (R00 + R10) + (R01 + R11) of the tile's regions.
+        float r[4][3];
+        for (int i{0}; i < 4; ++i) {
+            const int rx{tx * 2 + (i & 1)}, ry{ty * 2 + (i >> 1)};
+            const bool ok{rx < static_cast<int>(geo.rw) && ry < static_cast<int>(geo.rh)};
+            for (int c{0}; c < 3; ++c) {
+                r[i][c] = ok ? cfl_part[(static_cast<std::size_t>(ry) * geo.rw + rx) * 3 + c] : 0.0f;
+            }
+        }
+        float s[3];
+        for (int c{0}; c < 3; ++c) {
+            s[c] = (r[0][c] + r[1][c]) + (r[2][c] + r[3][c]);
+        }
+        const double sxy{s[0]}, syy{s[1]}, sby{s[2]};
+        const float fx{syy > 0.0 ? static_cast<float>(sxy / syy) : 0.0f};
+        const float db{syy > 0.0 ? static_cast<float>(sby / syy) : 0.0f};
+        int mx{static_cast<int>(__builtin_rintf(fx * 84))};
+        int mb{static_cast<int>(__builtin_rintf(db * 84))};
+        mx = mx < -128 ? -128 : (mx > 127 ? 127 : mx);
+        mb = mb < -128 ? -128 : (mb > 127 ? 127 : mb);
+        ytox_out[tile] = static_cast<std::int8_t>(mx);
+        ytob_out[tile] = static_cast<std::int8_t>(mb);
+        S.maps[0] = 0.0f + static_cast<float>(mx) * (1.0f / 84);
+        S.maps[1] = 1.0f + static_cast<float>(mb) * (1.0f / 84);
+    }
+    __syncthreads();
+
+    const float yx{S.maps[0]}, yb{S.maps[1]};
+    for (int j{0}; j < 16; ++j) {
+        const int p{j * 4 + w};
+        const int bx{tx * 8 + (p & 7)}, by{ty * 8 + (p >> 3)};
+        if (bx >= bw || by >= bh) {
+            continue;
+        }
+        const int own{S.own[p]};
+        const int n{own >> 8}, op{own & 255};
+        const int ox{tx * 8 + (op & 7)}, oy{ty * 8 + (op >> 3)};
+        const int m{n / 8};
+        const int e{((by - oy) * m + (bx - ox)) * 64 + lane};
+        const std::size_t slot{(static_cast<std::size_t>(by) * bw + bx) * 64 + lane};
+        const float xr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[slot]))};
+        const float yr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[cplane + slot]))};
+        const float br{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[2 * cplane + slot]))};
+        const bool llf{e / n < m && e % n < m};
+        std::int16_t q0{0}, q1{0}, q2{0};
+        if (llf) {
+            const int lp{(e / n) * m + e % n};
+            S.llf[op][0][lp] = xr;
+            S.llf[op][1][lp] = yr;
+            S.llf[op][2][lp] = br;
+        } else {
+            const float qgsf{static_cast<float>(qf[static_cast<std::size_t>(oy) * bw + ox]) *
+                             prm.global_scale_float};
+            const float wy{dequant_of(n, 1, e)};
+            const int qy{static_cast<int>(__builtin_rintf(yr * qgsf / wy))};
+            const float y_round{static_cast<float>(qy) * wy / qgsf};
+            const float wx{dequant_of(n, 0, e)};
+            const float wb{dequant_of(n, 2, e)};
+            q0 = static_cast<std::int16_t>(static_cast<int>(__builtin_rintf((xr - yx * y_round) * qgsf / wx)));
+            q1 = static_cast<std::int16_t>(qy);
+            q2 = static_cast<std::int16_t>(static_cast<int>(__builtin_rintf((br - yb * y_round) * qgsf / wb)));
+        }
+        ac[slot] = q0;
+        ac[cplane + slot] = q1;
+        ac[2 * cplane + slot] = q2;
+        const unsigned int c0{static_cast<unsigned int>(__popcll(__ballot(q0 != 0)))};
+        const unsigned int c1{static_cast<unsigned int>(__popcll(__ballot(q1 != 0)))};
+        const unsigned int c2{static_cast<unsigned int>(__popcll(__ballot(q2 != 0)))};
+        if (lane == 0) {
+            atomicAdd(&S.nz[op][0], c0);
+            atomicAdd(&S.nz[op][1], c1);
+            atomicAdd(&S.nz[op][2], c2);
+        }
+    }
+    __syncthreads();
+
+    if (tid < 64) {
+        const int p{tid};
+        const int bx{tx * 8 + (p & 7)}, by{ty * 8 + (p >> 3)};
+        if (bx < bw && by < bh) {
+            const int own{S.own[p]};
+            const int n{own >> 8}, op{own & 255};
+            const int ox{tx * 8 + (op & 7)}, oy{ty * 8 + (op >> 3)};
+            const int m{n / 8};
+            const int dx{bx - ox}, dy{by - oy};
+            float dcv[3];
+            for (int c{0}; c < 3; ++c) {
+                const float* l{S.llf[op][c]};
+                if (m == 1) {
+                    dcv[c] = l[0];
+                } else {
+                    const double* t{m == 2 ? g_tables.llf2 : g_tables.llf4};
+                    double sum{0.0};
+                    for (int fx{0}; fx < m; ++fx) {
+                        for (int fy{0}; fy < m; ++fy) {
+                            sum += t[fx * m + dx] * t[fy * m + dy] * static_cast<double>(l[fx * m + fy]);
+                        }
+                    }
+                    dcv[c] = static_cast<float>(static_cast<double>(m) * sum);
+                }
+            }
+            const float dcq_x{4096.0f * prm.dc_scale}, dcq_y{512.0f * prm.dc_scale};
+            const float dcq_b{256.0f * prm.dc_scale};
+            const std::size_t b{static_cast<std::size_t>(by) * bw + bx};
+            const int qdy{static_cast<int>(__builtin_rintf(dcv[1] * dcq_y))};
+            const float y_dc_round{static_cast<float>(qdy) / dcq_y};
+            dc[nblk + b] = qdy;
+            dc[b] = static_cast<int>(__builtin_rintf(dcv[0] * dcq_x));
+            dc[2 * nblk + b] = static_cast<int>(__builtin_rintf((dcv[2] - 1.0f * y_dc_round) * dcq_b));
+            const unsigned int l2{n == 16 ? 2u : (n == 32 ? 4u : 0u)};
+            for (int c{0}; c < 3; ++c) {
+                const unsigned int cnt{S.nz[op][c]};
+                nz_grid[c * nblk + b] = static_cast<std::uint8_t>((cnt + m * m - 1) >> l2);
+                nz_count[c * nblk + b] = op == p ? static_cast<std::uint16_t>(cnt) : 0;
+            }
+        }
+    }
+}
+
+}  // namespace cjx

@@ -1,0 +1,77 @@
+// Host-callable launchers for the device kernels (device.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "common.h"
+
+namespace cjx {
+
+struct Constants;
+struct CodeSet;
+struct Layout;
+struct DcBlobs;
+
+std::size_t sizeof_constants();
+std::size_t sizeof_codeset();
+std::size_t sizeof_layout();
+std::size_t sizeof_dc_blobs();
+std::uint32_t dc_chunks_per_group();
+std::uint32_t blob_words_per_group();
+// Host view of Constants to fill before upload.
+void fill_constants(void* dst, const std::uint32_t* head, std::uint32_t head_bits,
+                    const std::uint32_t* dc_global, std::uint32_t dc_global_bits,
+                    const std::uint32_t* ac_prefix, std::uint32_t ac_prefix_bits,
+                    const std::uint32_t* dc_pre, std::uint32_t dc_pre_bits,
+                    const std::uint32_t* dc_mid, std::uint32_t dc_mid_bits);
+bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::uint32_t dc_pre_bits,
+                   std::uint32_t dc_mid_bits);
+std::uint64_t layout_total_offset();
+
+void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
+                   std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
+                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part);
+void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
+                          const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
+                          const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
+                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count);
+void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+               std::uint8_t* nz_grid, std::uint16_t* nz_count);
+void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v);
+void launch_ac_hist(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* hist);
+void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
+                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                    std::uint32_t* dc_hist, std::uint32_t* meta_hist, std::uint32_t* fb_count);
+void launch_huffman(hipStream_t s, int count, const std::uint32_t* hists, CodeSet* cs);
+void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
+                     const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words);
+void launch_ac_bits(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
+                    std::uint32_t* item_bits);
+void launch_ac_group_scan(hipStream_t s, const Geometry& geo, const std::uint32_t* item_bits,
+                          std::uint32_t* item_off, std::uint32_t* group_bytes, std::uint64_t* group_bits);
+void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
+                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                    const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
+                    std::uint32_t* group_bytes, std::uint64_t* group_bits);
+void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
+                   const std::uint32_t* ac_group_bytes, const std::uint32_t* dc_group_bytes,
+                   const std::uint64_t* ac_group_bits, const std::uint64_t* dc_group_bits, const DcBlobs* blobs,
+                   Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base);
+void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* lay, std::uint32_t* out_words,
+                             std::uint64_t cap_words);
+void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
+                    const std::uint32_t* item_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
+                    std::uint64_t cap_words);
+void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
+                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                    const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
+                    const std::uint64_t* chunk_off, const std::uint64_t* dc_base, std::uint32_t* out_words,
+                    std::uint64_t cap_words);
+
+}  // namespace cjx

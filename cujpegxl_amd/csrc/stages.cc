@@ -1,0 +1,140 @@
+// Stage harness (cujpegxl_stages.h).
+#include "cujpegxl/cujpegxl_stages.h"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "common.h"
+#include "encoder.h"
+#include "handle.h"
+
+
+extern "C" {
+
+size_t cjx_stage_buffer_bytes(cujpegxl_encoder* e, int which) {
+    if (e == nullptr || which < 0 || which >= static_cast<int>(cjx::Buffer::kCount)) {
+        return 0;
+    }
+    return e->engine.buffer_bytes(static_cast<cjx::Buffer>(which));
+}
+
+int cjx_stage_fetch(cujpegxl_encoder* e, int which, void* host, size_t bytes) {
+    if (e == nullptr || host == nullptr || which < 0 || which >= static_cast<int>(cjx::Buffer::kCount)) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return e->engine.fetch(static_cast<cjx::Buffer>(which), host, bytes) ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_stage_encode_coefficients(cujpegxl_encoder* e, const int16_t* ac, const int32_t* dc, const int8_t* acs,
+                                  const int32_t* qf, const int8_t* ytox, const int8_t* ytob,
+                                  uint32_t global_scale, uint32_t quant_dc, uint8_t* out, size_t cap,
+                                  size_t* size) {
+    if (e == nullptr || ac == nullptr || dc == nullptr || acs == nullptr || qf == nullptr || size == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    if (!e->engine.encode_coefficients(ac, dc, acs, qf, ytox, ytob, global_scale, quant_dc, out, cap, size)) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    return *size > cap ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
+}
+
+int cjx_stage_encoder_create_any(const cujpegxl_config* c, cujpegxl_encoder** out) {
+    if (c == nullptr || out == nullptr || !(c->distance > 0.0f) || !std::isfinite(c->distance)) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    if (c->width == 0 || c->height == 0 || c->width % 8 || c->height % 8) {
+        return CUJPEGXL_UNSUPPORTED_RESOLUTION;
+    }
+    if (!cjx::device_exists(c->device_ordinal)) {
+        return CUJPEGXL_NO_DEVICE;
+    }
+    auto* e{new (std::nothrow) cujpegxl_encoder{}};
+    if (e == nullptr) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    e->config = *c;
+    if (!e->engine.configure(c->width, c->height, c->distance, c->device_ordinal)) {
+        delete e;
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    *out = e;
+    return CUJPEGXL_OK;
+}
+
+int cjx_stage_set_profiling(cujpegxl_encoder* e, int on) {
+    if (e == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    e->engine.set_profiling(on != 0);
+    return CUJPEGXL_OK;
+}
+
+int cjx_stage_timings(cujpegxl_encoder* e, float* ms, int count) {
+    if (e == nullptr || ms == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const auto& t{e->engine.stage_ms()};
+    for (int i{0}; i < count && i < static_cast<int>(t.size()); ++i) {
+        ms[i] = t[i];
+    }
+    return CUJPEGXL_OK;
+}
+
+void* cjx_device_alloc(size_t bytes) {
+    void* p{nullptr};
+    return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+}
+
+void cjx_device_free(void* p) {
+    (void)hipFree(p);
+}
+
+int cjx_copy_to_device(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_copy_to_host(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_device_synchronize(void) {
+    return hipDeviceSynchronize() == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+size_t cjx_host_table(int which, void* out, size_t cap) {
+    const cjx::DeviceTables& t{cjx::host_tables()};
+    const void* src{nullptr};
+    size_t n{0};
+    float bases[64 + 256 + 1024];
+    std::uint16_t orders[64 + 256 + 1024];
+    switch (which) {
+        case 0:
+            src = &t.dequant8[0][0];
+            n = sizeof(t.dequant8) + sizeof(t.dequant16) + sizeof(t.dequant32);
+            break;
+        case 1:
+            std::memcpy(bases, t.basis8, sizeof(t.basis8));
+            std::memcpy(bases + 64, t.basis16, sizeof(t.basis16));
+            std::memcpy(bases + 320, t.basis32, sizeof(t.basis32));
+            src = bases;
+            n = sizeof(bases);
+            break;
+        case 2:
+            std::memcpy(orders, t.order8, sizeof(t.order8));
+            std::memcpy(orders + 64, t.order16, sizeof(t.order16));
+            std::memcpy(orders + 320, t.order32, sizeof(t.order32));
+            src = orders;
+            n = sizeof(orders);
+            break;
+        default:
+            return 0;
+    }
+    if (out != nullptr && cap >= n) {
+        std::memcpy(out, src, n);
+    }
+    return n;
+}
+
+}  // extern "C"

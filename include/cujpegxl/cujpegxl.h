@@ -1,0 +1,92 @@
+/* cujpegxl C ABI, implemented for AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in for the public interface of m-pilia/cujpegxl: the same types, enum
+ * values, struct layouts and entry points (include/cujpegxl/cujpegxl.h:24-82
+ * there), exported from libcujpegxl.so.1 under the version node CUJPEGXL_1.
+ * The header is free of HIP types: device memory is passed as integer
+ * addresses.
+ */
+#ifndef CUJPEGXL_CUJPEGXL_H_
+#define CUJPEGXL_CUJPEGXL_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* cujpegxl.h:24 */
+#define CUJPEGXL_API_VERSION 3u
+
+/* cujpegxl.h:26-34; values are ABI. */
+typedef enum cujpegxl_status {
+    CUJPEGXL_OK = 0,
+    CUJPEGXL_NOT_IMPLEMENTED = 1,
+    CUJPEGXL_INVALID_ARGUMENT = 2,
+    CUJPEGXL_UNSUPPORTED_RESOLUTION = 3,
+    CUJPEGXL_NO_DEVICE = 4,
+    CUJPEGXL_INTERNAL_ERROR = 5,
+    CUJPEGXL_BUFFER_TOO_SMALL = 6
+} cujpegxl_status;
+
+/* cujpegxl.h:36-40.  A gfx9 device with matrix cores reports FP16_TENSOR. */
+typedef enum cujpegxl_backend {
+    CUJPEGXL_BACKEND_UNKNOWN = 0,
+    CUJPEGXL_BACKEND_FP32_SIMT = 1,
+    CUJPEGXL_BACKEND_FP16_TENSOR = 2
+} cujpegxl_backend;
+
+typedef struct cujpegxl_encoder cujpegxl_encoder;
+
+/* cujpegxl.h:44-49 */
+typedef struct cujpegxl_config {
+    uint32_t width;
+    uint32_t height;
+    float distance;
+    int32_t device_ordinal;
+} cujpegxl_config;
+
+/* cujpegxl.h:53-58: NV12 (BT.709 full range) in device memory. */
+typedef struct cujpegxl_nv12_input {
+    uintptr_t luma;
+    uintptr_t chroma;
+    size_t luma_pitch;
+    size_t chroma_pitch;
+} cujpegxl_nv12_input;
+
+/* cujpegxl.h:60-64: caller-owned host buffer; the library writes `size`. */
+typedef struct cujpegxl_output {
+    uint8_t* data;
+    size_t size;
+    size_t capacity;
+} cujpegxl_output;
+
+/* cujpegxl.h:66 */
+uint32_t cujpegxl_api_version(void);
+/* cujpegxl.h:67 */
+const char* cujpegxl_status_string(cujpegxl_status status);
+/* cujpegxl.h:68 */
+const char* cujpegxl_backend_string(cujpegxl_backend backend);
+/* cujpegxl.h:70 */
+cujpegxl_status cujpegxl_query_backend(int32_t device_ordinal, cujpegxl_backend* out_backend);
+/* cujpegxl.h:74: worst-case output bytes for `config`. */
+cujpegxl_status cujpegxl_max_output_size(const cujpegxl_config* config, size_t* out_bytes);
+/* cujpegxl.h:76-77 */
+cujpegxl_status cujpegxl_encoder_create(const cujpegxl_config* config,
+                                        cujpegxl_encoder** out_encoder);
+/* cujpegxl.h:78-79 */
+cujpegxl_status cujpegxl_encoder_configure(cujpegxl_encoder* encoder,
+                                           const cujpegxl_config* config);
+/* cujpegxl.h:80-81: synchronous encode of one device-resident NV12 frame into
+ * a `.jxl` (ISOBMFF container) in `output`. */
+cujpegxl_status cujpegxl_encoder_encode(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
+                                        cujpegxl_output* output);
+/* cujpegxl.h:82 */
+void cujpegxl_encoder_destroy(cujpegxl_encoder* encoder);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* CUJPEGXL_CUJPEGXL_H_ */

@@ -1,0 +1,72 @@
+/* Stage harness for libcujpegxl_stages.so: the encode pipeline's internals,
+ * exposed to parity tests and the benchmark.
+ *
+ * It plays the role of the reference's dump tools and round-trip wrappers
+ * (the src/ dump tools, src/entropy_roundtrip.h:19-40, src/dc_entropy_roundtrip.h,
+ * tools/benchmark/encode_bench.cc:462-527): run a stage on given data and read
+ * its intermediate buffers back.  libcujpegxl_stages.so also exports the
+ * public cujpegxl_* entry points; an encoder from either is usable here.
+ */
+#ifndef CUJPEGXL_CUJPEGXL_STAGES_H_
+#define CUJPEGXL_CUJPEGXL_STAGES_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "cujpegxl/cujpegxl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Buffers readable after an encode (layouts as in the front end):
+ *  0 coefficients  uint16 FP16 bits [3][blocks*64] covered-block layout
+ *  1 transform map int8 [blocks] (8/16/32 first-block, 0 covered)
+ *  2 quant field   int32 [blocks]
+ *  3 CfL partials  float [regions][3]
+ *  4 YtoX map      int8 [tiles]      5 YtoB map int8 [tiles]
+ *  6 AC            int16 [3][blocks*64]
+ *  7 DC            int32 [3][blocks]
+ *  8 nz grid       uint8 [3][blocks] 9 nz count uint16 [3][blocks]
+ * 10 histograms    uint32 [32 + 2*dc_groups][256]
+ * 11 AC group bytes uint32 [ac_groups] 12 DC group bytes uint32 [dc_groups]
+ * 13 code depths   uint8 [32 + 2*dc_groups][256]
+ * 14 code words    uint16 [32 + 2*dc_groups][256] */
+size_t cjx_stage_buffer_bytes(cujpegxl_encoder* encoder, int which);
+int cjx_stage_fetch(cujpegxl_encoder* encoder, int which, void* host, size_t bytes);
+
+/* Entropy coding and assembly of host-supplied coefficients (frame of the
+ * encoder's configured size).  `ytox`/`ytob` may be NULL (all zero).
+ * Returns a cujpegxl_status; *size receives the file size. */
+int cjx_stage_encode_coefficients(cujpegxl_encoder* encoder, const int16_t* ac, const int32_t* dc,
+                                  const int8_t* acs, const int32_t* quant_field, const int8_t* ytox,
+                                  const int8_t* ytob, uint32_t global_scale, uint32_t quant_dc,
+                                  uint8_t* out, size_t capacity, size_t* size);
+
+/* Creates an encoder without the C ABI's more-than-one-AC-group rule, for
+ * single-group frames (one combined TOC section). */
+int cjx_stage_encoder_create_any(const cujpegxl_config* config, cujpegxl_encoder** out_encoder);
+
+/* Per-stage device time of the following encodes (HIP events on the
+ * encoder's stream): region, quantize, histograms, codes, sizes, emit,
+ * readback.  `ms` receives 7 floats for the last encode. */
+int cjx_stage_set_profiling(cujpegxl_encoder* encoder, int on);
+int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
+
+/* Device memory for harnesses that have no other allocator. */
+void* cjx_device_alloc(size_t bytes);
+void cjx_device_free(void* ptr);
+int cjx_copy_to_device(void* dst, const void* src, size_t bytes);
+int cjx_copy_to_host(void* dst, const void* src, size_t bytes);
+int cjx_device_synchronize(void);
+
+/* Constant tables as the device uses them, for table parity tests:
+ * 0 dequant weights float [3*(64+256+1024)], 1 DCT bases float [64+256+1024],
+ * 2 natural orders uint16 [64+256+1024].  Host-side; needs no device. */
+size_t cjx_host_table(int which, void* out, size_t capacity);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* CUJPEGXL_CUJPEGXL_STAGES_H_ */

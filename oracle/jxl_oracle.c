@@ -969,9 +969,9 @@ static inline int raw_is_llf(uint32_t e, int n) {
 }
 
 /* Per 32x32 region FP32 regression partials (Sxy, Syy, S(b-y)y) over non-LLF
- * FP16 coefficients, in the device layout: wave w = block column, iteration
- * k = block row, lane = slot; per-thread sum over k, tree64, then
- * (P0 + P1) + (P2 + P3). */
+ * FP16 coefficients, in the device layout: iteration k = 16x16 quad (raster),
+ * wave w = 8x8 block within the quad (raster), lane = slot; per-thread sum
+ * over k, tree64, then (P0 + P1) + (P2 + P3). */
 void jo_cfl_region_partials(const uint16_t* c16, const int8_t* acs, uint32_t w, uint32_t h,
                             float* partials) {
     const uint32_t bw = w / 8, bh = h / 8, rw = (bw + 3) / 4, rh = (bh + 3) / 4;
@@ -984,7 +984,8 @@ void jo_cfl_region_partials(const uint16_t* c16, const int8_t* acs, uint32_t w, 
                 for (uint32_t l = 0; l < 64; ++l) {
                     float acc[3] = {0.0f, 0.0f, 0.0f};
                     for (uint32_t k = 0; k < 4; ++k) {
-                        const uint32_t bx = rx * 4 + wv, by = ry * 4 + k;
+                        const uint32_t bx = rx * 4 + 2 * (k % 2) + wv % 2;
+                        const uint32_t by = ry * 4 + 2 * (k / 2) + wv / 2;
                         float v[3] = {0.0f, 0.0f, 0.0f};
                         if (bx < bw && by < bh) {
                             uint32_t ox, oy;

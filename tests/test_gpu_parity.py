@@ -1,0 +1,180 @@
+"""GPU: the HIP path against the oracle, bit for bit.
+
+Each case runs the real device pipeline through the C ABI (or the stage
+harness for intermediate buffers and coefficient-only inputs), then compares
+with the oracle on the same seeded input: the quant field, transform map, FP16
+coefficients, CfL partials and maps, quantized AC/DC, non-zero grid,
+histograms and the final `.jxl` bytes.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+import cujpegxl_amd as cj
+import patterns
+from cujpegxl_amd.synth import synthetic_nv12
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg: str) -> None:
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def oracle_frontend(jo, luma, chroma, w, h, d):
+    xyb = jo.nv12_to_xyb(luma, chroma, w, h)
+    qf = jo.adaptive_quant(xyb, d)
+    sharp = jo.gaborish(xyb)
+    acs = jo.select_transforms(sharp[1], d)
+    c16 = jo.to_f16(jo.forward_dct(sharp, acs))
+    part, ytox, ytob = jo.cfl(c16, acs, w, h)
+    ac, dc, nz = jo.quantize(c16, acs, ytox, ytob, qf, w, h, d)
+    return dict(qf=qf, acs=acs, c16=c16, part=part, ytox=ytox, ytob=ytob, ac=ac, dc=dc, nz=nz)
+
+
+def device_encode(w, h, d, luma, chroma, any_size=False):
+    enc = cj.Encoder(w, h, d, stages=True, any_size=any_size)
+    nv12 = cj.DeviceBuffer.from_array(np.concatenate([luma.ravel(), chroma.ravel()]))
+    data = enc.encode_device(nv12.ptr, nv12.ptr + w * h, w, w)
+    return enc, data, nv12
+
+
+@pytest.mark.parametrize("w,h,d,seed", [
+    (512, 264, 1.0, 1),
+    (264, 520, 3.0, 2),
+    (1024, 576, 1.0, 3),
+    (776, 456, 2.0, 4),
+])
+def test_pipeline_matches_oracle(jo, w, h, d, seed):
+    say(f"pipeline {w}x{h} d={d} seed={seed}")
+    luma, chroma = synthetic_nv12(w, h, seed)
+    o = oracle_frontend(jo, luma, chroma, w, h, d)
+    enc, data, _buf = device_encode(w, h, d, luma, chroma)
+    np.testing.assert_array_equal(enc.fetch("quant_field", np.int32), o["qf"], err_msg="AQ field")
+    np.testing.assert_array_equal(enc.fetch("acs", np.int8), o["acs"], err_msg="transform map")
+    np.testing.assert_array_equal(enc.fetch("coeffs", np.uint16), o["c16"], err_msg="coefficients")
+    np.testing.assert_array_equal(enc.fetch("cfl_partials", np.float32), o["part"], err_msg="CfL partials")
+    np.testing.assert_array_equal(enc.fetch("ytox", np.int8), o["ytox"])
+    np.testing.assert_array_equal(enc.fetch("ytob", np.int8), o["ytob"])
+    np.testing.assert_array_equal(enc.fetch("ac", np.int16), o["ac"], err_msg="quantized AC")
+    np.testing.assert_array_equal(enc.fetch("dc", np.int32), o["dc"], err_msg="quantized DC")
+    nzg = enc.fetch("nz_grid", np.uint8)
+    np.testing.assert_array_equal(nzg, o["nz"], err_msg="non-zero grid")
+    want = jo.encode_nv12(luma, chroma, w, h, d)
+    assert data == want, f"file differs: {len(data)} vs {len(want)} bytes"
+
+
+@pytest.mark.parametrize("name,maker", [
+    ("ac_mixed_512x384", lambda: patterns.mixed_ac_frame(512, 384)),
+    ("cs_asm_2560x256", lambda: patterns.assembly_frame(2560, 256)),
+    ("cs_asm_640x384", lambda: patterns.assembly_frame(640, 384)),
+    ("cs_mixeddc_2560x2304", lambda: patterns.mixed_dc_frame(2560, 2304)),
+    ("cs_asm_256", lambda: patterns.assembly_frame(256, 256)),
+])
+def test_entropy_on_reference_test_frames(jo, golden, name, maker):
+    """The reference's entropy/assembly test frames: the device codestream is
+    the reference host writer's (golden SHA-256), inside the container."""
+    say(f"entropy {name}")
+    fc = maker()
+    acs = fc.acs if fc.acs is not None else np.full(fc.bw * fc.bh, 8, np.int8)
+    enc = cj.Encoder(fc.w, fc.h, 1.0, any_size=True)
+    data = enc.encode_coefficients(fc.ac.ravel().astype(np.int16), fc.dc.ravel(), acs, fc.qf,
+                                   fc.ytox, fc.ytob, fc.gs, fc.qdc)
+    cs = jo.write_codestream(jo.make_frame(fc.w, fc.h, fc.gs, fc.qdc, fc.dc.ravel(), fc.ac.ravel(),
+                                           acs, fc.qf, fc.ytox, fc.ytob))
+    assert data[40:] == cs
+    key = f"{name}_sha"
+    if key in golden:
+        assert hashlib.sha256(data[40:]).digest() == golden[key].tobytes()
+    assert data[:12] == b"\x00\x00\x00\x0cJXL \x0d\x0a\x87\x0a"
+
+
+def test_histograms_and_codes_match_oracle(jo):
+    say("histograms 512x384 mixed")
+    fc = patterns.mixed_dc_frame(512, 384)
+    ac = patterns.mixed_ac_frame(512, 384).ac
+    enc = cj.Encoder(512, 384, 1.0, stages=True)
+    enc.encode_coefficients(ac.ravel().astype(np.int16), fc.dc.ravel(), fc.acs, fc.qf, fc.ytox,
+                            fc.ytob, fc.gs, fc.qdc)
+    f = jo.make_frame(512, 384, fc.gs, fc.qdc, fc.dc.ravel(), ac.ravel(), fc.acs, fc.qf, fc.ytox,
+                      fc.ytob)
+    import ctypes as C
+    hist = np.zeros(32 * 256, np.uint32)
+    jo.lib().jo_ac_histogram(C.byref(f[0]), jo.ptr(hist))
+    dch = np.zeros(256, np.uint32)
+    mth = np.zeros(256, np.uint32)
+    jo.lib().jo_dc_histograms(C.byref(f[0]), jo.ptr(dch), jo.ptr(mth))
+    got = enc.fetch("histograms", np.uint32).reshape(-1, 256)
+    np.testing.assert_array_equal(got[:32].ravel(), hist)
+    np.testing.assert_array_equal(got[32], dch)
+    np.testing.assert_array_equal(got[33], mth)
+    depth = enc.fetch("code_depth", np.uint8).reshape(-1, 256)
+    bits = enc.fetch("code_bits", np.uint16).reshape(-1, 256)
+    for row in range(34):
+        h = got[row]
+        nzs = np.nonzero(h)[0]
+        alpha = int(nzs[-1]) + 1 if nzs.size else 1
+        if alpha > 1:
+            d, b, _, _ = _huff(jo, np.ascontiguousarray(h[:alpha]))
+            np.testing.assert_array_equal(depth[row, :alpha], d, err_msg=f"row {row}")
+            np.testing.assert_array_equal(bits[row, :alpha], b, err_msg=f"row {row}")
+
+
+def _huff(jo, h):
+    import ctypes as C
+    d = np.zeros(h.size, np.uint8)
+    b = np.zeros(h.size, np.uint16)
+    out = np.zeros(4096, np.uint8)
+    nb = C.c_size_t()
+    jo.lib().jo_huffman(jo.ptr(h), h.size, jo.ptr(d), jo.ptr(b), jo.ptr(out), out.size, C.byref(nb))
+    return d, b, out, nb.value
+
+
+def test_single_group_frame(jo):
+    """256x256 (one AC group, one combined section) through the harness."""
+    say("single group 256x256")
+    luma, chroma = synthetic_nv12(256, 256, 5)
+    enc, data, _buf = device_encode(256, 256, 1.0, luma, chroma, any_size=True)
+    assert data == jo.encode_nv12(luma, chroma, 256, 256, 1.0)
+
+
+def test_c_abi_behaviour():
+    say("C ABI")
+    assert cj.query_backend(0) == "fp16-tensor"
+    with pytest.raises(RuntimeError):
+        cj.query_backend(9999)
+    luma, chroma = synthetic_nv12(512, 512, 6)
+    nv12 = np.concatenate([luma.ravel(), chroma.ravel()])
+    a = cj.encode(nv12, 512, 512)
+    b = cj.encode(nv12, 512, 512)
+    assert a == b and a.startswith(b"\x00\x00\x00\x0cJXL ")
+    data, stages = cj.encode_with_stats(nv12, 512, 512)
+    assert data == a and [s["name"] for s in stages] == ["frontend", "entropy", "assembly"]
+    enc = cj.Encoder(512, 512, 1.0)
+    dev = cj.DeviceBuffer.from_array(nv12)
+    s, size = enc.encode_status(dev.ptr, dev.ptr + 512 * 512, 512, 512, 16)
+    assert s == cj.BUFFER_TOO_SMALL and size == len(a)
+    # Pitched input: rows padded to 640 bytes encode to the same file.
+    pl = np.zeros((512, 640), np.uint8)
+    pl[:, :512] = luma
+    pc = np.zeros((256, 640), np.uint8)
+    pc[:, :512] = chroma
+    dl, dcb = cj.DeviceBuffer.from_array(pl), cj.DeviceBuffer.from_array(pc)
+    assert enc.encode_device(dl.ptr, dcb.ptr, 640, 640) == a
+
+
+def test_4k_frame_matches_oracle(jo):
+    """The benchmark workload (3840x2160, d = 1.0) byte for byte."""
+    say("4K d=1.0")
+    luma, chroma = synthetic_nv12(3840, 2160, 0)
+    enc, data, _buf = device_encode(3840, 2160, 1.0, luma, chroma)
+    want = jo.encode_nv12(luma, chroma, 3840, 2160, 1.0)
+    assert data == want, f"{len(data)} vs {len(want)}"
+    again = enc.encode_device(_buf.ptr, _buf.ptr + 3840 * 2160, 3840, 3840)
+    assert again == data

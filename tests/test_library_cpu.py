@@ -1,0 +1,95 @@
+"""CPU: the built libraries load, export exactly the declared C ABI, and their
+host-side tables and argument checks match the reference without a device."""
+
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cujpegxl_amd as cj
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared(header: str, prefix: str) -> set[str]:
+    text = (ROOT / "include" / "cujpegxl" / header).read_text()
+    return set(re.findall(rf"\b({prefix}\w+)\s*\(", text))
+
+
+def exported(lib: pathlib.Path) -> set[str]:
+    out = subprocess.run(["nm", "-D", "--defined-only", str(lib)], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1].split("@")[0] for line in out.splitlines() if " T " in line}
+
+
+def test_public_library_exports_exactly_the_abi():
+    want = declared("cujpegxl.h", "cujpegxl_")
+    assert len(want) == 9  # cujpegxl.h:66-82
+    assert exported(cj.PUBLIC_LIB) == want
+    lib = cj.public_lib()
+    for name in want:
+        assert hasattr(lib, name)
+
+
+def test_stage_library_exports_abi_and_harness():
+    want = declared("cujpegxl.h", "cujpegxl_") | declared("cujpegxl_stages.h", "cjx_")
+    assert exported(cj.STAGES_LIB) == want
+
+
+def test_status_and_backend_strings():
+    # cujpegxl.cc:69-99
+    names = ["ok", "not-implemented", "invalid-argument", "unsupported-resolution", "no-device",
+             "internal-error", "buffer-too-small"]
+    assert [cj.status_string(i) for i in range(7)] == names
+    lib = cj.public_lib()
+    assert [lib.cujpegxl_backend_string(i).decode() for i in range(3)] == ["unknown", "fp32-simt",
+                                                                            "fp16-tensor"]
+    assert cj.api_version() == 3
+
+
+def test_max_output_size_and_validation():
+    # cujpegxl.cc:50-59 bound, and the resolution / distance rules (:36-48).
+    bw, bh = 480, 270
+    assert cj.max_output_size(3840, 2160) == 3 * bw * bh * 384 + 48 * bw * bh + (135 + 4) * 2048 + 16384
+    lib = cj.public_lib()
+    n = C.c_size_t()
+    for (w, h, d, want) in ((100, 100, 1.0, cj.UNSUPPORTED_RESOLUTION),
+                            (256, 256, 1.0, cj.UNSUPPORTED_RESOLUTION),  # one AC group
+                            (512, 512, 0.0, cj.INVALID_ARGUMENT),
+                            (512, 512, float("nan"), cj.INVALID_ARGUMENT),
+                            (512, 520, 1.0, cj.OK)):
+        assert lib.cujpegxl_max_output_size(C.byref(cj.Config(w, h, d, 0)), C.byref(n)) == want
+    assert lib.cujpegxl_max_output_size(None, C.byref(n)) == cj.INVALID_ARGUMENT
+    h = C.c_void_p()
+    assert lib.cujpegxl_encoder_create(C.byref(cj.Config(100, 100, 1.0, 0)), C.byref(h)) == \
+        cj.UNSUPPORTED_RESOLUTION and not h.value
+    assert lib.cujpegxl_encoder_encode(None, None, None) == cj.INVALID_ARGUMENT
+
+
+def test_host_tables_match_reference_fixtures(jo, golden):
+    L = cj.stages_lib()
+    w = np.zeros(3 * (64 + 256 + 1024), np.float32)
+    assert L.cjx_host_table(0, w.ctypes.data, w.nbytes) == w.nbytes
+    np.testing.assert_array_equal(w, np.fromfile(ROOT / "tests/golden/dequant_weights.bin", "<f4"))
+    o = np.zeros(64 + 256 + 1024, np.uint16)
+    L.cjx_host_table(2, o.ctypes.data, o.nbytes)
+    np.testing.assert_array_equal(o, np.concatenate([golden["order8"], golden["order16"],
+                                                     golden["order32"]]))
+    b = np.zeros(64 + 256 + 1024, np.float32)
+    L.cjx_host_table(1, b.ctypes.data, b.nbytes)
+    for n, off in ((8, 0), (16, 64), (32, 320)):
+        ob = np.zeros(n * n, np.float32)
+        jo.lib().jo_dct_basis(n, jo.ptr(ob))
+        np.testing.assert_array_equal(b[off : off + n * n], ob)
+
+
+def test_python_binding_argument_errors():
+    with pytest.raises(ValueError):
+        cj.encode(np.zeros(10, np.uint8), 512, 512)
+    with pytest.raises(ValueError):
+        cj.encode(np.zeros((512 * 512 * 3 // 2) * 2, np.uint8)[::2], 512, 512)
