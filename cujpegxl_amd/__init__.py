@@ -92,6 +92,7 @@ def stages_lib() -> C.CDLL:
         L.cjx_stage_encode_coefficients.argtypes = [P, P, P, P, P, P, P, C.c_uint32, C.c_uint32, P,
                                                     C.c_size_t, C.POINTER(C.c_size_t)]
         L.cjx_stage_encoder_create_any.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+        L.cjx_stage_encode_nv12.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
         L.cjx_stage_timings.argtypes = [P, C.POINTER(C.c_float), C.c_int]
         L.cjx_device_alloc.restype = P
@@ -182,6 +183,7 @@ class Encoder:
         if s != OK:
             raise RuntimeError(f"encoder_create failed: {status_string(s)}")
         self.handle = h
+        self._encode = self.lib.cjx_stage_encode_nv12 if any_size else self.lib.cujpegxl_encoder_encode
         bw, bh = width // 8, height // 8
         self.capacity = 3 * bw * bh * 384 + 48 * bw * bh + (1 << 20)
         self.out = np.empty(self.capacity, np.uint8)
@@ -190,7 +192,7 @@ class Encoder:
                       chroma_pitch: int) -> bytes:
         inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
         out = Output(self.out.ctypes.data, 0, self.capacity)
-        s = self.lib.cujpegxl_encoder_encode(self.handle, C.byref(inp), C.byref(out))
+        s = self._encode(self.handle, C.byref(inp), C.byref(out))
         if s != OK:
             raise RuntimeError(f"encode failed: {status_string(s)}")
         return self.out[: out.size].tobytes()

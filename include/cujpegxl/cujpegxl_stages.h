@@ -47,6 +47,10 @@ int cjx_stage_encode_coefficients(cujpegxl_encoder* encoder, const int16_t* ac, 
  * single-group frames (one combined TOC section). */
 int cjx_stage_encoder_create_any(const cujpegxl_config* config, cujpegxl_encoder** out_encoder);
 
+/* cujpegxl_encoder_encode without that rule (any multiple-of-8 size). */
+int cjx_stage_encode_nv12(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
+                          cujpegxl_output* output);
+
 /* Per-stage device time of the following encodes (HIP events on the
  * encoder's stream): region, quantize, histograms, codes, sizes, emit,
  * readback.  `ms` receives 7 floats for the last encode. */

@@ -89,9 +89,8 @@ def test_entropy_on_reference_test_frames(jo, golden, name, maker):
     cs = jo.write_codestream(jo.make_frame(fc.w, fc.h, fc.gs, fc.qdc, fc.dc.ravel(), fc.ac.ravel(),
                                            acs, fc.qf, fc.ytox, fc.ytob))
     assert data[40:] == cs
-    key = f"{name}_sha"
-    if key in golden:
-        assert hashlib.sha256(data[40:]).digest() == golden[key].tobytes()
+    if name.startswith("cs_"):  # whole-codestream vectors (write_vardct_codestream)
+        assert hashlib.sha256(data[40:]).digest() == golden[f"{name}_sha"].tobytes()
     assert data[:12] == b"\x00\x00\x00\x0cJXL \x0d\x0a\x87\x0a"
 
 
