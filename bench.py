@@ -27,6 +27,20 @@ sys.path.insert(0, HERE)
 
 WIDTH, HEIGHT, DISTANCE, SEED = 3840, 2160, 1.0, 0
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_4k_d1.txt")
+
+
+def region_traffic_bytes() -> int | None:
+    """HBM bytes per region_kernel launch on this workload, from the committed
+    FETCH_SIZE/WRITE_SIZE summary (counters scaled by a copy of known size)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("region_kernel traffic_bytes="):
+                    return int(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(luma: np.ndarray, chroma: np.ndarray, budget_s: float = 12.0) -> dict:
@@ -51,6 +65,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build encodes on a single MI355X "
@@ -59,6 +75,14 @@ def main() -> int:
 
     import cujpegxl_amd as cj
     from cujpegxl_amd.synth import synthetic_nv12
+
+    if args.copy_bytes:
+        src, dst = cj.DeviceBuffer(args.copy_bytes), cj.DeviceBuffer(args.copy_bytes)
+        for _ in range(max(1, args.steps)):
+            cj.stages_lib().cjx_device_copy(dst.ptr, src.ptr, args.copy_bytes)
+        cj.stages_lib().cjx_device_synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "copies": max(1, args.steps)}))
+        return 0
 
     luma, chroma = synthetic_nv12(WIDTH, HEIGHT, SEED)
     nv12 = np.concatenate([luma.ravel(), chroma.ravel()])
@@ -116,7 +140,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
-            "traffic": None,
+            "traffic": region_traffic_bytes(),
         },
     }
     if not args.no_cpu_baseline:

@@ -101,6 +101,7 @@ def stages_lib() -> C.CDLL:
         L.cjx_device_free.restype = None
         L.cjx_copy_to_device.argtypes = [P, P, C.c_size_t]
         L.cjx_copy_to_host.argtypes = [P, P, C.c_size_t]
+        L.cjx_device_copy.argtypes = [P, P, C.c_size_t]
         L.cjx_host_table.restype = C.c_size_t
         L.cjx_host_table.argtypes = [C.c_int, P, C.c_size_t]
         _stages = L

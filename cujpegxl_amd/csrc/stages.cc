@@ -114,6 +114,11 @@ int cjx_copy_to_host(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
 }
 
+int cjx_device_copy(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice) == hipSuccess ? CUJPEGXL_OK
+                                                                             : CUJPEGXL_INTERNAL_ERROR;
+}
+
 int cjx_device_synchronize(void) {
     return hipDeviceSynchronize() == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
 }

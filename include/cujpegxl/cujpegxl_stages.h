@@ -63,6 +63,9 @@ void cjx_device_free(void* ptr);
 int cjx_copy_to_device(void* dst, const void* src, size_t bytes);
 int cjx_copy_to_host(void* dst, const void* src, size_t bytes);
 int cjx_device_synchronize(void);
+/* Device-to-device copy (a plain copy kernel of known size, used to calibrate
+ * the HBM byte counters against the encoder's kernels). */
+int cjx_device_copy(void* dst, const void* src, size_t bytes);
 
 /* Constant tables as the device uses them, for table parity tests:
  * 0 dequant weights float [3*(64+256+1024)], 1 DCT bases float [64+256+1024],
