@@ -351,3 +351,50 @@ def test_quantization_is_rint_of_coeff_over_step(jo):
                     assert got == np.rint(val), (blk, e, val, got)
                     checked += 1
         assert checked > 1000 and ties < checked // 100
+
+
+def test_dct8_uniform_quantizer_is_rint_of_coeff_over_step(jo):
+    """quant.cu:30-97 semantics (DCT8 everywhere, uniform quant field, base
+    Y-to-B correlation 1.0), checked as quant_selfconsistency_test.py:41-102
+    and :148-179 do: every integer is rint(residual / step) in float64 with
+    step = w / (raw_quant_field * global_scale_float) for AC and
+    1 / (DC_INV_QUANT[c] * global_scale_float * quant_dc) for DC."""
+    import ctypes as C
+
+    w8, _, _ = jo.dequant_weights()
+    inv_dc = (4096.0, 512.0, 256.0)
+    for seed, d, (w, h) in ((4, 1.0, (64, 64)), (5, 3.0, (96, 64)), (6, 0.5, (64, 32))):
+        luma, chroma = _nv12(w, h, seed)
+        xyb = jo.gaborish(jo.nv12_to_xyb(luma, chroma, w, h))
+        coef = jo.forward_dct(xyb, np.full((w // 8) * (h // 8), 8, np.int8))
+        nblk = (w // 8) * (h // 8)
+        ac = np.zeros(3 * nblk * 64, np.int32)
+        dc = np.zeros(3 * nblk, np.int32)
+        jo.lib().jo_quantize_dct8_uniform(jo.ptr(coef), w, h, C.c_float(d), jo.ptr(ac), jo.ptr(dc))
+        cal = jo.calibrate(d)
+        gsf, rq, qdc = float(cal.global_scale_float), float(cal.raw_quant_field), float(cal.quant_dc)
+        v = coef.astype(np.float64).reshape(3, nblk, 64)
+        a = ac.reshape(3, nblk, 64)
+        dq = dc.reshape(3, nblk)
+        ties = checked = 0
+
+        def check(val, got):
+            nonlocal ties, checked
+            frac = np.abs(np.abs(val - np.floor(val)) - 0.5)
+            ok = frac >= 1e-4
+            ties += int(np.count_nonzero(~ok))
+            np.testing.assert_array_equal(got[ok], np.rint(val[ok]))
+            checked += int(np.count_nonzero(ok))
+
+        step = w8 / (rq * gsf)
+        qy = np.rint(v[1, :, 1:] / step[1, 1:])
+        check(v[1, :, 1:] / step[1, 1:], a[1, :, 1:])
+        check(v[0, :, 1:] / step[0, 1:], a[0, :, 1:])
+        check((v[2, :, 1:] - qy * step[1, 1:]) / step[2, 1:], a[2, :, 1:])
+        dstep = [1.0 / (inv_dc[c] * gsf * qdc) for c in range(3)]
+        qdy = np.rint(v[1, :, 0] / dstep[1])
+        check(v[1, :, 0] / dstep[1], dq[1])
+        check(v[0, :, 0] / dstep[0], dq[0])
+        check((v[2, :, 0] - qdy * dstep[1]) / dstep[2], dq[2])
+        assert np.all(a[:, :, 0] == 0)
+        assert checked > 3 * nblk * 60 and ties < checked // 100
