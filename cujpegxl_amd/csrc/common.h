@@ -85,6 +85,7 @@ struct DeviceTables {
     std::uint16_t order32[1024];
     double llf2[4];        // T[f][o] = ortho_2(f, o) * resample_scale_2[f]
     double llf4[16];
+    float rate[256];       // selection rate term 1 + log2(1 + k) for |q| = k >= 1; 0 for k = 0
 };
 
 extern __constant__ DeviceTables g_tables;

@@ -106,8 +106,11 @@ static unsigned int grid_for(std::size_t n, std::size_t per) {
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part) {
-    region_kernel<<<geo.rw * geo.rh, 256, 0, s>>>(geo, prm, luma, luma_pitch, chroma, chroma_pitch, coeffs,
-                                                  acs, qf, cfl_part);
+    // Eight equal runs of regions, one per XCD (the kernel maps workgroup i to
+    // run i mod 8); workgroups past the last region exit at once.
+    const unsigned int per_xcd{(geo.rw * geo.rh + 7) / 8};
+    region_kernel<<<8 * per_xcd, 256, 0, s>>>(geo, prm, luma, luma_pitch, chroma, chroma_pitch, coeffs, acs,
+                                              qf, cfl_part);
 }
 
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,

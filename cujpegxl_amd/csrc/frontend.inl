@@ -155,31 +155,74 @@ __device__ float erosion_cell(const float* pre, int cx, int cy, const FrameParam
     return p.km[0] * m0 + p.km[1] * m1 + p.km[2] * m2 + p.km[3] * m3;
 }
 
-// Block (bx, by) of the region (quad-major element layout) for size n.
-__device__ __forceinline__ void element_block(int n, int i, int& bx, int& by, int& e) {
-    if (n == 8) {
-        const int b{i >> 6};
-        const int quad{b >> 2}, j{b & 3};
-        bx = 2 * (quad & 1) + (j & 1);
-        by = 2 * (quad >> 1) + (j >> 1);
-        e = i & 63;
-    } else if (n == 16) {
-        const int quad{i >> 8};
-        bx = 2 * (quad & 1);
-        by = 2 * (quad >> 1);
-        e = i & 255;
+// Separable DCT building block.  One lane owns one line of N inputs (a pixel
+// row for the row pass, a row-pass output line for the column pass) and
+// produces F consecutive frequencies k0..k0+F-1 of it:
+//   out[f] = sum_{c = 0..N-1} A[k0 + f][c] * in[c]   (fmaf chain in c order).
+// k0 is wave-uniform, so every basis value is a scalar operand; the line is
+// read with 16-byte LDS loads.
+template <int N>
+__device__ __forceinline__ const float* basis_n() {
+    if constexpr (N == 8) {
+        return g_tables.basis8;
+    } else if constexpr (N == 16) {
+        return g_tables.basis16;
     } else {
-        bx = 0;
-        by = 0;
-        e = i;
+        return g_tables.basis32;
     }
 }
 
+template <int N, int F>
+__device__ __forceinline__ void dct_line(const float* __restrict__ src, int k0, float (&out)[F]) {
+    float p[N];
+#pragma unroll
+    for (int c{0}; c < N; c += 4) {
+        const float4 v{*reinterpret_cast<const float4*>(src + c)};
+        p[c] = v.x;
+        p[c + 1] = v.y;
+        p[c + 2] = v.z;
+        p[c + 3] = v.w;
+    }
+    const float* A{basis_n<N>() + k0 * N};
+#pragma unroll
+    for (int f{0}; f < F; ++f) {
+        float s{0.0f};
+#pragma unroll
+        for (int c{0}; c < N; ++c) {
+            s = __builtin_fmaf(A[f * N + c], p[c], s);
+        }
+        out[f] = s;
+    }
+}
+
+// Row-pass buffer geometry (floats): line (block b, frequency k) of size n
+// starts at b * BS + k * RS.  RS is a multiple of 4 (16-byte loads) chosen so
+// that eight consecutive lines start in eight different 4-bank groups.
+template <int N>
+struct RowBuf;
+template <>
+struct RowBuf<8> {
+    static constexpr int RS = 12, BS = 104;
+};
+template <>
+struct RowBuf<16> {
+    static constexpr int RS = 20, BS = 320;
+};
+template <>
+struct RowBuf<32> {
+    static constexpr int RS = 36, BS = 1152;
+};
+
+constexpr int kSP = 36;  // sharpened-plane row stride (floats)
+
 struct RegionShared {
-    float xyb[3][kTileW * kTileW];
-    float sharp[3][32 * 32];
-    float rows[1024];
-    float coefy[3][1024];  // selection coefficients of Y for n = 8, 16, 32
+    union {
+        float xyb[3][kTileW * kTileW];  // NV12 -> XYB tile with halo (steps 1-3)
+        float scratch[3 * kTileW * kTileW];  // row-pass lines and selection costs (steps 4-5)
+    } u;
+    alignas(16) float sharp[3][32 * kSP];
+    _Float16 coefy[3][1024];   // Y coefficients of each candidate size, element order
+    _Float16 coefxb[2][1024];  // X and B coefficients of the chosen transforms
     float mask[1024];
     float cells[64];
     float part[4][4][2];   // [wave][quad][d|r]
@@ -190,6 +233,24 @@ struct RegionShared {
     int acs[16];           // region-local raster
 };
 
+// Selection terms of one Y coefficient (transform_select_impl.cuh:49-108):
+// squared quantization error and rate proxy; LLF coefficients contribute 0.
+__device__ __forceinline__ void selection_terms(float s, int e, int n, const float* wsel, float qgsf,
+                                                float& d, float& rt) {
+    const int m{n / 8};
+    if (e / n < m && e % n < m) {
+        d = 0.0f;
+        rt = 0.0f;
+        return;
+    }
+    const float wq{wsel[e]};
+    const float q{__builtin_rintf(s * qgsf / wq)};
+    const float err{q * wq / qgsf - s};
+    d = err * err;
+    const float aq{fabsf(q)};
+    rt = aq < 256.0f ? g_tables.rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
+}
+
 __global__ __launch_bounds__(256) void region_kernel(
     Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
@@ -198,8 +259,16 @@ __global__ __launch_bounds__(256) void region_kernel(
     __shared__ RegionShared S;
     const int tid{static_cast<int>(threadIdx.x)};
     const int w{tid >> 6};
+    const int wu{__builtin_amdgcn_readfirstlane(w)};
     const int lane{tid & 63};
-    const int region{static_cast<int>(blockIdx.x)};
+    // Workgroup i runs on XCD i mod 8: give each XCD a contiguous run of
+    // regions so that rows shared by neighbouring regions stay in one L2.
+    const int nreg{static_cast<int>(geo.rw * geo.rh)};
+    const int per_xcd{(nreg + 7) / 8};
+    const int region{static_cast<int>(blockIdx.x & 7u) * per_xcd + static_cast<int>(blockIdx.x >> 3)};
+    if (region >= nreg) {
+        return;
+    }
     const int rx{region % static_cast<int>(geo.rw)};
     const int ry{region / static_cast<int>(geo.rw)};
     const int px0{rx * 32}, py0{ry * 32};
@@ -231,7 +300,7 @@ __global__ __launch_bounds__(256) void region_kernel(
                        (1.0f / 255.0f)};
         const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
         const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
-        pixel_to_xyb(yv, cb, cr, S.xyb[0][i], S.xyb[1][i], S.xyb[2][i]);
+        pixel_to_xyb(yv, cb, cr, S.u.xyb[0][i], S.u.xyb[1][i], S.u.xyb[2][i]);
     }
     if (tid < 16) {
         S.acs[tid] = 8;
@@ -247,7 +316,7 @@ __global__ __launch_bounds__(256) void region_kernel(
         constexpr float wc{nm}, wr{nm * k0}, wr2{nm * k2}, wd{nm * k1}, wl{nm * k3}, wd2{nm * k4};
         for (int i{tid}; i < 3 * 1024; i += 256) {
             const int c{i >> 10}, p{i & 1023};
-            const float* s{&S.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+            const float* s{&S.u.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
 #define AT(dx, dy) s[(dy) * kTileW + (dx)]
             const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
             const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
@@ -255,12 +324,12 @@ __global__ __launch_bounds__(256) void region_kernel(
             const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
                            AT(-1, 2) + AT(1, 2)};
             const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
-            S.sharp[c][p] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
+            S.sharp[c][(p >> 5) * kSP + (p & 31)] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
 #undef AT
         }
         // 3a. AQ pre-erosion terms per interior pixel (frontend_fused.cu:118-137).
         for (int p{tid}; p < 1024; p += 256) {
-            const float* s{&S.xyb[1][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+            const float* s{&S.u.xyb[1][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
             const float yv{s[0]};
             const float base{0.25f * (s[kTileW] + s[-kTileW] + s[-1] + s[1])};
             float diff{ratio_cbrt_gamma(yv + 0.019f, false) * (yv - base)};
@@ -291,13 +360,13 @@ __global__ __launch_bounds__(256) void region_kernel(
         }
         const int fx{lane & 7}, fy{lane >> 3};
         const int idx{(lby * 8 + fy + kHalo) * kTileW + lbx * 8 + fx + kHalo};
-        const float yv{S.xyb[1][idx]}, xv{S.xyb[0][idx]}, bv{S.xyb[2][idx]};
+        const float yv{S.u.xyb[1][idx]}, xv{S.u.xyb[0][idx]}, bv{S.u.xyb[2][idx]};
         float hf{0.0f};
         if (fx < 7) {
-            hf += fminf(0.0206f, fabsf(yv - S.xyb[1][idx + 1]));
+            hf += fminf(0.0206f, fabsf(yv - S.u.xyb[1][idx + 1]));
         }
         if (fy < 7) {
-            hf += fminf(0.0206f, fabsf(yv - S.xyb[1][idx + kTileW]));
+            hf += fminf(0.0206f, fabsf(yv - S.u.xyb[1][idx + kTileW]));
         }
         const float iny{yv + 0.16f};
         const float gm{ratio_cbrt_gamma(iny - xv, true) + ratio_cbrt_gamma(iny + xv, true)};
@@ -331,48 +400,95 @@ __global__ __launch_bounds__(256) void region_kernel(
     }
 
     // 4. RD transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
+    //    Each size: row pass, column pass (the Y coefficients go to coefy in
+    //    element order), selection terms scattered to element order, then the
+    //    per-block / per-quad / per-region sums over the element layout.
+    __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
+    float* const lines{S.u.scratch};
+    float* const dsc{S.u.scratch + 16 * RowBuf<8>::BS};
+    float* const rsc{S.u.scratch + 16 * RowBuf<8>::BS + 1024};
+    const float qgsf{prm.qgsf_select};
     for (int sz{0}; sz < 3; ++sz) {
         const int n{8 << sz};
-        const float* A{basis_of(n)};
+        if (n == 8) {
+            // 16 blocks x 8 rows; waves {0,1} / {2,3} take frequencies 0-3 / 4-7.
+            const int job{((wu & 1) << 6) | lane};
+            const int b{job >> 3}, r{job & 7};
+            const int k0{(wu >> 1) * 4};
+            float o[4];
+            dct_line<8, 4>(&S.sharp[1][((b >> 2) * 8 + r) * kSP + (b & 3) * 8], k0, o);
 #pragma unroll
-        for (int k{0}; k < 4; ++k) {
-            const int i{256 * k + 64 * w + lane};
-            int bx, by, e;
-            element_block(n, i, bx, by, e);
-            const int fx{e / n}, r{e % n};
-            const float* prow{&S.sharp[1][(by * 8 + r) * 32 + bx * 8]};
-            float s{0.0f};
-            for (int c{0}; c < n; ++c) {
-                s = __builtin_fmaf(A[fx * n + c], prow[c], s);
+            for (int f{0}; f < 4; ++f) {
+                lines[b * RowBuf<8>::BS + (k0 + f) * RowBuf<8>::RS + r] = o[f];
             }
-            S.rows[i] = s;
+            __syncthreads();
+            const int fx{job & 7};
+            const int bx{b & 3}, by{b >> 2};
+            const int base{256 * ((by >> 1) * 2 + (bx >> 1)) + 64 * ((by & 1) * 2 + (bx & 1))};
+            dct_line<8, 4>(&lines[b * RowBuf<8>::BS + fx * RowBuf<8>::RS], k0, o);
+#pragma unroll
+            for (int f{0}; f < 4; ++f) {
+                const int e{fx * 8 + k0 + f};
+                float d, rt;
+                selection_terms(o[f], e, 8, g_tables.wsel8, qgsf, d, rt);
+                dsc[base + e] = d;
+                rsc[base + e] = rt;
+                S.coefy[0][base + e] = static_cast<_Float16>(o[f]);
+            }
+        } else if (n == 16) {
+            // 4 quads x 16 rows; wave w takes frequencies 4w..4w+3.
+            const int q{lane >> 4}, r{lane & 15};
+            const int k0{wu * 4};
+            float o[4];
+            dct_line<16, 4>(&S.sharp[1][((q >> 1) * 16 + r) * kSP + (q & 1) * 16], k0, o);
+#pragma unroll
+            for (int f{0}; f < 4; ++f) {
+                lines[q * RowBuf<16>::BS + (k0 + f) * RowBuf<16>::RS + r] = o[f];
+            }
+            __syncthreads();
+            const int fx{lane & 15};
+            dct_line<16, 4>(&lines[q * RowBuf<16>::BS + fx * RowBuf<16>::RS], k0, o);
+#pragma unroll
+            for (int f{0}; f < 4; ++f) {
+                const int e{fx * 16 + k0 + f};
+                float d, rt;
+                selection_terms(o[f], e, 16, g_tables.wsel16, qgsf, d, rt);
+                dsc[256 * q + e] = d;
+                rsc[256 * q + e] = rt;
+                S.coefy[1][256 * q + e] = static_cast<_Float16>(o[f]);
+            }
+        } else {
+            // 32 rows on lanes 0-31; wave w takes frequencies 8w..8w+7.
+            const int k0{wu * 8};
+            float o[8];
+            if (lane < 32) {
+                dct_line<32, 8>(&S.sharp[1][lane * kSP], k0, o);
+#pragma unroll
+                for (int f{0}; f < 8; ++f) {
+                    lines[(k0 + f) * RowBuf<32>::RS + lane] = o[f];
+                }
+            }
+            __syncthreads();
+            if (lane < 32) {
+                dct_line<32, 8>(&lines[lane * RowBuf<32>::RS], k0, o);
+#pragma unroll
+                for (int f{0}; f < 8; ++f) {
+                    const int e{lane * 32 + k0 + f};
+                    float d, rt;
+                    selection_terms(o[f], e, 32, g_tables.wsel32, qgsf, d, rt);
+                    dsc[e] = d;
+                    rsc[e] = rt;
+                    S.coefy[2][e] = static_cast<_Float16>(o[f]);
+                }
+            }
         }
         __syncthreads();
         float d[4], rt[4];
-        const float* wsel{wsel_of(n)};
 #pragma unroll
         for (int k{0}; k < 4; ++k) {
             const int i{256 * k + 64 * w + lane};
-            int bx, by, e;
-            element_block(n, i, bx, by, e);
-            const int fx{e / n}, fy{e % n};
-            const float* rr{&S.rows[i - e + fx * n]};
-            float s{0.0f};
-            for (int r{0}; r < n; ++r) {
-                s = __builtin_fmaf(A[fy * n + r], rr[r], s);
-            }
-            S.coefy[sz][i] = s;
-            const int m{n / 8};
-            if (fx < m && fy < m) {
-                d[k] = 0.0f;
-                rt[k] = 0.0f;
-            } else {
-                const float wq{wsel[e]};
-                const float q{__builtin_rintf(s * prm.qgsf_select / wq)};
-                const float err{q * wq / prm.qgsf_select - s};
-                d[k] = err * err;
-                rt[k] = q != 0.0f ? 1.0f + log2_det(1.0f + fabsf(q)) : 0.0f;
-            }
+            d[k] = dsc[i];
+            rt[k] = rsc[i];
         }
         if (n == 8) {
 #pragma unroll
@@ -480,74 +596,101 @@ __global__ __launch_bounds__(256) void region_kernel(
         }
     }
 
-    // 5. Chosen DCTs of X and B (Y reuses the selection coefficients), FP16 out,
-    //    CfL partials (frontend_quantize.cu:49-117 on this region).
-    int own_n[4], own_e[4], own_idx[4], own_ox[4], own_oy[4];
-    bool inside[4];
+    // 5. Chosen DCTs of X and B (Y reuses the selection coefficients) into
+    //    coefxb in element order, then FP16 out and the CfL partials
+    //    (frontend_quantize.cu:49-117 on this region).
+    const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
+    if (acs0 == 32) {
+        // X rows on lanes 0-31, B rows on lanes 32-63; wave w takes frequencies 8w..8w+7.
+        const int ch{lane >> 5}, r{lane & 31};
+        const int k0{wu * 8};
+        float o[8];
+        dct_line<32, 8>(&S.sharp[2 * ch][r * kSP], k0, o);
 #pragma unroll
-    for (int k{0}; k < 4; ++k) {
-        const int bx{2 * (k & 1) + (w & 1)}, by{2 * (k >> 1) + (w >> 1)};
-        inside[k] = bx0 + bx < bw && by0 + by < bh;
-        int n{8}, ox{bx}, oy{by};
-        if (S.acs[0] == 32) {
-            n = 32;
-            ox = 0;
-            oy = 0;
-        } else if (S.acs[(by & ~1) * 4 + (bx & ~1)] == 16) {
-            n = 16;
-            ox = bx & ~1;
-            oy = by & ~1;
-        }
-        const int q{(by - oy) * (n / 8) + (bx - ox)};
-        const int e{q * 64 + lane};
-        own_n[k] = n;
-        own_e[k] = e;
-        own_ox[k] = ox;
-        own_oy[k] = oy;
-        own_idx[k] = n == 32 ? e : 256 * k + 64 * w + lane;
-    }
-    float xc[4], bc[4];
-    for (int ch{0}; ch < 3; ch += 2) {
-#pragma unroll
-        for (int k{0}; k < 4; ++k) {
-            const int n{own_n[k]}, e{own_e[k]};
-            const float* A{basis_of(n)};
-            const int fx{e / n}, r{e % n};
-            const float* prow{&S.sharp[ch][(own_oy[k] * 8 + r) * 32 + own_ox[k] * 8]};
-            float s{0.0f};
-            for (int c{0}; c < n; ++c) {
-                s = __builtin_fmaf(A[fx * n + c], prow[c], s);
-            }
-            S.rows[own_idx[k]] = s;
+        for (int f{0}; f < 8; ++f) {
+            lines[ch * RowBuf<32>::BS + (k0 + f) * RowBuf<32>::RS + r] = o[f];
         }
         __syncthreads();
+        dct_line<32, 8>(&lines[ch * RowBuf<32>::BS + r * RowBuf<32>::RS], k0, o);
 #pragma unroll
-        for (int k{0}; k < 4; ++k) {
-            const int n{own_n[k]}, e{own_e[k]};
-            const float* A{basis_of(n)};
-            const int fx{e / n}, fy{e % n};
-            const float* rr{&S.rows[own_idx[k] - e + fx * n]};
-            float s{0.0f};
-            for (int r{0}; r < n; ++r) {
-                s = __builtin_fmaf(A[fy * n + r], rr[r], s);
+        for (int f{0}; f < 8; ++f) {
+            S.coefxb[ch][r * 32 + k0 + f] = static_cast<_Float16>(o[f]);
+        }
+    } else {
+        // Wave w transforms quad w: one DCT16 (X rows on lanes 0-15, B rows on
+        // 16-31) or four DCT8s (lanes: channel, block, row).
+        constexpr int kQuadLines{2 * 4 * RowBuf<8>::BS};
+        float* const ql{lines + wu * kQuadLines};
+        const int q{wu};
+        const int n16{__builtin_amdgcn_readfirstlane(S.acs[(q >> 1) * 8 + (q & 1) * 2])};
+        if (n16 == 16) {
+            if (lane < 32) {
+                const int ch{lane >> 4}, r{lane & 15};
+                float o[16];
+                dct_line<16, 16>(&S.sharp[2 * ch][((q >> 1) * 16 + r) * kSP + (q & 1) * 16], 0, o);
+#pragma unroll
+                for (int f{0}; f < 16; ++f) {
+                    ql[ch * RowBuf<16>::BS + f * RowBuf<16>::RS + r] = o[f];
+                }
             }
-            (ch == 0 ? xc : bc)[k] = s;
+        } else {
+            const int ch{lane >> 5}, j{(lane >> 3) & 3}, r{lane & 7};
+            const int bx{2 * (q & 1) + (j & 1)}, by{2 * (q >> 1) + (j >> 1)};
+            float o[8];
+            dct_line<8, 8>(&S.sharp[2 * ch][(by * 8 + r) * kSP + bx * 8], 0, o);
+#pragma unroll
+            for (int f{0}; f < 8; ++f) {
+                ql[(ch * 4 + j) * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
+            }
         }
         __syncthreads();
+        if (n16 == 16) {
+            if (lane < 32) {
+                const int ch{lane >> 4}, fx{lane & 15};
+                float o[16];
+                dct_line<16, 16>(&ql[ch * RowBuf<16>::BS + fx * RowBuf<16>::RS], 0, o);
+#pragma unroll
+                for (int f{0}; f < 16; ++f) {
+                    S.coefxb[ch][256 * q + fx * 16 + f] = static_cast<_Float16>(o[f]);
+                }
+            }
+        } else {
+            const int ch{lane >> 5}, j{(lane >> 3) & 3}, fx{lane & 7};
+            float o[8];
+            dct_line<8, 8>(&ql[(ch * 4 + j) * RowBuf<8>::BS + fx * RowBuf<8>::RS], 0, o);
+#pragma unroll
+            for (int f{0}; f < 8; ++f) {
+                S.coefxb[ch][256 * q + 64 * j + fx * 8 + f] = static_cast<_Float16>(o[f]);
+            }
+        }
     }
+    __syncthreads();
+
     const std::size_t cplane{static_cast<std::size_t>(geo.nblocks) * 64};
     float acc[3]{0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int k{0}; k < 4; ++k) {
-        const int n{own_n[k]}, e{own_e[k]};
+        const int bxl{2 * (k & 1) + (w & 1)}, byl{2 * (k >> 1) + (w >> 1)};
+        int n{8}, ox{bxl}, oy{byl};
+        if (S.acs[0] == 32) {
+            n = 32;
+            ox = 0;
+            oy = 0;
+        } else if (S.acs[(byl & ~1) * 4 + (bxl & ~1)] == 16) {
+            n = 16;
+            ox = bxl & ~1;
+            oy = byl & ~1;
+        }
+        const int qc{(byl - oy) * (n / 8) + (bxl - ox)};
+        const int e{qc * 64 + lane};
+        const int idx{n == 32 ? e : 256 * k + 64 * w + lane};
         const int sz{n == 32 ? 2 : (n == 16 ? 1 : 0)};
-        const _Float16 hx{static_cast<_Float16>(xc[k])};
-        const _Float16 hy{static_cast<_Float16>(S.coefy[sz][own_idx[k]])};
-        const _Float16 hb{static_cast<_Float16>(bc[k])};
+        const _Float16 hx{S.coefxb[0][idx]};
+        const _Float16 hy{S.coefy[sz][idx]};
+        const _Float16 hb{S.coefxb[1][idx]};
         float v[3]{0.0f, 0.0f, 0.0f};
-        if (inside[k]) {
-            const int bx{bx0 + 2 * (k & 1) + (w & 1)}, by{by0 + 2 * (k >> 1) + (w >> 1)};
-            const std::size_t slot{(static_cast<std::size_t>(by) * bw + bx) * 64 + lane};
+        if (bx0 + bxl < bw && by0 + byl < bh) {
+            const std::size_t slot{(static_cast<std::size_t>(by0 + byl) * bw + bx0 + bxl) * 64 + lane};
             coeffs[slot] = __builtin_bit_cast(std::uint16_t, hx);
             coeffs[cplane + slot] = __builtin_bit_cast(std::uint16_t, hy);
             coeffs[2 * cplane + slot] = __builtin_bit_cast(std::uint16_t, hb);

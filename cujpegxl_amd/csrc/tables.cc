@@ -91,6 +91,10 @@ DeviceTables build_tables() {
     natural_order(32, t.order32);
     llf_table(2, t.llf2);
     llf_table(4, t.llf4);
+    t.rate[0] = 0.0f;
+    for (int k{1}; k < 256; ++k) {
+        t.rate[k] = 1.0f + log2_det(1.0f + static_cast<float>(k));
+    }
     return t;
 }
 
