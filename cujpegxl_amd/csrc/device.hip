@@ -197,10 +197,11 @@ void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* la
 
 void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                     const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
-                    const std::uint32_t* item_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
-                    std::uint64_t cap_words) {
-    ac_emit_kernel<<<item_grid(geo), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, &cs->depth[0][0],
-                                                  &cs->code[0][0], item_off, ac_base, out_words, cap_words);
+                    const std::uint32_t* item_off, const std::uint32_t* item_bits, const std::uint64_t* ac_base,
+                    std::uint32_t* out_words, std::uint64_t cap_words) {
+    ac_emit_kernel<<<geo.num_ac * kAcGroupBlocks, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, &cs->depth[0][0],
+                                                               &cs->code[0][0], item_off, item_bits, ac_base,
+                                                               out_words, cap_words);
 }
 
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,

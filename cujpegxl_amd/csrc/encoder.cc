@@ -481,7 +481,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
                   lay, ac_base_, dc_base_);
     mark(Stage::kEmit);
     launch_zero_and_headers(stream_, k, lay, out_, out_cap_ / 4);
-    launch_ac_emit(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, codes, item_off_, ac_base_, out_, out_cap_ / 4);
+    launch_ac_emit(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, codes, item_off_, item_bits_, ac_base_, out_,
+                   out_cap_ / 4);
     launch_dc_emit(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_base_,
                    out_, out_cap_ / 4);
     mark(Stage::kReadback);

@@ -318,7 +318,15 @@ __global__ __launch_bounds__(1024) void ac_group_scan_kernel(Geometry geo,
     }
 }
 
-// Pass 3: emission at group byte offset + item bit offset.
+// Pass 3: emission.  One workgroup per (AC group, block row of the group):
+// the row's items, in decoder order (blocks left to right, channels Y, X, B),
+// occupy one contiguous bit range of the group's section.  The workgroup ORs
+// its tokens into an LDS copy of that range, then writes the interior words
+// with plain stores; only the first and last word, which a neighbouring range
+// may share, are combined with atomicOr into the zeroed output.  A range
+// longer than the LDS copy is emitted with global atomics instead.
+constexpr int kEmitWords = 2048;
+
 __global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
                                                       const std::int8_t* __restrict__ acs,
                                                       const std::uint8_t* __restrict__ nz_grid,
@@ -326,21 +334,44 @@ __global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::i
                                                       const std::uint8_t* __restrict__ depth,
                                                       const std::uint16_t* __restrict__ codes,
                                                       const std::uint32_t* __restrict__ item_off,
+                                                      const std::uint32_t* __restrict__ item_bits,
                                                       const std::uint64_t* __restrict__ ac_group_base_bits,
                                                       std::uint32_t* __restrict__ out_words,
                                                       std::uint64_t cap_words) {
-    const int lane{static_cast<int>(threadIdx.x) & 63};
-    const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
-    const std::uint32_t nwaves{gridDim.x * 4};
-    const std::uint32_t items{geo.nblocks * 3};
-    for (std::uint32_t item{wave}; item < items; item += nwaves) {
+    __shared__ std::uint32_t buf[kEmitWords];
+    const int tid{static_cast<int>(threadIdx.x)};
+    const int lane{tid & 63};
+    const int wave{tid >> 6};
+    const std::uint32_t g{blockIdx.x / kAcGroupBlocks};
+    const int row{static_cast<int>(blockIdx.x % kAcGroupBlocks)};
+    const int bx0{static_cast<int>(g % geo.xg) * kAcGroupBlocks};
+    const int by{static_cast<int>(g / geo.xg) * kAcGroupBlocks + row};
+    if (by >= static_cast<int>(geo.bh)) {
+        return;
+    }
+    const int gbw{min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0)};
+    const std::size_t first{(static_cast<std::size_t>(by) * geo.bw + bx0) * 3};
+    const std::size_t last{first + 3 * static_cast<std::size_t>(gbw) - 1};
+    const std::uint64_t gbase{ac_group_base_bits[g]};
+    const std::uint64_t start{gbase + item_off[first]};
+    const std::uint64_t end{gbase + item_off[last] + item_bits[last]};
+    const std::uint64_t w0{start >> 5};
+    const std::uint64_t nw{((end + 31) >> 5) - w0};
+    const bool staged{nw <= static_cast<std::uint64_t>(kEmitWords)};
+    if (staged) {
+        for (int i{tid}; i < static_cast<int>(nw); i += 256) {
+            buf[i] = 0;
+        }
+    }
+    __syncthreads();
+    const std::uint64_t lbase{w0 << 5};
+    for (int i{wave}; i < 3 * gbw; i += 4) {
+        const std::uint32_t item{static_cast<std::uint32_t>(first) + static_cast<std::uint32_t>(i)};
         const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
         if (it.n == 0) {
             continue;
         }
-        const std::uint32_t g{static_cast<std::uint32_t>(it.by / kAcGroupBlocks) * geo.xg +
-                              static_cast<std::uint32_t>(it.bx / kAcGroupBlocks)};
-        std::uint64_t pos{ac_group_base_bits[g] + item_off[item]};
+        std::uint64_t pos{gbase + item_off[item]};
         tokenize_item(geo, ac, it, lane,
                       [&](bool valid, int cl, std::uint32_t sym, std::uint32_t nb, std::uint32_t extra) {
                           std::uint32_t len{0}, val{0};
@@ -351,9 +382,45 @@ __global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::i
                               val = static_cast<std::uint32_t>(codes[idx]) | (extra << d);
                           }
                           const int pre{wave_excl_scan(static_cast<int>(len), lane)};
-                          or_bits(out_words, cap_words, pos + static_cast<std::uint64_t>(pre), val, len);
+                          const std::uint64_t at{pos + static_cast<std::uint64_t>(pre)};
+                          if (staged) {
+                              if (len != 0) {
+                                  if (len < 32) {
+                                      val &= (1u << len) - 1u;
+                                  }
+                                  const std::uint64_t rel{at - lbase};
+                                  const std::uint32_t wi{static_cast<std::uint32_t>(rel >> 5)};
+                                  const std::uint32_t b{static_cast<std::uint32_t>(rel & 31u)};
+                                  if (wi < static_cast<std::uint32_t>(kEmitWords)) {
+                                      atomicOr(&buf[wi], val << b);
+                                  }
+                                  if (b + len > 32 && wi + 1 < static_cast<std::uint32_t>(kEmitWords)) {
+                                      atomicOr(&buf[wi + 1], val >> (32 - b));
+                                  }
+                              }
+                          } else {
+                              or_bits(out_words, cap_words, at, val, len);
+                          }
                           pos += static_cast<std::uint64_t>(wave_sum_int(static_cast<int>(len)));
                       });
+    }
+    if (!staged) {
+        return;
+    }
+    __syncthreads();
+    for (int i{tid}; i < static_cast<int>(nw); i += 256) {
+        const std::uint64_t gi{w0 + static_cast<std::uint64_t>(i)};
+        if (gi >= cap_words) {
+            continue;
+        }
+        const std::uint32_t v{buf[i]};
+        if (i == 0 || i == static_cast<int>(nw) - 1) {
+            if (v != 0) {
+                atomicOr(&out_words[gi], v);
+            }
+        } else {
+            out_words[gi] = v;
+        }
     }
 }
 

@@ -66,8 +66,8 @@ void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* la
                              std::uint64_t cap_words);
 void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                     const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
-                    const std::uint32_t* item_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
-                    std::uint64_t cap_words);
+                    const std::uint32_t* item_off, const std::uint32_t* item_bits, const std::uint64_t* ac_base,
+                    std::uint32_t* out_words, std::uint64_t cap_words);
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,

@@ -177,3 +177,28 @@ def test_4k_frame_matches_oracle(jo):
     assert data == want, f"{len(data)} vs {len(want)}"
     again = enc.encode_device(_buf.ptr, _buf.ptr + 3840 * 2160, 3840, 3840)
     assert again == data
+
+
+def test_dense_coefficients_match_oracle(jo):
+    """Large random coefficients in every slot: some block rows of an AC group
+    exceed the emitter's LDS range and take the global-atomic path, the rest
+    the staged path; the codestream is the oracle's either way."""
+    say("dense coefficients 512x264")
+    w, h = 512, 264
+    rng = np.random.default_rng(11)
+    nblk = (w // 8) * (h // 8)
+    acs = np.full(nblk, 8, np.int8)
+    ac = rng.integers(-3000, 3000, size=3 * nblk * 64).astype(np.int16)
+    ac.reshape(3, nblk, 64)[:, :, 0] = 0
+    ac.reshape(3, nblk, 64)[:, : nblk // 2, 8:] = 0  # sparse top half: staged rows
+    dc = rng.integers(-500, 500, size=3 * nblk).astype(np.int32)
+    qf = rng.integers(1, 256, size=nblk).astype(np.int32)
+    tiles = ((w // 8 + 7) // 8) * ((h // 8 + 7) // 8)
+    ytox = rng.integers(-128, 128, size=tiles).astype(np.int8)
+    ytob = rng.integers(-128, 128, size=tiles).astype(np.int8)
+    gs, qdc = 2048, 300
+    enc = cj.Encoder(w, h, 1.0, any_size=True)
+    data = enc.encode_coefficients(ac, dc, acs, qf, ytox, ytob, gs, qdc)
+    cs = jo.write_codestream(jo.make_frame(w, h, gs, qdc, dc, ac, acs, qf, ytox, ytob))
+    assert len(cs) > 300_000
+    assert data[40:] == cs
