@@ -96,6 +96,7 @@ def stages_lib() -> C.CDLL:
         L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
         L.cjx_stage_timings.argtypes = [P, C.POINTER(C.c_float), C.c_int]
         L.cjx_device_alloc.restype = P
+        L.cjx_stage_build_codes.argtypes = [C.c_int32, P, C.c_uint32, P, P, P, P, P]
         L.cjx_device_alloc.argtypes = [C.c_size_t]
         L.cjx_device_free.argtypes = [P]
         L.cjx_device_free.restype = None
@@ -130,6 +131,31 @@ def max_output_size(width: int, height: int, distance: float = 1.0) -> int:
     if s != OK:
         raise ValueError(status_string(s))
     return n.value
+
+
+def build_codes(hists: np.ndarray, device_ordinal: int = 0) -> list[dict]:
+    """Prefix codes of up to 96 histograms (rows of 256 uint32 bins) built by
+    the device code kernel: alphabet size, depths, codes and header bytes."""
+    h = np.ascontiguousarray(hists, dtype=np.uint32).reshape(-1, 256)
+    n = h.shape[0]
+    if not 0 < n <= 96:
+        raise ValueError("between 1 and 96 histograms")
+    alpha = np.zeros(n, np.uint32)
+    depth = np.zeros((n, 256), np.uint8)
+    bits = np.zeros((n, 256), np.uint16)
+    hdr = np.zeros((n, 64), np.uint32)
+    nbits = np.zeros(n, np.uint32)
+    s = stages_lib().cjx_stage_build_codes(device_ordinal, h.ctypes.data, n, alpha.ctypes.data,
+                                           depth.ctypes.data, bits.ctypes.data, hdr.ctypes.data,
+                                           nbits.ctypes.data)
+    if s != OK:
+        raise RuntimeError(f"build_codes failed: {status_string(s)}")
+    out = []
+    for i in range(n):
+        nb = int(nbits[i])
+        out.append({"alpha": int(alpha[i]), "depth": depth[i], "bits": bits[i],
+                    "header": hdr[i].view(np.uint8)[: (nb + 7) // 8].tobytes(), "header_bits": nb})
+    return out
 
 
 class DeviceBuffer:

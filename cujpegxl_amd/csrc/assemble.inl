@@ -47,46 +47,47 @@ struct Layout {
     std::uint32_t head_total_bits;
 };
 
+// One wave per histogram: alphabet size, depths, codes and header.
 __global__ __launch_bounds__(64) void huffman_kernel(const std::uint32_t* __restrict__ hists, CodeSet* __restrict__ cs) {
-    __shared__ HuffScratch scratch;
+    __shared__ WaveCodeScratch scratch;
     __shared__ std::uint32_t hist[kHistStride];
     __shared__ std::uint8_t depth[kHistStride];
     __shared__ std::uint16_t code[kHistStride];
     __shared__ std::uint32_t hdr[kHdrWords];
-    __shared__ int alpha_s;
     const int h{static_cast<int>(blockIdx.x)};
     const int t{static_cast<int>(threadIdx.x)};
+    std::uint32_t top{0};
     for (int i{t}; i < kHistStride; i += 64) {
-        hist[i] = hists[h * kHistStride + i];
+        const std::uint32_t v{hists[h * kHistStride + i]};
+        hist[i] = v;
         depth[i] = 0;
         code[i] = 0;
-    }
-    hdr[t] = 0;
-    if (t == 0) {
-        alpha_s = 1;
-    }
-    __syncthreads();
-    for (int i{t}; i < kHistStride; i += 64) {
-        if (hist[i]) {
-            atomicMax(&alpha_s, i + 1);
+        if (v != 0) {
+            top = static_cast<std::uint32_t>(i) + 1u;
         }
     }
+    for (int i{t}; i < kHdrWords; i += 64) {
+        hdr[i] = 0;
+    }
+    const int alpha{static_cast<int>(wave_max_u32(top > 1u ? top : 1u))};
     __syncthreads();
-    const int alpha{alpha_s};
-    if (t == 0) {
-        BitSink w{hdr, 0, kHdrWords * 32};
-        if (alpha > 1) {
-            build_and_store_huffman(hist, alpha, depth, code, w, scratch);
-        }
-        cs->hdr_bits[h] = w.ok() ? w.pos : 0u;
-        cs->alpha[h] = static_cast<std::uint32_t>(alpha);
+    std::uint32_t nbits{0};
+    if (alpha > 1) {
+        nbits = alpha <= 64 ? wave_build_code<1>(hist, alpha, depth, code, hdr, kHdrWords, scratch)
+                            : wave_build_code<4>(hist, alpha, depth, code, hdr, kHdrWords, scratch);
     }
     __syncthreads();
     for (int i{t}; i < kHistStride; i += 64) {
         cs->depth[h][i] = depth[i];
         cs->code[h][i] = code[i];
     }
-    cs->hdr[h][t] = hdr[t];
+    for (int i{t}; i < kHdrWords; i += 64) {
+        cs->hdr[h][i] = hdr[i];
+    }
+    if (t == 0) {
+        cs->hdr_bits[h] = nbits;
+        cs->alpha[h] = static_cast<std::uint32_t>(alpha);
+    }
 }
 
 // Per DC group: blob_pre = constant prefix + data alphabet + code header;

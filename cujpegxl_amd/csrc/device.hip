@@ -14,6 +14,7 @@ __constant__ DeviceTables g_tables;
 
 #include "frontend.inl"
 #include "entropy.inl"
+#include "huffman.inl"
 #include "assemble.inl"
 
 namespace cjx {
@@ -152,6 +153,36 @@ void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 
 void launch_huffman(hipStream_t s, int count, const std::uint32_t* hists, CodeSet* cs) {
     huffman_kernel<<<count, 64, 0, s>>>(hists, cs);
+}
+
+bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint32_t* alpha, std::uint8_t* depth,
+                      std::uint16_t* bits, std::uint32_t* hdr_words, std::uint32_t* hdr_bits) {
+    if (count == 0 || count > 96) {
+        return false;
+    }
+    std::uint32_t* dh{nullptr};
+    CodeSet* dcs{nullptr};
+    const std::size_t hbytes{static_cast<std::size_t>(count) * kHistStride * sizeof(std::uint32_t)};
+    bool ok{hipMalloc(&dh, hbytes) == hipSuccess && hipMalloc(&dcs, sizeof(CodeSet)) == hipSuccess &&
+            hipMemcpy(dh, hists, hbytes, hipMemcpyHostToDevice) == hipSuccess};
+    auto* host{new CodeSet};
+    if (ok) {
+        huffman_kernel<<<count, 64, 0, nullptr>>>(dh, dcs);
+        ok = hipGetLastError() == hipSuccess && hipMemcpy(host, dcs, sizeof(CodeSet), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (ok) {
+        for (std::uint32_t h{0}; h < count; ++h) {
+            alpha[h] = host->alpha[h];
+            hdr_bits[h] = host->hdr_bits[h];
+            std::memcpy(depth + h * kHistStride, host->depth[h], kHistStride);
+            std::memcpy(bits + h * kHistStride, host->code[h], kHistStride * sizeof(std::uint16_t));
+            std::memcpy(hdr_words + h * kHdrWords, host->hdr[h], kHdrWords * sizeof(std::uint32_t));
+        }
+    }
+    delete host;
+    (void)hipFree(dh);
+    (void)hipFree(dcs);
+    return ok;
 }
 
 void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,

@@ -1,0 +1,454 @@
+// Length-limited prefix codes built by one wave (included into device.hip).
+//
+// Same results as the serial builder in prefix.h (prefix_code.cc:36-397):
+// leaves in descending symbol order, stable sort by count, two-queue merge,
+// count floor doubled until the depth limit holds, canonical LSB-first codes,
+// simple / code-length-code headers.  The parallel parts:
+//   - leaf positions from ballots, the stable sort as a rank computation over
+//     (count, leaf position);
+//   - the two-queue merge and the top-down depth walk stay sequential, but
+//     their queues live in VGPRs (one value per lane, read with readlane and
+//     written with a lane select at a wave-uniform index), so no step waits on
+//     memory;
+//   - canonical codes from per-depth ballot ranks;
+//   - the code-length-coded header as a wave scan of per-symbol bit lengths.
+// Every lane runs the same control flow; loop bounds and queue indices are
+// wave-uniform.
+
+namespace cjx {
+
+constexpr std::uint32_t kQueueEnd = 0xffffffffu;
+
+// R values per lane: element idx lives in register idx / 64 of lane idx % 64.
+template <int R>
+struct LaneArray {
+    std::uint32_t v[R];
+
+    __device__ __forceinline__ std::uint32_t get(int idx) const {
+        const int l{idx & 63};
+        if constexpr (R == 1) {
+            return static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v[0]), l));
+        } else {
+            const int r{idx >> 6};
+            const int x{r == 0   ? __builtin_amdgcn_readlane(static_cast<int>(v[0]), l)
+                        : r == 1 ? __builtin_amdgcn_readlane(static_cast<int>(v[1]), l)
+                        : r == 2 ? __builtin_amdgcn_readlane(static_cast<int>(v[2]), l)
+                                 : __builtin_amdgcn_readlane(static_cast<int>(v[3]), l)};
+            return static_cast<std::uint32_t>(x);
+        }
+    }
+
+    // Lane idx % 64 of register idx / 64 takes x (a compare and a select).
+    __device__ __forceinline__ void set(int idx, std::uint32_t x) {
+        const bool mine{(static_cast<int>(threadIdx.x) & 63) == (idx & 63)};
+        if constexpr (R == 1) {
+            v[0] = mine ? x : v[0];
+        } else {
+            const int r{idx >> 6};
+#pragma unroll
+            for (int k{0}; k < R; ++k) {
+                v[k] = (mine && r == k) ? x : v[k];
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ std::uint32_t wave_max_u32(std::uint32_t v) {
+#pragma unroll
+    for (int off{32}; off > 0; off >>= 1) {
+        const std::uint32_t o{static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), off, 64))};
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// LDS scratch of one wave's build.
+struct WaveHuffScratch {
+    std::uint32_t leaf_total[256];  // input order
+    std::uint32_t sorted_total[256];
+    std::uint32_t sorted_symbol[256];
+};
+
+// create_huffman_tree (prefix_code.cc:36-96).  data[0..length) and
+// depth[0..length) are in LDS; depth is zero on entry for absent symbols.
+// R * 64 >= length.
+template <int R>
+__device__ void wave_create_tree(const std::uint32_t* data, int length, int limit, std::uint8_t* depth,
+                                 WaveHuffScratch& s) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    std::uint32_t cnt[R];
+    std::uint64_t present[R];
+    int n{0};
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        const int sym{64 * r + lane};
+        cnt[r] = sym < length ? data[sym] : 0u;
+        present[r] = __ballot(cnt[r] != 0);
+        n += __popcll(present[r]);
+    }
+    if (n == 0) {
+        return;
+    }
+    // Leaf position in input order (descending symbol): present symbols above.
+    const std::uint64_t above{lane == 63 ? 0ull : (~0ull << (lane + 1))};
+    int pos[R];
+    {
+        int higher{0};
+#pragma unroll
+        for (int r{R - 1}; r >= 0; --r) {
+            pos[r] = higher + __popcll(present[r] & above);
+            higher += __popcll(present[r]);
+        }
+    }
+    if (n == 1) {
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            if (cnt[r] != 0) {
+                depth[64 * r + lane] = 1;
+            }
+        }
+        return;
+    }
+    for (std::uint32_t count_limit{1};; count_limit *= 2) {
+        std::uint32_t tot[R];
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            tot[r] = cnt[r] > count_limit - 1 ? cnt[r] : count_limit - 1;
+            if (cnt[r] != 0) {
+                s.leaf_total[pos[r]] = tot[r];
+            }
+        }
+        __syncthreads();
+        // Stable ascending sort: rank = smaller totals + equal totals earlier.
+        int rank[R];
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            rank[r] = 0;
+        }
+        for (int b{0}; b < n; ++b) {
+            const std::uint32_t t{s.leaf_total[b]};
+#pragma unroll
+            for (int r{0}; r < R; ++r) {
+                rank[r] += (t < tot[r] || (t == tot[r] && b < pos[r])) ? 1 : 0;
+            }
+        }
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            if (cnt[r] != 0) {
+                s.sorted_total[rank[r]] = tot[r];
+                s.sorted_symbol[rank[r]] = static_cast<std::uint32_t>(64 * r + lane);
+            }
+        }
+        __syncthreads();
+        LaneArray<R> leaf{}, sym{};
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            const int t{64 * r + lane};
+            leaf.v[r] = t < n ? s.sorted_total[t] : kQueueEnd;
+            sym.v[r] = t < n ? s.sorted_symbol[t] : 0u;
+        }
+        // Two-queue merge: leaves (tree index i < n) and internal nodes
+        // (internal k = tree index n + 1 + k), smaller total first, leaf on ties.
+        LaneArray<R> inner{}, kid_l{}, kid_r{};
+        int i{0}, j{0};
+        for (int k{0}; k < n - 1; ++k) {
+            std::uint32_t a{i < n ? leaf.get(i) : kQueueEnd};
+            std::uint32_t b{j < k ? inner.get(j) : kQueueEnd};
+            int left, right;
+            std::uint32_t lt, rt;
+            if (a <= b) {
+                left = i;
+                lt = a;
+                ++i;
+                a = i < n ? leaf.get(i) : kQueueEnd;
+            } else {
+                left = n + 1 + j;
+                lt = b;
+                ++j;
+                b = j < k ? inner.get(j) : kQueueEnd;
+            }
+            if (a <= b) {
+                right = i;
+                rt = a;
+                ++i;
+            } else {
+                right = n + 1 + j;
+                rt = b;
+                ++j;
+            }
+            inner.set(k, lt + rt);
+            kid_l.set(k, static_cast<std::uint32_t>(left));
+            kid_r.set(k, static_cast<std::uint32_t>(right));
+        }
+        // Depths top-down from the root (internal n - 2): every child was
+        // created before its parent.
+        LaneArray<R> node_depth{}, leaf_depth{};
+        node_depth.set(n - 2, 0u);
+        for (int k{n - 2}; k >= 0; --k) {
+            const std::uint32_t d{node_depth.get(k) + 1u};
+            const int cl{static_cast<int>(kid_l.get(k))}, cr{static_cast<int>(kid_r.get(k))};
+            if (cl > n) {
+                node_depth.set(cl - n - 1, d);
+            } else {
+                leaf_depth.set(cl, d);
+            }
+            if (cr > n) {
+                node_depth.set(cr - n - 1, d);
+            } else {
+                leaf_depth.set(cr, d);
+            }
+        }
+        std::uint32_t maxd{0};
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            if (64 * r + lane < n) {
+                depth[sym.v[r]] = static_cast<std::uint8_t>(leaf_depth.v[r]);
+                maxd = leaf_depth.v[r] > maxd ? leaf_depth.v[r] : maxd;
+            }
+        }
+        maxd = wave_max_u32(maxd);
+        __syncthreads();
+        if (static_cast<int>(maxd) <= limit) {
+            return;
+        }
+    }
+}
+
+// Canonical codes (prefix_code.cc:111-130): bits[s] for depth[s] > 0.
+template <int R>
+__device__ void wave_depths_to_codes(const std::uint8_t* depth, int len, std::uint16_t* bits) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    const std::uint64_t below{(1ull << lane) - 1ull};
+    int dep[R];
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        const int sym{64 * r + lane};
+        dep[r] = sym < len ? depth[sym] : 0;
+    }
+    std::uint32_t next[16];
+    {
+        std::uint32_t code{0}, prev{0};
+        next[0] = 0;
+        for (int d{1}; d < 16; ++d) {
+            code = (code + prev) << 1;
+            next[d] = code;
+            std::uint32_t c{0};
+#pragma unroll
+            for (int r{0}; r < R; ++r) {
+                c += static_cast<std::uint32_t>(__popcll(__ballot(dep[r] == d)));
+            }
+            prev = c;
+        }
+    }
+    for (int d{1}; d < 16; ++d) {
+        std::uint32_t base{next[d]};
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            const std::uint64_t m{__ballot(dep[r] == d)};
+            if (dep[r] == d) {
+                const std::uint32_t c{base + static_cast<std::uint32_t>(__popcll(m & below))};
+                bits[64 * r + lane] = static_cast<std::uint16_t>(__builtin_bitreverse32(c) >> (32 - d));
+            }
+            base += static_cast<std::uint32_t>(__popcll(m));
+        }
+    }
+}
+
+// ORs `len` (<= 32) bits at bit `pos` of an LDS word buffer of `cap` words.
+__device__ __forceinline__ void lds_or_bits(std::uint32_t* words, std::uint32_t cap, std::uint32_t pos,
+                                            std::uint32_t value, std::uint32_t len) {
+    if (len == 0) {
+        return;
+    }
+    if (len < 32) {
+        value &= (1u << len) - 1u;
+    }
+    const std::uint32_t wi{pos >> 5}, b{pos & 31u};
+    if (wi < cap) {
+        atomicOr(&words[wi], value << b);
+    }
+    if (b + len > 32 && wi + 1 < cap) {
+        atomicOr(&words[wi + 1], value >> (32 - b));
+    }
+}
+
+struct WaveCodeScratch {
+    WaveHuffScratch tree;
+    std::uint32_t cl_hist[32];
+    std::uint8_t cl_depth[32];
+    std::uint16_t cl_bits[32];
+    std::uint32_t pos;
+};
+
+// store_huffman_tree (prefix_code.cc:211-354) for depth[0..num): returns the
+// header length in bits written into hdr (cap words, zeroed).
+template <int R>
+__device__ std::uint32_t wave_store_tree(const std::uint8_t* depth, int num, std::uint32_t* hdr,
+                                         std::uint32_t cap, WaveCodeScratch& s) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    int dep[R];
+    std::uint32_t last{0};
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        const int sym{64 * r + lane};
+        dep[r] = sym < num ? depth[sym] : 0;
+        if (dep[r] != 0) {
+            last = static_cast<std::uint32_t>(sym) + 1u;
+        }
+    }
+    const int new_len{static_cast<int>(wave_max_u32(last))};
+    // Histogram of depth values over [0, new_len).
+    for (int v{0}; v < 18; ++v) {
+        std::uint32_t c{0};
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            c += static_cast<std::uint32_t>(__popcll(__ballot(64 * r + lane < new_len && dep[r] == v)));
+        }
+        if (lane == 0) {
+            s.cl_hist[v] = c;
+        }
+    }
+    if (lane < 18) {
+        s.cl_depth[lane] = 0;
+        s.cl_bits[lane] = 0;
+    }
+    __syncthreads();
+    int num_codes{0}, code{0};
+    for (int i{0}; i < 18; ++i) {
+        if (s.cl_hist[i]) {
+            if (num_codes == 0) {
+                code = i;
+                num_codes = 1;
+            } else if (num_codes == 1) {
+                num_codes = 2;
+                break;
+            }
+        }
+    }
+    wave_create_tree<1>(s.cl_hist, 18, 5, s.cl_depth, s.tree);
+    __syncthreads();
+    wave_depths_to_codes<1>(s.cl_depth, 18, s.cl_bits);
+    __syncthreads();
+    if (lane == 0) {
+        constexpr std::uint8_t order[18]{1, 2, 3, 4, 0, 5, 17, 6, 16, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+        constexpr std::uint8_t sym[6]{0, 7, 3, 2, 1, 15};
+        constexpr std::uint8_t len[6]{2, 4, 3, 2, 2, 4};
+        int to_store{18};
+        if (num_codes > 1) {
+            for (; to_store > 0; --to_store) {
+                if (s.cl_depth[order[to_store - 1]] != 0) {
+                    break;
+                }
+            }
+        }
+        int skip{0};
+        if (s.cl_depth[order[0]] == 0 && s.cl_depth[order[1]] == 0) {
+            skip = 2;
+            if (s.cl_depth[order[2]] == 0) {
+                skip = 3;
+            }
+        }
+        BitSink w{hdr, 0, cap * 32};
+        w.write(2, static_cast<std::uint32_t>(skip));
+        for (int i{skip}; i < to_store; ++i) {
+            const std::uint8_t l{s.cl_depth[order[i]]};
+            w.write(len[l], sym[l]);
+        }
+        if (num_codes == 1) {
+            s.cl_depth[code] = 0;
+        }
+        s.pos = w.pos;
+    }
+    __syncthreads();
+    std::uint32_t at{s.pos};
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        const bool in{64 * r + lane < new_len};
+        const std::uint32_t l{in ? static_cast<std::uint32_t>(s.cl_depth[dep[r]]) : 0u};
+        const std::uint32_t v{in ? static_cast<std::uint32_t>(s.cl_bits[dep[r]]) : 0u};
+        const std::uint32_t pre{static_cast<std::uint32_t>(wave_excl_scan(static_cast<int>(l), lane))};
+        if (at + pre + l <= cap * 32) {
+            lds_or_bits(hdr, cap, at + pre, v, l);
+        }
+        at += static_cast<std::uint32_t>(wave_sum_int(static_cast<int>(l)));
+    }
+    return at;
+}
+
+// build_and_store_huffman_tree (prefix_code.cc:358-397) for hist[0..alpha),
+// alpha >= 2; depth and bits are zeroed by the caller.  Returns the header
+// length in bits, or 0 if it would not fit `cap` words.
+template <int R>
+__device__ std::uint32_t wave_build_code(const std::uint32_t* hist, int alpha, std::uint8_t* depth,
+                                         std::uint16_t* bits, std::uint32_t* hdr, std::uint32_t cap,
+                                         WaveCodeScratch& s) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    std::uint64_t present[R];
+    int count{0};
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        const int sym{64 * r + lane};
+        present[r] = __ballot(sym < alpha && hist[sym] != 0);
+        count += __popcll(present[r]);
+    }
+    // The first four present symbols, ascending.
+    int s4[4]{0, 0, 0, 0};
+    {
+        int got{0};
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            std::uint64_t m{present[r]};
+            while (m != 0 && got < 4) {
+                s4[got++] = 64 * r + static_cast<int>(__builtin_ctzll(m));
+                m &= m - 1;
+            }
+        }
+    }
+    int max_bits{0};
+    for (int c{alpha - 1}; c; c >>= 1) {
+        ++max_bits;
+    }
+    if (count <= 1) {
+        std::uint32_t pos{0};
+        if (lane == 0) {
+            BitSink w{hdr, 0, cap * 32};
+            w.write(4, 1);
+            w.write(static_cast<std::uint32_t>(max_bits), static_cast<std::uint32_t>(s4[0]));
+            pos = w.ok() ? w.pos : 0u;
+        }
+        return static_cast<std::uint32_t>(__shfl(static_cast<int>(pos), 0, 64));
+    }
+    wave_create_tree<R>(hist, alpha, 15, depth, s.tree);
+    __syncthreads();
+    wave_depths_to_codes<R>(depth, alpha, bits);
+    __syncthreads();
+    if (count <= 4) {
+        std::uint32_t pos{0};
+        if (lane == 0) {
+            BitSink w{hdr, 0, cap * 32};
+            w.write(2, 1);
+            w.write(2, static_cast<std::uint32_t>(count - 1));
+            for (int i{0}; i < count; ++i) {
+                for (int j{i + 1}; j < count; ++j) {
+                    if (depth[s4[j]] < depth[s4[i]]) {
+                        const int t{s4[j]};
+                        s4[j] = s4[i];
+                        s4[i] = t;
+                    }
+                }
+            }
+            for (int i{0}; i < count; ++i) {
+                w.write(static_cast<std::uint32_t>(max_bits), static_cast<std::uint32_t>(s4[i]));
+            }
+            if (count == 4) {
+                w.write(1, depth[s4[0]] == 1 ? 1u : 0u);
+            }
+            pos = w.ok() ? w.pos : 0u;
+        }
+        return static_cast<std::uint32_t>(__shfl(static_cast<int>(pos), 0, 64));
+    }
+    const std::uint32_t nb{wave_store_tree<R>(depth, alpha, hdr, cap, s)};
+    return nb <= cap * 32 ? nb : 0u;
+}
+
+}  // namespace cjx

@@ -46,6 +46,10 @@ void launch_ac_hist(hipStream_t s, const Geometry& geo, const std::int16_t* ac, 
 void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     std::uint32_t* dc_hist, std::uint32_t* meta_hist, std::uint32_t* fb_count);
+// Builds the codes of `count` host histograms on the current device and copies
+// the results back (stage harness; allocates).
+bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint32_t* alpha, std::uint8_t* depth,
+                      std::uint16_t* bits, std::uint32_t* hdr_words, std::uint32_t* hdr_bits);
 void launch_huffman(hipStream_t s, int count, const std::uint32_t* hists, CodeSet* cs);
 void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                      const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words);

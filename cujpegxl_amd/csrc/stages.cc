@@ -8,6 +8,7 @@
 #include "common.h"
 #include "encoder.h"
 #include "handle.h"
+#include "kernels.h"
 
 
 extern "C" {
@@ -95,6 +96,19 @@ int cjx_stage_timings(cujpegxl_encoder* e, float* ms, int count) {
         ms[i] = t[i];
     }
     return CUJPEGXL_OK;
+}
+
+int cjx_stage_build_codes(int32_t device_ordinal, const uint32_t* hists, uint32_t count, uint32_t* alpha,
+                          uint8_t* depth, uint16_t* bits, uint32_t* hdr_words, uint32_t* hdr_bits) {
+    if (hists == nullptr || alpha == nullptr || depth == nullptr || bits == nullptr || hdr_words == nullptr ||
+        hdr_bits == nullptr || count == 0 || count > 96) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    if (!cjx::device_exists(device_ordinal) || hipSetDevice(device_ordinal) != hipSuccess) {
+        return CUJPEGXL_NO_DEVICE;
+    }
+    return cjx::build_codes_host(hists, count, alpha, depth, bits, hdr_words, hdr_bits) ? CUJPEGXL_OK
+                                                                                       : CUJPEGXL_INTERNAL_ERROR;
 }
 
 void* cjx_device_alloc(size_t bytes) {

@@ -57,6 +57,13 @@ int cjx_stage_encode_nv12(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* 
 int cjx_stage_set_profiling(cujpegxl_encoder* encoder, int on);
 int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
 
+/* Prefix codes of `count` (<= 96) histograms of 256 uint32 bins, built on
+ * device `device_ordinal` by the encoder's code kernel (prefix_code.cc:36-397
+ * semantics): per histogram the alphabet size, 256 depths, 256 LSB-first
+ * codes, the header (64 words, LSB-first) and its length in bits. */
+int cjx_stage_build_codes(int32_t device_ordinal, const uint32_t* hists, uint32_t count, uint32_t* alpha,
+                          uint8_t* depth, uint16_t* bits, uint32_t* hdr_words, uint32_t* hdr_bits);
+
 /* Device memory for harnesses that have no other allocator. */
 void* cjx_device_alloc(size_t bytes);
 void cjx_device_free(void* ptr);

@@ -202,3 +202,56 @@ def test_dense_coefficients_match_oracle(jo):
     cs = jo.write_codestream(jo.make_frame(w, h, gs, qdc, dc, ac, acs, qf, ytox, ytob))
     assert len(cs) > 300_000
     assert data[40:] == cs
+
+
+def _huffman_cases():
+    rng = np.random.default_rng(21)
+    cases = []
+    fib = [1, 1]
+    while len(fib) < 40:
+        fib.append(fib[-1] + fib[-2])
+    for lo in (0, 30, 200):  # depth limit retries, with and without > 64 symbols
+        h = np.zeros(256, np.uint32)
+        h[lo:lo + 40] = fib
+        cases.append(h)
+    for k in (1, 2, 3, 4, 5, 6, 17, 64, 65, 200, 256):
+        h = np.zeros(256, np.uint32)
+        idx = rng.choice(256, size=k, replace=False)
+        h[idx] = rng.integers(1, 1000, size=k)
+        cases.append(h)
+    h = np.zeros(256, np.uint32)
+    h[:64] = 7  # all ties
+    cases.append(h)
+    h = np.zeros(256, np.uint32)
+    h[0] = 5  # alphabet of one
+    cases.append(h)
+    cases.append(np.zeros(256, np.uint32))
+    h = np.zeros(256, np.uint32)
+    h[255] = 3  # single symbol at the top
+    cases.append(h)
+    for _ in range(40):
+        k = int(rng.integers(2, 257))
+        h = np.zeros(256, np.uint32)
+        h[:k] = rng.geometric(rng.uniform(0.02, 0.6), size=k) * rng.integers(0, 2, size=k)
+        cases.append(h)
+    return np.stack(cases)
+
+
+def test_prefix_codes_match_oracle(jo):
+    """The device code kernel on adversarial histograms (depth-limit retries,
+    ties, 1-6 symbols, alphabets past 64) against the oracle's builder."""
+    say("prefix codes")
+    hists = _huffman_cases()
+    got = cj.build_codes(hists)
+    for i, h in enumerate(hists):
+        nzs = np.nonzero(h)[0]
+        alpha = int(nzs[-1]) + 1 if nzs.size else 1
+        assert got[i]["alpha"] == alpha, i
+        if alpha <= 1:
+            assert got[i]["header_bits"] == 0 and not got[i]["depth"].any()
+            continue
+        d, b, out, nb = _huff(jo, np.ascontiguousarray(h[:alpha]))
+        np.testing.assert_array_equal(got[i]["depth"][:alpha], d, err_msg=f"case {i}")
+        np.testing.assert_array_equal(got[i]["bits"][:alpha], b, err_msg=f"case {i}")
+        assert got[i]["header_bits"] == nb, (i, got[i]["header_bits"], nb)
+        assert got[i]["header"] == out[: (nb + 7) // 8].tobytes(), i
