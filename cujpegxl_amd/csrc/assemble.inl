@@ -131,102 +131,201 @@ __global__ __launch_bounds__(64) void dc_blobs_kernel(Geometry geo, const Consta
     }
 }
 
-// AcGlobal, section sizes, TOC, head, framing and every section's position.
-// `single` frames (one AC group) concatenate DcGlobal, the DC group, AcGlobal
-// and the AC tokens into one section without padding in between.
-__global__ __launch_bounds__(64) void layout_kernel(Geometry geo, const Constants* __restrict__ k,
-                                                    const CodeSet* __restrict__ cs,
-                                                    const std::uint32_t* __restrict__ ac_group_bytes,
-                                                    const std::uint32_t* __restrict__ dc_group_bytes,
-                                                    const std::uint64_t* __restrict__ ac_single_bits,
-                                                    const std::uint64_t* __restrict__ dc_single_bits,
-                                                    const DcBlobs* __restrict__ blobs,
-                                                    Layout* __restrict__ lay, std::uint64_t* __restrict__ ac_base,
-                                                    std::uint64_t* __restrict__ dc_base) {
-    const int t{static_cast<int>(threadIdx.x)};
-    for (int i{t}; i < kAcGlobalWords; i += 64) {
-        lay->ac_global_words[i] = 0;
+// U32 field coder choice (field_coder.h:46-107) as one code word: selector
+// in the low 2 bits, then the offset bits.
+__device__ __forceinline__ void u32_code(const Distr* d, std::uint32_t value, std::uint32_t& code,
+                                         std::uint32_t& len) {
+    std::uint32_t best{0}, best_bits{64};
+    for (std::uint32_t s{0}; s < 4; ++s) {
+        if (d[s].direct) {
+            if (d[s].v == value) {
+                best = s;
+                best_bits = 2;
+                break;
+            }
+            continue;
+        }
+        if (value < d[s].v ||
+            static_cast<std::uint64_t>(value) >= static_cast<std::uint64_t>(d[s].v) + (1ull << d[s].bits)) {
+            continue;
+        }
+        if (2 + d[s].bits < best_bits) {
+            best = s;
+            best_bits = 2 + d[s].bits;
+        }
     }
-    for (int i{t}; i < kHeadWords; i += 64) {
-        lay->head_words[i] = 0;
+    if (d[best].direct) {
+        code = best;
+        len = 2;
+        return;
+    }
+    const std::uint32_t nb{d[best].bits};
+    const std::uint32_t off{nb < 32 ? (value - d[best].v) & ((1u << nb) - 1u) : value - d[best].v};
+    code = best | (off << 2);
+    len = 2 + nb;
+}
+
+// AcGlobal, section sizes, TOC, head, framing and every section's position,
+// by one 256-thread workgroup.  AcGlobal (constant prefix, 32 alphabet sizes,
+// 32 code headers) and the head (constant bits, TOC) are assembled in LDS:
+// every piece's bit offset comes from a block scan of the piece lengths, and
+// the section bases from scans of the section sizes.  `single` frames (one AC
+// group) concatenate DcGlobal, the DC group, AcGlobal and the AC tokens into
+// one section without padding in between.
+__global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constants* __restrict__ k,
+                                                     const CodeSet* __restrict__ cs,
+                                                     const std::uint32_t* __restrict__ ac_group_bytes,
+                                                     const std::uint32_t* __restrict__ dc_group_bytes,
+                                                     const std::uint64_t* __restrict__ ac_single_bits,
+                                                     const std::uint64_t* __restrict__ dc_single_bits,
+                                                     const DcBlobs* __restrict__ blobs,
+                                                     Layout* __restrict__ lay, std::uint64_t* __restrict__ ac_base,
+                                                     std::uint64_t* __restrict__ dc_base) {
+    __shared__ std::uint32_t agw[kAcGlobalWords];
+    __shared__ std::uint32_t hw[kHeadWords];
+    __shared__ std::uint64_t sh[17];
+    const int t{static_cast<int>(threadIdx.x)};
+    for (int i{t}; i < kAcGlobalWords; i += 256) {
+        agw[i] = 0;
+    }
+    for (int i{t}; i < kHeadWords; i += 256) {
+        hw[i] = 0;
     }
     __syncthreads();
-    if (t != 0) {
-        return;
-    }
-    BitSink a{lay->ac_global_words, 0, kAcGlobalWords * 32};
-    copy_bits(a, k->ac_prefix_words, k->ac_prefix_bits);
-    for (int c{0}; c < kNumClusters; ++c) {
-        write_var_len_u16(a, cs->alpha[c] - 1);
-    }
-    for (int c{0}; c < kNumClusters; ++c) {
-        copy_bits(a, cs->hdr[c], cs->hdr_bits[c]);
-    }
-    lay->ac_global_bits = a.pos;
-    const bool single{geo.num_ac == 1};
-    std::uint32_t overflow{a.ok() ? 0u : 1u};
-    overflow |= blobs->overflow;
+    std::uint32_t overflow{blobs->overflow};
 
-    // Section sizes in codestream order: DcGlobal, DcGroups, AcGlobal, AcGroups.
-    const Distr toc[4]{{false, 0, 10}, {false, 1024, 14}, {false, 17408, 22}, {false, 4211712, 30}};
-    BitSink hw{lay->head_words, 0, kHeadWords * 32};
-    copy_bits(hw, k->head_words, k->head_bits);
-    hw.write(1, 0);                 // no permutation
-    hw.pos = (hw.pos + 7u) & ~7u;   // byte boundary
-    std::uint64_t body{0};
-    if (single) {
-        const std::uint64_t bits{k->dc_global_bits + dc_single_bits[0] + a.pos + ac_single_bits[0]};
-        const std::uint32_t bytes{static_cast<std::uint32_t>((bits + 7) / 8)};
-        write_u32(hw, toc, bytes);
-        body = bytes;
-    } else {
-        const std::uint32_t dcg{(k->dc_global_bits + 7) / 8};
-        write_u32(hw, toc, dcg);
-        body += dcg;
-        for (std::uint32_t g{0}; g < geo.num_dc; ++g) {
-            write_u32(hw, toc, dc_group_bytes[g]);
-            body += dc_group_bytes[g];
+    // AcGlobal.
+    const std::uint32_t pre_bits{k->ac_prefix_bits};
+    for (std::uint32_t i{static_cast<std::uint32_t>(t)}; i < (pre_bits + 31) / 32; i += 256) {
+        const std::uint32_t rem{pre_bits - 32 * i};
+        const std::uint32_t w{k->ac_prefix_words[i]};
+        agw[i] = rem >= 32 ? w : (w & ((1u << rem) - 1u));
+    }
+    std::uint32_t len{0}, val{0};
+    if (t < kNumClusters) {
+        const std::uint32_t n{cs->alpha[t] - 1};
+        if (n == 0) {
+            len = 1;
+        } else {
+            const std::uint32_t nb{31u - static_cast<std::uint32_t>(__builtin_clz(n))};
+            len = 5 + nb;
+            val = 1u | (nb << 1) | ((n - (1u << nb)) << 5);
         }
-        const std::uint32_t acg{(a.pos + 7) / 8};
-        write_u32(hw, toc, acg);
-        body += acg;
-        for (std::uint32_t g{0}; g < geo.num_ac; ++g) {
-            write_u32(hw, toc, ac_group_bytes[g]);
-            body += ac_group_bytes[g];
+    } else if (t < 2 * kNumClusters) {
+        len = cs->hdr_bits[t - kNumClusters];
+    }
+    std::uint64_t pieces{0};
+    const std::uint64_t off{pre_bits + block_excl_scan(len, sh, pieces)};
+    const std::uint64_t abits{pre_bits + pieces};
+    if (abits > static_cast<std::uint64_t>(kAcGlobalWords) * 32) {
+        overflow = 1;
+    } else if (t < kNumClusters) {
+        lds_or_bits(agw, kAcGlobalWords, static_cast<std::uint32_t>(off), val, len);
+    } else if (t < 2 * kNumClusters) {
+        const std::uint32_t* src{cs->hdr[t - kNumClusters]};
+        for (std::uint32_t i{0}; i < len; i += 32) {
+            lds_or_bits(agw, kAcGlobalWords, static_cast<std::uint32_t>(off) + i, src[i >> 5],
+                        len - i < 32 ? len - i : 32u);
         }
     }
-    hw.pos = (hw.pos + 7u) & ~7u;
-    overflow |= hw.ok() ? 0u : 1u;
-    const std::uint32_t head_bytes{hw.pos / 8};
+
+    // Head: constant bits, no permutation (one 0 bit), byte-aligned TOC.
+    const std::uint32_t hb{k->head_bits};
+    for (std::uint32_t i{static_cast<std::uint32_t>(t)}; i < (hb + 31) / 32; i += 256) {
+        const std::uint32_t rem{hb - 32 * i};
+        const std::uint32_t w{k->head_words[i]};
+        hw[i] = rem >= 32 ? w : (w & ((1u << rem) - 1u));
+    }
+    const Distr toc[4]{{false, 0, 10}, {false, 1024, 14}, {false, 17408, 22}, {false, 4211712, 30}};
+    const bool single{geo.num_ac == 1};
+    const std::uint32_t dcg{(k->dc_global_bits + 7) / 8};
+    const std::uint32_t acg{static_cast<std::uint32_t>((abits + 7) / 8)};
+    const std::uint32_t entries{single ? 1u : 2u + geo.num_dc + geo.num_ac};
+    std::uint64_t at{(hb + 1u + 7u) & ~7u};
+    std::uint64_t body{0};
+    for (std::uint32_t e0{0}; e0 < entries; e0 += 256) {
+        const std::uint32_t e{e0 + static_cast<std::uint32_t>(t)};
+        std::uint32_t size{0}, code{0}, clen{0};
+        if (e < entries) {
+            if (single) {
+                const std::uint64_t bits{k->dc_global_bits + dc_single_bits[0] + abits + ac_single_bits[0]};
+                size = static_cast<std::uint32_t>((bits + 7) / 8);
+            } else if (e == 0) {
+                size = dcg;
+            } else if (e <= geo.num_dc) {
+                size = dc_group_bytes[e - 1];
+            } else if (e == geo.num_dc + 1) {
+                size = acg;
+            } else {
+                size = ac_group_bytes[e - geo.num_dc - 2];
+            }
+            u32_code(toc, size, code, clen);
+        }
+        std::uint64_t chunk_bits{0}, chunk_bytes{0};
+        const std::uint64_t p{at + block_excl_scan(clen, sh, chunk_bits)};
+        (void)block_excl_scan(size, sh, chunk_bytes);
+        if (clen != 0 && p + clen <= static_cast<std::uint64_t>(kHeadWords) * 32) {
+            lds_or_bits(hw, kHeadWords, static_cast<std::uint32_t>(p), code, clen);
+        }
+        at += chunk_bits;
+        body += chunk_bytes;
+    }
+    at = (at + 7) & ~7ull;
+    if (at > static_cast<std::uint64_t>(kHeadWords) * 32) {
+        overflow = 1;
+    }
+    const std::uint32_t head_bytes{static_cast<std::uint32_t>(at / 8)};
     const std::uint64_t cs_bytes{head_bytes + body};
     const std::uint32_t framing{cs_bytes >= 0x100000000ull - 8 ? 48u : 40u};
-    lay->head_total_bits = hw.pos;
-    lay->head_bytes = head_bytes;
-    lay->framing_bytes = framing;
-    lay->codestream_bytes = cs_bytes;
-    lay->total_bytes = overflow ? ~0ull : framing + cs_bytes;
-    lay->overflow = overflow;
-    std::uint64_t at{(static_cast<std::uint64_t>(framing) + head_bytes) * 8};
-    lay->dc_global_bit = at;
+
+    // Section positions (absolute bits in the file).
+    const std::uint64_t dcg_bit{(static_cast<std::uint64_t>(framing) + head_bytes) * 8};
+    std::uint64_t ac_global_bit{0};
     if (single) {
-        at += k->dc_global_bits;
-        dc_base[0] = at;
-        at += dc_single_bits[0];
-        lay->ac_global_bit = at;
-        at += a.pos;
-        ac_base[0] = at;
-        return;
+        if (t == 0) {
+            dc_base[0] = dcg_bit + k->dc_global_bits;
+            ac_global_bit = dcg_bit + k->dc_global_bits + dc_single_bits[0];
+            ac_base[0] = ac_global_bit + abits;
+        }
+    } else {
+        const std::uint64_t first{dcg_bit + static_cast<std::uint64_t>(dcg) * 8};
+        std::uint64_t dc_total{0};
+        const std::uint32_t g{static_cast<std::uint32_t>(t)};
+        const std::uint64_t dsz{g < geo.num_dc ? static_cast<std::uint64_t>(dc_group_bytes[g]) * 8 : 0ull};
+        const std::uint64_t dpre{block_excl_scan(dsz, sh, dc_total)};
+        if (g < geo.num_dc) {
+            dc_base[g] = first + dpre;
+        }
+        ac_global_bit = first + dc_total;
+        std::uint64_t run{ac_global_bit + static_cast<std::uint64_t>(acg) * 8};
+        for (std::uint32_t g0{0}; g0 < geo.num_ac; g0 += 256) {
+            const std::uint32_t ga{g0 + static_cast<std::uint32_t>(t)};
+            const std::uint64_t asz{ga < geo.num_ac ? static_cast<std::uint64_t>(ac_group_bytes[ga]) * 8 : 0ull};
+            std::uint64_t chunk{0};
+            const std::uint64_t apre{block_excl_scan(asz, sh, chunk)};
+            if (ga < geo.num_ac) {
+                ac_base[ga] = run + apre;
+            }
+            run += chunk;
+        }
     }
-    at += ((k->dc_global_bits + 7) / 8) * 8;
-    for (std::uint32_t g{0}; g < geo.num_dc; ++g) {
-        dc_base[g] = at;
-        at += static_cast<std::uint64_t>(dc_group_bytes[g]) * 8;
+    __syncthreads();
+    for (int i{t}; i < kAcGlobalWords; i += 256) {
+        lay->ac_global_words[i] = agw[i];
     }
-    lay->ac_global_bit = at;
-    at += ((a.pos + 7) / 8) * 8;
-    for (std::uint32_t g{0}; g < geo.num_ac; ++g) {
-        ac_base[g] = at;
-        at += static_cast<std::uint64_t>(ac_group_bytes[g]) * 8;
+    for (int i{t}; i < kHeadWords; i += 256) {
+        lay->head_words[i] = hw[i];
+    }
+    if (t == 0) {
+        lay->ac_global_bits = static_cast<std::uint32_t>(abits);
+        lay->head_total_bits = static_cast<std::uint32_t>(at);
+        lay->head_bytes = head_bytes;
+        lay->framing_bytes = framing;
+        lay->codestream_bytes = cs_bytes;
+        lay->total_bytes = overflow ? ~0ull : framing + cs_bytes;
+        lay->overflow = overflow;
+        lay->dc_global_bit = dcg_bit;
+        lay->ac_global_bit = ac_global_bit;
     }
 }
 
@@ -245,30 +344,31 @@ __global__ __launch_bounds__(64) void headers_kernel(const Constants* __restrict
                                                      std::uint32_t* __restrict__ out_words,
                                                      std::uint64_t cap_words) {
     const int t{static_cast<int>(threadIdx.x)};
-    if (t == 0) {
+    // The framing is 40 or 48 bytes, so it owns whole words: lane i stores
+    // bytes 4i..4i+3.
+    if (t < static_cast<int>(lay->framing_bytes / 4) && static_cast<std::uint64_t>(t) < cap_words) {
         constexpr std::uint8_t hdr[32]{0,   0,   0, 0x0c, 'J', 'X', 'L', ' ', 0x0d, 0x0a, 0x87,
                                        0x0a, 0,  0, 0,    0x14, 'f', 't', 'y', 'p', 'j',  'x',
                                        'l', ' ', 0, 0,    0,    0,   'j', 'x', 'l', ' '};
-        std::uint8_t f[48];
-        for (int i{0}; i < 32; ++i) {
-            f[i] = hdr[i];
-        }
         const bool large{lay->framing_bytes == 48};
         const std::uint64_t box{lay->codestream_bytes + (large ? 16 : 8)};
         const std::uint32_t field{large ? 1u : static_cast<std::uint32_t>(box)};
-        for (int i{0}; i < 4; ++i) {
-            f[32 + i] = static_cast<std::uint8_t>(field >> (8 * (3 - i)));
+        std::uint32_t word{0};
+        for (int j{0}; j < 4; ++j) {
+            const int i{4 * t + j};
+            std::uint8_t b;
+            if (i < 32) {
+                b = hdr[i];
+            } else if (i < 36) {
+                b = static_cast<std::uint8_t>(field >> (8 * (35 - i)));
+            } else if (i < 40) {
+                b = static_cast<std::uint8_t>("jxlc"[i - 36]);
+            } else {
+                b = static_cast<std::uint8_t>(box >> (8 * (47 - i)));
+            }
+            word |= static_cast<std::uint32_t>(b) << (8 * j);
         }
-        f[36] = 'j';
-        f[37] = 'x';
-        f[38] = 'l';
-        f[39] = 'c';
-        for (int i{0}; i < 8; ++i) {
-            f[40 + i] = static_cast<std::uint8_t>(box >> (8 * (7 - i)));
-        }
-        for (std::uint32_t i{0}; i < lay->framing_bytes; ++i) {
-            or_bits(out_words, cap_words, static_cast<std::uint64_t>(i) * 8, f[i], 8);
-        }
+        out_words[t] = word;
     }
     const std::uint64_t head_bit{static_cast<std::uint64_t>(lay->framing_bytes) * 8};
     for (std::uint32_t i{static_cast<std::uint32_t>(t) * 32}; i < lay->head_total_bits; i += 64 * 32) {

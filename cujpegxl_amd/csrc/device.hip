@@ -216,7 +216,7 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
                    const std::uint32_t* ac_group_bytes, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* ac_group_bits, const std::uint64_t* dc_group_bits, const DcBlobs* blobs,
                    Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base) {
-    layout_kernel<<<1, 64, 0, s>>>(geo, k, cs, ac_group_bytes, dc_group_bytes, ac_group_bits, dc_group_bits,
+    layout_kernel<<<1, 256, 0, s>>>(geo, k, cs, ac_group_bytes, dc_group_bytes, ac_group_bits, dc_group_bits,
                                    blobs, lay, ac_base, dc_base);
 }
 
