@@ -132,16 +132,19 @@ void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32
     fill_u32_kernel<<<g > 0 ? g : 1, 256, 0, s>>>(p, n, v);
 }
 
-static unsigned int item_grid(const Geometry& geo) {
-    const std::size_t waves{3ull * geo.nblocks};
-    const unsigned int g{grid_for(waves, 4)};
-    return g < 4096 ? g : 4096;
+
+void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
+                      const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* tokens,
+                      std::uint16_t* item_tokens, std::uint32_t* partial, std::uint32_t* hist) {
+    const unsigned int waves{(3 * geo.nblocks + 3) / 4};
+    const unsigned int grid{waves < static_cast<unsigned int>(kTokenGrid) ? waves : static_cast<unsigned int>(kTokenGrid)};
+    ac_tokens_kernel<<<grid, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, tokens, item_tokens, partial);
+    ac_hist_reduce_kernel<<<dim3(kNumClusters * kAcSymbols / 256, 32), 256, 0, s>>>(partial, static_cast<int>(grid),
+                                                                                    hist);
 }
 
-void launch_ac_hist(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
-                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* hist) {
-    ac_hist_kernel<<<item_grid(geo) < 1024 ? item_grid(geo) : 1024, 256, 0, s>>>(geo, ac, acs, nz_grid,
-                                                                               nz_count, hist);
+std::size_t ac_partial_words() {
+    return static_cast<std::size_t>(kTokenGrid) * kNumClusters * kAcSymbols;
 }
 
 void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
@@ -190,15 +193,16 @@ void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, con
     dc_blobs_kernel<<<geo.num_dc, 64, 0, s>>>(geo, k, cs, fb_count, blobs, blob_words);
 }
 
-void launch_ac_bits(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
-                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
-                    std::uint32_t* item_bits) {
-    ac_bits_kernel<<<item_grid(geo), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, &cs->depth[0][0], item_bits);
+void launch_ac_seg_bits(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
+                        const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
+                        std::uint32_t* seg_bits) {
+    ac_seg_bits_kernel<<<geo.num_ac * kAcGroupBlocks, 256, 0, s>>>(geo, acs, item_tokens, tokens, &cs->depth[0][0],
+                                                                   seg_bits);
 }
 
-void launch_ac_group_scan(hipStream_t s, const Geometry& geo, const std::uint32_t* item_bits,
-                          std::uint32_t* item_off, std::uint32_t* group_bytes, std::uint64_t* group_bits) {
-    ac_group_scan_kernel<<<geo.num_ac, 1024, 0, s>>>(geo, item_bits, item_off, group_bytes, group_bits);
+void launch_ac_group_scan(hipStream_t s, const Geometry& geo, const std::uint32_t* seg_bits,
+                          std::uint32_t* seg_off, std::uint32_t* group_bytes, std::uint64_t* group_bits) {
+    ac_group_scan_kernel<<<geo.num_ac, 64, 0, s>>>(geo, seg_bits, seg_off, group_bytes, group_bits);
 }
 
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
@@ -226,12 +230,12 @@ void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* la
     headers_kernel<<<1, 64, 0, s>>>(k, lay, out_words, cap_words);
 }
 
-void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
-                    const std::uint8_t* nz_grid, const std::uint16_t* nz_count, const CodeSet* cs,
-                    const std::uint32_t* item_off, const std::uint32_t* item_bits, const std::uint64_t* ac_base,
-                    std::uint32_t* out_words, std::uint64_t cap_words) {
-    ac_emit_kernel<<<geo.num_ac * kAcGroupBlocks, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, &cs->depth[0][0],
-                                                               &cs->code[0][0], item_off, item_bits, ac_base,
+void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs, const std::uint16_t* item_tokens,
+                    const std::uint32_t* tokens, const CodeSet* cs, const std::uint32_t* seg_bits,
+                    const std::uint32_t* seg_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
+                    std::uint64_t cap_words) {
+    ac_emit_kernel<<<geo.num_ac * kAcGroupBlocks, 256, 0, s>>>(geo, acs, item_tokens, tokens, &cs->depth[0][0],
+                                                               &cs->code[0][0], seg_bits, seg_off, ac_base,
                                                                out_words, cap_words);
 }
 

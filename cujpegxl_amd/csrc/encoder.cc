@@ -226,6 +226,16 @@ void Encoder::release() {
         (void)hipHostFree(total_host_);
         total_host_ = nullptr;
     }
+    if (stage_host_ != nullptr) {
+        (void)hipHostFree(stage_host_);
+        stage_host_ = nullptr;
+    }
+    for (hipEvent_t& e : stage_ready_) {
+        if (e != nullptr) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    }
     for (hipEvent_t& e : events_) {
         if (e != nullptr) {
             (void)hipEventDestroy(e);
@@ -270,8 +280,14 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
             return false;
         }
     }
-    if (hipHostMalloc(reinterpret_cast<void**>(&total_host_), sizeof(std::uint64_t)) != hipSuccess) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&total_host_), sizeof(std::uint64_t)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&stage_host_), 2 * kStageChunk) != hipSuccess) {
         return false;
+    }
+    for (hipEvent_t& e : stage_ready_) {
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            return false;
+        }
     }
     return alloc();
 }
@@ -308,8 +324,11 @@ bool Encoder::alloc() {
     get(constants_, sizeof_constants());
     get(blobs_, sizeof_dc_blobs());
     get(blob_words_, static_cast<std::size_t>(geo_.num_dc) * blob_words_per_group() * sizeof(std::uint32_t));
-    get(item_bits_, 3 * nblk * sizeof(std::uint32_t));
-    get(item_off_, 3 * nblk * sizeof(std::uint32_t));
+    get(tokens_, 3 * nblk * 64 * sizeof(std::uint32_t));
+    get(item_tokens_, 3 * nblk * sizeof(std::uint16_t));
+    get(hist_partial_, ac_partial_words() * sizeof(std::uint32_t));
+    get(seg_bits_, static_cast<std::size_t>(geo_.num_ac) * kAcGroupBlocks * sizeof(std::uint32_t));
+    get(seg_off_, static_cast<std::size_t>(geo_.num_ac) * kAcGroupBlocks * sizeof(std::uint32_t));
     get(ac_group_bytes_, geo_.num_ac * sizeof(std::uint32_t));
     get(ac_group_bits_, geo_.num_ac * sizeof(std::uint64_t));
     get(chunk_bits_, static_cast<std::size_t>(geo_.num_dc) * dc_chunks_per_group() * sizeof(std::uint64_t));
@@ -414,6 +433,39 @@ void Encoder::mark(Stage s) {
     }
 }
 
+// Device file -> caller's host buffer through the two pinned slots: chunk k
+// goes to slot k % 2; the host copies a chunk out while the next one is in
+// flight, then reuses its slot for chunk k + 2.
+bool Encoder::copy_out(std::uint8_t* dst, std::size_t bytes) {
+    const auto* src{reinterpret_cast<const std::uint8_t*>(out_)};
+    std::size_t issued{0}, done{0};
+    std::size_t len[2]{0, 0};
+    auto issue = [&](int slot) {
+        len[slot] = std::min(kStageChunk, bytes - issued);
+        const bool ok{hipMemcpyAsync(stage_host_ + slot * kStageChunk, src + issued, len[slot],
+                                     hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+                      hipEventRecord(stage_ready_[slot], stream_) == hipSuccess};
+        issued += len[slot];
+        return ok;
+    };
+    for (int slot{0}; slot < 2 && issued < bytes; ++slot) {
+        if (!issue(slot)) {
+            return false;
+        }
+    }
+    for (int slot{0}; done < bytes; slot ^= 1) {
+        if (hipEventSynchronize(stage_ready_[slot]) != hipSuccess) {
+            return false;
+        }
+        std::memcpy(dst + done, stage_host_ + slot * kStageChunk, len[slot]);
+        done += len[slot];
+        if (issued < bytes && !issue(slot)) {
+            return false;
+        }
+    }
+    return true;
+}
+
 bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                           std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
     if (hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
@@ -466,22 +518,22 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     launch_fill_u32(stream_, hists_, nh * kHistStride, 0u);
     launch_fill_u32(stream_, fb_count_, 64, 0u);
     launch_fill_u32(stream_, static_cast<std::uint32_t*>(blobs_), sizeof_dc_blobs() / 4, 0u);
-    launch_ac_hist(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, hists_);
+    launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_);
     launch_dc_hist(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
                    hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
     mark(Stage::kCodes);
     launch_huffman(stream_, static_cast<int>(nh), hists_, codes);
     launch_dc_blobs(stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
     mark(Stage::kSizes);
-    launch_ac_bits(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, codes, item_bits_);
-    launch_ac_group_scan(stream_, geo_, item_bits_, item_off_, ac_group_bytes_, ac_group_bits_);
+    launch_ac_seg_bits(stream_, geo_, acs_, item_tokens_, tokens_, codes, seg_bits_);
+    launch_ac_group_scan(stream_, geo_, seg_bits_, seg_off_, ac_group_bytes_, ac_group_bits_);
     launch_dc_bits(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
                    dc_group_bits_);
     launch_layout(stream_, geo_, k, codes, ac_group_bytes_, dc_group_bytes_, ac_group_bits_, dc_group_bits_, blobs,
                   lay, ac_base_, dc_base_);
     mark(Stage::kEmit);
     launch_zero_and_headers(stream_, k, lay, out_, out_cap_ / 4);
-    launch_ac_emit(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, codes, item_off_, item_bits_, ac_base_, out_,
+    launch_ac_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, seg_bits_, seg_off_, ac_base_, out_,
                    out_cap_ / 4);
     launch_dc_emit(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_base_,
                    out_, out_cap_ / 4);
@@ -497,10 +549,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     if (total > out_cap_) {
         return false;
     }
-    if (total <= cap && out != nullptr) {
-        if (hipMemcpyAsync(out, out_, total, hipMemcpyDeviceToHost, stream_) != hipSuccess) {
-            return false;
-        }
+    if (total <= cap && out != nullptr && !copy_out(out, static_cast<std::size_t>(total))) {
+        return false;
     }
     mark(Stage::kCount);
     if (hipStreamSynchronize(stream_) != hipSuccess) {

@@ -69,6 +69,7 @@ private:
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
     bool run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size);
     void mark(Stage s);
+    bool copy_out(std::uint8_t* dst, std::size_t bytes);
 
     int device_{-1};
     float distance_{1.0f};
@@ -97,8 +98,11 @@ private:
     void* constants_{nullptr};
     void* blobs_{nullptr};
     std::uint32_t* blob_words_{nullptr};
-    std::uint32_t* item_bits_{nullptr};
-    std::uint32_t* item_off_{nullptr};
+    std::uint32_t* tokens_{nullptr};       // token records in coefficient-slot layout
+    std::uint16_t* item_tokens_{nullptr};  // tokens per (block, channel) item
+    std::uint32_t* hist_partial_{nullptr}; // per-workgroup AC histograms
+    std::uint32_t* seg_bits_{nullptr};     // bits per AC-group block row
+    std::uint32_t* seg_off_{nullptr};      // bit offset of each row in its group
     std::uint32_t* ac_group_bytes_{nullptr};
     std::uint64_t* ac_group_bits_{nullptr};
     std::uint64_t* chunk_bits_{nullptr};
@@ -110,6 +114,11 @@ private:
     std::uint32_t* out_{nullptr};
     std::size_t out_cap_{0};
     std::uint64_t* total_host_{nullptr};  // pinned
+    // Readback: the file is copied in chunks through two pinned slots, the
+    // device-to-host copy of one chunk overlapping the host copy of the last.
+    static constexpr std::size_t kStageChunk = 512 * 1024;
+    std::uint8_t* stage_host_{nullptr};   // pinned, 2 * kStageChunk
+    std::array<hipEvent_t, 2> stage_ready_{};
     std::vector<void*> allocations_{};
 };
 

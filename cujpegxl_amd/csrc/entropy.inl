@@ -134,16 +134,18 @@ __device__ __forceinline__ AcItem ac_item(const Geometry& geo, const std::int8_t
     return it;
 }
 
-// Tokenizes one item across the wave; `emit(valid, cluster, sym, nbits, extra)`
-// is called once per lane per chunk (lane 0 of the first call carries the
-// non-zero-count token).
+// Tokenizes one item across the wave; `emit(valid, k, cluster, sym, nbits,
+// extra)` is called once per lane per chunk, k being the token's index in the
+// item: 0 for the non-zero-count token (lane 0 of the first call), then one
+// token per scan position from the first non-LLF position up to the last
+// non-zero coefficient, so the valid lanes of a chunk are a prefix.
 template <typename Emit>
 __device__ __forceinline__ void tokenize_item(const Geometry& geo, const std::int16_t* ac,
                                               const AcItem& it, int lane, Emit&& emit) {
     {
         std::uint32_t sym, nb, bits;
         hybrid420(it.nz, sym, nb, bits);
-        emit(lane == 0, ac_cluster(it.nz_ctx), sym, nb, bits);
+        emit(lane == 0, 0u, ac_cluster(it.nz_ctx), sym, nb, bits);
     }
     const int n{it.n};
     const int side{n / 8};
@@ -179,21 +181,70 @@ __device__ __forceinline__ void tokenize_item(const Geometry& geo, const std::in
             cl = static_cast<std::uint32_t>(ac_cluster(ctx));
             hybrid420(pack_signed(v), sym, nb, bits);
         }
-        emit(valid, static_cast<int>(cl), sym, nb, bits);
+        emit(valid, 1u + (k - cb), static_cast<int>(cl), sym, nb, bits);
         seen += static_cast<std::uint32_t>(__popcll(mask));
         prev_carry = static_cast<std::uint32_t>((mask >> 63) & 1ull);
     }
 }
 
-// Pass 1: per-cluster token histograms.  Grid-stride over items by wave;
-// each workgroup accumulates in LDS and flushes its non-zero bins.
-__global__ __launch_bounds__(256) void ac_hist_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
-                                                      const std::int8_t* __restrict__ acs,
-                                                      const std::uint8_t* __restrict__ nz_grid,
-                                                      const std::uint16_t* __restrict__ nz_count,
-                                                      std::uint32_t* __restrict__ hist) {
-    __shared__ std::uint32_t h[kNumClusters * kHistStride];
-    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kHistStride; i += 256) {
+// AC segments: one block row of one AC group.  Its items, in decoder order
+// (blocks left to right, channels Y, X, B), form one contiguous bit range of
+// the group's section.
+struct AcSegment {
+    std::uint32_t first;  // first item
+    int items;            // 0 for a row below the frame
+    std::uint32_t g;      // AC group
+};
+
+__device__ __forceinline__ AcSegment ac_segment(const Geometry& geo, std::uint32_t s) {
+    AcSegment seg{};
+    seg.g = s / kAcGroupBlocks;
+    const int row{static_cast<int>(s % kAcGroupBlocks)};
+    const int bx0{static_cast<int>(seg.g % geo.xg) * kAcGroupBlocks};
+    const int by{static_cast<int>(seg.g / geo.xg) * kAcGroupBlocks + row};
+    if (by >= static_cast<int>(geo.bh)) {
+        return seg;
+    }
+    seg.first = (static_cast<std::uint32_t>(by) * geo.bw + static_cast<std::uint32_t>(bx0)) * 3u;
+    seg.items = 3 * min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0);
+    return seg;
+}
+
+// Token records: cluster | symbol << 5 | extra-bit count << 13 | extra << 18
+// (hybrid-uint extra bits of 16-bit magnitudes fit in 14 bits).  Token k of
+// an item lives in the k-th coefficient slot of the item's covered blocks
+// (an item has at most 1 + n*n - (n/8)^2 <= n*n tokens).
+__device__ __forceinline__ std::uint32_t token_record(int cl, std::uint32_t sym, std::uint32_t nb,
+                                                      std::uint32_t extra) {
+    return static_cast<std::uint32_t>(cl) | (sym << 5) | (nb << 13) | (extra << 18);
+}
+
+__device__ __forceinline__ std::size_t token_slot(const Geometry& geo, int c, int bx, int by, int m,
+                                                  std::uint32_t k) {
+    const int q{static_cast<int>(k >> 6)};
+    const int l2m{m >> 1};  // m in {1, 2, 4}
+    return static_cast<std::size_t>(c) * geo.nblocks * 64 +
+           (static_cast<std::size_t>(by + (q >> l2m)) * geo.bw + static_cast<std::size_t>(bx + (q & (m - 1)))) * 64 +
+           (k & 63u);
+}
+
+// Pass 1: tokenize every item once: token records, tokens per item and the
+// per-cluster histograms.  AC symbols are below 64 (hybrid-uint tokens of
+// 16-bit magnitudes and of counts <= 1024), so each workgroup counts into a
+// 32 x 64 LDS histogram and writes it out whole as one partial; a second
+// kernel sums the partials.  Grid-stride over items, one wave per item.
+constexpr int kAcSymbols = 64;
+constexpr int kTokenGrid = 2048;
+
+__global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
+                                                        const std::int8_t* __restrict__ acs,
+                                                        const std::uint8_t* __restrict__ nz_grid,
+                                                        const std::uint16_t* __restrict__ nz_count,
+                                                        std::uint32_t* __restrict__ tokens,
+                                                        std::uint16_t* __restrict__ item_tokens,
+                                                        std::uint32_t* __restrict__ partial) {
+    __shared__ std::uint32_t h[kNumClusters * kAcSymbols];
+    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
         h[i] = 0;
     }
     __syncthreads();
@@ -203,50 +254,44 @@ __global__ __launch_bounds__(256) void ac_hist_kernel(Geometry geo, const std::i
     const std::uint32_t items{geo.nblocks * 3};
     for (std::uint32_t item{wave}; item < items; item += nwaves) {
         const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
-        if (it.n == 0) {
-            continue;
+        std::uint32_t count{0};
+        if (it.n != 0) {
+            const int m{it.n / 8};
+            tokenize_item(geo, ac, it, lane,
+                          [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
+                              std::uint32_t extra) {
+                              if (valid) {
+                                  atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
+                                  tokens[token_slot(geo, it.c, it.bx, it.by, m, k)] = token_record(cl, sym, nb, extra);
+                              }
+                              count += static_cast<std::uint32_t>(__popcll(__ballot(valid)));
+                          });
         }
-        tokenize_item(geo, ac, it, lane,
-                      [&](bool valid, int cl, std::uint32_t sym, std::uint32_t, std::uint32_t) {
-                          if (valid) {
-                              atomicAdd(&h[cl * kHistStride + sym], 1u);
-                          }
-                      });
+        if (lane == 0) {
+            item_tokens[item] = static_cast<std::uint16_t>(count);
+        }
     }
     __syncthreads();
-    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kHistStride; i += 256) {
-        if (h[i]) {
-            atomicAdd(&hist[i], h[i]);
-        }
+    std::uint32_t* mine{partial + static_cast<std::size_t>(blockIdx.x) * kNumClusters * kAcSymbols};
+    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
+        mine[i] = h[i];
     }
 }
 
-// Pass 2: bits per item under the built codes.
-__global__ __launch_bounds__(256) void ac_bits_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
-                                                      const std::int8_t* __restrict__ acs,
-                                                      const std::uint8_t* __restrict__ nz_grid,
-                                                      const std::uint16_t* __restrict__ nz_count,
-                                                      const std::uint8_t* __restrict__ depth,
-                                                      std::uint32_t* __restrict__ item_bits) {
-    const int lane{static_cast<int>(threadIdx.x) & 63};
-    const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
-    const std::uint32_t nwaves{gridDim.x * 4};
-    const std::uint32_t items{geo.nblocks * 3};
-    for (std::uint32_t item{wave}; item < items; item += nwaves) {
-        const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
-        int total{0};
-        if (it.n != 0) {
-            tokenize_item(geo, ac, it, lane,
-                          [&](bool valid, int cl, std::uint32_t sym, std::uint32_t nb, std::uint32_t) {
-                              if (valid) {
-                                  total += depth[cl * kHistStride + sym] + static_cast<int>(nb);
-                              }
-                          });
-        }
-        total = wave_sum_int(total);
-        if (lane == 0) {
-            item_bits[item] = static_cast<std::uint32_t>(total);
-        }
+// Sums `parts` partial histograms of 32 x 64 bins into the 32 x 256 cluster
+// histograms: grid (bins / 256, slices); each thread adds its slice.
+__global__ __launch_bounds__(256) void ac_hist_reduce_kernel(const std::uint32_t* __restrict__ partial, int parts,
+                                                             std::uint32_t* __restrict__ hist) {
+    const int bin{static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x)};
+    const int per{(parts + static_cast<int>(gridDim.y) - 1) / static_cast<int>(gridDim.y)};
+    const int p0{static_cast<int>(blockIdx.y) * per};
+    const int p1{min(parts, p0 + per)};
+    std::uint32_t sum{0};
+    for (int p{p0}; p < p1; ++p) {
+        sum += partial[static_cast<std::size_t>(p) * kNumClusters * kAcSymbols + bin];
+    }
+    if (sum != 0) {
+        atomicAdd(&hist[(bin / kAcSymbols) * kHistStride + bin % kAcSymbols], sum);
     }
 }
 
@@ -282,79 +327,132 @@ __device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std
     return r;
 }
 
-// Per AC group: bit offsets of its items in decoder order (group raster of
-// blocks, channels Y, X, B) and the group's byte size.  One 1024-thread
-// workgroup per group, one block per thread.
-__global__ __launch_bounds__(1024) void ac_group_scan_kernel(Geometry geo,
-                                                             const std::uint32_t* __restrict__ item_bits,
-                                                             std::uint32_t* __restrict__ item_off,
-                                                             std::uint32_t* __restrict__ group_bytes,
-                                                             std::uint64_t* __restrict__ group_bits) {
-    __shared__ std::uint64_t sh[17];
-    const int g{static_cast<int>(blockIdx.x)};
-    const int gx{g % static_cast<int>(geo.xg)}, gy{g / static_cast<int>(geo.xg)};
-    const int bx0{gx * 32}, by0{gy * 32};
-    const int gbw{min(32, static_cast<int>(geo.bw) - bx0)}, gbh{min(32, static_cast<int>(geo.bh) - by0)};
+// A segment's items in LDS: token prefix and the slot geometry of each item.
+struct SegItems {
+    std::uint32_t prefix[3 * kAcGroupBlocks + 1];
+    std::int16_t c[3 * kAcGroupBlocks], bx[3 * kAcGroupBlocks], by[3 * kAcGroupBlocks];
+    std::int8_t m[3 * kAcGroupBlocks];
+    std::uint64_t sh[5];
+};
+
+// Loads the segment's item table (256 threads); returns the token count.
+__device__ std::uint32_t load_segment(const Geometry& geo, const AcSegment& seg, const std::int8_t* acs,
+                                      const std::uint16_t* item_tokens, SegItems& S) {
     const int t{static_cast<int>(threadIdx.x)};
-    const bool has{t < gbw * gbh};
-    std::uint32_t b3[3]{0, 0, 0};
-    std::size_t pos{0};
-    if (has) {
-        pos = static_cast<std::size_t>(by0 + t / gbw) * geo.bw + bx0 + t % gbw;
-        for (int ci{0}; ci < 3; ++ci) {
-            b3[ci] = item_bits[pos * 3 + ci];
-        }
+    std::uint32_t cnt{0};
+    if (t < seg.items) {
+        const std::uint32_t item{seg.first + static_cast<std::uint32_t>(t)};
+        const std::uint32_t pos{item / 3};
+        const int ci{static_cast<int>(item % 3)};
+        const int n{acs[pos]};
+        S.c[t] = static_cast<std::int16_t>(ci == 0 ? 1 : (ci == 1 ? 0 : 2));
+        S.bx[t] = static_cast<std::int16_t>(pos % geo.bw);
+        S.by[t] = static_cast<std::int16_t>(pos / geo.bw);
+        S.m[t] = static_cast<std::int8_t>(n / 8);
+        cnt = n != 0 ? item_tokens[item] : 0u;
     }
     std::uint64_t total{0};
-    const std::uint64_t base{block_excl_scan(static_cast<std::uint64_t>(b3[0]) + b3[1] + b3[2], sh, total)};
-    if (has) {
-        item_off[pos * 3 + 0] = static_cast<std::uint32_t>(base);
-        item_off[pos * 3 + 1] = static_cast<std::uint32_t>(base + b3[0]);
-        item_off[pos * 3 + 2] = static_cast<std::uint32_t>(base + b3[0] + b3[1]);
+    const std::uint64_t p{block_excl_scan(cnt, S.sh, total)};
+    if (t < seg.items) {
+        S.prefix[t] = static_cast<std::uint32_t>(p);
     }
     if (t == 0) {
-        group_bytes[g] = static_cast<std::uint32_t>((total + 7) / 8);
-        group_bits[g] = total;
+        S.prefix[seg.items] = static_cast<std::uint32_t>(total);
+    }
+    __syncthreads();
+    return static_cast<std::uint32_t>(total);
+}
+
+// Token f of the segment: the last item whose prefix is <= f (empty items
+// share their successor's prefix and are skipped by that rule).
+__device__ __forceinline__ std::uint32_t segment_token(const Geometry& geo, const SegItems& S, int items,
+                                                       std::uint32_t f, const std::uint32_t* tokens) {
+    int lo{0}, hi{items - 1};
+    while (lo < hi) {
+        const int mid{(lo + hi + 1) >> 1};
+        if (S.prefix[mid] <= f) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return tokens[token_slot(geo, S.c[lo], S.bx[lo], S.by[lo], S.m[lo], f - S.prefix[lo])];
+}
+
+// Pass 2: bits per segment under the built codes.
+__global__ __launch_bounds__(256) void ac_seg_bits_kernel(Geometry geo, const std::int8_t* __restrict__ acs,
+                                                          const std::uint16_t* __restrict__ item_tokens,
+                                                          const std::uint32_t* __restrict__ tokens,
+                                                          const std::uint8_t* __restrict__ depth,
+                                                          std::uint32_t* __restrict__ seg_bits) {
+    __shared__ SegItems S;
+    const AcSegment seg{ac_segment(geo, blockIdx.x)};
+    if (seg.items == 0) {
+        if (threadIdx.x == 0) {
+            seg_bits[blockIdx.x] = 0;
+        }
+        return;
+    }
+    const std::uint32_t total{load_segment(geo, seg, acs, item_tokens, S)};
+    std::uint64_t bits{0};
+    for (std::uint32_t f{threadIdx.x}; f < total; f += 256) {
+        const std::uint32_t r{segment_token(geo, S, seg.items, f, tokens)};
+        bits += depth[(r & 31u) * kHistStride + ((r >> 5) & 255u)] + ((r >> 13) & 31u);
+    }
+    std::uint64_t sum{0};
+    (void)block_excl_scan(bits, S.sh, sum);
+    if (threadIdx.x == 0) {
+        seg_bits[blockIdx.x] = static_cast<std::uint32_t>(sum);
     }
 }
 
-// Pass 3: emission.  One workgroup per (AC group, block row of the group):
-// the row's items, in decoder order (blocks left to right, channels Y, X, B),
-// occupy one contiguous bit range of the group's section.  The workgroup ORs
-// its tokens into an LDS copy of that range, then writes the interior words
-// with plain stores; only the first and last word, which a neighbouring range
-// may share, are combined with atomicOr into the zeroed output.  A range
-// longer than the LDS copy is emitted with global atomics instead.
+// Per AC group: bit offsets of its 32 segments, the group's bits and bytes.
+__global__ __launch_bounds__(64) void ac_group_scan_kernel(Geometry geo, const std::uint32_t* __restrict__ seg_bits,
+                                                           std::uint32_t* __restrict__ seg_off,
+                                                           std::uint32_t* __restrict__ group_bytes,
+                                                           std::uint64_t* __restrict__ group_bits) {
+    const int g{static_cast<int>(blockIdx.x)};
+    const int lane{static_cast<int>(threadIdx.x)};
+    const std::uint32_t s{static_cast<std::uint32_t>(g) * kAcGroupBlocks + static_cast<std::uint32_t>(lane)};
+    const int v{lane < kAcGroupBlocks ? static_cast<int>(seg_bits[s]) : 0};
+    const int pre{wave_excl_scan(v, lane)};
+    const int total{wave_sum_int(v)};
+    if (lane < kAcGroupBlocks) {
+        seg_off[s] = static_cast<std::uint32_t>(pre);
+    }
+    if (lane == 0) {
+        group_bytes[g] = static_cast<std::uint32_t>((static_cast<std::uint64_t>(total) + 7) / 8);
+        group_bits[g] = static_cast<std::uint64_t>(total);
+    }
+}
+
+// Pass 3: emission.  One workgroup per segment, one thread per token: bit
+// positions from a block scan of the token lengths, tokens ORed into an LDS
+// copy of the segment's bit range, interior words written with plain stores
+// and only the first and last word (shared with neighbouring ranges)
+// combined with atomicOr into the zeroed output.  A range longer than the
+// LDS copy is emitted with global atomics instead.
 constexpr int kEmitWords = 2048;
 
-__global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
-                                                      const std::int8_t* __restrict__ acs,
-                                                      const std::uint8_t* __restrict__ nz_grid,
-                                                      const std::uint16_t* __restrict__ nz_count,
+__global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::int8_t* __restrict__ acs,
+                                                      const std::uint16_t* __restrict__ item_tokens,
+                                                      const std::uint32_t* __restrict__ tokens,
                                                       const std::uint8_t* __restrict__ depth,
                                                       const std::uint16_t* __restrict__ codes,
-                                                      const std::uint32_t* __restrict__ item_off,
-                                                      const std::uint32_t* __restrict__ item_bits,
+                                                      const std::uint32_t* __restrict__ seg_bits,
+                                                      const std::uint32_t* __restrict__ seg_off,
                                                       const std::uint64_t* __restrict__ ac_group_base_bits,
                                                       std::uint32_t* __restrict__ out_words,
                                                       std::uint64_t cap_words) {
     __shared__ std::uint32_t buf[kEmitWords];
+    __shared__ SegItems S;
     const int tid{static_cast<int>(threadIdx.x)};
-    const int lane{tid & 63};
-    const int wave{tid >> 6};
-    const std::uint32_t g{blockIdx.x / kAcGroupBlocks};
-    const int row{static_cast<int>(blockIdx.x % kAcGroupBlocks)};
-    const int bx0{static_cast<int>(g % geo.xg) * kAcGroupBlocks};
-    const int by{static_cast<int>(g / geo.xg) * kAcGroupBlocks + row};
-    if (by >= static_cast<int>(geo.bh)) {
+    const AcSegment seg{ac_segment(geo, blockIdx.x)};
+    if (seg.items == 0) {
         return;
     }
-    const int gbw{min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0)};
-    const std::size_t first{(static_cast<std::size_t>(by) * geo.bw + bx0) * 3};
-    const std::size_t last{first + 3 * static_cast<std::size_t>(gbw) - 1};
-    const std::uint64_t gbase{ac_group_base_bits[g]};
-    const std::uint64_t start{gbase + item_off[first]};
-    const std::uint64_t end{gbase + item_off[last] + item_bits[last]};
+    const std::uint64_t start{ac_group_base_bits[seg.g] + seg_off[blockIdx.x]};
+    const std::uint64_t end{start + seg_bits[blockIdx.x]};
     const std::uint64_t w0{start >> 5};
     const std::uint64_t nw{((end + 31) >> 5) - w0};
     const bool staged{nw <= static_cast<std::uint64_t>(kEmitWords)};
@@ -363,46 +461,40 @@ __global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::i
             buf[i] = 0;
         }
     }
-    __syncthreads();
+    const std::uint32_t total{load_segment(geo, seg, acs, item_tokens, S)};
     const std::uint64_t lbase{w0 << 5};
-    for (int i{wave}; i < 3 * gbw; i += 4) {
-        const std::uint32_t item{static_cast<std::uint32_t>(first) + static_cast<std::uint32_t>(i)};
-        const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
-        if (it.n == 0) {
-            continue;
+    std::uint64_t run{start};
+    for (std::uint32_t f0{0}; f0 < total; f0 += 256) {
+        const std::uint32_t f{f0 + static_cast<std::uint32_t>(tid)};
+        std::uint32_t len{0}, val{0};
+        if (f < total) {
+            const std::uint32_t r{segment_token(geo, S, seg.items, f, tokens)};
+            const std::uint32_t idx{(r & 31u) * kHistStride + ((r >> 5) & 255u)};
+            const std::uint32_t d{depth[idx]};
+            len = d + ((r >> 13) & 31u);
+            val = static_cast<std::uint32_t>(codes[idx]) | ((r >> 18) << d);
         }
-        std::uint64_t pos{gbase + item_off[item]};
-        tokenize_item(geo, ac, it, lane,
-                      [&](bool valid, int cl, std::uint32_t sym, std::uint32_t nb, std::uint32_t extra) {
-                          std::uint32_t len{0}, val{0};
-                          if (valid) {
-                              const int idx{cl * kHistStride + static_cast<int>(sym)};
-                              const std::uint32_t d{depth[idx]};
-                              len = d + nb;
-                              val = static_cast<std::uint32_t>(codes[idx]) | (extra << d);
-                          }
-                          const int pre{wave_excl_scan(static_cast<int>(len), lane)};
-                          const std::uint64_t at{pos + static_cast<std::uint64_t>(pre)};
-                          if (staged) {
-                              if (len != 0) {
-                                  if (len < 32) {
-                                      val &= (1u << len) - 1u;
-                                  }
-                                  const std::uint64_t rel{at - lbase};
-                                  const std::uint32_t wi{static_cast<std::uint32_t>(rel >> 5)};
-                                  const std::uint32_t b{static_cast<std::uint32_t>(rel & 31u)};
-                                  if (wi < static_cast<std::uint32_t>(kEmitWords)) {
-                                      atomicOr(&buf[wi], val << b);
-                                  }
-                                  if (b + len > 32 && wi + 1 < static_cast<std::uint32_t>(kEmitWords)) {
-                                      atomicOr(&buf[wi + 1], val >> (32 - b));
-                                  }
-                              }
-                          } else {
-                              or_bits(out_words, cap_words, at, val, len);
-                          }
-                          pos += static_cast<std::uint64_t>(wave_sum_int(static_cast<int>(len)));
-                      });
+        std::uint64_t chunk{0};
+        const std::uint64_t at{run + block_excl_scan(len, S.sh, chunk)};
+        if (len != 0) {
+            if (staged) {
+                if (len < 32) {
+                    val &= (1u << len) - 1u;
+                }
+                const std::uint64_t rel{at - lbase};
+                const std::uint32_t wi{static_cast<std::uint32_t>(rel >> 5)};
+                const std::uint32_t b{static_cast<std::uint32_t>(rel & 31u)};
+                if (wi < static_cast<std::uint32_t>(kEmitWords)) {
+                    atomicOr(&buf[wi], val << b);
+                }
+                if (b + len > 32 && wi + 1 < static_cast<std::uint32_t>(kEmitWords)) {
+                    atomicOr(&buf[wi + 1], val >> (32 - b));
+                }
+            } else {
+                or_bits(out_words, cap_words, at, val, len);
+            }
+        }
+        run += chunk;
     }
     if (!staged) {
         return;

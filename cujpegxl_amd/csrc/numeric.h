@@ -1,7 +1,7 @@
 // Float primitives with one definition on host and device.
 //
-// Everything here is built from IEEE-754 basic operations (add, mul,
-// correctly rounded div/sqrt, fma, round-to-nearest-even) so the host and
+// Everything here is built from IEEE-754 basic operations (add, mul, fma,
+// round-to-nearest-even, integer bit manipulation) so the host and
 // gfx950 evaluate them to identical bits when compiled with contraction off.
 // The encoder's integer outputs (quant field, transform map, quantized
 // coefficients) depend on these values, which is why no libm or ocml
@@ -23,8 +23,9 @@ CJX_HD float u2f(std::uint32_t u) {
     return __builtin_bit_cast(float, u);
 }
 
-// log2 for positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)];
-// ln m = 2 atanh(t), t = (m - 1) / (m + 1), odd series to t^9.
+// log2 for positive normal x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// f = m - 1 (exact), log2(1 + f) = f P(f), P a degree-9 fit of
+// log2(1 + f) / f on that interval (no division; <= 1.7 ulp).
 CJX_HD float log2_det(float x) {
     const std::uint32_t u{f2u(x)};
     int e{static_cast<int>((u >> 23) & 0xffu) - 127};
@@ -33,15 +34,18 @@ CJX_HD float log2_det(float x) {
         m = m * 0.5f;
         e = e + 1;
     }
-    const float t{(m - 1.0f) / (m + 1.0f)};
-    const float t2{t * t};
-    float p{0.111111111f};
-    p = __builtin_fmaf(p, t2, 0.142857143f);
-    p = __builtin_fmaf(p, t2, 0.2f);
-    p = __builtin_fmaf(p, t2, 0.333333333f);
-    p = __builtin_fmaf(p, t2, 1.0f);
-    const float ln_m{(2.0f * t) * p};
-    return __builtin_fmaf(ln_m, 1.44269504f, static_cast<float>(e));
+    const float f{m - 1.0f};
+    float p{-0.11296797543764114f};
+    p = __builtin_fmaf(p, f, 0.18781770765781403f);
+    p = __builtin_fmaf(p, f, -0.19061124324798584f);
+    p = __builtin_fmaf(p, f, 0.20429740846157074f);
+    p = __builtin_fmaf(p, f, -0.2396368831396103f);
+    p = __builtin_fmaf(p, f, 0.28858521580696106f);
+    p = __builtin_fmaf(p, f, -0.360696405172348f);
+    p = __builtin_fmaf(p, f, 0.4808979034423828f);
+    p = __builtin_fmaf(p, f, -0.7213473320007324f);
+    p = __builtin_fmaf(p, f, 1.4426950216293335f);
+    return __builtin_fmaf(f, p, static_cast<float>(e));
 }
 
 // 2^x on [-126, 127]: x = n + f, |f| <= 1/2, degree-7 Taylor of e^(f ln 2),
@@ -62,17 +66,25 @@ CJX_HD float exp2_det(float x) {
     return p * u2f(static_cast<std::uint32_t>(static_cast<int>(n) + 127) << 23);
 }
 
-// Cube root of x >= 0: exponent-thirding seed and three Newton steps.
+// Cube root of x >= 0 without division: y ~ x^(-1/3) from an exponent-
+// thirding seed and three Newton steps y <- y (4 - x y^3) / 3, then c = x y^2
+// and one Newton correction c <- c - (c^3 - x) y^2 / 3 (<= 0.75 ulp on
+// [1e-6, 64]).
 CJX_HD float cbrt_det(float x) {
     if (!(x > 0.0f)) {
         return 0.0f;
     }
-    float y{u2f(f2u(x) / 3u + 0x2a5137a0u)};
+    float y{u2f(0x54a23300u - f2u(x) / 3u)};
+    const float x3{x * (1.0f / 3.0f)};
 #pragma unroll
     for (int i{0}; i < 3; ++i) {
-        y = (2.0f * y + x / (y * y)) * 0.333333333f;
+        const float y3{(y * y) * y};
+        y = y * __builtin_fmaf(-x3, y3, 4.0f / 3.0f);
     }
-    return y;
+    float c{(x * y) * y};
+    const float r{__builtin_fmaf(c * c, c, -x)};
+    c = __builtin_fmaf(-r, (y * y) * (1.0f / 3.0f), c);
+    return c;
 }
 
 CJX_HD float pow24_det(float x) {
