@@ -62,6 +62,7 @@ struct FrameParams {
     float global_scale_float;
     float inv_global_scale;
     float qgsf_select;  // raw_quant_field * global_scale_float
+    float inv_qgsf_select;  // RN(1 / qgsf_select)
     float km[4];        // fuzzy-erosion weights
     float mul_pb, add_pb;
     float dc_scale;     // global_scale_float * quant_dc
@@ -80,6 +81,12 @@ struct DeviceTables {
     float dequant8[3][64];
     float dequant16[3][256];
     float dequant32[3][1024];
+    float wsel_inv8[64];   // RN(1 / weight) of the tables above
+    float wsel_inv16[256];
+    float wsel_inv32[1024];
+    float dequant_inv8[3][64];
+    float dequant_inv16[3][256];
+    float dequant_inv32[3][1024];
     std::uint16_t order8[64];     // natural scan order, raw index per scan position
     std::uint16_t order16[256];
     std::uint16_t order32[1024];
@@ -103,6 +110,13 @@ __device__ __forceinline__ const float* wsel_of(int n) {
 __device__ __forceinline__ float dequant_of(int n, int c, int e) {
     return n == 32 ? g_tables.dequant32[c][e]
                    : (n == 16 ? g_tables.dequant16[c][e] : g_tables.dequant8[c][e]);
+}
+__device__ __forceinline__ const float* wsel_inv_of(int n) {
+    return n == 32 ? g_tables.wsel_inv32 : (n == 16 ? g_tables.wsel_inv16 : g_tables.wsel_inv8);
+}
+__device__ __forceinline__ float dequant_inv_of(int n, int c, int e) {
+    return n == 32 ? g_tables.dequant_inv32[c][e]
+                   : (n == 16 ? g_tables.dequant_inv16[c][e] : g_tables.dequant_inv8[c][e]);
 }
 __device__ __forceinline__ const std::uint16_t* order_of(int n) {
     return n == 32 ? g_tables.order32 : (n == 16 ? g_tables.order16 : g_tables.order8);

@@ -97,6 +97,24 @@ __device__ __forceinline__ void or_bits(std::uint32_t* words, std::uint64_t cap_
     }
 }
 
+// ORs `len` (<= 32) bits at bit `pos` of an LDS word buffer of `cap` words.
+__device__ __forceinline__ void lds_or_bits(std::uint32_t* words, std::uint32_t cap, std::uint32_t pos,
+                                            std::uint32_t value, std::uint32_t len) {
+    if (len == 0) {
+        return;
+    }
+    if (len < 32) {
+        value &= (1u << len) - 1u;
+    }
+    const std::uint32_t wi{pos >> 5}, b{pos & 31u};
+    if (wi < cap) {
+        atomicOr(&words[wi], value << b);
+    }
+    if (b + len > 32 && wi + 1 < cap) {
+        atomicOr(&words[wi + 1], value >> (32 - b));
+    }
+}
+
 struct AcItem {
     int n;             // transform side, 0 if covered
     int c;             // XYB channel
@@ -134,20 +152,25 @@ __device__ __forceinline__ AcItem ac_item(const Geometry& geo, const std::int8_t
     return it;
 }
 
-// Tokenizes one item across the wave; `emit(valid, k, cluster, sym, nbits,
-// extra)` is called once per lane per chunk, k being the token's index in the
-// item: 0 for the non-zero-count token (lane 0 of the first call), then one
-// token per scan position from the first non-LLF position up to the last
-// non-zero coefficient, so the valid lanes of a chunk are a prefix.
+// Tokenizes two items per wave, one per 32-lane half (item `it` of this
+// lane's half; `has` false for a half without an item).  `emit(valid, k,
+// cluster, sym, nbits, extra)` is called once per lane per chunk, k being the
+// token's index in the item: 0 for the non-zero-count token (lane 0 of the
+// half, first call), then one token per scan position from the first
+// non-LLF position up to the last non-zero coefficient, so the valid lanes of
+// a chunk are a prefix of the half.  A lane's remaining-non-zero count and
+// previous-coefficient flag (entropy.cu:648-681, ac_context.h:78-84) come
+// from popcounts of the half's ballot bits below it.
 template <typename Emit>
-__device__ __forceinline__ void tokenize_item(const Geometry& geo, const std::int16_t* ac,
-                                              const AcItem& it, int lane, Emit&& emit) {
+__device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const std::int16_t* ac,
+                                                   const AcItem& it, bool has, int lane, Emit&& emit) {
+    const int half{lane >> 5}, sub{lane & 31};
     {
-        std::uint32_t sym, nb, bits;
+        std::uint32_t sym{0}, nb{0}, bits{0};
         hybrid420(it.nz, sym, nb, bits);
-        emit(lane == 0, 0u, ac_cluster(it.nz_ctx), sym, nb, bits);
+        emit(has && sub == 0, 0u, has ? ac_cluster(it.nz_ctx) : 0, sym, nb, bits);
     }
-    const int n{it.n};
+    const int n{has ? it.n : 8};
     const int side{n / 8};
     const std::uint32_t cb{static_cast<std::uint32_t>(side * side)};
     const std::uint32_t size{static_cast<std::uint32_t>(n * n)};
@@ -157,22 +180,25 @@ __device__ __forceinline__ void tokenize_item(const Geometry& geo, const std::in
     const std::uint16_t* order{order_of(n)};
     std::uint32_t seen{0};
     std::uint32_t prev_carry{it.nz > size / 16 ? 0u : 1u};
-    const std::uint64_t below{(1ull << lane) - 1ull};
-    for (std::uint32_t base{cb}; base < size && seen < it.nz; base += 64) {
-        const std::uint32_t k{base + static_cast<std::uint32_t>(lane)};
+    const std::uint32_t below{(1u << sub) - 1u};
+    for (std::uint32_t base{cb};; base += 32) {
+        const bool active{has && base < size && seen < it.nz};
+        if (__ballot(active) == 0) {
+            break;
+        }
+        const std::uint32_t k{base + static_cast<std::uint32_t>(sub)};
         int v{0};
-        if (k < size) {
+        if (active && k < size) {
             const std::uint32_t e{order[k]};
             const std::uint32_t q{e >> 6};
-            const std::size_t slot{(static_cast<std::size_t>(it.by + q / side) * geo.bw + it.bx +
-                                    q % side) * 64 + (e & 63u)};
+            const std::size_t slot{(static_cast<std::size_t>(it.by + q / side) * geo.bw + it.bx + q % side) * 64 +
+                                   (e & 63u)};
             v = plane[slot];
         }
-        const std::uint64_t mask{__ballot(v != 0)};
-        const std::uint32_t before{seen + static_cast<std::uint32_t>(__popcll(mask & below))};
-        const std::uint32_t prev{lane == 0 ? prev_carry
-                                           : static_cast<std::uint32_t>((mask >> (lane - 1)) & 1ull)};
-        const bool valid{k < size && before < it.nz};
+        const std::uint32_t gm{static_cast<std::uint32_t>(__ballot(v != 0) >> (32 * half))};
+        const std::uint32_t before{seen + static_cast<std::uint32_t>(__popc(gm & below))};
+        const std::uint32_t prev{sub == 0 ? prev_carry : ((gm >> (sub - 1)) & 1u)};
+        const bool valid{active && k < size && before < it.nz};
         std::uint32_t cl{0}, sym{0}, nb{0}, bits{0};
         if (valid) {
             const std::uint32_t rem{((it.nz - before) + cb - 1) >> l2};
@@ -182,8 +208,8 @@ __device__ __forceinline__ void tokenize_item(const Geometry& geo, const std::in
             hybrid420(pack_signed(v), sym, nb, bits);
         }
         emit(valid, 1u + (k - cb), static_cast<int>(cl), sym, nb, bits);
-        seen += static_cast<std::uint32_t>(__popcll(mask));
-        prev_carry = static_cast<std::uint32_t>((mask >> 63) & 1ull);
+        seen += static_cast<std::uint32_t>(__popc(gm));
+        prev_carry = gm >> 31;
     }
 }
 
@@ -232,7 +258,7 @@ __device__ __forceinline__ std::size_t token_slot(const Geometry& geo, int c, in
 // per-cluster histograms.  AC symbols are below 64 (hybrid-uint tokens of
 // 16-bit magnitudes and of counts <= 1024), so each workgroup counts into a
 // 32 x 64 LDS histogram and writes it out whole as one partial; a second
-// kernel sums the partials.  Grid-stride over items, one wave per item.
+// kernel sums the partials.  Grid-stride over item pairs, one wave per pair.
 constexpr int kAcSymbols = 64;
 constexpr int kTokenGrid = 2048;
 
@@ -249,25 +275,28 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
     }
     __syncthreads();
     const int lane{static_cast<int>(threadIdx.x) & 63};
+    const int half{lane >> 5};
     const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
     const std::uint32_t nwaves{gridDim.x * 4};
     const std::uint32_t items{geo.nblocks * 3};
-    for (std::uint32_t item{wave}; item < items; item += nwaves) {
-        const AcItem it{ac_item(geo, acs, nz_grid, nz_count, item)};
+    for (std::uint32_t pair{2 * wave}; pair < items; pair += 2 * nwaves) {
+        const std::uint32_t item{pair + static_cast<std::uint32_t>(half)};
+        const bool in{item < items};
+        const AcItem it{in ? ac_item(geo, acs, nz_grid, nz_count, item) : AcItem{}};
+        const bool has{in && it.n != 0};
+        const int m{has ? it.n / 8 : 1};
         std::uint32_t count{0};
-        if (it.n != 0) {
-            const int m{it.n / 8};
-            tokenize_item(geo, ac, it, lane,
-                          [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
-                              std::uint32_t extra) {
-                              if (valid) {
-                                  atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
-                                  tokens[token_slot(geo, it.c, it.bx, it.by, m, k)] = token_record(cl, sym, nb, extra);
-                              }
-                              count += static_cast<std::uint32_t>(__popcll(__ballot(valid)));
-                          });
-        }
-        if (lane == 0) {
+        tokenize_item_pair(geo, ac, it, has, lane,
+                           [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
+                               std::uint32_t extra) {
+                               if (valid) {
+                                   atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
+                                   tokens[token_slot(geo, it.c, it.bx, it.by, m, k)] = token_record(cl, sym, nb, extra);
+                               }
+                               count += static_cast<std::uint32_t>(
+                                   __popc(static_cast<std::uint32_t>(__ballot(valid) >> (32 * half))));
+                           });
+        if ((lane & 31) == 0 && in) {
             item_tokens[item] = static_cast<std::uint16_t>(count);
         }
     }
@@ -556,49 +585,133 @@ __device__ __forceinline__ std::int32_t gradient_predict(bool hl, bool ht, std::
     return static_cast<std::int32_t>(static_cast<std::int64_t>(left) + top - tl);
 }
 
-// Token of DC-section item k (k != 0 and != 1 + 3n).  `kind` 0 = DC code,
-// 1 = AcMetadata code, 2 = absent (a covered block's strategy/QF slot).
-__device__ __forceinline__ int dc_item_token(const Geometry& geo, const DcGroupGeo& d, std::uint32_t k,
-                                             const std::int32_t* dc, const std::int8_t* acs,
-                                             const std::int32_t* qf, const std::int8_t* ytox,
-                                             const std::int8_t* ytob, std::uint32_t& sym,
-                                             std::uint32_t& nb, std::uint32_t& bits) {
+__device__ __forceinline__ std::uint32_t codes_or_zero(const std::uint16_t* codes, std::uint32_t idx) {
+    return codes != nullptr ? static_cast<std::uint32_t>(codes[idx]) : 0u;
+}
+
+// Walks the items of a DcGroup section in order.  Sections: 0 blob_pre,
+// 1 DC residuals (channel ci, block x, y), 2 blob_mid, 3 YtoX and 4 YtoB
+// (tile x, y), 5 strategy and 6 quant field (block x, y), 7 EPF zeros.
+// Positioning at item k divides once; advancing is incremental.
+struct DcCursor {
+    int sec, ci, x, y;
+};
+
+__device__ __forceinline__ DcCursor dc_cursor_at(const DcGroupGeo& d, std::uint32_t k) {
+    DcCursor c{};
     const std::uint32_t n{static_cast<std::uint32_t>(d.n)};
-    if (k >= 1 && k < 1 + 3 * n) {
+    const std::uint32_t cwch{static_cast<std::uint32_t>(d.cw * d.ch)};
+    if (k == 0) {
+        c.sec = 0;
+    } else if (k < 1 + 3 * n) {
         const std::uint32_t i{k - 1};
-        const int ci{static_cast<int>(i / n)};
-        const int c{ci == 0 ? 1 : (ci == 1 ? 0 : 2)};
-        const int p{static_cast<int>(i % n)};
-        const int x{p % d.dgw}, y{p / d.dgw};
-        const std::int32_t* pl{dc + static_cast<std::size_t>(c) * geo.nblocks};
-        const std::size_t b{static_cast<std::size_t>(d.by0 + y) * geo.bw + d.bx0 + x};
-        const bool hl{x > 0}, ht{y > 0};
+        const std::uint32_t p{i % n};
+        c.sec = 1;
+        c.ci = static_cast<int>(i / n);
+        c.x = static_cast<int>(p % static_cast<std::uint32_t>(d.dgw));
+        c.y = static_cast<int>(p / static_cast<std::uint32_t>(d.dgw));
+    } else if (k == 1 + 3 * n) {
+        c.sec = 2;
+    } else {
+        const std::uint32_t j{k - (2 + 3 * n)};
+        if (j < 2 * cwch) {
+            const std::uint32_t jj{j < cwch ? j : j - cwch};
+            c.sec = j < cwch ? 3 : 4;
+            c.x = static_cast<int>(jj % static_cast<std::uint32_t>(d.cw));
+            c.y = static_cast<int>(jj / static_cast<std::uint32_t>(d.cw));
+        } else if (j < 2 * cwch + 2 * n) {
+            const std::uint32_t jj{(j - 2 * cwch) % n};
+            c.sec = j < 2 * cwch + n ? 5 : 6;
+            c.x = static_cast<int>(jj % static_cast<std::uint32_t>(d.dgw));
+            c.y = static_cast<int>(jj / static_cast<std::uint32_t>(d.dgw));
+        } else {
+            c.sec = 7;
+        }
+    }
+    return c;
+}
+
+__device__ __forceinline__ void dc_cursor_next(const DcGroupGeo& d, DcCursor& c) {
+    switch (c.sec) {
+        case 0:
+        case 2:
+            ++c.sec;
+            c.ci = 0;
+            c.x = 0;
+            c.y = 0;
+            break;
+        case 1:
+            if (++c.x == d.dgw) {
+                c.x = 0;
+                if (++c.y == d.dgh) {
+                    c.y = 0;
+                    if (++c.ci == 3) {
+                        c.sec = 2;
+                    }
+                }
+            }
+            break;
+        case 3:
+        case 4:
+            if (++c.x == d.cw) {
+                c.x = 0;
+                if (++c.y == d.ch) {
+                    c.y = 0;
+                    ++c.sec;
+                }
+            }
+            break;
+        case 5:
+        case 6:
+            if (++c.x == d.dgw) {
+                c.x = 0;
+                if (++c.y == d.dgh) {
+                    c.y = 0;
+                    ++c.sec;
+                }
+            }
+            break;
+        default:
+            break;
+    }
+}
+
+// Token of the cursor's item (sections 1 and 3-7).  Returns 0 for a DC code,
+// 1 for an AcMetadata code, 2 for an absent item (a covered block's
+// strategy / quant-field slot).
+__device__ __forceinline__ int dc_cursor_token(const Geometry& geo, const DcGroupGeo& d, const DcCursor& c,
+                                               const std::int32_t* dc, const std::int8_t* acs,
+                                               const std::int32_t* qf, const std::int8_t* ytox,
+                                               const std::int8_t* ytob, std::uint32_t& sym, std::uint32_t& nb,
+                                               std::uint32_t& bits) {
+    if (c.sec == 1) {
+        const int ch{c.ci == 0 ? 1 : (c.ci == 1 ? 0 : 2)};
+        const std::int32_t* pl{dc + static_cast<std::size_t>(ch) * geo.nblocks};
+        const std::size_t b{static_cast<std::size_t>(d.by0 + c.y) * geo.bw + d.bx0 + c.x};
+        const bool hl{c.x > 0}, ht{c.y > 0};
         const std::int32_t pred{gradient_predict(hl, ht, hl ? pl[b - 1] : 0, ht ? pl[b - geo.bw] : 0,
                                                  (hl && ht) ? pl[b - geo.bw - 1] : 0)};
         hybrid420(pack_signed(pl[b] - pred), sym, nb, bits);
         return 0;
     }
-    const std::uint32_t j{k - (2 + 3 * n)};
-    const std::uint32_t cwch{static_cast<std::uint32_t>(d.cw * d.ch)};
     std::int32_t v{0};
-    if (j < 2 * cwch) {
-        const std::uint32_t jj{j < cwch ? j : j - cwch};
-        const std::size_t t{static_cast<std::size_t>(d.by0 / 8 + jj / d.cw) * geo.cmw + d.bx0 / 8 + jj % d.cw};
-        v = j < cwch ? ytox[t] : ytob[t];
-    } else if (j < 2 * cwch + 2 * n) {
-        const std::uint32_t jj{(j - 2 * cwch) % n};
-        const std::size_t b{static_cast<std::size_t>(d.by0 + jj / d.dgw) * geo.bw + d.bx0 + jj % d.dgw};
+    if (c.sec == 3 || c.sec == 4) {
+        const std::size_t t{static_cast<std::size_t>(d.by0 / 8 + c.y) * geo.cmw + d.bx0 / 8 + c.x};
+        v = c.sec == 3 ? ytox[t] : ytob[t];
+    } else if (c.sec == 5 || c.sec == 6) {
+        const std::size_t b{static_cast<std::size_t>(d.by0 + c.y) * geo.bw + d.bx0 + c.x};
         const int side{acs[b]};
         if (side == 0) {
             return 2;
         }
-        v = j < 2 * cwch + n ? (side == 16 ? 4 : (side == 32 ? 5 : 0)) : qf[b] - 1;
+        v = c.sec == 5 ? (side == 16 ? 4 : (side == 32 ? 5 : 0)) : qf[b] - 1;
     }
     hybrid420(pack_signed(v), sym, nb, bits);
     return 1;
 }
 
-constexpr int kDcChunk = 4096;  // items per workgroup in the DC passes
+constexpr int kDcChunk = 4096;               // items per workgroup in the DC passes
+constexpr int kDcPerThread = kDcChunk / 256;  // consecutive items per thread
 
 // DC and AcMetadata token histograms per group, plus the first-block count.
 __global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::int32_t* __restrict__ dc,
@@ -620,23 +733,32 @@ __global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::i
         cnt = 0;
     }
     __syncthreads();
-    const std::uint32_t k0{blockIdx.x * kDcChunk};
-    const std::uint32_t mid{1u + 3u * static_cast<std::uint32_t>(d.n)};
-    const std::uint32_t fb_lo{2u + 3u * static_cast<std::uint32_t>(d.n) + 2u * static_cast<std::uint32_t>(d.cw * d.ch)};
-    for (std::uint32_t k{k0 + threadIdx.x}; k < min(k0 + kDcChunk, d.items); k += 256) {
-        if (k == 0 || k == mid) {
-            continue;
-        }
-        std::uint32_t sym, nb, bits;
-        const int kind{dc_item_token(geo, d, k, dc, acs, qf, ytox, ytob, sym, nb, bits)};
-        if (kind == 0) {
-            atomicAdd(&h[sym], 1u);
-        } else if (kind == 1) {
-            atomicAdd(&h[kHistStride + sym], 1u);
-            if (k >= fb_lo && k < fb_lo + static_cast<std::uint32_t>(d.n)) {
-                atomicAdd(&cnt, 1u);
+    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * kDcPerThread};
+    const std::uint32_t k1{min(k0 + kDcPerThread, d.items)};
+    std::uint32_t fb{0};
+    DcCursor c{k0 < k1 ? dc_cursor_at(d, k0) : DcCursor{}};
+    for (std::uint32_t i{0}; i < static_cast<std::uint32_t>(kDcPerThread); ++i) {
+        const std::uint32_t k{k0 + i};
+        bool on{false};
+        int key{0};
+        if (k < k1 && c.sec != 0 && c.sec != 2) {
+            std::uint32_t sym, nb, bits;
+            const int kind{dc_cursor_token(geo, d, c, dc, acs, qf, ytox, ytob, sym, nb, bits)};
+            if (kind != 2) {
+                on = true;
+                key = (kind == 0 ? 0 : kHistStride) + static_cast<int>(sym);
+                fb += (kind == 1 && c.sec == 5) ? 1u : 0u;
             }
         }
+        if (on) {
+            atomicAdd(&h[key], 1u);
+        }
+        if (k < k1) {
+            dc_cursor_next(d, c);
+        }
+    }
+    if (fb != 0) {
+        atomicAdd(&cnt, fb);
     }
     __syncthreads();
     for (int i{static_cast<int>(threadIdx.x)}; i < kHistStride; i += 256) {
@@ -659,24 +781,24 @@ struct DcBlobs {
     std::uint32_t overflow;  // a header blob exceeded its slot
 };
 
-__device__ __forceinline__ std::uint32_t dc_item_bits(const Geometry& geo, const DcGroupGeo& d,
-                                                      std::uint32_t k, int g, const std::int32_t* dc,
-                                                      const std::int8_t* acs, const std::int32_t* qf,
-                                                      const std::int8_t* ytox, const std::int8_t* ytob,
-                                                      const std::uint8_t* dc_depth,
-                                                      const std::uint8_t* meta_depth, const DcBlobs* blobs) {
-    if (k == 0) {
-        return blobs->pre_bits[g];
-    }
-    if (k == 1u + 3u * static_cast<std::uint32_t>(d.n)) {
-        return blobs->mid_bits[g];
-    }
+// Code length and code of the cursor's item (0 bits for an absent item).
+__device__ __forceinline__ std::uint32_t dc_cursor_code(const Geometry& geo, const DcGroupGeo& d,
+                                                        const DcCursor& c, int g, const std::int32_t* dc,
+                                                        const std::int8_t* acs, const std::int32_t* qf,
+                                                        const std::int8_t* ytox, const std::int8_t* ytob,
+                                                        const std::uint8_t* dc_depth, const std::uint16_t* dc_codes,
+                                                        const std::uint8_t* meta_depth,
+                                                        const std::uint16_t* meta_codes, std::uint32_t& val) {
     std::uint32_t sym, nb, bits;
-    const int kind{dc_item_token(geo, d, k, dc, acs, qf, ytox, ytob, sym, nb, bits)};
+    const int kind{dc_cursor_token(geo, d, c, dc, acs, qf, ytox, ytob, sym, nb, bits)};
     if (kind == 2) {
+        val = 0;
         return 0;
     }
-    return (kind == 0 ? dc_depth : meta_depth)[g * kHistStride + sym] + nb;
+    const std::uint32_t idx{static_cast<std::uint32_t>(g) * kHistStride + sym};
+    const std::uint32_t dd{(kind == 0 ? dc_depth : meta_depth)[idx]};
+    val = codes_or_zero(kind == 0 ? dc_codes : meta_codes, idx) | (bits << dd);
+    return dd + nb;
 }
 
 // Chunk bit totals of every DC group (grid: chunks x groups).
@@ -689,10 +811,20 @@ __global__ __launch_bounds__(256) void dc_chunk_bits_kernel(
     __shared__ std::uint64_t sh[5];
     const int g{static_cast<int>(blockIdx.y)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
-    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * (kDcChunk / 256)};
+    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * kDcPerThread};
+    const std::uint32_t k1{min(k0 + kDcPerThread, d.items)};
     std::uint64_t run{0};
-    for (std::uint32_t k{k0}; k < min(k0 + kDcChunk / 256, d.items); ++k) {
-        run += dc_item_bits(geo, d, k, g, dc, acs, qf, ytox, ytob, dc_depth, meta_depth, blobs);
+    if (k0 < k1) {
+        DcCursor c{dc_cursor_at(d, k0)};
+        for (std::uint32_t k{k0}; k < k1; ++k, dc_cursor_next(d, c)) {
+            if (c.sec == 0 || c.sec == 2) {
+                run += c.sec == 0 ? blobs->pre_bits[g] : blobs->mid_bits[g];
+                continue;
+            }
+            std::uint32_t val;
+            run += dc_cursor_code(geo, d, c, g, dc, acs, qf, ytox, ytob, dc_depth, nullptr, meta_depth, nullptr,
+                                  val);
+        }
     }
     std::uint64_t total{0};
     block_excl_scan(run, sh, total);
@@ -726,6 +858,12 @@ __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::
     }
 }
 
+// Emission of one chunk of a DC group: the chunk's items occupy one
+// contiguous bit range, assembled in LDS (interior words plain-stored, the two
+// boundary words ORed into the zeroed output); a range longer than the LDS
+// copy is emitted with global atomics instead.
+constexpr int kDcEmitWords = 2048;
+
 __global__ __launch_bounds__(256) void dc_emit_kernel(
     Geometry geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
     const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
@@ -736,43 +874,86 @@ __global__ __launch_bounds__(256) void dc_emit_kernel(
     std::uint32_t chunks_per_group, const std::uint64_t* __restrict__ dc_group_base_bits,
     std::uint32_t* __restrict__ out_words, std::uint64_t cap_words) {
     __shared__ std::uint64_t sh[5];
+    __shared__ std::uint32_t buf[kDcEmitWords];
     const int g{static_cast<int>(blockIdx.y)};
+    const int tid{static_cast<int>(threadIdx.x)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
-    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * (kDcChunk / 256)};
-    const std::uint32_t k1{min(k0 + kDcChunk / 256, d.items)};
+    if (blockIdx.x * kDcChunk >= d.items) {
+        return;
+    }
+    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * kDcPerThread};
+    const std::uint32_t k1{min(k0 + kDcPerThread, d.items)};
     std::uint64_t run{0};
-    for (std::uint32_t k{k0}; k < k1; ++k) {
-        run += dc_item_bits(geo, d, k, g, dc, acs, qf, ytox, ytob, dc_depth, meta_depth, blobs);
+    if (k0 < k1) {
+        DcCursor c{dc_cursor_at(d, k0)};
+        for (std::uint32_t k{k0}; k < k1; ++k, dc_cursor_next(d, c)) {
+            if (c.sec == 0 || c.sec == 2) {
+                run += c.sec == 0 ? blobs->pre_bits[g] : blobs->mid_bits[g];
+                continue;
+            }
+            std::uint32_t val;
+            run += dc_cursor_code(geo, d, c, g, dc, acs, qf, ytox, ytob, dc_depth, nullptr, meta_depth, nullptr,
+                                  val);
+        }
     }
     std::uint64_t total{0};
-    std::uint64_t pos{block_excl_scan(run, sh, total)};
-    pos += dc_group_base_bits[g] + chunk_off[static_cast<std::size_t>(g) * chunks_per_group + blockIdx.x];
-    const std::uint32_t mid{1u + 3u * static_cast<std::uint32_t>(d.n)};
-    for (std::uint32_t k{k0}; k < k1; ++k) {
-        if (k == 0 || k == mid) {
-            const std::uint32_t nbits{k == 0 ? blobs->pre_bits[g] : blobs->mid_bits[g]};
-            const std::uint32_t* src{blob_words + (k == 0 ? blobs->pre_off[g] : blobs->mid_off[g])};
-            std::uint32_t i{0};
-            for (; i + 32 <= nbits; i += 32) {
-                or_bits(out_words, cap_words, pos + i, src[i >> 5], 32);
+    const std::uint64_t rel0{block_excl_scan(run, sh, total)};
+    const std::uint64_t start{dc_group_base_bits[g] + chunk_off[static_cast<std::size_t>(g) * chunks_per_group +
+                                                                 blockIdx.x]};
+    const std::uint64_t w0{start >> 5};
+    const std::uint64_t nw{((start + total + 31) >> 5) - w0};
+    const bool staged{nw <= static_cast<std::uint64_t>(kDcEmitWords)};
+    if (staged) {
+        for (int i{tid}; i < static_cast<int>(nw); i += 256) {
+            buf[i] = 0;
+        }
+    }
+    __syncthreads();
+    const std::uint64_t lbase{w0 << 5};
+    auto put = [&](std::uint64_t at, std::uint32_t v, std::uint32_t len) {
+        if (staged) {
+            lds_or_bits(buf, kDcEmitWords, static_cast<std::uint32_t>(at - lbase), v, len);
+        } else {
+            or_bits(out_words, cap_words, at, v, len);
+        }
+    };
+    std::uint64_t pos{start + rel0};
+    if (k0 < k1) {
+        DcCursor c{dc_cursor_at(d, k0)};
+        for (std::uint32_t k{k0}; k < k1; ++k, dc_cursor_next(d, c)) {
+            if (c.sec == 0 || c.sec == 2) {
+                const std::uint32_t nbits{c.sec == 0 ? blobs->pre_bits[g] : blobs->mid_bits[g]};
+                const std::uint32_t* src{blob_words + (c.sec == 0 ? blobs->pre_off[g] : blobs->mid_off[g])};
+                for (std::uint32_t i{0}; i < nbits; i += 32) {
+                    put(pos + i, src[i >> 5], nbits - i < 32 ? nbits - i : 32u);
+                }
+                pos += nbits;
+                continue;
             }
-            if (i < nbits) {
-                or_bits(out_words, cap_words, pos + i, src[i >> 5], nbits - i);
-            }
-            pos += nbits;
+            std::uint32_t val;
+            const std::uint32_t len{dc_cursor_code(geo, d, c, g, dc, acs, qf, ytox, ytob, dc_depth, dc_codes,
+                                                   meta_depth, meta_codes, val)};
+            put(pos, val, len);
+            pos += len;
+        }
+    }
+    if (!staged) {
+        return;
+    }
+    __syncthreads();
+    for (int i{tid}; i < static_cast<int>(nw); i += 256) {
+        const std::uint64_t gi{w0 + static_cast<std::uint64_t>(i)};
+        if (gi >= cap_words) {
             continue;
         }
-        std::uint32_t sym, nb, bits;
-        const int kind{dc_item_token(geo, d, k, dc, acs, qf, ytox, ytob, sym, nb, bits)};
-        if (kind == 2) {
-            continue;
+        const std::uint32_t v{buf[i]};
+        if (i == 0 || i == static_cast<int>(nw) - 1) {
+            if (v != 0) {
+                atomicOr(&out_words[gi], v);
+            }
+        } else {
+            out_words[gi] = v;
         }
-        const std::uint8_t* dep{kind == 0 ? dc_depth : meta_depth};
-        const std::uint16_t* cod{kind == 0 ? dc_codes : meta_codes};
-        const std::uint32_t dd{dep[g * kHistStride + sym]};
-        or_bits(out_words, cap_words, pos, static_cast<std::uint32_t>(cod[g * kHistStride + sym]) | (bits << dd),
-                dd + nb);
-        pos += dd + nb;
     }
 }
 

@@ -235,8 +235,9 @@ struct RegionShared {
 
 // Selection terms of one Y coefficient (transform_select_impl.cuh:49-108):
 // squared quantization error and rate proxy; LLF coefficients contribute 0.
-__device__ __forceinline__ void selection_terms(float s, int e, int n, const float* wsel, float qgsf,
-                                                float& d, float& rt) {
+// The two quotients are the correctly rounded ones, by reciprocals.
+__device__ __forceinline__ void selection_terms(float s, int e, int n, const float* wsel, const float* wsel_inv,
+                                                const FrameParams& prm, float& d, float& rt) {
     const int m{n / 8};
     if (e / n < m && e % n < m) {
         d = 0.0f;
@@ -244,8 +245,8 @@ __device__ __forceinline__ void selection_terms(float s, int e, int n, const flo
         return;
     }
     const float wq{wsel[e]};
-    const float q{__builtin_rintf(s * qgsf / wq)};
-    const float err{q * wq / qgsf - s};
+    const float q{__builtin_rintf(div_by_recip(s * prm.qgsf_select, wq, wsel_inv[e]))};
+    const float err{div_by_recip(q * wq, prm.qgsf_select, prm.inv_qgsf_select) - s};
     d = err * err;
     const float aq{fabsf(q)};
     rt = aq < 256.0f ? g_tables.rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
@@ -407,7 +408,6 @@ __global__ __launch_bounds__(256) void region_kernel(
     float* const lines{S.u.scratch};
     float* const dsc{S.u.scratch + 16 * RowBuf<8>::BS};
     float* const rsc{S.u.scratch + 16 * RowBuf<8>::BS + 1024};
-    const float qgsf{prm.qgsf_select};
     for (int sz{0}; sz < 3; ++sz) {
         const int n{8 << sz};
         if (n == 8) {
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             for (int f{0}; f < 4; ++f) {
                 const int e{fx * 8 + k0 + f};
                 float d, rt;
-                selection_terms(o[f], e, 8, g_tables.wsel8, qgsf, d, rt);
+                selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, prm, d, rt);
                 dsc[base + e] = d;
                 rsc[base + e] = rt;
                 S.coefy[0][base + e] = static_cast<_Float16>(o[f]);
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             for (int f{0}; f < 4; ++f) {
                 const int e{fx * 16 + k0 + f};
                 float d, rt;
-                selection_terms(o[f], e, 16, g_tables.wsel16, qgsf, d, rt);
+                selection_terms(o[f], e, 16, g_tables.wsel16, g_tables.wsel_inv16, prm, d, rt);
                 dsc[256 * q + e] = d;
                 rsc[256 * q + e] = rt;
                 S.coefy[1][256 * q + e] = static_cast<_Float16>(o[f]);
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(256) void region_kernel(
                 for (int f{0}; f < 8; ++f) {
                     const int e{lane * 32 + k0 + f};
                     float d, rt;
-                    selection_terms(o[f], e, 32, g_tables.wsel32, qgsf, d, rt);
+                    selection_terms(o[f], e, 32, g_tables.wsel32, g_tables.wsel_inv32, prm, d, rt);
                     dsc[e] = d;
                     rsc[e] = rt;
                     S.coefy[2][e] = static_cast<_Float16>(o[f]);
@@ -829,14 +829,18 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
         } else {
             const float qgsf{static_cast<float>(qf[static_cast<std::size_t>(oy) * bw + ox]) *
                              prm.global_scale_float};
+            // Quotients by the table weights are the correctly rounded ones, by
+            // reciprocals; y_round divides by the block's own qgsf.
             const float wy{dequant_of(n, 1, e)};
-            const int qy{static_cast<int>(__builtin_rintf(yr * qgsf / wy))};
+            const int qy{static_cast<int>(__builtin_rintf(div_by_recip(yr * qgsf, wy, dequant_inv_of(n, 1, e))))};
             const float y_round{static_cast<float>(qy) * wy / qgsf};
             const float wx{dequant_of(n, 0, e)};
             const float wb{dequant_of(n, 2, e)};
-            q0 = static_cast<std::int16_t>(static_cast<int>(__builtin_rintf((xr - yx * y_round) * qgsf / wx)));
+            q0 = static_cast<std::int16_t>(static_cast<int>(
+                __builtin_rintf(div_by_recip((xr - yx * y_round) * qgsf, wx, dequant_inv_of(n, 0, e)))));
             q1 = static_cast<std::int16_t>(qy);
-            q2 = static_cast<std::int16_t>(static_cast<int>(__builtin_rintf((br - yb * y_round) * qgsf / wb)));
+            q2 = static_cast<std::int16_t>(static_cast<int>(
+                __builtin_rintf(div_by_recip((br - yb * y_round) * qgsf, wb, dequant_inv_of(n, 2, e)))));
         }
         ac[slot] = q0;
         ac[cplane + slot] = q1;

@@ -69,6 +69,81 @@ struct WaveHuffScratch {
     std::uint32_t sorted_symbol[256];
 };
 
+// Two-queue merge and top-down depths of the n (>= 2) sorted leaves in
+// s.sorted_total / s.sorted_symbol; writes depth[symbol] and returns this
+// lane's largest leaf depth.  Q * 64 >= n.
+template <int Q>
+__device__ std::uint32_t merge_depths(const WaveHuffScratch& s, int n, std::uint8_t* depth) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    LaneArray<Q> leaf{}, sym{};
+#pragma unroll
+    for (int r{0}; r < Q; ++r) {
+        const int t{64 * r + lane};
+        leaf.v[r] = t < n ? s.sorted_total[t] : kQueueEnd;
+        sym.v[r] = t < n ? s.sorted_symbol[t] : 0u;
+    }
+    // Two-queue merge: leaves (tree index i < n) and internal nodes
+    // (internal k = tree index n + 1 + k), smaller total first, leaf on ties.
+    LaneArray<Q> inner{}, kid_l{}, kid_r{};
+    int i{0}, j{0};
+    for (int k{0}; k < n - 1; ++k) {
+        std::uint32_t a{i < n ? leaf.get(i) : kQueueEnd};
+        std::uint32_t b{j < k ? inner.get(j) : kQueueEnd};
+        int left, right;
+        std::uint32_t lt, rt;
+        if (a <= b) {
+            left = i;
+            lt = a;
+            ++i;
+            a = i < n ? leaf.get(i) : kQueueEnd;
+        } else {
+            left = n + 1 + j;
+            lt = b;
+            ++j;
+            b = j < k ? inner.get(j) : kQueueEnd;
+        }
+        if (a <= b) {
+            right = i;
+            rt = a;
+            ++i;
+        } else {
+            right = n + 1 + j;
+            rt = b;
+            ++j;
+        }
+        inner.set(k, lt + rt);
+        kid_l.set(k, static_cast<std::uint32_t>(left));
+        kid_r.set(k, static_cast<std::uint32_t>(right));
+    }
+    // Depths top-down from the root (internal n - 2): every child was
+    // created before its parent.
+    LaneArray<Q> node_depth{}, leaf_depth{};
+    node_depth.set(n - 2, 0u);
+    for (int k{n - 2}; k >= 0; --k) {
+        const std::uint32_t d{node_depth.get(k) + 1u};
+        const int cl{static_cast<int>(kid_l.get(k))}, cr{static_cast<int>(kid_r.get(k))};
+        if (cl > n) {
+            node_depth.set(cl - n - 1, d);
+        } else {
+            leaf_depth.set(cl, d);
+        }
+        if (cr > n) {
+            node_depth.set(cr - n - 1, d);
+        } else {
+            leaf_depth.set(cr, d);
+        }
+    }
+    std::uint32_t maxd{0};
+#pragma unroll
+    for (int r{0}; r < Q; ++r) {
+        if (64 * r + lane < n) {
+            depth[sym.v[r]] = static_cast<std::uint8_t>(leaf_depth.v[r]);
+            maxd = leaf_depth.v[r] > maxd ? leaf_depth.v[r] : maxd;
+        }
+    }
+    return maxd;
+}
+
 // create_huffman_tree (prefix_code.cc:36-96).  data[0..length) and
 // depth[0..length) are in LDS; depth is zero on entry for absent symbols.
 // R * 64 >= length.
@@ -140,72 +215,7 @@ __device__ void wave_create_tree(const std::uint32_t* data, int length, int limi
             }
         }
         __syncthreads();
-        LaneArray<R> leaf{}, sym{};
-#pragma unroll
-        for (int r{0}; r < R; ++r) {
-            const int t{64 * r + lane};
-            leaf.v[r] = t < n ? s.sorted_total[t] : kQueueEnd;
-            sym.v[r] = t < n ? s.sorted_symbol[t] : 0u;
-        }
-        // Two-queue merge: leaves (tree index i < n) and internal nodes
-        // (internal k = tree index n + 1 + k), smaller total first, leaf on ties.
-        LaneArray<R> inner{}, kid_l{}, kid_r{};
-        int i{0}, j{0};
-        for (int k{0}; k < n - 1; ++k) {
-            std::uint32_t a{i < n ? leaf.get(i) : kQueueEnd};
-            std::uint32_t b{j < k ? inner.get(j) : kQueueEnd};
-            int left, right;
-            std::uint32_t lt, rt;
-            if (a <= b) {
-                left = i;
-                lt = a;
-                ++i;
-                a = i < n ? leaf.get(i) : kQueueEnd;
-            } else {
-                left = n + 1 + j;
-                lt = b;
-                ++j;
-                b = j < k ? inner.get(j) : kQueueEnd;
-            }
-            if (a <= b) {
-                right = i;
-                rt = a;
-                ++i;
-            } else {
-                right = n + 1 + j;
-                rt = b;
-                ++j;
-            }
-            inner.set(k, lt + rt);
-            kid_l.set(k, static_cast<std::uint32_t>(left));
-            kid_r.set(k, static_cast<std::uint32_t>(right));
-        }
-        // Depths top-down from the root (internal n - 2): every child was
-        // created before its parent.
-        LaneArray<R> node_depth{}, leaf_depth{};
-        node_depth.set(n - 2, 0u);
-        for (int k{n - 2}; k >= 0; --k) {
-            const std::uint32_t d{node_depth.get(k) + 1u};
-            const int cl{static_cast<int>(kid_l.get(k))}, cr{static_cast<int>(kid_r.get(k))};
-            if (cl > n) {
-                node_depth.set(cl - n - 1, d);
-            } else {
-                leaf_depth.set(cl, d);
-            }
-            if (cr > n) {
-                node_depth.set(cr - n - 1, d);
-            } else {
-                leaf_depth.set(cr, d);
-            }
-        }
-        std::uint32_t maxd{0};
-#pragma unroll
-        for (int r{0}; r < R; ++r) {
-            if (64 * r + lane < n) {
-                depth[sym.v[r]] = static_cast<std::uint8_t>(leaf_depth.v[r]);
-                maxd = leaf_depth.v[r] > maxd ? leaf_depth.v[r] : maxd;
-            }
-        }
+        std::uint32_t maxd{n <= 64 ? merge_depths<1>(s, n, depth) : merge_depths<4>(s, n, depth)};
         maxd = wave_max_u32(maxd);
         __syncthreads();
         if (static_cast<int>(maxd) <= limit) {
@@ -251,24 +261,6 @@ __device__ void wave_depths_to_codes(const std::uint8_t* depth, int len, std::ui
             }
             base += static_cast<std::uint32_t>(__popcll(m));
         }
-    }
-}
-
-// ORs `len` (<= 32) bits at bit `pos` of an LDS word buffer of `cap` words.
-__device__ __forceinline__ void lds_or_bits(std::uint32_t* words, std::uint32_t cap, std::uint32_t pos,
-                                            std::uint32_t value, std::uint32_t len) {
-    if (len == 0) {
-        return;
-    }
-    if (len < 32) {
-        value &= (1u << len) - 1u;
-    }
-    const std::uint32_t wi{pos >> 5}, b{pos & 31u};
-    if (wi < cap) {
-        atomicOr(&words[wi], value << b);
-    }
-    if (b + len > 32 && wi + 1 < cap) {
-        atomicOr(&words[wi + 1], value >> (32 - b));
     }
 }
 

@@ -87,6 +87,18 @@ CJX_HD float cbrt_det(float x) {
     return c;
 }
 
+// a / b correctly rounded, given r = RN(1 / b): two Markstein correction
+// steps (the first makes the quotient faithful, the second rounds it
+// correctly).  Valid when a / b and the remainders stay normal; the sign of
+// a zero quotient is not kept.
+CJX_HD float div_by_recip(float a, float b, float r) {
+    float q{a * r};
+    float e{__builtin_fmaf(-b, q, a)};
+    q = __builtin_fmaf(e, r, q);
+    e = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(e, r, q);
+}
+
 CJX_HD float pow24_det(float x) {
     return exp2_det(2.4f * log2_det(x));
 }

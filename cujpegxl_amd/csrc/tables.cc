@@ -91,6 +91,17 @@ DeviceTables build_tables() {
     natural_order(32, t.order32);
     llf_table(2, t.llf2);
     llf_table(4, t.llf4);
+    auto recip = [](const float* w, float* r, int n) {
+        for (int i{0}; i < n; ++i) {
+            r[i] = 1.0f / w[i];
+        }
+    };
+    recip(t.wsel8, t.wsel_inv8, 64);
+    recip(t.wsel16, t.wsel_inv16, 256);
+    recip(t.wsel32, t.wsel_inv32, 1024);
+    recip(&t.dequant8[0][0], &t.dequant_inv8[0][0], 3 * 64);
+    recip(&t.dequant16[0][0], &t.dequant_inv16[0][0], 3 * 256);
+    recip(&t.dequant32[0][0], &t.dequant_inv32[0][0], 3 * 1024);
     t.rate[0] = 0.0f;
     for (int k{1}; k < 256; ++k) {
         t.rate[k] = 1.0f + log2_det(1.0f + static_cast<float>(k));
@@ -133,6 +144,7 @@ FrameParams make_frame_params(float distance) {
     p.global_scale_float = static_cast<float>(global_scale) * (1.0f / (1 << 16));
     p.inv_global_scale = inv_global_scale;
     p.qgsf_select = static_cast<float>(p.raw_quant_field) * p.global_scale_float;
+    p.inv_qgsf_select = 1.0f / p.qgsf_select;
     p.dc_scale = p.global_scale_float * static_cast<float>(p.quant_dc);
 
     const float base_level{0.48f * quant_ac};
