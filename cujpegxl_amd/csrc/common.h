@@ -95,7 +95,7 @@ struct DeviceTables {
     float rate[256];       // selection rate term 1 + log2(1 + k) for |q| = k >= 1; 0 for k = 0
 };
 
-extern __constant__ DeviceTables g_tables;
+extern __constant__ __align__(16) DeviceTables g_tables;
 
 // Host mirror of the tables (same values), built once.
 const DeviceTables& host_tables();

@@ -9,7 +9,7 @@
 #include "prefix.h"
 
 namespace cjx {
-__constant__ DeviceTables g_tables;
+__constant__ __align__(16) DeviceTables g_tables;
 }  // namespace cjx
 
 #include "frontend.inl"

@@ -161,9 +161,34 @@ __device__ __forceinline__ AcItem ac_item(const Geometry& geo, const std::int8_t
 // a chunk are a prefix of the half.  A lane's remaining-non-zero count and
 // previous-coefficient flag (entropy.cu:648-681, ac_context.h:78-84) come
 // from popcounts of the half's ballot bits below it.
+// The tokenizer's lookup tables, staged in LDS once per workgroup (the
+// lookups are per-lane gathers).
+struct TokenTables {
+    std::uint16_t order8[64];
+    std::uint16_t order16[256];
+    std::uint16_t order32[1024];
+    std::uint16_t nz_ctx[64];
+    std::uint16_t freq_ctx[64];
+};
+
+__device__ __forceinline__ void load_token_tables(TokenTables& T) {
+    for (int i{static_cast<int>(threadIdx.x)}; i < 1024; i += static_cast<int>(blockDim.x)) {
+        T.order32[i] = g_tables.order32[i];
+        if (i < 256) {
+            T.order16[i] = g_tables.order16[i];
+        }
+        if (i < 64) {
+            T.order8[i] = g_tables.order8[i];
+            T.nz_ctx[i] = c_nz_ctx[i];
+            T.freq_ctx[i] = c_freq_ctx[i];
+        }
+    }
+}
+
 template <typename Emit>
 __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const std::int16_t* ac,
-                                                   const AcItem& it, bool has, int lane, Emit&& emit) {
+                                                   const TokenTables& T, const AcItem& it, bool has, int lane,
+                                                   Emit&& emit) {
     const int half{lane >> 5}, sub{lane & 31};
     {
         std::uint32_t sym{0}, nb{0}, bits{0};
@@ -177,7 +202,7 @@ __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const st
     const std::uint32_t l2{n == 16 ? 2u : (n == 32 ? 4u : 0u)};
     const std::uint32_t off{15u * 37u + 458u * static_cast<std::uint32_t>(it.bctx)};
     const std::int16_t* plane{ac + static_cast<std::size_t>(it.c) * geo.nblocks * 64};
-    const std::uint16_t* order{order_of(n)};
+    const std::uint16_t* order{n == 32 ? T.order32 : (n == 16 ? T.order16 : T.order8)};
     std::uint32_t seen{0};
     std::uint32_t prev_carry{it.nz > size / 16 ? 0u : 1u};
     const std::uint32_t below{(1u << sub) - 1u};
@@ -203,7 +228,7 @@ __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const st
         if (valid) {
             const std::uint32_t rem{((it.nz - before) + cb - 1) >> l2};
             const std::uint32_t kk{k >> l2};
-            const std::uint32_t ctx{off + (c_nz_ctx[rem] + c_freq_ctx[kk]) * 2 + prev};
+            const std::uint32_t ctx{off + (T.nz_ctx[rem] + T.freq_ctx[kk]) * 2u + prev};
             cl = static_cast<std::uint32_t>(ac_cluster(ctx));
             hybrid420(pack_signed(v), sym, nb, bits);
         }
@@ -270,9 +295,11 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
                                                         std::uint16_t* __restrict__ item_tokens,
                                                         std::uint32_t* __restrict__ partial) {
     __shared__ std::uint32_t h[kNumClusters * kAcSymbols];
+    __shared__ TokenTables T;
     for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
         h[i] = 0;
     }
+    load_token_tables(T);
     __syncthreads();
     const int lane{static_cast<int>(threadIdx.x) & 63};
     const int half{lane >> 5};
@@ -286,7 +313,7 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
         const bool has{in && it.n != 0};
         const int m{has ? it.n / 8 : 1};
         std::uint32_t count{0};
-        tokenize_item_pair(geo, ac, it, has, lane,
+        tokenize_item_pair(geo, ac, T, it, has, lane,
                            [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
                                std::uint32_t extra) {
                                if (valid) {
@@ -733,28 +760,21 @@ __global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::i
         cnt = 0;
     }
     __syncthreads();
-    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * kDcPerThread};
-    const std::uint32_t k1{min(k0 + kDcPerThread, d.items)};
+    // Lane-strided items (neighbouring lanes read neighbouring blocks); the
+    // histogram does not depend on the order.
+    const std::uint32_t k0{blockIdx.x * kDcChunk};
+    const std::uint32_t k1{min(k0 + static_cast<std::uint32_t>(kDcChunk), d.items)};
     std::uint32_t fb{0};
-    DcCursor c{k0 < k1 ? dc_cursor_at(d, k0) : DcCursor{}};
-    for (std::uint32_t i{0}; i < static_cast<std::uint32_t>(kDcPerThread); ++i) {
-        const std::uint32_t k{k0 + i};
-        bool on{false};
-        int key{0};
-        if (k < k1 && c.sec != 0 && c.sec != 2) {
-            std::uint32_t sym, nb, bits;
-            const int kind{dc_cursor_token(geo, d, c, dc, acs, qf, ytox, ytob, sym, nb, bits)};
-            if (kind != 2) {
-                on = true;
-                key = (kind == 0 ? 0 : kHistStride) + static_cast<int>(sym);
-                fb += (kind == 1 && c.sec == 5) ? 1u : 0u;
-            }
+    for (std::uint32_t k{k0 + threadIdx.x}; k < k1; k += 256) {
+        const DcCursor c{dc_cursor_at(d, k)};
+        if (c.sec == 0 || c.sec == 2) {
+            continue;
         }
-        if (on) {
-            atomicAdd(&h[key], 1u);
-        }
-        if (k < k1) {
-            dc_cursor_next(d, c);
+        std::uint32_t sym, nb, bits;
+        const int kind{dc_cursor_token(geo, d, c, dc, acs, qf, ytox, ytob, sym, nb, bits)};
+        if (kind != 2) {
+            atomicAdd(&h[(kind == 0 ? 0 : kHistStride) + static_cast<int>(sym)], 1u);
+            fb += (kind == 1 && c.sec == 5) ? 1u : 0u;
         }
     }
     if (fb != 0) {

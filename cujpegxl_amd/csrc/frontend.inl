@@ -195,6 +195,34 @@ __device__ __forceinline__ void dct_line(const float* __restrict__ src, int k0, 
     }
 }
 
+// Separable DCT lines on the matrix cores.  One K=1 step of
+// v_mfma_f32_16x16x1_4b_f32 adds A[b][i] * B[b][j] to D[b][i][j] for four
+// independent 16x16 blocks b, rounding like fmaf (the hardware's result for
+// every K=1 step equals fmaf(a, b, d) bit for bit), so N steps over c
+// reproduce dct_line's fmaf chain exactly.  Lanes 16b..16b+15 supply block b:
+// lane 16b + t gives A[b][t] = a_row[c] and B[b][t] = b_row[c].  On return
+// lane l holds D[b][4 (l / 16) + u][l % 16] in element 4 b + u.
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+template <int N>
+__device__ __forceinline__ v16f mfma_lines(const float* __restrict__ a_row, const float* __restrict__ b_row) {
+    v16f acc;
+#pragma unroll
+    for (int i{0}; i < 16; ++i) {
+        acc[i] = 0.0f;
+    }
+#pragma unroll
+    for (int c{0}; c < N; c += 4) {
+        const float4 a{*reinterpret_cast<const float4*>(a_row + c)};
+        const float4 b{*reinterpret_cast<const float4*>(b_row + c)};
+        acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.x, b.x, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.y, b.y, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.z, b.z, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.w, b.w, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 // Row-pass buffer geometry (floats): line (block b, frequency k) of size n
 // starts at b * BS + k * RS.  RS is a multiple of 4 (16-byte loads) chosen so
 // that eight consecutive lines start in eight different 4-bank groups.
@@ -214,17 +242,24 @@ struct RowBuf<32> {
 };
 
 constexpr int kSP = 36;  // sharpened-plane row stride (floats)
+// Row-pass lines of the three selection sizes side by side.
+constexpr int kLines32 = 0;
+constexpr int kLines16 = kLines32 + RowBuf<32>::BS;
+constexpr int kLines8 = kLines16 + 4 * RowBuf<16>::BS;
+constexpr int kScratch = kLines8 + 16 * RowBuf<8>::BS;
 
 struct RegionShared {
     union {
         float xyb[3][kTileW * kTileW];  // NV12 -> XYB tile with halo (steps 1-3)
-        float scratch[3 * kTileW * kTileW];  // row-pass lines and selection costs (steps 4-5)
+        float scratch[kScratch];        // row-pass lines and selection costs (steps 4-5)
     } u;
     alignas(16) float sharp[3][32 * kSP];
     _Float16 coefy[3][1024];   // Y coefficients of each candidate size, element order
     _Float16 coefxb[2][1024];  // X and B coefficients of the chosen transforms
     float mask[1024];
     float cells[64];
+    float ero[64];         // fuzzy erosion of each cell
+    float blk[16][3];      // per block: HF sum, gamma, blue sum
     float part[4][4][2];   // [wave][quad][d|r]
     float cost8[16];       // quad-major
     float cost16[4];
@@ -353,12 +388,16 @@ __global__ __launch_bounds__(256) void region_kernel(
     __syncthreads();
 
     // 3b. Per-block modulation and the quant integer (frontend_fused.cu:117-203).
+    //     The 64 cells' fuzzy erosion and the 16 blocks' final step each run
+    //     once, in parallel lanes; the per-pixel sums are wave sums as before
+    //     (a block's four erosion terms sum as (e0 + e2) + (e1 + e3), which is
+    //     the wave tree over four non-zero lanes).
+    if (tid < 64) {
+        S.ero[tid] = erosion_cell(S.cells, tid & 7, tid >> 3, prm);
+    }
     for (int j{0}; j < 4; ++j) {
         const int lb{j * 4 + w};
         const int lbx{lb & 3}, lby{lb >> 2};
-        if (bx0 + lbx >= bw || by0 + lby >= bh) {
-            continue;
-        }
         const int fx{lane & 7}, fy{lane >> 3};
         const int idx{(lby * 8 + fy + kHalo) * kTileW + lbx * 8 + fx + kHalo};
         const float yv{S.u.xyb[1][idx]}, xv{S.u.xyb[0][idx]}, bv{S.u.xyb[2][idx]};
@@ -373,14 +412,24 @@ __global__ __launch_bounds__(256) void region_kernel(
         const float gm{ratio_cbrt_gamma(iny - xv, true) + ratio_cbrt_gamma(iny + xv, true)};
         const float pye{yv + 0.084381641171960495f + fabsf(xv)};
         const float bl{bv > pye ? fminf(bv - pye, 0.027121074570634722f) : 0.0f};
-        const float er{lane < 4 ? erosion_cell(S.cells, 2 * lbx + (lane & 1), 2 * lby + (lane >> 1),
-                                                prm)
-                                : 0.0f};
         const float hf_sum{wave_sum(hf)};
         const float gamma_overall{wave_sum(gm) * (0.5f / 64.0f)};
-        float blue_sum{wave_sum(bl)};
-        const float aq{wave_sum(er)};
+        const float blue_sum{wave_sum(bl)};
         if (lane == 0) {
+            S.blk[lb][0] = hf_sum;
+            S.blk[lb][1] = gamma_overall;
+            S.blk[lb][2] = blue_sum;
+        }
+    }
+    __syncthreads();
+    if (tid < 16) {
+        const int lb{tid};
+        const int lbx{lb & 3}, lby{lb >> 2};
+        if (bx0 + lbx < bw && by0 + lby < bh) {
+            const float e0{S.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.ero[(2 * lby) * 8 + 2 * lbx + 1]};
+            const float e2{S.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
+            const float aq{(e0 + e2) + (e1 + e3)};
+            float blue_sum{S.blk[lb][2]};
             constexpr float k_limit{0.027121074570634722f};
             if (blue_sum >= 32.0f * k_limit) {
                 blue_sum = 64.0f * k_limit - blue_sum;
@@ -390,8 +439,8 @@ __global__ __launch_bounds__(256) void region_kernel(
                 blue_sum = k_max_limit * k_limit;
             }
             float ov{compute_mask(aq)};
-            ov += hf_sum * -0.38f + 0.42f;
-            ov += 0.1005613337192697f * log2_det(gamma_overall);
+            ov += S.blk[lb][0] * -0.38f + 0.42f;
+            ov += 0.1005613337192697f * log2_det(S.blk[lb][1]);
             ov += blue_sum * 0.14207000358439159f;
             const float qval{exp_det(ov) * prm.mul_pb + prm.add_pb};
             int qi{static_cast<int>(qval * prm.inv_global_scale + 0.5f)};
@@ -401,84 +450,110 @@ __global__ __launch_bounds__(256) void region_kernel(
     }
 
     // 4. RD transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
-    //    Each size: row pass, column pass (the Y coefficients go to coefy in
-    //    element order), selection terms scattered to element order, then the
-    //    per-block / per-quad / per-region sums over the element layout.
+    //    The three sizes run side by side: wave 0 the DCT32 and wave 1 the four
+    //    DCT16s on the matrix cores, waves 2-3 the sixteen DCT8s on the VALU.
+    //    Each wave keeps its coefficients' selection terms in registers; per
+    //    size, the terms are scattered to element order and summed over the
+    //    element layout as before.
     __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
     float* const lines{S.u.scratch};
-    float* const dsc{S.u.scratch + 16 * RowBuf<8>::BS};
-    float* const rsc{S.u.scratch + 16 * RowBuf<8>::BS + 1024};
+    float* const dsc{S.u.scratch};
+    float* const rsc{S.u.scratch + 1024};
+    const int grp{lane >> 4}, t16{lane & 15};
+    if (wu == 0) {
+        // DCT32 as four 16x16 quadrants: block (fh, rh) = (grp >> 1, grp & 1).
+        const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                      &S.sharp[1][(16 * (grp & 1) + t16) * kSP])};
+#pragma unroll
+        for (int b{0}; b < 4; ++b) {
+#pragma unroll
+            for (int u{0}; u < 4; ++u) {
+                const int f{16 * (b >> 1) + 4 * grp + u}, r{16 * (b & 1) + t16};
+                lines[kLines32 + f * RowBuf<32>::RS + r] = acc[4 * b + u];
+            }
+        }
+    } else if (wu == 1) {
+        // Block q = quad q.
+        const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                      &S.sharp[1][((grp >> 1) * 16 + t16) * kSP + (grp & 1) * 16])};
+#pragma unroll
+        for (int q{0}; q < 4; ++q) {
+#pragma unroll
+            for (int u{0}; u < 4; ++u) {
+                lines[kLines16 + q * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * q + u];
+            }
+        }
+    } else {
+        // 16 blocks x 8 rows over 128 lanes, all 8 frequencies per lane.
+        const int job{((wu - 2) << 6) | lane};
+        const int b{job >> 3}, r{job & 7};
+        float o[8];
+        dct_line<8, 8>(&S.sharp[1][((b >> 2) * 8 + r) * kSP + (b & 3) * 8], 0, o);
+#pragma unroll
+        for (int f{0}; f < 8; ++f) {
+            lines[kLines8 + b * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
+        }
+    }
+    __syncthreads();
+    // Column passes: 16 terms per lane on waves 0-1, 8 on waves 2-3.
+    float td[16], tr[16];
+    int ti[16];
+    if (wu == 0) {
+        const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                      &lines[kLines32 + (16 * (grp & 1) + t16) * RowBuf<32>::RS])};
+#pragma unroll
+        for (int b{0}; b < 4; ++b) {
+#pragma unroll
+            for (int u{0}; u < 4; ++u) {
+                const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
+                const int e{fx * 32 + fy};
+                const float v{acc[4 * b + u]};
+                selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, prm, td[4 * b + u], tr[4 * b + u]);
+                ti[4 * b + u] = e;
+                S.coefy[2][e] = static_cast<_Float16>(v);
+            }
+        }
+    } else if (wu == 1) {
+        const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                      &lines[kLines16 + grp * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
+#pragma unroll
+        for (int q{0}; q < 4; ++q) {
+#pragma unroll
+            for (int u{0}; u < 4; ++u) {
+                const int fy{4 * grp + u}, fx{t16};
+                const int e{fx * 16 + fy};
+                const float v{acc[4 * q + u]};
+                selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, prm, td[4 * q + u], tr[4 * q + u]);
+                ti[4 * q + u] = 256 * q + e;
+                S.coefy[1][256 * q + e] = static_cast<_Float16>(v);
+            }
+        }
+    } else {
+        const int job{((wu - 2) << 6) | lane};
+        const int b{job >> 3}, fx{job & 7};
+        const int bx{b & 3}, by{b >> 2};
+        const int base{256 * ((by >> 1) * 2 + (bx >> 1)) + 64 * ((by & 1) * 2 + (bx & 1))};
+        float o[8];
+        dct_line<8, 8>(&lines[kLines8 + b * RowBuf<8>::BS + fx * RowBuf<8>::RS], 0, o);
+#pragma unroll
+        for (int f{0}; f < 8; ++f) {
+            const int e{fx * 8 + f};
+            selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, prm, td[f], tr[f]);
+            ti[f] = base + e;
+            S.coefy[0][base + e] = static_cast<_Float16>(o[f]);
+        }
+    }
+    __syncthreads();  // row-pass lines are dead; dsc / rsc reuse the space
     for (int sz{0}; sz < 3; ++sz) {
         const int n{8 << sz};
-        if (n == 8) {
-            // 16 blocks x 8 rows; waves {0,1} / {2,3} take frequencies 0-3 / 4-7.
-            const int job{((wu & 1) << 6) | lane};
-            const int b{job >> 3}, r{job & 7};
-            const int k0{(wu >> 1) * 4};
-            float o[4];
-            dct_line<8, 4>(&S.sharp[1][((b >> 2) * 8 + r) * kSP + (b & 3) * 8], k0, o);
+        const bool owner{sz == 0 ? wu >= 2 : (sz == 1 ? wu == 1 : wu == 0)};
+        if (owner) {
+            const int cnt{sz == 0 ? 8 : 16};
 #pragma unroll
-            for (int f{0}; f < 4; ++f) {
-                lines[b * RowBuf<8>::BS + (k0 + f) * RowBuf<8>::RS + r] = o[f];
-            }
-            __syncthreads();
-            const int fx{job & 7};
-            const int bx{b & 3}, by{b >> 2};
-            const int base{256 * ((by >> 1) * 2 + (bx >> 1)) + 64 * ((by & 1) * 2 + (bx & 1))};
-            dct_line<8, 4>(&lines[b * RowBuf<8>::BS + fx * RowBuf<8>::RS], k0, o);
-#pragma unroll
-            for (int f{0}; f < 4; ++f) {
-                const int e{fx * 8 + k0 + f};
-                float d, rt;
-                selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, prm, d, rt);
-                dsc[base + e] = d;
-                rsc[base + e] = rt;
-                S.coefy[0][base + e] = static_cast<_Float16>(o[f]);
-            }
-        } else if (n == 16) {
-            // 4 quads x 16 rows; wave w takes frequencies 4w..4w+3.
-            const int q{lane >> 4}, r{lane & 15};
-            const int k0{wu * 4};
-            float o[4];
-            dct_line<16, 4>(&S.sharp[1][((q >> 1) * 16 + r) * kSP + (q & 1) * 16], k0, o);
-#pragma unroll
-            for (int f{0}; f < 4; ++f) {
-                lines[q * RowBuf<16>::BS + (k0 + f) * RowBuf<16>::RS + r] = o[f];
-            }
-            __syncthreads();
-            const int fx{lane & 15};
-            dct_line<16, 4>(&lines[q * RowBuf<16>::BS + fx * RowBuf<16>::RS], k0, o);
-#pragma unroll
-            for (int f{0}; f < 4; ++f) {
-                const int e{fx * 16 + k0 + f};
-                float d, rt;
-                selection_terms(o[f], e, 16, g_tables.wsel16, g_tables.wsel_inv16, prm, d, rt);
-                dsc[256 * q + e] = d;
-                rsc[256 * q + e] = rt;
-                S.coefy[1][256 * q + e] = static_cast<_Float16>(o[f]);
-            }
-        } else {
-            // 32 rows on lanes 0-31; wave w takes frequencies 8w..8w+7.
-            const int k0{wu * 8};
-            float o[8];
-            if (lane < 32) {
-                dct_line<32, 8>(&S.sharp[1][lane * kSP], k0, o);
-#pragma unroll
-                for (int f{0}; f < 8; ++f) {
-                    lines[(k0 + f) * RowBuf<32>::RS + lane] = o[f];
-                }
-            }
-            __syncthreads();
-            if (lane < 32) {
-                dct_line<32, 8>(&lines[lane * RowBuf<32>::RS], k0, o);
-#pragma unroll
-                for (int f{0}; f < 8; ++f) {
-                    const int e{lane * 32 + k0 + f};
-                    float d, rt;
-                    selection_terms(o[f], e, 32, g_tables.wsel32, g_tables.wsel_inv32, prm, d, rt);
-                    dsc[e] = d;
-                    rsc[e] = rt;
-                    S.coefy[2][e] = static_cast<_Float16>(o[f]);
+            for (int i{0}; i < 16; ++i) {
+                if (i < cnt) {
+                    dsc[ti[i]] = td[i];
+                    rsc[ti[i]] = tr[i];
                 }
             }
         }
@@ -601,36 +676,51 @@ __global__ __launch_bounds__(256) void region_kernel(
     //    (frontend_quantize.cu:49-117 on this region).
     const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
     if (acs0 == 32) {
-        // X rows on lanes 0-31, B rows on lanes 32-63; wave w takes frequencies 8w..8w+7.
-        const int ch{lane >> 5}, r{lane & 31};
-        const int k0{wu * 8};
-        float o[8];
-        dct_line<32, 8>(&S.sharp[2 * ch][r * kSP], k0, o);
+        // Waves 0 and 1 transform X and B on the matrix cores (DCT32 as four
+        // 16x16 quadrants); waves 2-3 wait.
+        const int ch{wu};
+        float* const cl{lines + ch * RowBuf<32>::BS};
+        if (wu < 2) {
+            const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                          &S.sharp[2 * ch][(16 * (grp & 1) + t16) * kSP])};
 #pragma unroll
-        for (int f{0}; f < 8; ++f) {
-            lines[ch * RowBuf<32>::BS + (k0 + f) * RowBuf<32>::RS + r] = o[f];
+            for (int b{0}; b < 4; ++b) {
+#pragma unroll
+                for (int u{0}; u < 4; ++u) {
+                    cl[(16 * (b >> 1) + 4 * grp + u) * RowBuf<32>::RS + 16 * (b & 1) + t16] = acc[4 * b + u];
+                }
+            }
         }
         __syncthreads();
-        dct_line<32, 8>(&lines[ch * RowBuf<32>::BS + r * RowBuf<32>::RS], k0, o);
+        if (wu < 2) {
+            const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                          &cl[(16 * (grp & 1) + t16) * RowBuf<32>::RS])};
 #pragma unroll
-        for (int f{0}; f < 8; ++f) {
-            S.coefxb[ch][r * 32 + k0 + f] = static_cast<_Float16>(o[f]);
+            for (int b{0}; b < 4; ++b) {
+#pragma unroll
+                for (int u{0}; u < 4; ++u) {
+                    const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
+                    S.coefxb[ch][fx * 32 + fy] = static_cast<_Float16>(acc[4 * b + u]);
+                }
+            }
         }
     } else {
-        // Wave w transforms quad w: one DCT16 (X rows on lanes 0-15, B rows on
-        // 16-31) or four DCT8s (lanes: channel, block, row).
+        // Wave w transforms quad w: one DCT16 on the matrix cores (block 0 = X,
+        // block 1 = B, blocks 2-3 unused) or four DCT8s on the VALU (lanes:
+        // channel, block, row).
         constexpr int kQuadLines{2 * 4 * RowBuf<8>::BS};
         float* const ql{lines + wu * kQuadLines};
         const int q{wu};
         const int n16{__builtin_amdgcn_readfirstlane(S.acs[(q >> 1) * 8 + (q & 1) * 2])};
+        const int ch16{grp & 1};  // lanes 32-63 repeat blocks 0-1; their results are dropped
         if (n16 == 16) {
-            if (lane < 32) {
-                const int ch{lane >> 4}, r{lane & 15};
-                float o[16];
-                dct_line<16, 16>(&S.sharp[2 * ch][((q >> 1) * 16 + r) * kSP + (q & 1) * 16], 0, o);
+            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                          &S.sharp[2 * ch16][((q >> 1) * 16 + t16) * kSP + (q & 1) * 16])};
 #pragma unroll
-                for (int f{0}; f < 16; ++f) {
-                    ql[ch * RowBuf<16>::BS + f * RowBuf<16>::RS + r] = o[f];
+            for (int b{0}; b < 2; ++b) {
+#pragma unroll
+                for (int u{0}; u < 4; ++u) {
+                    ql[b * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * b + u];
                 }
             }
         } else {
@@ -645,13 +735,13 @@ __global__ __launch_bounds__(256) void region_kernel(
         }
         __syncthreads();
         if (n16 == 16) {
-            if (lane < 32) {
-                const int ch{lane >> 4}, fx{lane & 15};
-                float o[16];
-                dct_line<16, 16>(&ql[ch * RowBuf<16>::BS + fx * RowBuf<16>::RS], 0, o);
+            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                          &ql[ch16 * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
 #pragma unroll
-                for (int f{0}; f < 16; ++f) {
-                    S.coefxb[ch][256 * q + fx * 16 + f] = static_cast<_Float16>(o[f]);
+            for (int b{0}; b < 2; ++b) {
+#pragma unroll
+                for (int u{0}; u < 4; ++u) {
+                    S.coefxb[b][256 * q + t16 * 16 + 4 * grp + u] = static_cast<_Float16>(acc[4 * b + u]);
                 }
             }
         } else {
