@@ -47,14 +47,16 @@ struct Layout {
     std::uint32_t head_total_bits;
 };
 
-// One wave per histogram: alphabet size, depths, codes and header.
-__global__ __launch_bounds__(64) void huffman_kernel(const std::uint32_t* __restrict__ hists, CodeSet* __restrict__ cs) {
+// One wave per histogram (histograms first, first + 1, ...): alphabet size,
+// depths, codes and header.
+__global__ __launch_bounds__(64) void huffman_kernel(const std::uint32_t* __restrict__ hists, CodeSet* __restrict__ cs,
+                                                     int first) {
     __shared__ WaveCodeScratch scratch;
     __shared__ std::uint32_t hist[kHistStride];
     __shared__ std::uint8_t depth[kHistStride];
     __shared__ std::uint16_t code[kHistStride];
     __shared__ std::uint32_t hdr[kHdrWords];
-    const int h{static_cast<int>(blockIdx.x)};
+    const int h{first + static_cast<int>(blockIdx.x)};
     const int t{static_cast<int>(threadIdx.x)};
     std::uint32_t top{0};
     for (int i{t}; i < kHistStride; i += 64) {

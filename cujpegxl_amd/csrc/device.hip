@@ -154,8 +154,8 @@ void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
     dc_hist_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, dc_hist, meta_hist, fb_count);
 }
 
-void launch_huffman(hipStream_t s, int count, const std::uint32_t* hists, CodeSet* cs) {
-    huffman_kernel<<<count, 64, 0, s>>>(hists, cs);
+void launch_huffman(hipStream_t s, int first, int count, const std::uint32_t* hists, CodeSet* cs) {
+    huffman_kernel<<<count, 64, 0, s>>>(hists, cs, first);
 }
 
 bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint32_t* alpha, std::uint8_t* depth,
@@ -170,7 +170,7 @@ bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint
             hipMemcpy(dh, hists, hbytes, hipMemcpyHostToDevice) == hipSuccess};
     auto* host{new CodeSet};
     if (ok) {
-        huffman_kernel<<<count, 64, 0, nullptr>>>(dh, dcs);
+        huffman_kernel<<<count, 64, 0, nullptr>>>(dh, dcs, 0);
         ok = hipGetLastError() == hipSuccess && hipMemcpy(host, dcs, sizeof(CodeSet), hipMemcpyDeviceToHost) == hipSuccess;
     }
     if (ok) {

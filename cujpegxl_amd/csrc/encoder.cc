@@ -242,6 +242,16 @@ void Encoder::release() {
             e = nullptr;
         }
     }
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_sizes_, &ev_headers_, &ev_dc_emit_}) {
+        if (*e != nullptr) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
+    }
+    if (dc_stream_ != nullptr) {
+        (void)hipStreamDestroy(dc_stream_);
+        dc_stream_ = nullptr;
+    }
     if (stream_ != nullptr) {
         (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
@@ -272,8 +282,14 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
     if (!g_tables_ok[device & 63]) {
         return false;
     }
-    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess) {
         return false;
+    }
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_sizes_, &ev_headers_, &ev_dc_emit_}) {
+        if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+            return false;
+        }
     }
     for (hipEvent_t& e : events_) {
         if (hipEventCreate(&e) != hipSuccess) {
@@ -509,34 +525,58 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
 }
 
 bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size) {
-    const std::size_t nh{static_cast<std::size_t>(kNumClusters) + 2 * geo_.num_dc};
     auto* codes{static_cast<CodeSet*>(codes_)};
     auto* k{static_cast<const Constants*>(constants_)};
     auto* blobs{static_cast<DcBlobs*>(blobs_)};
     auto* lay{static_cast<Layout*>(layout_)};
+    // The DC chain (histograms, codes, header blobs, chunk sizes) depends on
+    // nothing of the AC chain: it runs on the second stream and joins before
+    // the layout; DC emission runs beside AC emission.
     mark(Stage::kHistograms);
-    launch_fill_u32(stream_, hists_, nh * kHistStride, 0u);
-    launch_fill_u32(stream_, fb_count_, 64, 0u);
-    launch_fill_u32(stream_, static_cast<std::uint32_t*>(blobs_), sizeof_dc_blobs() / 4, 0u);
-    launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_);
-    launch_dc_hist(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
+    const int ndc{static_cast<int>(2 * geo_.num_dc)};
+    if (hipEventRecord(ev_fork_, stream_) != hipSuccess || hipStreamWaitEvent(dc_stream_, ev_fork_, 0) != hipSuccess) {
+        return false;
+    }
+    launch_fill_u32(dc_stream_, hists_ + kNumClusters * kHistStride, static_cast<std::size_t>(ndc) * kHistStride, 0u);
+    launch_fill_u32(dc_stream_, fb_count_, 64, 0u);
+    launch_fill_u32(dc_stream_, static_cast<std::uint32_t*>(blobs_), sizeof_dc_blobs() / 4, 0u);
+    launch_dc_hist(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
                    hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
+    launch_huffman(dc_stream_, kNumClusters, ndc, hists_, codes);
+    launch_dc_blobs(dc_stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
+    launch_dc_bits(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
+                   dc_group_bits_);
+    if (hipEventRecord(ev_dc_sizes_, dc_stream_) != hipSuccess) {
+        return false;
+    }
+    launch_fill_u32(stream_, hists_, static_cast<std::size_t>(kNumClusters) * kHistStride, 0u);
+    launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_);
     mark(Stage::kCodes);
-    launch_huffman(stream_, static_cast<int>(nh), hists_, codes);
-    launch_dc_blobs(stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
+    launch_huffman(stream_, 0, kNumClusters, hists_, codes);
     mark(Stage::kSizes);
     launch_ac_seg_bits(stream_, geo_, acs_, item_tokens_, tokens_, codes, seg_bits_);
     launch_ac_group_scan(stream_, geo_, seg_bits_, seg_off_, ac_group_bytes_, ac_group_bits_);
-    launch_dc_bits(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
-                   dc_group_bits_);
+    if (hipStreamWaitEvent(stream_, ev_dc_sizes_, 0) != hipSuccess) {
+        return false;
+    }
     launch_layout(stream_, geo_, k, codes, ac_group_bytes_, dc_group_bytes_, ac_group_bits_, dc_group_bits_, blobs,
                   lay, ac_base_, dc_base_);
     mark(Stage::kEmit);
     launch_zero_and_headers(stream_, k, lay, out_, out_cap_ / 4);
+    if (hipEventRecord(ev_headers_, stream_) != hipSuccess ||
+        hipStreamWaitEvent(dc_stream_, ev_headers_, 0) != hipSuccess) {
+        return false;
+    }
+    launch_dc_emit(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_base_,
+                   out_, out_cap_ / 4);
+    if (hipEventRecord(ev_dc_emit_, dc_stream_) != hipSuccess) {
+        return false;
+    }
     launch_ac_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, seg_bits_, seg_off_, ac_base_, out_,
                    out_cap_ / 4);
-    launch_dc_emit(stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_base_,
-                   out_, out_cap_ / 4);
+    if (hipStreamWaitEvent(stream_, ev_dc_emit_, 0) != hipSuccess) {
+        return false;
+    }
     mark(Stage::kReadback);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),

@@ -76,6 +76,8 @@ private:
     Geometry geo_{};
     FrameParams prm_{};
     hipStream_t stream_{nullptr};
+    hipStream_t dc_stream_{nullptr};  // DC entropy chain, joined by events
+    hipEvent_t ev_fork_{nullptr}, ev_dc_sizes_{nullptr}, ev_headers_{nullptr}, ev_dc_emit_{nullptr};
     bool profiling_{false};
     std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
     std::array<float, static_cast<int>(Stage::kCount)> stage_ms_{};

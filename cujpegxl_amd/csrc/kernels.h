@@ -52,7 +52,7 @@ void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 // the results back (stage harness; allocates).
 bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint32_t* alpha, std::uint8_t* depth,
                       std::uint16_t* bits, std::uint32_t* hdr_words, std::uint32_t* hdr_bits);
-void launch_huffman(hipStream_t s, int count, const std::uint32_t* hists, CodeSet* cs);
+void launch_huffman(hipStream_t s, int first, int count, const std::uint32_t* hists, CodeSet* cs);
 void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                      const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words);
 void launch_ac_seg_bits(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
