@@ -47,44 +47,106 @@ struct Layout {
     std::uint32_t head_total_bits;
 };
 
-// One wave per histogram (histograms first, first + 1, ...): alphabet size,
-// depths, codes and header.
-__global__ __launch_bounds__(64) void huffman_kernel(const std::uint32_t* __restrict__ hists, CodeSet* __restrict__ cs,
-                                                     int first) {
-    __shared__ WaveCodeScratch scratch;
-    __shared__ std::uint32_t hist[kHistStride];
-    __shared__ std::uint8_t depth[kHistStride];
-    __shared__ std::uint16_t code[kHistStride];
-    __shared__ std::uint32_t hdr[kHdrWords];
+// One 16-wave workgroup per histogram (histograms first, first + 1, ...):
+// create_huffman_tree's count floors 1, 2, 4, ... are independent trials, so
+// wave w tries floor 2^w and the smallest floor whose tree fits 15 bits wins
+// (the serial builder stops at that same floor; floors above 2^15 continue on
+// wave 0).  Wave 0 then builds the codes and the header.
+constexpr int kTrials = 16;
+
+struct HuffmanShared {
+    WaveHuffScratch trial[kTrials];
+    std::uint8_t trial_depth[kTrials][kHistStride];
+    std::uint32_t trial_max[kTrials];
+    WaveCodeScratch code_scratch;
+    std::uint32_t hist[kHistStride];
+    std::uint8_t depth[kHistStride];
+    std::uint16_t code[kHistStride];
+    std::uint32_t hdr[kHdrWords];
+};
+
+template <int R>
+__device__ void huffman_depths(HuffmanShared& S, int alpha) {
+    const int w{static_cast<int>(threadIdx.x) >> 6};
+    const Leaves<R> L{wave_leaves<R>(S.hist, alpha)};
+    if (L.n < 2) {
+        return;  // one symbol or none: the simple header, all depths stay 0
+    }
+    const std::uint32_t maxd{tree_trial<R>(L, 1u << w, S.trial_depth[w], S.trial[w])};
+    if ((threadIdx.x & 63) == 0) {
+        S.trial_max[w] = maxd;
+    }
+    __syncthreads();
+    int win{-1};
+    for (int i{0}; i < kTrials; ++i) {
+        if (S.trial_max[i] <= 15u) {
+            win = i;
+            break;
+        }
+    }
+    if (win < 0 && w == 0) {
+        for (std::uint32_t count_limit{1u << kTrials};; count_limit *= 2) {
+            for (int i{static_cast<int>(threadIdx.x)}; i < kHistStride; i += 64) {
+                S.trial_depth[0][i] = 0;
+            }
+            wave_sync();
+            if (tree_trial<R>(L, count_limit, S.trial_depth[0], S.trial[0]) <= 15u) {
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    const int src{win < 0 ? 0 : win};
+    for (int i{static_cast<int>(threadIdx.x)}; i < kHistStride; i += 64 * kTrials) {
+        S.depth[i] = S.trial_depth[src][i];
+    }
+}
+
+__global__ __launch_bounds__(64 * kTrials) void huffman_kernel(const std::uint32_t* __restrict__ hists,
+                                                               CodeSet* __restrict__ cs, int first) {
+    __shared__ HuffmanShared S;
     const int h{first + static_cast<int>(blockIdx.x)};
     const int t{static_cast<int>(threadIdx.x)};
+    const int w{t >> 6};
+    for (int i{t}; i < kHistStride; i += 64 * kTrials) {
+        S.hist[i] = hists[h * kHistStride + i];
+        S.depth[i] = 0;
+        S.code[i] = 0;
+    }
+    for (int i{t}; i < kTrials * kHistStride; i += 64 * kTrials) {
+        S.trial_depth[i / kHistStride][i % kHistStride] = 0;
+    }
+    if (t < kHdrWords) {
+        S.hdr[t] = 0;
+    }
+    __syncthreads();
     std::uint32_t top{0};
-    for (int i{t}; i < kHistStride; i += 64) {
-        const std::uint32_t v{hists[h * kHistStride + i]};
-        hist[i] = v;
-        depth[i] = 0;
-        code[i] = 0;
-        if (v != 0) {
+    for (int i{t & 63}; i < kHistStride; i += 64) {
+        if (S.hist[i] != 0) {
             top = static_cast<std::uint32_t>(i) + 1u;
         }
     }
-    for (int i{t}; i < kHdrWords; i += 64) {
-        hdr[i] = 0;
-    }
     const int alpha{static_cast<int>(wave_max_u32(top > 1u ? top : 1u))};
+    if (alpha > 1) {
+        if (alpha <= 64) {
+            huffman_depths<1>(S, alpha);
+        } else {
+            huffman_depths<4>(S, alpha);
+        }
+    }
     __syncthreads();
     std::uint32_t nbits{0};
-    if (alpha > 1) {
-        nbits = alpha <= 64 ? wave_build_code<1>(hist, alpha, depth, code, hdr, kHdrWords, scratch)
-                            : wave_build_code<4>(hist, alpha, depth, code, hdr, kHdrWords, scratch);
+    if (w == 0 && alpha > 1) {
+        nbits = alpha <= 64 ? wave_finish_code<1>(S.hist, alpha, S.depth, S.code, S.hdr, kHdrWords, S.code_scratch)
+                            : wave_finish_code<4>(S.hist, alpha, S.depth, S.code, S.hdr, kHdrWords, S.code_scratch);
     }
     __syncthreads();
-    for (int i{t}; i < kHistStride; i += 64) {
-        cs->depth[h][i] = depth[i];
-        cs->code[h][i] = code[i];
+    for (int i{t}; i < kHistStride; i += 64 * kTrials) {
+        cs->depth[h][i] = S.depth[i];
+        cs->code[h][i] = S.code[i];
     }
-    for (int i{t}; i < kHdrWords; i += 64) {
-        cs->hdr[h][i] = hdr[i];
+    if (t < kHdrWords) {
+        cs->hdr[h][t] = S.hdr[t];
     }
     if (t == 0) {
         cs->hdr_bits[h] = nbits;

@@ -155,7 +155,7 @@ void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 }
 
 void launch_huffman(hipStream_t s, int first, int count, const std::uint32_t* hists, CodeSet* cs) {
-    huffman_kernel<<<count, 64, 0, s>>>(hists, cs, first);
+    huffman_kernel<<<count, 64 * kTrials, 0, s>>>(hists, cs, first);
 }
 
 bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint32_t* alpha, std::uint8_t* depth,
@@ -170,7 +170,7 @@ bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint
             hipMemcpy(dh, hists, hbytes, hipMemcpyHostToDevice) == hipSuccess};
     auto* host{new CodeSet};
     if (ok) {
-        huffman_kernel<<<count, 64, 0, nullptr>>>(dh, dcs, 0);
+        huffman_kernel<<<count, 64 * kTrials, 0, nullptr>>>(dh, dcs, 0);
         ok = hipGetLastError() == hipSuccess && hipMemcpy(host, dcs, sizeof(CodeSet), hipMemcpyDeviceToHost) == hipSuccess;
     }
     if (ok) {

@@ -19,6 +19,13 @@ namespace cjx {
 
 constexpr std::uint32_t kQueueEnd = 0xffffffffu;
 
+// Orders one wave's LDS accesses (its LDS instructions execute in order; this
+// keeps the compiler from moving them across the point).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // R values per lane: element idx lives in register idx / 64 of lane idx % 64.
 template <int R>
 struct LaneArray {
@@ -144,81 +151,103 @@ __device__ std::uint32_t merge_depths(const WaveHuffScratch& s, int n, std::uint
     return maxd;
 }
 
-// create_huffman_tree (prefix_code.cc:36-96).  data[0..length) and
-// depth[0..length) are in LDS; depth is zero on entry for absent symbols.
-// R * 64 >= length.
+// Leaves of data[0..length) in create_huffman_tree's input order (descending
+// symbol): per lane, the counts of symbols 64 r + lane, whether present, and
+// the input position of each present one.
+template <int R>
+struct Leaves {
+    std::uint32_t cnt[R];
+    int pos[R];
+    int n;
+};
+
+template <int R>
+__device__ __forceinline__ Leaves<R> wave_leaves(const std::uint32_t* data, int length) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    Leaves<R> L{};
+    std::uint64_t present[R];
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        const int sym{64 * r + lane};
+        L.cnt[r] = sym < length ? data[sym] : 0u;
+        present[r] = __ballot(L.cnt[r] != 0);
+        L.n += __popcll(present[r]);
+    }
+    const std::uint64_t above{lane == 63 ? 0ull : (~0ull << (lane + 1))};
+    int higher{0};
+#pragma unroll
+    for (int r{R - 1}; r >= 0; --r) {
+        L.pos[r] = higher + __popcll(present[r] & above);
+        higher += __popcll(present[r]);
+    }
+    return L;
+}
+
+// One pass of create_huffman_tree's loop (prefix_code.cc:36-96) with count
+// floor `count_limit` for n >= 2 leaves: writes depth[symbol] of every present
+// symbol and returns the largest depth (wave-uniform).
+template <int R>
+__device__ std::uint32_t tree_trial(const Leaves<R>& L, std::uint32_t count_limit, std::uint8_t* depth,
+                                    WaveHuffScratch& s) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    const int n{L.n};
+    std::uint32_t tot[R];
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        tot[r] = L.cnt[r] > count_limit - 1 ? L.cnt[r] : count_limit - 1;
+        if (L.cnt[r] != 0) {
+            s.leaf_total[L.pos[r]] = tot[r];
+        }
+    }
+    wave_sync();
+    // Stable ascending sort: rank = smaller totals + equal totals earlier.
+    int rank[R];
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        rank[r] = 0;
+    }
+    for (int b{0}; b < n; ++b) {
+        const std::uint32_t t{s.leaf_total[b]};
+#pragma unroll
+        for (int r{0}; r < R; ++r) {
+            rank[r] += (t < tot[r] || (t == tot[r] && b < L.pos[r])) ? 1 : 0;
+        }
+    }
+#pragma unroll
+    for (int r{0}; r < R; ++r) {
+        if (L.cnt[r] != 0) {
+            s.sorted_total[rank[r]] = tot[r];
+            s.sorted_symbol[rank[r]] = static_cast<std::uint32_t>(64 * r + lane);
+        }
+    }
+    wave_sync();
+    const std::uint32_t maxd{n <= 64 ? merge_depths<1>(s, n, depth) : merge_depths<4>(s, n, depth)};
+    wave_sync();
+    return wave_max_u32(maxd);
+}
+
+// create_huffman_tree (prefix_code.cc:36-96) by one wave, floors tried in
+// order.  data[0..length) and depth[0..length) are in LDS; depth is zero on
+// entry for absent symbols.  R * 64 >= length.
 template <int R>
 __device__ void wave_create_tree(const std::uint32_t* data, int length, int limit, std::uint8_t* depth,
                                  WaveHuffScratch& s) {
     const int lane{static_cast<int>(threadIdx.x) & 63};
-    std::uint32_t cnt[R];
-    std::uint64_t present[R];
-    int n{0};
-#pragma unroll
-    for (int r{0}; r < R; ++r) {
-        const int sym{64 * r + lane};
-        cnt[r] = sym < length ? data[sym] : 0u;
-        present[r] = __ballot(cnt[r] != 0);
-        n += __popcll(present[r]);
-    }
-    if (n == 0) {
+    const Leaves<R> L{wave_leaves<R>(data, length)};
+    if (L.n == 0) {
         return;
     }
-    // Leaf position in input order (descending symbol): present symbols above.
-    const std::uint64_t above{lane == 63 ? 0ull : (~0ull << (lane + 1))};
-    int pos[R];
-    {
-        int higher{0};
-#pragma unroll
-        for (int r{R - 1}; r >= 0; --r) {
-            pos[r] = higher + __popcll(present[r] & above);
-            higher += __popcll(present[r]);
-        }
-    }
-    if (n == 1) {
+    if (L.n == 1) {
 #pragma unroll
         for (int r{0}; r < R; ++r) {
-            if (cnt[r] != 0) {
+            if (L.cnt[r] != 0) {
                 depth[64 * r + lane] = 1;
             }
         }
         return;
     }
     for (std::uint32_t count_limit{1};; count_limit *= 2) {
-        std::uint32_t tot[R];
-#pragma unroll
-        for (int r{0}; r < R; ++r) {
-            tot[r] = cnt[r] > count_limit - 1 ? cnt[r] : count_limit - 1;
-            if (cnt[r] != 0) {
-                s.leaf_total[pos[r]] = tot[r];
-            }
-        }
-        __syncthreads();
-        // Stable ascending sort: rank = smaller totals + equal totals earlier.
-        int rank[R];
-#pragma unroll
-        for (int r{0}; r < R; ++r) {
-            rank[r] = 0;
-        }
-        for (int b{0}; b < n; ++b) {
-            const std::uint32_t t{s.leaf_total[b]};
-#pragma unroll
-            for (int r{0}; r < R; ++r) {
-                rank[r] += (t < tot[r] || (t == tot[r] && b < pos[r])) ? 1 : 0;
-            }
-        }
-#pragma unroll
-        for (int r{0}; r < R; ++r) {
-            if (cnt[r] != 0) {
-                s.sorted_total[rank[r]] = tot[r];
-                s.sorted_symbol[rank[r]] = static_cast<std::uint32_t>(64 * r + lane);
-            }
-        }
-        __syncthreads();
-        std::uint32_t maxd{n <= 64 ? merge_depths<1>(s, n, depth) : merge_depths<4>(s, n, depth)};
-        maxd = wave_max_u32(maxd);
-        __syncthreads();
-        if (static_cast<int>(maxd) <= limit) {
+        if (static_cast<int>(tree_trial<R>(L, count_limit, depth, s)) <= limit) {
             return;
         }
     }
@@ -304,7 +333,7 @@ __device__ std::uint32_t wave_store_tree(const std::uint8_t* depth, int num, std
         s.cl_depth[lane] = 0;
         s.cl_bits[lane] = 0;
     }
-    __syncthreads();
+    wave_sync();
     int num_codes{0}, code{0};
     for (int i{0}; i < 18; ++i) {
         if (s.cl_hist[i]) {
@@ -318,9 +347,9 @@ __device__ std::uint32_t wave_store_tree(const std::uint8_t* depth, int num, std
         }
     }
     wave_create_tree<1>(s.cl_hist, 18, 5, s.cl_depth, s.tree);
-    __syncthreads();
+    wave_sync();
     wave_depths_to_codes<1>(s.cl_depth, 18, s.cl_bits);
-    __syncthreads();
+    wave_sync();
     if (lane == 0) {
         constexpr std::uint8_t order[18]{1, 2, 3, 4, 0, 5, 17, 6, 16, 7, 8, 9, 10, 11, 12, 13, 14, 15};
         constexpr std::uint8_t sym[6]{0, 7, 3, 2, 1, 15};
@@ -351,7 +380,7 @@ __device__ std::uint32_t wave_store_tree(const std::uint8_t* depth, int num, std
         }
         s.pos = w.pos;
     }
-    __syncthreads();
+    wave_sync();
     std::uint32_t at{s.pos};
 #pragma unroll
     for (int r{0}; r < R; ++r) {
@@ -368,12 +397,14 @@ __device__ std::uint32_t wave_store_tree(const std::uint8_t* depth, int num, std
 }
 
 // build_and_store_huffman_tree (prefix_code.cc:358-397) for hist[0..alpha),
-// alpha >= 2; depth and bits are zeroed by the caller.  Returns the header
-// length in bits, or 0 if it would not fit `cap` words.
+// alpha >= 2, with depth[] already holding create_huffman_tree's result when
+// more than one symbol is present (all zero otherwise); bits are zeroed by the
+// caller.  One wave.  Returns the header length in bits, or 0 if it would not
+// fit `cap` words.
 template <int R>
-__device__ std::uint32_t wave_build_code(const std::uint32_t* hist, int alpha, std::uint8_t* depth,
-                                         std::uint16_t* bits, std::uint32_t* hdr, std::uint32_t cap,
-                                         WaveCodeScratch& s) {
+__device__ std::uint32_t wave_finish_code(const std::uint32_t* hist, int alpha, const std::uint8_t* depth,
+                                          std::uint16_t* bits, std::uint32_t* hdr, std::uint32_t cap,
+                                          WaveCodeScratch& s) {
     const int lane{static_cast<int>(threadIdx.x) & 63};
     std::uint64_t present[R];
     int count{0};
@@ -410,10 +441,8 @@ __device__ std::uint32_t wave_build_code(const std::uint32_t* hist, int alpha, s
         }
         return static_cast<std::uint32_t>(__shfl(static_cast<int>(pos), 0, 64));
     }
-    wave_create_tree<R>(hist, alpha, 15, depth, s.tree);
-    __syncthreads();
     wave_depths_to_codes<R>(depth, alpha, bits);
-    __syncthreads();
+    wave_sync();
     if (count <= 4) {
         std::uint32_t pos{0};
         if (lane == 0) {

@@ -214,6 +214,11 @@ def _huffman_cases():
         h = np.zeros(256, np.uint32)
         h[lo:lo + 40] = fib
         cases.append(h)
+    while len(fib) < 45:
+        fib.append(fib[-1] + fib[-2])
+    h = np.zeros(256, np.uint32)
+    h[:45] = fib  # totals near 3e9: the count floor climbs past 2^15
+    cases.append(h)
     for k in (1, 2, 3, 4, 5, 6, 17, 64, 65, 200, 256):
         h = np.zeros(256, np.uint32)
         idx = rng.choice(256, size=k, replace=False)
@@ -255,3 +260,18 @@ def test_prefix_codes_match_oracle(jo):
         np.testing.assert_array_equal(got[i]["bits"][:alpha], b, err_msg=f"case {i}")
         assert got[i]["header_bits"] == nb, (i, got[i]["header_bits"], nb)
         assert got[i]["header"] == out[: (nb + 7) // 8].tobytes(), i
+
+
+def test_single_symbol_histogram_has_zero_depths(jo):
+    """A histogram with one present symbol takes the simple header and keeps
+    every depth at 0 (prefix_code.cc build_and_store_huffman_tree)."""
+    say("single-symbol histogram")
+    h = np.zeros((2, 256), np.uint32)
+    h[0, 16] = 1
+    h[1, 200] = 12345
+    got = cj.build_codes(h)
+    for i in range(2):
+        a = int(np.nonzero(h[i])[0][-1]) + 1
+        d, b, out, nb = _huff(jo, np.ascontiguousarray(h[i, :a]))
+        assert not got[i]["depth"].any() and not d.any()
+        assert got[i]["header_bits"] == nb and got[i]["header"] == out[: (nb + 7) // 8].tobytes()
