@@ -193,16 +193,21 @@ void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, con
     dc_blobs_kernel<<<geo.num_dc, 64, 0, s>>>(geo, k, cs, fb_count, blobs, blob_words);
 }
 
-void launch_ac_seg_bits(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
-                        const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
-                        std::uint32_t* seg_bits) {
-    ac_seg_bits_kernel<<<geo.num_ac * kAcGroupBlocks, 256, 0, s>>>(geo, acs, item_tokens, tokens, &cs->depth[0][0],
-                                                                   seg_bits);
+std::size_t ac_group_slot_words() {
+    return kGroupSlotWords;
 }
 
-void launch_ac_group_scan(hipStream_t s, const Geometry& geo, const std::uint32_t* seg_bits,
-                          std::uint32_t* seg_off, std::uint32_t* group_bytes, std::uint64_t* group_bits) {
-    ac_group_scan_kernel<<<geo.num_ac, 64, 0, s>>>(geo, seg_bits, seg_off, group_bytes, group_bits);
+void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
+                          const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
+                          std::uint32_t* staging, std::uint32_t* group_bytes, std::uint64_t* group_bits) {
+    ac_group_emit_kernel<<<geo.num_ac, kGroupThreads, 0, s>>>(geo, acs, item_tokens, tokens, &cs->depth[0][0],
+                                                              &cs->code[0][0], staging, group_bytes, group_bits);
+}
+
+void launch_ac_group_place(hipStream_t s, const Geometry& geo, const std::uint32_t* staging,
+                           const std::uint64_t* group_bits, const std::uint64_t* ac_base, std::uint32_t* out_words,
+                           std::uint64_t cap_words) {
+    ac_group_place_kernel<<<geo.num_ac, 256, 0, s>>>(staging, group_bits, ac_base, out_words, cap_words);
 }
 
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
@@ -228,15 +233,6 @@ void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* la
                              std::uint64_t cap_words) {
     zero_kernel<<<1024, 256, 0, s>>>(lay, out_words, cap_words);
     headers_kernel<<<1, 64, 0, s>>>(k, lay, out_words, cap_words);
-}
-
-void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs, const std::uint16_t* item_tokens,
-                    const std::uint32_t* tokens, const CodeSet* cs, const std::uint32_t* seg_bits,
-                    const std::uint32_t* seg_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
-                    std::uint64_t cap_words) {
-    ac_emit_kernel<<<geo.num_ac * kAcGroupBlocks, 256, 0, s>>>(geo, acs, item_tokens, tokens, &cs->depth[0][0],
-                                                               &cs->code[0][0], seg_bits, seg_off, ac_base,
-                                                               out_words, cap_words);
 }
 
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,

@@ -343,8 +343,7 @@ bool Encoder::alloc() {
     get(tokens_, 3 * nblk * 64 * sizeof(std::uint32_t));
     get(item_tokens_, 3 * nblk * sizeof(std::uint16_t));
     get(hist_partial_, ac_partial_words() * sizeof(std::uint32_t));
-    get(seg_bits_, static_cast<std::size_t>(geo_.num_ac) * kAcGroupBlocks * sizeof(std::uint32_t));
-    get(seg_off_, static_cast<std::size_t>(geo_.num_ac) * kAcGroupBlocks * sizeof(std::uint32_t));
+    get(ac_staging_, static_cast<std::size_t>(geo_.num_ac) * ac_group_slot_words() * sizeof(std::uint32_t));
     get(ac_group_bytes_, geo_.num_ac * sizeof(std::uint32_t));
     get(ac_group_bits_, geo_.num_ac * sizeof(std::uint64_t));
     get(chunk_bits_, static_cast<std::size_t>(geo_.num_dc) * dc_chunks_per_group() * sizeof(std::uint64_t));
@@ -554,8 +553,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     mark(Stage::kCodes);
     launch_huffman(stream_, 0, kNumClusters, hists_, codes);
     mark(Stage::kSizes);
-    launch_ac_seg_bits(stream_, geo_, acs_, item_tokens_, tokens_, codes, seg_bits_);
-    launch_ac_group_scan(stream_, geo_, seg_bits_, seg_off_, ac_group_bytes_, ac_group_bits_);
+    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_group_bytes_,
+                         ac_group_bits_);
     if (hipStreamWaitEvent(stream_, ev_dc_sizes_, 0) != hipSuccess) {
         return false;
     }
@@ -572,8 +571,7 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     if (hipEventRecord(ev_dc_emit_, dc_stream_) != hipSuccess) {
         return false;
     }
-    launch_ac_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, seg_bits_, seg_off_, ac_base_, out_,
-                   out_cap_ / 4);
+    launch_ac_group_place(stream_, geo_, ac_staging_, ac_group_bits_, ac_base_, out_, out_cap_ / 4);
     if (hipStreamWaitEvent(stream_, ev_dc_emit_, 0) != hipSuccess) {
         return false;
     }

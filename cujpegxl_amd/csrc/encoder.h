@@ -103,8 +103,7 @@ private:
     std::uint32_t* tokens_{nullptr};       // token records in coefficient-slot layout
     std::uint16_t* item_tokens_{nullptr};  // tokens per (block, channel) item
     std::uint32_t* hist_partial_{nullptr}; // per-workgroup AC histograms
-    std::uint32_t* seg_bits_{nullptr};     // bits per AC-group block row
-    std::uint32_t* seg_off_{nullptr};      // bit offset of each row in its group
+    std::uint32_t* ac_staging_{nullptr};   // each AC group's stream at group-relative bits
     std::uint32_t* ac_group_bytes_{nullptr};
     std::uint64_t* ac_group_bits_{nullptr};
     std::uint64_t* chunk_bits_{nullptr};

@@ -238,29 +238,6 @@ __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const st
     }
 }
 
-// AC segments: one block row of one AC group.  Its items, in decoder order
-// (blocks left to right, channels Y, X, B), form one contiguous bit range of
-// the group's section.
-struct AcSegment {
-    std::uint32_t first;  // first item
-    int items;            // 0 for a row below the frame
-    std::uint32_t g;      // AC group
-};
-
-__device__ __forceinline__ AcSegment ac_segment(const Geometry& geo, std::uint32_t s) {
-    AcSegment seg{};
-    seg.g = s / kAcGroupBlocks;
-    const int row{static_cast<int>(s % kAcGroupBlocks)};
-    const int bx0{static_cast<int>(seg.g % geo.xg) * kAcGroupBlocks};
-    const int by{static_cast<int>(seg.g / geo.xg) * kAcGroupBlocks + row};
-    if (by >= static_cast<int>(geo.bh)) {
-        return seg;
-    }
-    seg.first = (static_cast<std::uint32_t>(by) * geo.bw + static_cast<std::uint32_t>(bx0)) * 3u;
-    seg.items = 3 * min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0);
-    return seg;
-}
-
 // Token records: cluster | symbol << 5 | extra-bit count << 13 | extra << 18
 // (hybrid-uint extra bits of 16-bit magnitudes fit in 14 bits).  Token k of
 // an item lives in the k-th coefficient slot of the item's covered blocks
@@ -383,191 +360,217 @@ __device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std
     return r;
 }
 
-// A segment's items in LDS: token prefix and the slot geometry of each item.
-struct SegItems {
-    std::uint32_t prefix[3 * kAcGroupBlocks + 1];
-    std::int16_t c[3 * kAcGroupBlocks], bx[3 * kAcGroupBlocks], by[3 * kAcGroupBlocks];
-    std::int8_t m[3 * kAcGroupBlocks];
-    std::uint64_t sh[5];
+// ---- AC groups emitted whole -------------------------------------------------
+//
+// One 1024-thread workgroup per AC group writes the group's token stream at
+// group-relative bit positions into its staging slot: the group's items in
+// decoder order (block rows, blocks left to right, channels Y, X, B) and
+// their token counts give a flat token numbering (token f belongs to the last
+// item whose prefix is <= f); block scans of the token lengths give the
+// positions; the stream is assembled in LDS and stored once.  The group's
+// bit and byte sizes fall out of the same pass.  A stream longer than the LDS
+// copy is emitted again with atomicOr into the zeroed slot.
+constexpr int kGroupThreads = 1024;
+constexpr int kGroupItems = 3 * kAcGroupBlocks * kAcGroupBlocks;
+constexpr int kGroupWords = 16384;
+constexpr int kGroupPerThread = 8;
+// Bound on one group's stream: 28 bits (15-bit code + 13 extra bits) per
+// coefficient slot of its 1024 blocks in three channels.
+constexpr std::size_t kGroupSlotWords = 3ull * kAcGroupBlocks * kAcGroupBlocks * 64 * 28 / 32 + 64;
+
+struct GroupShared {
+    std::uint32_t prefix[kGroupItems + 1];
+    std::int8_t m[kGroupItems];
+    std::uint32_t buf[kGroupWords];
+    std::uint64_t sh[17];
+    std::uint32_t overflow;
 };
 
-// Loads the segment's item table (256 threads); returns the token count.
-__device__ std::uint32_t load_segment(const Geometry& geo, const AcSegment& seg, const std::int8_t* acs,
-                                      const std::uint16_t* item_tokens, SegItems& S) {
-    const int t{static_cast<int>(threadIdx.x)};
-    std::uint32_t cnt{0};
-    if (t < seg.items) {
-        const std::uint32_t item{seg.first + static_cast<std::uint32_t>(t)};
-        const std::uint32_t pos{item / 3};
-        const int ci{static_cast<int>(item % 3)};
-        const int n{acs[pos]};
-        S.c[t] = static_cast<std::int16_t>(ci == 0 ? 1 : (ci == 1 ? 0 : 2));
-        S.bx[t] = static_cast<std::int16_t>(pos % geo.bw);
-        S.by[t] = static_cast<std::int16_t>(pos / geo.bw);
-        S.m[t] = static_cast<std::int8_t>(n / 8);
-        cnt = n != 0 ? item_tokens[item] : 0u;
-    }
-    std::uint64_t total{0};
-    const std::uint64_t p{block_excl_scan(cnt, S.sh, total)};
-    if (t < seg.items) {
-        S.prefix[t] = static_cast<std::uint32_t>(p);
-    }
-    if (t == 0) {
-        S.prefix[seg.items] = static_cast<std::uint32_t>(total);
-    }
-    __syncthreads();
-    return static_cast<std::uint32_t>(total);
-}
-
-// Token f of the segment: the last item whose prefix is <= f (empty items
-// share their successor's prefix and are skipped by that rule).
-__device__ __forceinline__ std::uint32_t segment_token(const Geometry& geo, const SegItems& S, int items,
-                                                       std::uint32_t f, const std::uint32_t* tokens) {
-    int lo{0}, hi{items - 1};
-    while (lo < hi) {
-        const int mid{(lo + hi + 1) >> 1};
-        if (S.prefix[mid] <= f) {
-            lo = mid;
-        } else {
-            hi = mid - 1;
-        }
-    }
-    return tokens[token_slot(geo, S.c[lo], S.bx[lo], S.by[lo], S.m[lo], f - S.prefix[lo])];
-}
-
-// Pass 2: bits per segment under the built codes.
-__global__ __launch_bounds__(256) void ac_seg_bits_kernel(Geometry geo, const std::int8_t* __restrict__ acs,
-                                                          const std::uint16_t* __restrict__ item_tokens,
-                                                          const std::uint32_t* __restrict__ tokens,
-                                                          const std::uint8_t* __restrict__ depth,
-                                                          std::uint32_t* __restrict__ seg_bits) {
-    __shared__ SegItems S;
-    const AcSegment seg{ac_segment(geo, blockIdx.x)};
-    if (seg.items == 0) {
-        if (threadIdx.x == 0) {
-            seg_bits[blockIdx.x] = 0;
-        }
-        return;
-    }
-    const std::uint32_t total{load_segment(geo, seg, acs, item_tokens, S)};
-    std::uint64_t bits{0};
-    for (std::uint32_t f{threadIdx.x}; f < total; f += 256) {
-        const std::uint32_t r{segment_token(geo, S, seg.items, f, tokens)};
-        bits += depth[(r & 31u) * kHistStride + ((r >> 5) & 255u)] + ((r >> 13) & 31u);
-    }
-    std::uint64_t sum{0};
-    (void)block_excl_scan(bits, S.sh, sum);
-    if (threadIdx.x == 0) {
-        seg_bits[blockIdx.x] = static_cast<std::uint32_t>(sum);
-    }
-}
-
-// Per AC group: bit offsets of its 32 segments, the group's bits and bytes.
-__global__ __launch_bounds__(64) void ac_group_scan_kernel(Geometry geo, const std::uint32_t* __restrict__ seg_bits,
-                                                           std::uint32_t* __restrict__ seg_off,
-                                                           std::uint32_t* __restrict__ group_bytes,
-                                                           std::uint64_t* __restrict__ group_bits) {
-    const int g{static_cast<int>(blockIdx.x)};
-    const int lane{static_cast<int>(threadIdx.x)};
-    const std::uint32_t s{static_cast<std::uint32_t>(g) * kAcGroupBlocks + static_cast<std::uint32_t>(lane)};
-    const int v{lane < kAcGroupBlocks ? static_cast<int>(seg_bits[s]) : 0};
-    const int pre{wave_excl_scan(v, lane)};
-    const int total{wave_sum_int(v)};
-    if (lane < kAcGroupBlocks) {
-        seg_off[s] = static_cast<std::uint32_t>(pre);
-    }
-    if (lane == 0) {
-        group_bytes[g] = static_cast<std::uint32_t>((static_cast<std::uint64_t>(total) + 7) / 8);
-        group_bits[g] = static_cast<std::uint64_t>(total);
-    }
-}
-
-// Pass 3: emission.  One workgroup per segment, one thread per token: bit
-// positions from a block scan of the token lengths, tokens ORed into an LDS
-// copy of the segment's bit range, interior words written with plain stores
-// and only the first and last word (shared with neighbouring ranges)
-// combined with atomicOr into the zeroed output.  A range longer than the
-// LDS copy is emitted with global atomics instead.
-constexpr int kEmitWords = 2048;
-
-__global__ __launch_bounds__(256) void ac_emit_kernel(Geometry geo, const std::int8_t* __restrict__ acs,
-                                                      const std::uint16_t* __restrict__ item_tokens,
-                                                      const std::uint32_t* __restrict__ tokens,
-                                                      const std::uint8_t* __restrict__ depth,
-                                                      const std::uint16_t* __restrict__ codes,
-                                                      const std::uint32_t* __restrict__ seg_bits,
-                                                      const std::uint32_t* __restrict__ seg_off,
-                                                      const std::uint64_t* __restrict__ ac_group_base_bits,
-                                                      std::uint32_t* __restrict__ out_words,
-                                                      std::uint64_t cap_words) {
-    __shared__ std::uint32_t buf[kEmitWords];
-    __shared__ SegItems S;
+__global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
+    Geometry geo, const std::int8_t* __restrict__ acs, const std::uint16_t* __restrict__ item_tokens,
+    const std::uint32_t* __restrict__ tokens, const std::uint8_t* __restrict__ depth,
+    const std::uint16_t* __restrict__ codes, std::uint32_t* __restrict__ staging,
+    std::uint32_t* __restrict__ group_bytes, std::uint64_t* __restrict__ group_bits) {
+    __shared__ GroupShared S;
     const int tid{static_cast<int>(threadIdx.x)};
-    const AcSegment seg{ac_segment(geo, blockIdx.x)};
-    if (seg.items == 0) {
-        return;
-    }
-    const std::uint64_t start{ac_group_base_bits[seg.g] + seg_off[blockIdx.x]};
-    const std::uint64_t end{start + seg_bits[blockIdx.x]};
-    const std::uint64_t w0{start >> 5};
-    const std::uint64_t nw{((end + 31) >> 5) - w0};
-    const bool staged{nw <= static_cast<std::uint64_t>(kEmitWords)};
-    if (staged) {
-        for (int i{tid}; i < static_cast<int>(nw); i += 256) {
-            buf[i] = 0;
+    const int g{static_cast<int>(blockIdx.x)};
+    const int bx0{(g % static_cast<int>(geo.xg)) * kAcGroupBlocks};
+    const int by0{(g / static_cast<int>(geo.xg)) * kAcGroupBlocks};
+    const int gbw{min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0)};
+    const int gbh{min(kAcGroupBlocks, static_cast<int>(geo.bh) - by0)};
+    const int row_items{3 * gbw};
+    const int items{row_items * gbh};
+    std::uint32_t* const slot{staging + static_cast<std::size_t>(g) * kGroupSlotWords};
+
+    // 1. Token prefix over the group's items.
+    std::uint32_t carry{0};
+    for (int base{0}; base < items; base += kGroupThreads) {
+        const int i{base + tid};
+        std::uint32_t cnt{0};
+        if (i < items) {
+            const int row{i / row_items}, rem{i % row_items};
+            const std::size_t pos{static_cast<std::size_t>(by0 + row) * geo.bw + bx0 + rem / 3};
+            const int n{acs[pos]};
+            S.m[i] = static_cast<std::int8_t>(n / 8);
+            cnt = n != 0 ? item_tokens[pos * 3 + rem % 3] : 0u;
         }
-    }
-    const std::uint32_t total{load_segment(geo, seg, acs, item_tokens, S)};
-    const std::uint64_t lbase{w0 << 5};
-    std::uint64_t run{start};
-    for (std::uint32_t f0{0}; f0 < total; f0 += 256) {
-        const std::uint32_t f{f0 + static_cast<std::uint32_t>(tid)};
-        std::uint32_t len{0}, val{0};
-        if (f < total) {
-            const std::uint32_t r{segment_token(geo, S, seg.items, f, tokens)};
-            const std::uint32_t idx{(r & 31u) * kHistStride + ((r >> 5) & 255u)};
-            const std::uint32_t d{depth[idx]};
-            len = d + ((r >> 13) & 31u);
-            val = static_cast<std::uint32_t>(codes[idx]) | ((r >> 18) << d);
+        std::uint64_t total{0};
+        const std::uint64_t p{block_excl_scan(cnt, S.sh, total)};
+        if (i < items) {
+            S.prefix[i] = carry + static_cast<std::uint32_t>(p);
         }
-        std::uint64_t chunk{0};
-        const std::uint64_t at{run + block_excl_scan(len, S.sh, chunk)};
-        if (len != 0) {
-            if (staged) {
-                if (len < 32) {
-                    val &= (1u << len) - 1u;
-                }
-                const std::uint64_t rel{at - lbase};
-                const std::uint32_t wi{static_cast<std::uint32_t>(rel >> 5)};
-                const std::uint32_t b{static_cast<std::uint32_t>(rel & 31u)};
-                if (wi < static_cast<std::uint32_t>(kEmitWords)) {
-                    atomicOr(&buf[wi], val << b);
-                }
-                if (b + len > 32 && wi + 1 < static_cast<std::uint32_t>(kEmitWords)) {
-                    atomicOr(&buf[wi + 1], val >> (32 - b));
-                }
-            } else {
-                or_bits(out_words, cap_words, at, val, len);
-            }
-        }
-        run += chunk;
+        carry += static_cast<std::uint32_t>(total);
     }
-    if (!staged) {
-        return;
+    if (tid == 0) {
+        S.prefix[items] = carry;
+        S.overflow = 0;
+    }
+    for (int i{tid}; i < kGroupWords; i += kGroupThreads) {
+        S.buf[i] = 0;
     }
     __syncthreads();
-    for (int i{tid}; i < static_cast<int>(nw); i += 256) {
-        const std::uint64_t gi{w0 + static_cast<std::uint64_t>(i)};
-        if (gi >= cap_words) {
+
+    // 2. Token lengths, positions and bits; twice if the LDS copy overflows.
+    // Each thread takes kGroupPerThread consecutive tokens: one binary search
+    // for the first, then a forward walk over the item prefix.
+    std::uint64_t run{0};
+    for (int pass{0}; pass < 2; ++pass) {
+        const bool global{pass == 1};
+        if (global) {
+            if (S.overflow == 0) {
+                break;
+            }
+            const std::uint64_t nw{(run + 31) / 32};
+            for (std::uint64_t i{static_cast<std::uint64_t>(tid)}; i < nw && i < kGroupSlotWords; i += kGroupThreads) {
+                slot[i] = 0;
+            }
+            __syncthreads();
+            run = 0;
+        }
+        for (std::uint32_t f0{0}; f0 < carry; f0 += kGroupThreads * kGroupPerThread) {
+            const std::uint32_t fb{f0 + static_cast<std::uint32_t>(tid) * kGroupPerThread};
+            std::uint32_t rec[kGroupPerThread];
+            int lo{0};
+            if (fb < carry) {
+                int hi{items - 1};
+                while (lo < hi) {
+                    const int mid{(lo + hi + 1) >> 1};
+                    if (S.prefix[mid] <= fb) {
+                        lo = mid;
+                    } else {
+                        hi = mid - 1;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j{0}; j < kGroupPerThread; ++j) {
+                const std::uint32_t f{fb + static_cast<std::uint32_t>(j)};
+                rec[j] = 0xffffffffu;
+                if (f < carry) {
+                    while (S.prefix[lo + 1] <= f) {
+                        ++lo;
+                    }
+                    const int row{lo / row_items}, rem{lo % row_items};
+                    const int ci{rem % 3};
+                    const int c{ci == 0 ? 1 : (ci == 1 ? 0 : 2)};
+                    rec[j] = tokens[token_slot(geo, c, bx0 + rem / 3, by0 + row, S.m[lo], f - S.prefix[lo])];
+                }
+            }
+            std::uint32_t len[kGroupPerThread], val[kGroupPerThread];
+            std::uint32_t sum{0};
+#pragma unroll
+            for (int j{0}; j < kGroupPerThread; ++j) {
+                len[j] = 0;
+                val[j] = 0;
+                const std::uint32_t r{rec[j]};
+                if (r != 0xffffffffu) {
+                    const std::uint32_t idx{(r & 31u) * kHistStride + ((r >> 5) & 255u)};
+                    const std::uint32_t d{depth[idx]};
+                    len[j] = d + ((r >> 13) & 31u);
+                    val[j] = (static_cast<std::uint32_t>(codes[idx]) | ((r >> 18) << d)) & ((1u << len[j]) - 1u);
+                }
+                sum += len[j];
+            }
+            std::uint64_t chunk{0};
+            std::uint64_t at{run + block_excl_scan(sum, S.sh, chunk)};
+#pragma unroll
+            for (int j{0}; j < kGroupPerThread; ++j) {
+                if (len[j] != 0) {
+                    const std::uint64_t wi{at >> 5};
+                    const std::uint32_t b{static_cast<std::uint32_t>(at & 31u)};
+                    const bool two{b + len[j] > 32};
+                    if (global) {
+                        if (wi < kGroupSlotWords) {
+                            atomicOr(&slot[wi], val[j] << b);
+                        }
+                        if (two && wi + 1 < kGroupSlotWords) {
+                            atomicOr(&slot[wi + 1], val[j] >> (32 - b));
+                        }
+                    } else if (wi + (two ? 1 : 0) < static_cast<std::uint64_t>(kGroupWords)) {
+                        atomicOr(&S.buf[wi], val[j] << b);
+                        if (two) {
+                            atomicOr(&S.buf[wi + 1], val[j] >> (32 - b));
+                        }
+                    } else {
+                        S.overflow = 1;
+                    }
+                }
+                at += len[j];
+            }
+            run += chunk;
+        }
+        __syncthreads();
+    }
+    if (S.overflow == 0) {
+        const std::uint64_t nw{(run + 31) / 32};
+        for (std::uint64_t i{static_cast<std::uint64_t>(tid)}; i < nw; i += kGroupThreads) {
+            slot[i] = S.buf[i];
+        }
+    }
+    if (tid == 0) {
+        group_bits[g] = run;
+        group_bytes[g] = static_cast<std::uint32_t>((run + 7) / 8);
+    }
+}
+
+// Places each staged group at its section's bit position (byte-aligned
+// except in single-group frames): words strictly inside the range are
+// stored, the first and last are ORed into the zeroed output.
+__global__ __launch_bounds__(256) void ac_group_place_kernel(const std::uint32_t* __restrict__ staging,
+                                                             const std::uint64_t* __restrict__ group_bits,
+                                                             const std::uint64_t* __restrict__ base_bits,
+                                                             std::uint32_t* __restrict__ out_words,
+                                                             std::uint64_t cap_words) {
+    const int g{static_cast<int>(blockIdx.x)};
+    const std::uint32_t* src{staging + static_cast<std::size_t>(g) * kGroupSlotWords};
+    const std::uint64_t bits{group_bits[g]};
+    if (bits == 0) {
+        return;
+    }
+    const std::uint64_t start{base_bits[g]};
+    const std::uint64_t w0{start >> 5};
+    const std::uint32_t sh{static_cast<std::uint32_t>(start & 31u)};
+    const std::uint64_t nsrc{(bits + 31) / 32};
+    const std::uint64_t nout{((start + bits + 31) >> 5) - w0};
+    for (std::uint64_t i{threadIdx.x}; i < nout; i += 256) {
+        std::uint32_t v;
+        if (sh == 0) {
+            v = i < nsrc ? src[i] : 0u;
+        } else {
+            const std::uint32_t lo{i >= 1 && i - 1 < nsrc ? src[i - 1] >> (32 - sh) : 0u};
+            const std::uint32_t hi{i < nsrc ? src[i] << sh : 0u};
+            v = lo | hi;
+        }
+        const std::uint64_t o{w0 + i};
+        if (o >= cap_words) {
             continue;
         }
-        const std::uint32_t v{buf[i]};
-        if (i == 0 || i == static_cast<int>(nw) - 1) {
+        if (i == 0 || i + 1 == nout) {
             if (v != 0) {
-                atomicOr(&out_words[gi], v);
+                atomicOr(&out_words[o], v);
             }
         } else {
-            out_words[gi] = v;
+            out_words[o] = v;
         }
     }
 }

@@ -55,11 +55,14 @@ bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint
 void launch_huffman(hipStream_t s, int first, int count, const std::uint32_t* hists, CodeSet* cs);
 void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                      const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words);
-void launch_ac_seg_bits(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
-                        const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
-                        std::uint32_t* seg_bits);
-void launch_ac_group_scan(hipStream_t s, const Geometry& geo, const std::uint32_t* seg_bits,
-                          std::uint32_t* seg_off, std::uint32_t* group_bytes, std::uint64_t* group_bits);
+// Words of one AC group's staging slot (bound on its token stream).
+std::size_t ac_group_slot_words();
+void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
+                          const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
+                          std::uint32_t* staging, std::uint32_t* group_bytes, std::uint64_t* group_bits);
+void launch_ac_group_place(hipStream_t s, const Geometry& geo, const std::uint32_t* staging,
+                           const std::uint64_t* group_bits, const std::uint64_t* ac_base, std::uint32_t* out_words,
+                           std::uint64_t cap_words);
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
@@ -70,10 +73,6 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
                    Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base);
 void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* lay, std::uint32_t* out_words,
                              std::uint64_t cap_words);
-void launch_ac_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs, const std::uint16_t* item_tokens,
-                    const std::uint32_t* tokens, const CodeSet* cs, const std::uint32_t* seg_bits,
-                    const std::uint32_t* seg_off, const std::uint64_t* ac_base, std::uint32_t* out_words,
-                    std::uint64_t cap_words);
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
