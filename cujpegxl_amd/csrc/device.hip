@@ -136,8 +136,22 @@ void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32
 void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                       const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* tokens,
                       std::uint16_t* item_tokens, std::uint32_t* partial, std::uint32_t* hist) {
+    // One resident round of workgroups (occupancy x CUs of the current
+    // device, at most kTokenGrid): a second partial round would double the
+    // kernel's time for a few waves.
+    static const unsigned int resident{[] {
+        int per_cu{0}, dev{0}, cus{0};
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ac_tokens_kernel, 256, 0) != hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            per_cu <= 0 || cus <= 0) {
+            return static_cast<unsigned int>(kTokenGrid);
+        }
+        const int n{per_cu * cus};
+        return static_cast<unsigned int>(n < kTokenGrid ? n : kTokenGrid);
+    }()};
     const unsigned int waves{(3 * geo.nblocks + 3) / 4};
-    const unsigned int grid{waves < static_cast<unsigned int>(kTokenGrid) ? waves : static_cast<unsigned int>(kTokenGrid)};
+    const unsigned int grid{waves < resident ? waves : resident};
     ac_tokens_kernel<<<grid, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, tokens, item_tokens, partial);
     ac_hist_reduce_kernel<<<dim3(kNumClusters * kAcSymbols / 256, 32), 256, 0, s>>>(partial, static_cast<int>(grid),
                                                                                     hist);

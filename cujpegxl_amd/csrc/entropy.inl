@@ -115,6 +115,13 @@ __device__ __forceinline__ void lds_or_bits(std::uint32_t* words, std::uint32_t 
     }
 }
 
+// Orders one wave's LDS accesses: writes before the barrier are seen by
+// every lane's reads after it.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 struct AcItem {
     int n;             // transform side, 0 if covered
     int c;             // XYB channel
@@ -153,7 +160,9 @@ __device__ __forceinline__ AcItem ac_item(const Geometry& geo, const std::int8_t
 }
 
 // Tokenizes two items per wave, one per 32-lane half (item `it` of this
-// lane's half; `has` false for a half without an item).  `emit(valid, k,
+// lane's half, its coefficients staged in `coef` block after block, so that
+// a scan-order entry (block << 6 | position) indexes it directly; `has`
+// false for a half without an item).  `emit(valid, k,
 // cluster, sym, nbits, extra)` is called once per lane per chunk, k being the
 // token's index in the item: 0 for the non-zero-count token (lane 0 of the
 // half, first call), then one token per scan position from the first
@@ -163,12 +172,17 @@ __device__ __forceinline__ AcItem ac_item(const Geometry& geo, const std::int8_t
 // from popcounts of the half's ballot bits below it.
 // The tokenizer's lookup tables, staged in LDS once per workgroup (the
 // lookups are per-lane gathers).
+// Zero-density contexts per block context: (nz_ctx[rem] + freq_ctx[k]) * 2 +
+// prev stays below 458 because rem + k <= 64 (ac_context.h:78-84).
+constexpr int kZeroDensityContexts = 458;
+
 struct TokenTables {
     std::uint16_t order8[64];
     std::uint16_t order16[256];
     std::uint16_t order32[1024];
     std::uint16_t nz_ctx[64];
     std::uint16_t freq_ctx[64];
+    std::uint8_t zero_band[kZeroDensityContexts];  // ac_cluster's band of each zero-density context
 };
 
 __device__ __forceinline__ void load_token_tables(TokenTables& T) {
@@ -182,13 +196,15 @@ __device__ __forceinline__ void load_token_tables(TokenTables& T) {
             T.nz_ctx[i] = c_nz_ctx[i];
             T.freq_ctx[i] = c_freq_ctx[i];
         }
+        if (i < kZeroDensityContexts) {
+            T.zero_band[i] = static_cast<std::uint8_t>(ac_cluster(15u * 37u + static_cast<std::uint32_t>(i)) - 8);
+        }
     }
 }
 
 template <typename Emit>
-__device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const std::int16_t* ac,
-                                                   const TokenTables& T, const AcItem& it, bool has, int lane,
-                                                   Emit&& emit) {
+__device__ __forceinline__ void tokenize_item_pair(const TokenTables& T, const std::int16_t* coef, const AcItem& it,
+                                                   bool has, int lane, Emit&& emit) {
     const int half{lane >> 5}, sub{lane & 31};
     {
         std::uint32_t sym{0}, nb{0}, bits{0};
@@ -200,8 +216,7 @@ __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const st
     const std::uint32_t cb{static_cast<std::uint32_t>(side * side)};
     const std::uint32_t size{static_cast<std::uint32_t>(n * n)};
     const std::uint32_t l2{n == 16 ? 2u : (n == 32 ? 4u : 0u)};
-    const std::uint32_t off{15u * 37u + 458u * static_cast<std::uint32_t>(it.bctx)};
-    const std::int16_t* plane{ac + static_cast<std::size_t>(it.c) * geo.nblocks * 64};
+    const int zgroup{it.bctx < 7 ? 8 : 20};  // 8 + 12 * (block context >= 7)
     const std::uint16_t* order{n == 32 ? T.order32 : (n == 16 ? T.order16 : T.order8)};
     std::uint32_t seen{0};
     std::uint32_t prev_carry{it.nz > size / 16 ? 0u : 1u};
@@ -214,11 +229,7 @@ __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const st
         const std::uint32_t k{base + static_cast<std::uint32_t>(sub)};
         int v{0};
         if (active && k < size) {
-            const std::uint32_t e{order[k]};
-            const std::uint32_t q{e >> 6};
-            const std::size_t slot{(static_cast<std::size_t>(it.by + q / side) * geo.bw + it.bx + q % side) * 64 +
-                                   (e & 63u)};
-            v = plane[slot];
+            v = coef[order[k]];
         }
         const std::uint32_t gm{static_cast<std::uint32_t>(__ballot(v != 0) >> (32 * half))};
         const std::uint32_t before{seen + static_cast<std::uint32_t>(__popc(gm & below))};
@@ -228,8 +239,7 @@ __device__ __forceinline__ void tokenize_item_pair(const Geometry& geo, const st
         if (valid) {
             const std::uint32_t rem{((it.nz - before) + cb - 1) >> l2};
             const std::uint32_t kk{k >> l2};
-            const std::uint32_t ctx{off + (T.nz_ctx[rem] + T.freq_ctx[kk]) * 2u + prev};
-            cl = static_cast<std::uint32_t>(ac_cluster(ctx));
+            cl = static_cast<std::uint32_t>(zgroup + T.zero_band[(T.nz_ctx[rem] + T.freq_ctx[kk]) * 2u + prev]);
             hybrid420(pack_signed(v), sym, nb, bits);
         }
         emit(valid, 1u + (k - cb), static_cast<int>(cl), sym, nb, bits);
@@ -260,9 +270,71 @@ __device__ __forceinline__ std::size_t token_slot(const Geometry& geo, int c, in
 // per-cluster histograms.  AC symbols are below 64 (hybrid-uint tokens of
 // 16-bit magnitudes and of counts <= 1024), so each workgroup counts into a
 // 32 x 64 LDS histogram and writes it out whole as one partial; a second
-// kernel sums the partials.  Grid-stride over item pairs, one wave per pair.
+// kernel sums the partials.  Grid-stride over item pairs, one wave per pair:
+// the descriptors of 32 pairs are loaded at once (one item per lane), and
+// each pair's coefficient blocks are read as 16-byte vectors while the
+// previous pair is tokenized from its LDS copy.
 constexpr int kAcSymbols = 64;
 constexpr int kTokenGrid = 2048;
+constexpr int kTokenBatch = 32;
+
+__device__ __forceinline__ AcItem shfl_item(const AcItem& d, int src) {
+    AcItem o;
+    o.n = __shfl(d.n, src, 64);
+    o.c = __shfl(d.c, src, 64);
+    o.bx = __shfl(d.bx, src, 64);
+    o.by = __shfl(d.by, src, 64);
+    o.nz = static_cast<std::uint32_t>(__shfl(static_cast<int>(d.nz), src, 64));
+    o.nz_ctx = static_cast<std::uint32_t>(__shfl(static_cast<int>(d.nz_ctx), src, 64));
+    o.bctx = __shfl(d.bctx, src, 64);
+    return o;
+}
+
+// The m x m coefficient blocks of an item (m = n / 8; 8 to 128 vectors of
+// 16 bytes), lane `sub` of the half taking vectors sub, sub + 32, sub + 64
+// and sub + 96 (four named registers, so the set never goes to scratch).
+struct ItemVecs {
+    uint4 r0, r1, r2, r3;
+};
+
+__device__ __forceinline__ int item_vecs(int n) {
+    return n == 32 ? 128 : (n == 16 ? 32 : (n == 8 ? 8 : 0));
+}
+
+__device__ __forceinline__ uint4 load_item_vec(const std::int16_t* plane, const Geometry& geo, const AcItem& it,
+                                               int l2s, int u) {
+    const int q{u >> 3};
+    const std::size_t blk{static_cast<std::size_t>(it.by + (q >> l2s)) * geo.bw +
+                          static_cast<std::size_t>(it.bx + (q & ((1 << l2s) - 1)))};
+    return *reinterpret_cast<const uint4*>(plane + blk * 64 + (u & 7) * 8);
+}
+
+__device__ __forceinline__ void load_item_coefs(const Geometry& geo, const std::int16_t* ac, const AcItem& it,
+                                                int sub, ItemVecs& v) {
+    const int l2s{it.n == 32 ? 2 : (it.n == 16 ? 1 : 0)};
+    const int vecs{item_vecs(it.n)};
+    const std::int16_t* plane{ac + static_cast<std::size_t>(it.c) * geo.nblocks * 64};
+    if (sub < vecs) {
+        v.r0 = load_item_vec(plane, geo, it, l2s, sub);
+    }
+    if (sub + 32 < vecs) {
+        v.r1 = load_item_vec(plane, geo, it, l2s, sub + 32);
+        v.r2 = load_item_vec(plane, geo, it, l2s, sub + 64);
+        v.r3 = load_item_vec(plane, geo, it, l2s, sub + 96);
+    }
+}
+
+__device__ __forceinline__ void store_item_coefs(std::int16_t* coef, const AcItem& it, int sub, const ItemVecs& v) {
+    const int vecs{item_vecs(it.n)};
+    if (sub < vecs) {
+        *reinterpret_cast<uint4*>(coef + sub * 8) = v.r0;
+    }
+    if (sub + 32 < vecs) {
+        *reinterpret_cast<uint4*>(coef + (sub + 32) * 8) = v.r1;
+        *reinterpret_cast<uint4*>(coef + (sub + 64) * 8) = v.r2;
+        *reinterpret_cast<uint4*>(coef + (sub + 96) * 8) = v.r3;
+    }
+}
 
 __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
                                                         const std::int8_t* __restrict__ acs,
@@ -273,35 +345,64 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
                                                         std::uint32_t* __restrict__ partial) {
     __shared__ std::uint32_t h[kNumClusters * kAcSymbols];
     __shared__ TokenTables T;
+    __shared__ __align__(16) std::int16_t stage[4][2][1024];
     for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
         h[i] = 0;
     }
     load_token_tables(T);
     __syncthreads();
     const int lane{static_cast<int>(threadIdx.x) & 63};
-    const int half{lane >> 5};
+    const int half{lane >> 5}, sub{lane & 31};
+    std::int16_t* const coef{stage[threadIdx.x >> 6][half]};
     const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
-    const std::uint32_t nwaves{gridDim.x * 4};
+    const std::uint32_t stride{2 * gridDim.x * 4};
     const std::uint32_t items{geo.nblocks * 3};
-    for (std::uint32_t pair{2 * wave}; pair < items; pair += 2 * nwaves) {
-        const std::uint32_t item{pair + static_cast<std::uint32_t>(half)};
-        const bool in{item < items};
-        const AcItem it{in ? ac_item(geo, acs, nz_grid, nz_count, item) : AcItem{}};
-        const bool has{in && it.n != 0};
-        const int m{has ? it.n / 8 : 1};
-        std::uint32_t count{0};
-        tokenize_item_pair(geo, ac, T, it, has, lane,
-                           [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
-                               std::uint32_t extra) {
-                               if (valid) {
-                                   atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
-                                   tokens[token_slot(geo, it.c, it.bx, it.by, m, k)] = token_record(cl, sym, nb, extra);
-                               }
-                               count += static_cast<std::uint32_t>(
-                                   __popc(static_cast<std::uint32_t>(__ballot(valid) >> (32 * half))));
-                           });
-        if ((lane & 31) == 0 && in) {
-            item_tokens[item] = static_cast<std::uint16_t>(count);
+    for (std::uint32_t base{2 * wave}; base < items; base += stride * kTokenBatch) {
+        const std::uint32_t li{base + stride * static_cast<std::uint32_t>(lane >> 1) +
+                               static_cast<std::uint32_t>(lane & 1)};
+        const AcItem d{li < items ? ac_item(geo, acs, nz_grid, nz_count, li) : AcItem{}};
+        const int npairs{static_cast<int>(min(static_cast<std::uint32_t>(kTokenBatch),
+                                              (items - base + stride - 1) / stride))};
+        AcItem cur{shfl_item(d, half)};
+        ItemVecs v{};
+        load_item_coefs(geo, ac, cur, sub, v);
+        for (int t{0}; t < npairs; ++t) {
+            const std::uint32_t item{base + stride * static_cast<std::uint32_t>(t) + static_cast<std::uint32_t>(half)};
+            wave_sync();
+            store_item_coefs(coef, cur, sub, v);
+            wave_sync();
+            AcItem nx{};
+            if (t + 1 < npairs) {
+                nx = shfl_item(d, 2 * (t + 1) + half);
+                load_item_coefs(geo, ac, nx, sub, v);
+            }
+            const bool has{cur.n != 0};
+            std::uint32_t count{0};
+            if (__ballot(has) != 0) {
+                const AcItem it{cur};
+                const std::uint32_t m{has ? static_cast<std::uint32_t>(it.n / 8) : 1u};
+                const std::uint32_t l2m{m >> 1};
+                // token_slot in 32 bits (3 * blocks * 64 < 2^32 for the
+                // at most 32 DC groups a frame may have).
+                const std::uint32_t slot0{static_cast<std::uint32_t>(token_slot(geo, has ? it.c : 0, it.bx, it.by, 1, 0))};
+                const std::uint32_t row{geo.bw * 64};
+                tokenize_item_pair(T, coef, it, has, lane,
+                                   [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
+                                       std::uint32_t extra) {
+                                       if (valid) {
+                                           atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
+                                           const std::uint32_t q{k >> 6};
+                                           tokens[slot0 + (q >> l2m) * row + ((q & (m - 1)) << 6) + (k & 63u)] =
+                                               token_record(cl, sym, nb, extra);
+                                       }
+                                       count += static_cast<std::uint32_t>(
+                                           __popc(static_cast<std::uint32_t>(__ballot(valid) >> (32 * half))));
+                                   });
+            }
+            if (sub == 0 && item < items) {
+                item_tokens[item] = static_cast<std::uint16_t>(count);
+            }
+            cur = nx;
         }
     }
     __syncthreads();
@@ -379,7 +480,9 @@ constexpr int kGroupPerThread = 8;
 constexpr std::size_t kGroupSlotWords = 3ull * kAcGroupBlocks * kAcGroupBlocks * 64 * 28 / 32 + 64;
 
 struct GroupShared {
+    std::uint32_t lut[kNumClusters * kAcSymbols];  // depth | code << 16
     std::uint32_t prefix[kGroupItems + 1];
+    std::uint32_t slot0[kGroupItems];  // token slot of the item's first token
     std::int8_t m[kGroupItems];
     std::uint32_t buf[kGroupWords];
     std::uint64_t sh[17];
@@ -411,8 +514,11 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
             const int row{i / row_items}, rem{i % row_items};
             const std::size_t pos{static_cast<std::size_t>(by0 + row) * geo.bw + bx0 + rem / 3};
             const int n{acs[pos]};
+            const int ci{rem % 3};
             S.m[i] = static_cast<std::int8_t>(n / 8);
-            cnt = n != 0 ? item_tokens[pos * 3 + rem % 3] : 0u;
+            S.slot0[i] = static_cast<std::uint32_t>(
+                token_slot(geo, ci == 0 ? 1 : (ci == 1 ? 0 : 2), bx0 + rem / 3, by0 + row, 1, 0));
+            cnt = n != 0 ? item_tokens[pos * 3 + ci] : 0u;
         }
         std::uint64_t total{0};
         const std::uint64_t p{block_excl_scan(cnt, S.sh, total)};
@@ -424,6 +530,10 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
     if (tid == 0) {
         S.prefix[items] = carry;
         S.overflow = 0;
+    }
+    for (int i{tid}; i < kNumClusters * kAcSymbols; i += kGroupThreads) {
+        const int at{(i / kAcSymbols) * kHistStride + i % kAcSymbols};
+        S.lut[i] = static_cast<std::uint32_t>(depth[at]) | (static_cast<std::uint32_t>(codes[at]) << 16);
     }
     for (int i{tid}; i < kGroupWords; i += kGroupThreads) {
         S.buf[i] = 0;
@@ -470,10 +580,10 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
                     while (S.prefix[lo + 1] <= f) {
                         ++lo;
                     }
-                    const int row{lo / row_items}, rem{lo % row_items};
-                    const int ci{rem % 3};
-                    const int c{ci == 0 ? 1 : (ci == 1 ? 0 : 2)};
-                    rec[j] = tokens[token_slot(geo, c, bx0 + rem / 3, by0 + row, S.m[lo], f - S.prefix[lo])];
+                    const std::uint32_t k{f - S.prefix[lo]};
+                    const std::uint32_t m{static_cast<std::uint32_t>(S.m[lo])};
+                    const std::uint32_t q{k >> 6};
+                    rec[j] = tokens[S.slot0[lo] + (q >> (m >> 1)) * (geo.bw * 64) + ((q & (m - 1)) << 6) + (k & 63u)];
                 }
             }
             std::uint32_t len[kGroupPerThread], val[kGroupPerThread];
@@ -484,10 +594,10 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
                 val[j] = 0;
                 const std::uint32_t r{rec[j]};
                 if (r != 0xffffffffu) {
-                    const std::uint32_t idx{(r & 31u) * kHistStride + ((r >> 5) & 255u)};
-                    const std::uint32_t d{depth[idx]};
+                    const std::uint32_t e{S.lut[(r & 31u) * kAcSymbols + ((r >> 5) & (kAcSymbols - 1))]};
+                    const std::uint32_t d{e & 0xffffu};
                     len[j] = d + ((r >> 13) & 31u);
-                    val[j] = (static_cast<std::uint32_t>(codes[idx]) | ((r >> 18) << d)) & ((1u << len[j]) - 1u);
+                    val[j] = ((e >> 16) | ((r >> 18) << d)) & ((1u << len[j]) - 1u);
                 }
                 sum += len[j];
             }

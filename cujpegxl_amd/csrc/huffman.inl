@@ -21,10 +21,6 @@ constexpr std::uint32_t kQueueEnd = 0xffffffffu;
 
 // Orders one wave's LDS accesses (its LDS instructions execute in order; this
 // keeps the compiler from moving them across the point).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // R values per lane: element idx lives in register idx / 64 of lane idx % 64.
 template <int R>
