@@ -45,6 +45,7 @@ struct Layout {
     std::uint32_t ac_global_words[kAcGlobalWords];
     std::uint32_t head_words[kHeadWords];
     std::uint32_t head_total_bits;
+    std::uint32_t framing_words[12];  // container framing, 40 or 48 bytes
 };
 
 // One 16-wave workgroup per histogram (histograms first, first + 1, ...):
@@ -373,6 +374,33 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
             run += chunk;
         }
     }
+    // Container framing (container.cc:35-54): the 32-byte signature and ftyp
+    // box, then the jxlc box header (8 bytes, or 16 with a 64-bit size).
+    // Lane i forms bytes 4i..4i+3.
+    if (t < static_cast<int>(framing / 4)) {
+        constexpr std::uint8_t sig[32]{0,   0,   0, 0x0c, 'J', 'X', 'L', ' ', 0x0d, 0x0a, 0x87,
+                                       0x0a, 0,  0, 0,    0x14, 'f', 't', 'y', 'p', 'j',  'x',
+                                       'l', ' ', 0, 0,    0,    0,   'j', 'x', 'l', ' '};
+        const bool large{framing == 48};
+        const std::uint64_t box{cs_bytes + (large ? 16 : 8)};
+        const std::uint32_t field{large ? 1u : static_cast<std::uint32_t>(box)};
+        std::uint32_t word{0};
+        for (int j{0}; j < 4; ++j) {
+            const int i{4 * t + j};
+            std::uint8_t b;
+            if (i < 32) {
+                b = sig[i];
+            } else if (i < 36) {
+                b = static_cast<std::uint8_t>(field >> (8 * (35 - i)));
+            } else if (i < 40) {
+                b = static_cast<std::uint8_t>("jxlc"[i - 36]);
+            } else {
+                b = static_cast<std::uint8_t>(box >> (8 * (47 - i)));
+            }
+            word |= static_cast<std::uint32_t>(b) << (8 * j);
+        }
+        lay->framing_words[t] = word;
+    }
     __syncthreads();
     for (int i{t}; i < kAcGlobalWords; i += 256) {
         lay->ac_global_words[i] = agw[i];
@@ -393,59 +421,106 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
     }
 }
 
-__global__ void zero_kernel(const Layout* __restrict__ lay, std::uint32_t* __restrict__ out_words,
-                            std::uint64_t cap_words) {
-    std::uint64_t words{(lay->total_bytes + 3) / 4};
-    words = words < cap_words ? words : cap_words;
-    for (std::uint64_t i{blockIdx.x * 256ull + threadIdx.x}; i < words; i += gridDim.x * 256ull) {
-        out_words[i] = 0;
+// The file is a sequence of bit strings ("pieces") laid end to end, each at
+// the position the layout gave it: framing, head (headers and TOC), DcGlobal,
+// the DC groups, AcGlobal, the AC groups.  Gaps between pieces (section
+// padding) are zero bits.  Each output word is built once from the pieces
+// that overlap it and stored whole, so the output needs no clearing and no
+// atomics.
+struct Pieces {
+    const std::uint32_t* dc_staging;
+    std::uint64_t dc_slot_words;
+    const std::uint32_t* ac_staging;
+    std::uint64_t ac_slot_words;
+    const std::uint64_t* dc_bits;
+    const std::uint64_t* ac_bits;
+    const std::uint64_t* dc_base;
+    const std::uint64_t* ac_base;
+};
+
+constexpr int kMaxPieces = 4 + 32 + 32 * 64;  // at most 32 DC groups of 64 AC groups
+
+struct Piece {
+    std::uint64_t dst;
+    std::uint64_t nbits;
+    const std::uint32_t* src;
+};
+
+__device__ __forceinline__ Piece piece_at(const Geometry& geo, const Constants* k, const Layout* lay,
+                                          const Pieces& P, int i) {
+    const int nd{static_cast<int>(geo.num_dc)};
+    if (i == 0) {
+        return {0, static_cast<std::uint64_t>(lay->framing_bytes) * 8, lay->framing_words};
     }
+    if (i == 1) {
+        return {static_cast<std::uint64_t>(lay->framing_bytes) * 8, lay->head_total_bits, lay->head_words};
+    }
+    if (i == 2) {
+        return {lay->dc_global_bit, k->dc_global_bits, k->dc_global_words};
+    }
+    if (i < 3 + nd) {
+        const int g{i - 3};
+        return {P.dc_base[g], P.dc_bits[g], P.dc_staging + static_cast<std::size_t>(g) * P.dc_slot_words};
+    }
+    if (i == 3 + nd) {
+        return {lay->ac_global_bit, lay->ac_global_bits, lay->ac_global_words};
+    }
+    const int g{i - 4 - nd};
+    return {P.ac_base[g], P.ac_bits[g], P.ac_staging + static_cast<std::size_t>(g) * P.ac_slot_words};
 }
 
-// Container framing (container.cc:35-54), head, DcGlobal, AcGlobal.
-__global__ __launch_bounds__(64) void headers_kernel(const Constants* __restrict__ k,
-                                                     const Layout* __restrict__ lay,
-                                                     std::uint32_t* __restrict__ out_words,
-                                                     std::uint64_t cap_words) {
-    const int t{static_cast<int>(threadIdx.x)};
-    // The framing is 40 or 48 bytes, so it owns whole words: lane i stores
-    // bytes 4i..4i+3.
-    if (t < static_cast<int>(lay->framing_bytes / 4) && static_cast<std::uint64_t>(t) < cap_words) {
-        constexpr std::uint8_t hdr[32]{0,   0,   0, 0x0c, 'J', 'X', 'L', ' ', 0x0d, 0x0a, 0x87,
-                                       0x0a, 0,  0, 0,    0x14, 'f', 't', 'y', 'p', 'j',  'x',
-                                       'l', ' ', 0, 0,    0,    0,   'j', 'x', 'l', ' '};
-        const bool large{lay->framing_bytes == 48};
-        const std::uint64_t box{lay->codestream_bytes + (large ? 16 : 8)};
-        const std::uint32_t field{large ? 1u : static_cast<std::uint32_t>(box)};
-        std::uint32_t word{0};
-        for (int j{0}; j < 4; ++j) {
-            const int i{4 * t + j};
-            std::uint8_t b;
-            if (i < 32) {
-                b = hdr[i];
-            } else if (i < 36) {
-                b = static_cast<std::uint8_t>(field >> (8 * (35 - i)));
-            } else if (i < 40) {
-                b = static_cast<std::uint8_t>("jxlc"[i - 36]);
+__global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Constants* __restrict__ k,
+                                                       const Layout* __restrict__ lay, Pieces P,
+                                                       std::uint32_t* __restrict__ out_words,
+                                                       std::uint64_t cap_words) {
+    __shared__ std::uint64_t end_bit[kMaxPieces];
+    if (lay->overflow != 0) {
+        return;
+    }
+    const int n{4 + static_cast<int>(geo.num_dc) + static_cast<int>(geo.num_ac)};
+    for (int i{static_cast<int>(threadIdx.x)}; i < n; i += 256) {
+        const Piece pc{piece_at(geo, k, lay, P, i)};
+        end_bit[i] = pc.dst + pc.nbits;
+    }
+    __syncthreads();
+    std::uint64_t words{(lay->total_bytes + 3) / 4};
+    words = words < cap_words ? words : cap_words;
+    for (std::uint64_t w{blockIdx.x * 256ull + threadIdx.x}; w < words; w += gridDim.x * 256ull) {
+        const std::uint64_t b0{w * 32};
+        // First piece ending after b0 (pieces are ordered and disjoint).
+        int lo{0}, hi{n};
+        while (lo < hi) {
+            const int mid{(lo + hi) >> 1};
+            if (end_bit[mid] > b0) {
+                hi = mid;
             } else {
-                b = static_cast<std::uint8_t>(box >> (8 * (47 - i)));
+                lo = mid + 1;
             }
-            word |= static_cast<std::uint32_t>(b) << (8 * j);
         }
-        out_words[t] = word;
-    }
-    const std::uint64_t head_bit{static_cast<std::uint64_t>(lay->framing_bytes) * 8};
-    for (std::uint32_t i{static_cast<std::uint32_t>(t) * 32}; i < lay->head_total_bits; i += 64 * 32) {
-        const std::uint32_t n{lay->head_total_bits - i < 32 ? lay->head_total_bits - i : 32};
-        or_bits(out_words, cap_words, head_bit + i, lay->head_words[i >> 5], n);
-    }
-    for (std::uint32_t i{static_cast<std::uint32_t>(t) * 32}; i < k->dc_global_bits; i += 64 * 32) {
-        const std::uint32_t n{k->dc_global_bits - i < 32 ? k->dc_global_bits - i : 32};
-        or_bits(out_words, cap_words, lay->dc_global_bit + i, k->dc_global_words[i >> 5], n);
-    }
-    for (std::uint32_t i{static_cast<std::uint32_t>(t) * 32}; i < lay->ac_global_bits; i += 64 * 32) {
-        const std::uint32_t n{lay->ac_global_bits - i < 32 ? lay->ac_global_bits - i : 32};
-        or_bits(out_words, cap_words, lay->ac_global_bit + i, lay->ac_global_words[i >> 5], n);
+        std::uint32_t v{0};
+        for (int i{lo}; i < n; ++i) {
+            const Piece pc{piece_at(geo, k, lay, P, i)};
+            if (pc.dst >= b0 + 32) {
+                break;
+            }
+            const std::uint64_t s{pc.dst > b0 ? pc.dst : b0};
+            const std::uint64_t e{pc.dst + pc.nbits < b0 + 32 ? pc.dst + pc.nbits : b0 + 32};
+            if (s >= e) {
+                continue;
+            }
+            const std::uint64_t o{s - pc.dst};
+            const std::uint32_t len{static_cast<std::uint32_t>(e - s)};
+            const std::uint32_t sh{static_cast<std::uint32_t>(o & 31u)};
+            std::uint32_t x{pc.src[o >> 5] >> sh};
+            if (sh != 0 && len > 32 - sh) {
+                x |= pc.src[(o >> 5) + 1] << (32 - sh);
+            }
+            if (len < 32) {
+                x &= (1u << len) - 1u;
+            }
+            v |= x << static_cast<std::uint32_t>(s - b0);
+        }
+        out_words[w] = v;
     }
 }
 

@@ -218,21 +218,21 @@ void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t*
                                                               &cs->code[0][0], staging, group_bytes, group_bits);
 }
 
-void launch_ac_group_place(hipStream_t s, const Geometry& geo, const std::uint32_t* staging,
-                           const std::uint64_t* group_bits, const std::uint64_t* ac_base, std::uint32_t* out_words,
-                           std::uint64_t cap_words) {
-    ac_group_place_kernel<<<geo.num_ac, 256, 0, s>>>(staging, group_bits, ac_base, out_words, cap_words);
+std::size_t dc_group_slot_words() {
+    // Every DC item is at most 32 bits; plus the two header blobs.
+    return static_cast<std::size_t>(dc_chunks_per_group()) * kDcChunk + 2 * kBlobWords + 64;
 }
 
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
-                    std::uint32_t* group_bytes, std::uint64_t* group_bits) {
+                    std::uint32_t* group_bytes, std::uint64_t* group_bits, std::uint32_t* staging) {
     const std::uint32_t cpg{dc_chunks_per_group()};
     const dim3 grid{cpg, geo.num_dc};
     dc_chunk_bits_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
                                               &cs->depth[kNumClusters + geo.num_dc][0], blobs, chunk_bits, cpg);
-    dc_chunk_scan_kernel<<<geo.num_dc, 1024, 0, s>>>(geo, chunk_bits, cpg, group_bytes, group_bits);
+    dc_chunk_scan_kernel<<<geo.num_dc, 1024, 0, s>>>(geo, chunk_bits, cpg, group_bytes, group_bits, staging,
+                                                     dc_group_slot_words());
 }
 
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
@@ -243,23 +243,24 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
                                    blobs, lay, ac_base, dc_base);
 }
 
-void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* lay, std::uint32_t* out_words,
-                             std::uint64_t cap_words) {
-    zero_kernel<<<1024, 256, 0, s>>>(lay, out_words, cap_words);
-    headers_kernel<<<1, 64, 0, s>>>(k, lay, out_words, cap_words);
+void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
+                     const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
+                     const std::uint64_t* ac_bits, const std::uint64_t* dc_base, const std::uint64_t* ac_base,
+                     std::uint32_t* out_words, std::uint64_t cap_words) {
+    const Pieces P{dc_staging, dc_group_slot_words(), ac_staging, kGroupSlotWords, dc_bits, ac_bits, dc_base, ac_base};
+    assemble_kernel<<<2048, 256, 0, s>>>(geo, k, lay, P, out_words, cap_words);
 }
 
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
-                    const std::uint64_t* chunk_off, const std::uint64_t* dc_base, std::uint32_t* out_words,
-                    std::uint64_t cap_words) {
+                    const std::uint64_t* chunk_off, std::uint32_t* staging) {
     const std::uint32_t cpg{dc_chunks_per_group()};
     const dim3 grid{cpg, geo.num_dc};
     dc_emit_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
                                         &cs->code[kNumClusters][0], &cs->depth[kNumClusters + geo.num_dc][0],
                                         &cs->code[kNumClusters + geo.num_dc][0], blobs, blob_words, chunk_off,
-                                        cpg, dc_base, out_words, cap_words);
+                                        cpg, staging, dc_group_slot_words());
 }
 
 }  // namespace cjx

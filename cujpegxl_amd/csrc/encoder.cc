@@ -242,7 +242,7 @@ void Encoder::release() {
             e = nullptr;
         }
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_sizes_, &ev_headers_, &ev_dc_emit_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_}) {
         if (*e != nullptr) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -286,7 +286,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess) {
         return false;
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_sizes_, &ev_headers_, &ev_dc_emit_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             return false;
         }
@@ -344,6 +344,7 @@ bool Encoder::alloc() {
     get(item_tokens_, 3 * nblk * sizeof(std::uint16_t));
     get(hist_partial_, ac_partial_words() * sizeof(std::uint32_t));
     get(ac_staging_, static_cast<std::size_t>(geo_.num_ac) * ac_group_slot_words() * sizeof(std::uint32_t));
+    get(dc_staging_, static_cast<std::size_t>(geo_.num_dc) * dc_group_slot_words() * sizeof(std::uint32_t));
     get(ac_group_bytes_, geo_.num_ac * sizeof(std::uint32_t));
     get(ac_group_bits_, geo_.num_ac * sizeof(std::uint64_t));
     get(chunk_bits_, static_cast<std::size_t>(geo_.num_dc) * dc_chunks_per_group() * sizeof(std::uint64_t));
@@ -544,8 +545,10 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     launch_huffman(dc_stream_, kNumClusters, ndc, hists_, codes);
     launch_dc_blobs(dc_stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
     launch_dc_bits(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
-                   dc_group_bits_);
-    if (hipEventRecord(ev_dc_sizes_, dc_stream_) != hipSuccess) {
+                   dc_group_bits_, dc_staging_);
+    launch_dc_emit(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_,
+                   dc_staging_);
+    if (hipEventRecord(ev_dc_done_, dc_stream_) != hipSuccess) {
         return false;
     }
     launch_fill_u32(stream_, hists_, static_cast<std::size_t>(kNumClusters) * kHistStride, 0u);
@@ -555,26 +558,14 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     mark(Stage::kSizes);
     launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_group_bytes_,
                          ac_group_bits_);
-    if (hipStreamWaitEvent(stream_, ev_dc_sizes_, 0) != hipSuccess) {
+    if (hipStreamWaitEvent(stream_, ev_dc_done_, 0) != hipSuccess) {
         return false;
     }
     launch_layout(stream_, geo_, k, codes, ac_group_bytes_, dc_group_bytes_, ac_group_bits_, dc_group_bits_, blobs,
                   lay, ac_base_, dc_base_);
     mark(Stage::kEmit);
-    launch_zero_and_headers(stream_, k, lay, out_, out_cap_ / 4);
-    if (hipEventRecord(ev_headers_, stream_) != hipSuccess ||
-        hipStreamWaitEvent(dc_stream_, ev_headers_, 0) != hipSuccess) {
-        return false;
-    }
-    launch_dc_emit(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_base_,
-                   out_, out_cap_ / 4);
-    if (hipEventRecord(ev_dc_emit_, dc_stream_) != hipSuccess) {
-        return false;
-    }
-    launch_ac_group_place(stream_, geo_, ac_staging_, ac_group_bits_, ac_base_, out_, out_cap_ / 4);
-    if (hipStreamWaitEvent(stream_, ev_dc_emit_, 0) != hipSuccess) {
-        return false;
-    }
+    launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, dc_base_,
+                    ac_base_, out_, out_cap_ / 4);
     mark(Stage::kReadback);
     if (hipGetLastError() != hipSuccess ||
         hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),

@@ -77,7 +77,7 @@ private:
     FrameParams prm_{};
     hipStream_t stream_{nullptr};
     hipStream_t dc_stream_{nullptr};  // DC entropy chain, joined by events
-    hipEvent_t ev_fork_{nullptr}, ev_dc_sizes_{nullptr}, ev_headers_{nullptr}, ev_dc_emit_{nullptr};
+    hipEvent_t ev_fork_{nullptr}, ev_dc_done_{nullptr};  // fork onto / join from the DC stream
     bool profiling_{false};
     std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
     std::array<float, static_cast<int>(Stage::kCount)> stage_ms_{};
@@ -104,6 +104,7 @@ private:
     std::uint16_t* item_tokens_{nullptr};  // tokens per (block, channel) item
     std::uint32_t* hist_partial_{nullptr}; // per-workgroup AC histograms
     std::uint32_t* ac_staging_{nullptr};   // each AC group's stream at group-relative bits
+    std::uint32_t* dc_staging_{nullptr};   // each DC group's section at group-relative bits
     std::uint32_t* ac_group_bytes_{nullptr};
     std::uint64_t* ac_group_bits_{nullptr};
     std::uint64_t* chunk_bits_{nullptr};

@@ -643,48 +643,6 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
     }
 }
 
-// Places each staged group at its section's bit position (byte-aligned
-// except in single-group frames): words strictly inside the range are
-// stored, the first and last are ORed into the zeroed output.
-__global__ __launch_bounds__(256) void ac_group_place_kernel(const std::uint32_t* __restrict__ staging,
-                                                             const std::uint64_t* __restrict__ group_bits,
-                                                             const std::uint64_t* __restrict__ base_bits,
-                                                             std::uint32_t* __restrict__ out_words,
-                                                             std::uint64_t cap_words) {
-    const int g{static_cast<int>(blockIdx.x)};
-    const std::uint32_t* src{staging + static_cast<std::size_t>(g) * kGroupSlotWords};
-    const std::uint64_t bits{group_bits[g]};
-    if (bits == 0) {
-        return;
-    }
-    const std::uint64_t start{base_bits[g]};
-    const std::uint64_t w0{start >> 5};
-    const std::uint32_t sh{static_cast<std::uint32_t>(start & 31u)};
-    const std::uint64_t nsrc{(bits + 31) / 32};
-    const std::uint64_t nout{((start + bits + 31) >> 5) - w0};
-    for (std::uint64_t i{threadIdx.x}; i < nout; i += 256) {
-        std::uint32_t v;
-        if (sh == 0) {
-            v = i < nsrc ? src[i] : 0u;
-        } else {
-            const std::uint32_t lo{i >= 1 && i - 1 < nsrc ? src[i - 1] >> (32 - sh) : 0u};
-            const std::uint32_t hi{i < nsrc ? src[i] << sh : 0u};
-            v = lo | hi;
-        }
-        const std::uint64_t o{w0 + i};
-        if (o >= cap_words) {
-            continue;
-        }
-        if (i == 0 || i + 1 == nout) {
-            if (v != 0) {
-                atomicOr(&out_words[o], v);
-            }
-        } else {
-            out_words[o] = v;
-        }
-    }
-}
-
 // ---- DC groups -------------------------------------------------------------
 
 // Item kinds of a DcGroup section, in order (vardct_frame.cc:543-554):
@@ -967,10 +925,14 @@ __global__ __launch_bounds__(256) void dc_chunk_bits_kernel(
 }
 
 // Exclusive scan of chunk totals per group (one workgroup per group).
+// Also clears the words of the group's staging slot that its emission will
+// OR into.
 __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::uint64_t* __restrict__ chunk_bits,
                                                              std::uint32_t chunks_per_group,
                                                              std::uint32_t* __restrict__ group_bytes,
-                                                             std::uint64_t* __restrict__ group_bits) {
+                                                             std::uint64_t* __restrict__ group_bits,
+                                                             std::uint32_t* __restrict__ staging,
+                                                             std::uint64_t slot_words) {
     __shared__ std::uint64_t sh[17];
     const int g{static_cast<int>(blockIdx.x)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
@@ -989,12 +951,18 @@ __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::
         group_bytes[g] = static_cast<std::uint32_t>((total + 7) / 8);
         group_bits[g] = total;
     }
+    std::uint32_t* const slot{staging + static_cast<std::size_t>(g) * slot_words};
+    const std::uint64_t nw{(total + 31) / 32 < slot_words ? (total + 31) / 32 : slot_words};
+    for (std::uint64_t i{t}; i < nw; i += 1024) {
+        slot[i] = 0;
+    }
 }
 
-// Emission of one chunk of a DC group: the chunk's items occupy one
-// contiguous bit range, assembled in LDS (interior words plain-stored, the two
-// boundary words ORed into the zeroed output); a range longer than the LDS
-// copy is emitted with global atomics instead.
+// Emission of one chunk of a DC group into the group's staging slot, at
+// group-relative bit positions: the chunk's items occupy one contiguous bit
+// range, assembled in LDS (interior words plain-stored, the two boundary
+// words ORed into the cleared slot); a range longer than the LDS copy is
+// emitted with global atomics instead.
 constexpr int kDcEmitWords = 2048;
 
 __global__ __launch_bounds__(256) void dc_emit_kernel(
@@ -1004,8 +972,7 @@ __global__ __launch_bounds__(256) void dc_emit_kernel(
     const std::uint16_t* __restrict__ dc_codes, const std::uint8_t* __restrict__ meta_depth,
     const std::uint16_t* __restrict__ meta_codes, const DcBlobs* __restrict__ blobs,
     const std::uint32_t* __restrict__ blob_words, const std::uint64_t* __restrict__ chunk_off,
-    std::uint32_t chunks_per_group, const std::uint64_t* __restrict__ dc_group_base_bits,
-    std::uint32_t* __restrict__ out_words, std::uint64_t cap_words) {
+    std::uint32_t chunks_per_group, std::uint32_t* __restrict__ staging, std::uint64_t slot_words) {
     __shared__ std::uint64_t sh[5];
     __shared__ std::uint32_t buf[kDcEmitWords];
     const int g{static_cast<int>(blockIdx.y)};
@@ -1031,8 +998,9 @@ __global__ __launch_bounds__(256) void dc_emit_kernel(
     }
     std::uint64_t total{0};
     const std::uint64_t rel0{block_excl_scan(run, sh, total)};
-    const std::uint64_t start{dc_group_base_bits[g] + chunk_off[static_cast<std::size_t>(g) * chunks_per_group +
-                                                                 blockIdx.x]};
+    std::uint32_t* const out_words{staging + static_cast<std::size_t>(g) * slot_words};
+    const std::uint64_t cap_words{slot_words};
+    const std::uint64_t start{chunk_off[static_cast<std::size_t>(g) * chunks_per_group + blockIdx.x]};
     const std::uint64_t w0{start >> 5};
     const std::uint64_t nw{((start + total + 31) >> 5) - w0};
     const bool staged{nw <= static_cast<std::uint64_t>(kDcEmitWords)};

@@ -60,23 +60,23 @@ std::size_t ac_group_slot_words();
 void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
                           const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
                           std::uint32_t* staging, std::uint32_t* group_bytes, std::uint64_t* group_bits);
-void launch_ac_group_place(hipStream_t s, const Geometry& geo, const std::uint32_t* staging,
-                           const std::uint64_t* group_bits, const std::uint64_t* ac_base, std::uint32_t* out_words,
-                           std::uint64_t cap_words);
+// Words of one DC group's staging slot.
+std::size_t dc_group_slot_words();
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
-                    std::uint32_t* group_bytes, std::uint64_t* group_bits);
+                    std::uint32_t* group_bytes, std::uint64_t* group_bits, std::uint32_t* staging);
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                    const std::uint32_t* ac_group_bytes, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* ac_group_bits, const std::uint64_t* dc_group_bits, const DcBlobs* blobs,
                    Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base);
-void launch_zero_and_headers(hipStream_t s, const Constants* k, const Layout* lay, std::uint32_t* out_words,
-                             std::uint64_t cap_words);
+void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
+                     const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
+                     const std::uint64_t* ac_bits, const std::uint64_t* dc_base, const std::uint64_t* ac_base,
+                     std::uint32_t* out_words, std::uint64_t cap_words);
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
-                    const std::uint64_t* chunk_off, const std::uint64_t* dc_base, std::uint32_t* out_words,
-                    std::uint64_t cap_words);
+                    const std::uint64_t* chunk_off, std::uint32_t* staging);
 
 }  // namespace cjx
