@@ -4,9 +4,9 @@ Mirrors the reference's `pycujpegxl` module (python/pycujpegxl.cc:31-90):
 `encode`, `encode_with_stats`, `query_backend`, `api_version`, `API_VERSION`,
 with the same argument meaning and errors (ValueError for a wrongly sized or
 non-contiguous array, RuntimeError for an encode or device failure).  Unlike
-the reference, encoding goes through the C ABI (libcujpegxl.so.1).  Host NV12
-is uploaded with the stage library's device allocator, so no framework is
-needed for device memory.
+the reference, encoding goes through the C ABI (libcujpegxl.so.1): host NV12
+through cujpegxl_encoder_encode_host, which uploads the planes itself, so no
+framework is needed for device memory.
 """
 
 from __future__ import annotations
@@ -21,7 +21,7 @@ LIB_DIR = PKG / "lib"
 PUBLIC_LIB = LIB_DIR / "libcujpegxl.so.1"
 STAGES_LIB = LIB_DIR / "libcujpegxl_stages.so"
 
-API_VERSION = 3
+API_VERSION = 4
 
 OK, NOT_IMPLEMENTED, INVALID_ARGUMENT, UNSUPPORTED_RESOLUTION, NO_DEVICE, INTERNAL_ERROR, \
     BUFFER_TOO_SMALL = range(7)
@@ -37,6 +37,11 @@ BUFFERS = {
 class Config(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("distance", C.c_float),
                 ("device_ordinal", C.c_int32)]
+
+
+class Nv12HostInput(C.Structure):
+    _fields_ = [("luma", C.c_void_p), ("chroma", C.c_void_p), ("luma_pitch", C.c_size_t),
+                ("chroma_pitch", C.c_size_t)]
 
 
 class Nv12Input(C.Structure):
@@ -65,6 +70,9 @@ def _bind_public(L: C.CDLL) -> C.CDLL:
     L.cujpegxl_encoder_encode.argtypes = [C.c_void_p, C.POINTER(Nv12Input), C.POINTER(Output)]
     L.cujpegxl_encoder_destroy.argtypes = [C.c_void_p]
     L.cujpegxl_encoder_destroy.restype = None
+    L.cujpegxl_encoder_encode_host.argtypes = [C.c_void_p, C.POINTER(Nv12HostInput), C.POINTER(Output)]
+    L.cujpegxl_encoder_encode_async.argtypes = [C.c_void_p, C.POINTER(Nv12Input), C.c_size_t]
+    L.cujpegxl_encoder_finish.argtypes = [C.c_void_p, C.POINTER(Output)]
     return L
 
 
@@ -98,6 +106,16 @@ def stages_lib() -> C.CDLL:
         L.cjx_device_alloc.restype = P
         L.cjx_stage_build_codes.argtypes = [C.c_int32, P, C.c_uint32, P, P, P, P, P]
         L.cjx_device_alloc.argtypes = [C.c_size_t]
+        L.cjx_stream_create.restype = C.c_size_t
+        L.cjx_stream_destroy.argtypes = [C.c_size_t]
+        L.cjx_stream_destroy.restype = None
+        L.cjx_stream_synchronize.argtypes = [C.c_size_t]
+        L.cjx_copy_async.argtypes = [P, P, C.c_size_t, C.c_size_t]
+        L.cjx_fill_async.argtypes = [P, C.c_int, C.c_size_t, C.c_size_t]
+        L.cjx_host_alloc.restype = P
+        L.cjx_host_alloc.argtypes = [C.c_size_t]
+        L.cjx_host_free.argtypes = [P]
+        L.cjx_host_free.restype = None
         L.cjx_device_free.argtypes = [P]
         L.cjx_device_free.restype = None
         L.cjx_copy_to_device.argtypes = [P, P, C.c_size_t]
@@ -193,6 +211,31 @@ class DeviceBuffer:
             pass
 
 
+class PinnedBuffer:
+    """Page-locked host memory from the runtime (hipHostMalloc), viewable as a
+    numpy array.  The library moves data into and out of such buffers by
+    direct DMA, skipping its own staging copy."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = nbytes
+        self.ptr = stages_lib().cjx_host_alloc(max(nbytes, 1))
+        if not self.ptr:
+            raise RuntimeError("pinned host allocation failed")
+        self.array = np.ctypeslib.as_array((C.c_uint8 * max(nbytes, 1)).from_address(self.ptr))[:nbytes]
+
+    def close(self) -> None:
+        if self.ptr:
+            self.array = None
+            stages_lib().cjx_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Encoder:
     """An encoder of the C ABI (or of the stage library with `stages=True`,
     which also exposes intermediate buffers and per-stage timings)."""
@@ -222,6 +265,42 @@ class Encoder:
         s = self._encode(self.handle, C.byref(inp), C.byref(out))
         if s != OK:
             raise RuntimeError(f"encode failed: {status_string(s)}")
+        return self.out[: out.size].tobytes()
+
+    def encode_into(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
+                    out_ptr: int, capacity: int) -> int:
+        """Device planes -> the file into caller memory at `out_ptr`; returns
+        its size."""
+        inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
+        out = Output(out_ptr, 0, capacity)
+        s = self._encode(self.handle, C.byref(inp), C.byref(out))
+        if s != OK:
+            raise RuntimeError(f"encode failed: {status_string(s)}")
+        return out.size
+
+    def encode_host(self, nv12: np.ndarray) -> bytes:
+        """Host NV12 array (luma rows, then interleaved CbCr rows) -> bytes,
+        through cujpegxl_encoder_encode_host."""
+        a = _planes(nv12, self.width, self.height)
+        base = a.ctypes.data
+        inp = Nv12HostInput(base, base + self.width * self.height, self.width, self.width)
+        out = Output(self.out.ctypes.data, 0, self.capacity)
+        s = self.lib.cujpegxl_encoder_encode_host(self.handle, C.byref(inp), C.byref(out))
+        if s != OK:
+            raise RuntimeError(f"encode failed: {status_string(s)}")
+        return self.out[: out.size].tobytes()
+
+    def encode_async(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
+                     stream: int = 0) -> int:
+        """Queues a frame after the work on `stream`; returns the status."""
+        inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
+        return self.lib.cujpegxl_encoder_encode_async(self.handle, C.byref(inp), stream)
+
+    def finish(self) -> bytes:
+        out = Output(self.out.ctypes.data, 0, self.capacity)
+        s = self.lib.cujpegxl_encoder_finish(self.handle, C.byref(out))
+        if s != OK:
+            raise RuntimeError(f"finish failed: {status_string(s)}")
         return self.out[: out.size].tobytes()
 
     def encode_status(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
@@ -286,14 +365,12 @@ def _planes(nv12: np.ndarray, width: int, height: int) -> np.ndarray:
 def _encode(nv12, width, height, distance, device_ordinal, stats: bool):
     a = _planes(nv12, width, height)
     enc = Encoder(width, height, distance, device_ordinal, stages=stats)
-    dev = DeviceBuffer.from_array(a)
     try:
         if stats:
             enc.set_profiling(True)
-        data = enc.encode_device(dev.ptr, dev.ptr + width * height, width, width)
+        data = enc.encode_host(a)
         t = enc.timings() if stats else None
     finally:
-        dev.close()
         enc.close()
     return data, t
 

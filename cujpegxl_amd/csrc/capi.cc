@@ -150,4 +150,63 @@ void cujpegxl_encoder_destroy(cujpegxl_encoder* encoder) {
     delete encoder;
 }
 
+cujpegxl_status cujpegxl_encoder_encode_host(cujpegxl_encoder* encoder, const cujpegxl_nv12_host_input* input,
+                                             cujpegxl_output* output) {
+    if (encoder == nullptr || input == nullptr || output == nullptr || output->data == nullptr ||
+        input->luma == nullptr || input->chroma == nullptr || encoder->engine.pending()) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const cujpegxl_status s{validate_config(encoder->config)};
+    if (s != CUJPEGXL_OK) {
+        return s;
+    }
+    if (input->luma_pitch < encoder->config.width || input->chroma_pitch < encoder->config.width) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    std::size_t size{0};
+    if (!encoder->engine.encode_host(input->luma, input->luma_pitch, input->chroma, input->chroma_pitch,
+                                     output->data, output->capacity, &size)) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    output->size = size;
+    return size > output->capacity ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
+}
+
+cujpegxl_status cujpegxl_encoder_encode_async(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
+                                              uintptr_t stream) {
+    if (encoder == nullptr || input == nullptr || input->luma == 0 || input->chroma == 0 ||
+        encoder->engine.pending()) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const cujpegxl_status s{validate_config(encoder->config)};
+    if (s != CUJPEGXL_OK) {
+        return s;
+    }
+    if (input->luma_pitch < encoder->config.width || input->chroma_pitch < encoder->config.width) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    if (!encoder->engine.encode_async(reinterpret_cast<const std::uint8_t*>(input->luma), input->luma_pitch,
+                                      reinterpret_cast<const std::uint8_t*>(input->chroma), input->chroma_pitch,
+                                      reinterpret_cast<hipStream_t>(stream))) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    return CUJPEGXL_OK;
+}
+
+cujpegxl_status cujpegxl_encoder_finish(cujpegxl_encoder* encoder, cujpegxl_output* output) {
+    if (encoder == nullptr || !encoder->engine.pending()) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const bool have_out{output != nullptr && output->data != nullptr};
+    std::size_t size{0};
+    if (!encoder->engine.finish(have_out ? output->data : nullptr, have_out ? output->capacity : 0, &size)) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    if (!have_out) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    output->size = size;
+    return size > output->capacity ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
+}
+
 }  // extern "C"

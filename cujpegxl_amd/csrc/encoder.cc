@@ -218,10 +218,18 @@ void Encoder::release() {
     if (device_ >= 0) {
         (void)hipSetDevice(device_);
     }
+    // A frame queued by encode_async and never finished may still run.
+    for (hipStream_t s : {stream_, dc_stream_}) {
+        if (s != nullptr) {
+            (void)hipStreamSynchronize(s);
+        }
+    }
+    pending_ = false;
     for (void* p : allocations_) {
         (void)hipFree(p);
     }
     allocations_.clear();
+    input_ = nullptr;
     if (total_host_ != nullptr) {
         (void)hipHostFree(total_host_);
         total_host_ = nullptr;
@@ -242,7 +250,7 @@ void Encoder::release() {
             e = nullptr;
         }
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_caller_, &ev_input_free_}) {
         if (*e != nullptr) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -286,7 +294,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess) {
         return false;
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_caller_, &ev_input_free_}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             return false;
         }
@@ -452,47 +460,214 @@ void Encoder::mark(Stage s) {
 // Device file -> caller's host buffer through the two pinned slots: chunk k
 // goes to slot k % 2; the host copies a chunk out while the next one is in
 // flight, then reuses its slot for chunk k + 2.
-bool Encoder::copy_out(std::uint8_t* dst, std::size_t bytes) {
-    const auto* src{reinterpret_cast<const std::uint8_t*>(out_)};
-    std::size_t issued{0}, done{0};
-    std::size_t len[2]{0, 0};
-    auto issue = [&](int slot) {
-        len[slot] = std::min(kStageChunk, bytes - issued);
-        const bool ok{hipMemcpyAsync(stage_host_ + slot * kStageChunk, src + issued, len[slot],
-                                     hipMemcpyDeviceToHost, stream_) == hipSuccess &&
-                      hipEventRecord(stage_ready_[slot], stream_) == hipSuccess};
-        issued += len[slot];
-        return ok;
-    };
-    for (int slot{0}; slot < 2 && issued < bytes; ++slot) {
-        if (!issue(slot)) {
-            return false;
+namespace {
+
+// True for host memory the runtime can DMA into directly (allocated or
+// registered through HIP); such an output skips the pinned staging copy.
+bool host_pinned(const void* p) {
+    hipPointerAttribute_t a{};
+    if (p == nullptr || hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+}  // namespace
+
+// The readback's first half, queued behind the frame: the total size, and,
+// when the output will go through the pinned staging buffer, the first two
+// chunks of the file before the size is known (bytes past the end are never
+// used), so the host's first wait returns with data already staged.
+bool Encoder::queue_readback(bool speculative) {
+    mark(Stage::kReadback);
+    staged_ = 0;
+    if (hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),
+                       sizeof(std::uint64_t), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+        hipEventRecord(ev_size_, stream_) != hipSuccess) {
+        return false;
+    }
+    if (speculative) {
+        for (int slot{0}; slot < 2 && staged_ < out_cap_; ++slot) {
+            if (!stage_chunk(slot, std::min(kStageChunk, out_cap_ - staged_))) {
+                return false;
+            }
         }
     }
-    for (int slot{0}; done < bytes; slot ^= 1) {
-        if (hipEventSynchronize(stage_ready_[slot]) != hipSuccess) {
-            return false;
-        }
-        std::memcpy(dst + done, stage_host_ + slot * kStageChunk, len[slot]);
-        done += len[slot];
-        if (issued < bytes && !issue(slot)) {
-            return false;
-        }
-    }
+    pending_ = true;
     return true;
 }
 
-bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
-                          std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
-    if (hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+bool Encoder::stage_chunk(int slot, std::size_t n) {
+    stage_len_[slot] = n;
+    const bool ok{hipMemcpyAsync(stage_host_ + slot * kStageChunk, reinterpret_cast<const std::uint8_t*>(out_) + staged_,
+                                 n, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+                  hipEventRecord(stage_ready_[slot], stream_) == hipSuccess};
+    staged_ += n;
+    return ok;
+}
+
+// The readback's second half: waits for the size, then moves the bytes into
+// `dst` -- one DMA when `dst` is pinned, else through the two staging chunks,
+// each host copy overlapping the next chunk's DMA.
+bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
+    if (!pending_) {
+        return false;
+    }
+    pending_ = false;
+    if (hipEventSynchronize(ev_size_) != hipSuccess) {
+        return false;
+    }
+    const std::uint64_t total{*total_host_};
+    *size = static_cast<std::size_t>(total);
+    bool ok{total <= out_cap_};
+    if (ok && total <= cap && dst != nullptr) {
+        if (host_pinned(dst)) {
+            ok = hipMemcpyAsync(dst, out_, total, hipMemcpyDeviceToHost, stream_) == hipSuccess;
+        } else {
+            std::size_t done{0};
+            for (int slot{0}; slot < 2 && staged_ < total && stage_len_[slot] == 0; ++slot) {
+                ok = ok && stage_chunk(slot, std::min<std::size_t>(kStageChunk, total - staged_));
+            }
+            for (int slot{0}; ok && done < total; slot ^= 1) {
+                if (hipEventSynchronize(stage_ready_[slot]) != hipSuccess) {
+                    return false;
+                }
+                const std::size_t n{std::min<std::size_t>(stage_len_[slot], total - done)};
+                std::memcpy(dst + done, stage_host_ + slot * kStageChunk, n);
+                done += n;
+                stage_len_[slot] = 0;
+                if (staged_ < total) {
+                    ok = stage_chunk(slot, std::min<std::size_t>(kStageChunk, total - staged_));
+                }
+            }
+        }
+    }
+    stage_len_[0] = stage_len_[1] = 0;
+    mark(Stage::kCount);
+    if (hipStreamSynchronize(stream_) != hipSuccess) {
+        return false;
+    }
+    if (profiling_) {
+        for (int i{0}; i < static_cast<int>(Stage::kCount); ++i) {
+            float ms{0.0f};
+            (void)hipEventElapsedTime(&ms, events_[i], events_[i + 1]);
+            stage_ms_[i] = ms;
+        }
+    }
+    return ok;
+}
+
+bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                          std::size_t chroma_pitch, hipStream_t caller, bool speculative) {
+    if (pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+        return false;
+    }
+    if (caller_set_ && (hipEventRecord(ev_caller_, caller) != hipSuccess ||
+                        hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess)) {
         return false;
     }
     mark(Stage::kRegion);
     launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_);
+    // The planes are read by the region kernel only: later work on the
+    // caller's stream may overwrite them once it has finished.
+    if (caller_set_ && (hipEventRecord(ev_input_free_, stream_) != hipSuccess ||
+                        hipStreamWaitEvent(caller, ev_input_free_, 0) != hipSuccess)) {
+        return false;
+    }
     mark(Stage::kQuantize);
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
                          nz_count_);
-    return run_entropy(out, cap, size);
+    return run_entropy() && queue_readback(speculative);
+}
+
+bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                          std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
+    caller_set_ = false;
+    if (!submit_nv12(luma, luma_pitch, chroma, chroma_pitch, nullptr, !host_pinned(out))) {
+        pending_ = false;
+        (void)hipStreamSynchronize(stream_);
+        return false;
+    }
+    return finish(out, cap, size);
+}
+
+bool Encoder::encode_async(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                           std::size_t chroma_pitch, hipStream_t caller) {
+    caller_set_ = true;
+    const bool ok{submit_nv12(luma, luma_pitch, chroma, chroma_pitch, caller, true)};
+    caller_set_ = false;
+    if (!ok && pending_) {
+        pending_ = false;
+        (void)hipStreamSynchronize(stream_);
+    }
+    return ok;
+}
+
+// Host planes -> the encoder's device NV12 frame (width pitch), row blocks
+// through the pinned staging chunks: the host copy of one chunk overlaps the
+// DMA of the other.  Pinned planes are copied by one 2D DMA each.
+bool Encoder::upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                          std::size_t chroma_pitch) {
+    if (pending_ || hipSetDevice(device_) != hipSuccess) {
+        return false;
+    }
+    const std::size_t w{geo_.width}, h{geo_.height};
+    if (input_ == nullptr) {
+        void* p{nullptr};
+        if (hipMalloc(&p, w * h * 3 / 2) != hipSuccess) {
+            return false;
+        }
+        allocations_.push_back(p);
+        input_ = static_cast<std::uint8_t*>(p);
+    }
+    struct Plane {
+        const std::uint8_t* src;
+        std::size_t pitch, rows;
+        std::uint8_t* dst;
+    };
+    const Plane planes[2]{{luma, luma_pitch, h, input_}, {chroma, chroma_pitch, h / 2, input_ + w * h}};
+    if (host_pinned(luma) && host_pinned(chroma)) {
+        for (const Plane& pl : planes) {
+            if (hipMemcpy2DAsync(pl.dst, w, pl.src, pl.pitch, w, pl.rows, hipMemcpyHostToDevice, stream_) !=
+                hipSuccess) {
+                return false;
+            }
+        }
+        return hipStreamSynchronize(stream_) == hipSuccess;
+    }
+    const std::size_t rows_per_chunk{kStageChunk / w};
+    if (rows_per_chunk == 0) {
+        return false;
+    }
+    bool busy[2]{false, false};
+    int slot{0};
+    for (const Plane& pl : planes) {
+        for (std::size_t r0{0}; r0 < pl.rows; r0 += rows_per_chunk) {
+            const std::size_t nr{std::min(rows_per_chunk, pl.rows - r0)};
+            if (busy[slot] && hipEventSynchronize(stage_ready_[slot]) != hipSuccess) {
+                return false;
+            }
+            std::uint8_t* st{stage_host_ + slot * kStageChunk};
+            for (std::size_t r{0}; r < nr; ++r) {
+                std::memcpy(st + r * w, pl.src + (r0 + r) * pl.pitch, w);
+            }
+            if (hipMemcpyAsync(pl.dst + r0 * w, st, nr * w, hipMemcpyHostToDevice, stream_) != hipSuccess ||
+                hipEventRecord(stage_ready_[slot], stream_) != hipSuccess) {
+                return false;
+            }
+            busy[slot] = true;
+            slot ^= 1;
+        }
+    }
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+
+bool Encoder::encode_host(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                          std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
+    return upload_nv12(luma, luma_pitch, chroma, chroma_pitch) &&
+           encode_nv12(input_, geo_.width, input_ + static_cast<std::size_t>(geo_.width) * geo_.height, geo_.width,
+                       out, cap, size);
 }
 
 bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc, const std::int8_t* acs,
@@ -521,10 +696,15 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_);
-    return run_entropy(out, cap, size);
+    if (!run_entropy() || !queue_readback(!host_pinned(out))) {
+        pending_ = false;
+        (void)hipStreamSynchronize(stream_);
+        return false;
+    }
+    return finish(out, cap, size);
 }
 
-bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size) {
+bool Encoder::run_entropy() {
     auto* codes{static_cast<CodeSet*>(codes_)};
     auto* k{static_cast<const Constants*>(constants_)};
     auto* blobs{static_cast<DcBlobs*>(blobs_)};
@@ -566,33 +746,7 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size)
     mark(Stage::kEmit);
     launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, dc_base_,
                     ac_base_, out_, out_cap_ / 4);
-    mark(Stage::kReadback);
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),
-                       sizeof(std::uint64_t), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-        hipStreamSynchronize(stream_) != hipSuccess) {
-        return false;
-    }
-    const std::uint64_t total{*total_host_};
-    *size = static_cast<std::size_t>(total);
-    if (total > out_cap_) {
-        return false;
-    }
-    if (total <= cap && out != nullptr && !copy_out(out, static_cast<std::size_t>(total))) {
-        return false;
-    }
-    mark(Stage::kCount);
-    if (hipStreamSynchronize(stream_) != hipSuccess) {
-        return false;
-    }
-    if (profiling_) {
-        for (int i{0}; i < static_cast<int>(Stage::kCount); ++i) {
-            float ms{0.0f};
-            (void)hipEventElapsedTime(&ms, events_[i], events_[i + 1]);
-            stage_ms_[i] = ms;
-        }
-    }
-    return true;
+    return hipGetLastError() == hipSuccess;
 }
 
 std::size_t Encoder::buffer_bytes(Buffer which) const {

@@ -45,6 +45,20 @@ public:
     bool encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size);
 
+    // Asynchronous form: the frame's work is queued on the encoder's stream
+    // after the work already on `caller` (which in turn waits for the region
+    // pass, the only reader of the planes, before its later work); finish()
+    // waits and delivers the bytes.  One frame may be pending at a time.
+    bool encode_async(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                      std::size_t chroma_pitch, hipStream_t caller);
+    bool finish(std::uint8_t* out, std::size_t cap, std::size_t* size);
+    bool pending() const { return pending_; }
+
+    // NV12 host planes: uploaded into an encoder-owned device frame, then
+    // encoded as encode_nv12 does.
+    bool encode_host(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                     std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size);
+
     // Entropy coding and assembly only, for coefficients supplied from the host
     // (same layouts as the front end's outputs).
     bool encode_coefficients(const std::int16_t* ac, const std::int32_t* dc, const std::int8_t* acs,
@@ -67,9 +81,14 @@ private:
     bool alloc();
     void release();
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
-    bool run_entropy(std::uint8_t* out, std::size_t cap, std::size_t* size);
+    bool run_entropy();
+    bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                     std::size_t chroma_pitch, hipStream_t caller, bool speculative);
+    bool queue_readback(bool speculative);
+    bool stage_chunk(int slot, std::size_t n);
+    bool upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                     std::size_t chroma_pitch);
     void mark(Stage s);
-    bool copy_out(std::uint8_t* dst, std::size_t bytes);
 
     int device_{-1};
     float distance_{1.0f};
@@ -78,6 +97,13 @@ private:
     hipStream_t stream_{nullptr};
     hipStream_t dc_stream_{nullptr};  // DC entropy chain, joined by events
     hipEvent_t ev_fork_{nullptr}, ev_dc_done_{nullptr};  // fork onto / join from the DC stream
+    hipEvent_t ev_size_{nullptr};                        // the total size has reached the host
+    hipEvent_t ev_caller_{nullptr}, ev_input_free_{nullptr};  // ordering against a caller's stream
+    bool caller_set_{false};
+    bool pending_{false};            // a frame's readback is queued, finish() not yet called
+    std::size_t staged_{0};          // output bytes queued into the staging chunks
+    std::size_t stage_len_[2]{0, 0};
+    std::uint8_t* input_{nullptr};   // device NV12 frame for host input (allocated on first use)
     bool profiling_{false};
     std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
     std::array<float, static_cast<int>(Stage::kCount)> stage_ms_{};

@@ -1,6 +1,21 @@
 CUJPEGXL_1 {
     global:
-        cujpegxl_*;
+        cujpegxl_api_version;
+        cujpegxl_status_string;
+        cujpegxl_backend_string;
+        cujpegxl_query_backend;
+        cujpegxl_max_output_size;
+        cujpegxl_encoder_create;
+        cujpegxl_encoder_configure;
+        cujpegxl_encoder_encode;
+        cujpegxl_encoder_destroy;
     local:
         *;
 };
+
+CUJPEGXL_2 {
+    global:
+        cujpegxl_encoder_encode_host;
+        cujpegxl_encoder_encode_async;
+        cujpegxl_encoder_finish;
+} CUJPEGXL_1;

@@ -120,6 +120,43 @@ void cjx_device_free(void* p) {
     (void)hipFree(p);
 }
 
+uintptr_t cjx_stream_create(void) {
+    hipStream_t s{nullptr};
+    return hipStreamCreateWithFlags(&s, hipStreamNonBlocking) == hipSuccess ? reinterpret_cast<uintptr_t>(s) : 0;
+}
+
+void cjx_stream_destroy(uintptr_t stream) {
+    if (stream != 0) {
+        (void)hipStreamDestroy(reinterpret_cast<hipStream_t>(stream));
+    }
+}
+
+int cjx_stream_synchronize(uintptr_t stream) {
+    return hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)) == hipSuccess ? CUJPEGXL_OK
+                                                                                      : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_copy_async(void* dst, const void* src, size_t bytes, uintptr_t stream) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, reinterpret_cast<hipStream_t>(stream)) == hipSuccess
+               ? CUJPEGXL_OK
+               : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_fill_async(void* dst, int value, size_t bytes, uintptr_t stream) {
+    return hipMemsetAsync(dst, value, bytes, reinterpret_cast<hipStream_t>(stream)) == hipSuccess
+               ? CUJPEGXL_OK
+               : CUJPEGXL_INTERNAL_ERROR;
+}
+
+void* cjx_host_alloc(size_t bytes) {
+    void* p{nullptr};
+    return hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+}
+
+void cjx_host_free(void* p) {
+    (void)hipHostFree(p);
+}
+
 int cjx_copy_to_device(void* dst, const void* src, size_t bytes) {
     return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
 }

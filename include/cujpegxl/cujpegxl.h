@@ -16,8 +16,9 @@
 extern "C" {
 #endif
 
-/* cujpegxl.h:24 */
-#define CUJPEGXL_API_VERSION 3u
+/* cujpegxl.h:24 defines version 3; version 4 is that ABI unchanged plus the
+ * host-input and asynchronous entry points at the end of this header. */
+#define CUJPEGXL_API_VERSION 4u
 
 /* cujpegxl.h:26-34; values are ABI. */
 typedef enum cujpegxl_status {
@@ -84,6 +85,41 @@ cujpegxl_status cujpegxl_encoder_encode(cujpegxl_encoder* encoder, const cujpegx
                                         cujpegxl_output* output);
 /* cujpegxl.h:82 */
 void cujpegxl_encoder_destroy(cujpegxl_encoder* encoder);
+
+/* ---- API version 4 additions (no counterpart in the reference) ---------- */
+
+/* NV12 (BT.709 full range) in host memory: plane addresses and pitches in
+ * bytes (pitches >= width). */
+typedef struct cujpegxl_nv12_host_input {
+    const uint8_t* luma;
+    const uint8_t* chroma;
+    size_t luma_pitch;
+    size_t chroma_pitch;
+} cujpegxl_nv12_host_input;
+
+/* Synchronous encode of a host-resident NV12 frame: the planes are uploaded
+ * into a device frame the encoder owns (through its pinned staging buffer, or
+ * by direct DMA when both planes are pinned host memory), then encoded as
+ * cujpegxl_encoder_encode does.  Same validation and statuses. */
+cujpegxl_status cujpegxl_encoder_encode_host(cujpegxl_encoder* encoder,
+                                             const cujpegxl_nv12_host_input* input,
+                                             cujpegxl_output* output);
+
+/* Asynchronous encode of a device-resident NV12 frame.  `stream` is a
+ * hipStream_t as an integer (0: the null stream).  The frame's work is
+ * ordered after the work already queued on `stream`, and later work queued on
+ * `stream` is ordered after the encoder has finished reading the planes.
+ * Returns once the work is queued; the file is delivered by
+ * cujpegxl_encoder_finish.  One frame may be pending per encoder: a second
+ * call before finish returns CUJPEGXL_INVALID_ARGUMENT. */
+cujpegxl_status cujpegxl_encoder_encode_async(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
+                                              uintptr_t stream);
+
+/* Waits for the pending frame and writes it to `output` (statuses as for
+ * cujpegxl_encoder_encode, including CUJPEGXL_BUFFER_TOO_SMALL with the
+ * required size in output->size; the frame is consumed either way).
+ * CUJPEGXL_INVALID_ARGUMENT when no frame is pending. */
+cujpegxl_status cujpegxl_encoder_finish(cujpegxl_encoder* encoder, cujpegxl_output* output);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -67,6 +67,16 @@ int cjx_stage_build_codes(int32_t device_ordinal, const uint32_t* hists, uint32_
 /* Device memory for harnesses that have no other allocator. */
 void* cjx_device_alloc(size_t bytes);
 void cjx_device_free(void* ptr);
+/* A HIP stream (as an integer, default priority) and ordered copies and
+ * fills on it, for tests of the asynchronous entry point. */
+uintptr_t cjx_stream_create(void);
+void cjx_stream_destroy(uintptr_t stream);
+int cjx_stream_synchronize(uintptr_t stream);
+int cjx_copy_async(void* dst, const void* src, size_t bytes, uintptr_t stream);
+int cjx_fill_async(void* dst, int value, size_t bytes, uintptr_t stream);
+/* Page-locked host memory (the library DMAs straight into or out of it). */
+void* cjx_host_alloc(size_t bytes);
+void cjx_host_free(void* ptr);
 int cjx_copy_to_device(void* dst, const void* src, size_t bytes);
 int cjx_copy_to_host(void* dst, const void* src, size_t bytes);
 int cjx_device_synchronize(void);

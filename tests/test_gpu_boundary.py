@@ -1,0 +1,129 @@
+"""GPU: the API version 4 entry points (host-memory input, asynchronous
+encode) and page-locked outputs deliver the same file as the synchronous
+device-input encode, whose bytes equal the oracle's."""
+
+from __future__ import annotations
+
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import cujpegxl_amd as cj
+from cujpegxl_amd.synth import synthetic_nv12
+
+pytestmark = pytest.mark.gpu
+
+W, H, D, SEED = 640, 384, 1.0, 7
+
+
+def say(msg: str) -> None:
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def frame(jo):
+    luma, chroma = synthetic_nv12(W, H, SEED)
+    nv12 = np.concatenate([luma.ravel(), chroma.ravel()])
+    want = jo.encode_nv12(luma, chroma, W, H, D)
+    return luma, chroma, nv12, want
+
+
+def test_host_input_matches_oracle(frame):
+    luma, chroma, nv12, want = frame
+    say(f"encode_host {W}x{H} contiguous, padded and pinned planes")
+    enc = cj.Encoder(W, H, D)
+    assert enc.encode_host(nv12) == want
+    # Padded pitches: the pad bytes must not reach the file.
+    pad = 72
+    lp = np.full((H, W + pad), 255, np.uint8)
+    lp[:, :W] = luma
+    cp = np.full((H // 2, W + pad), 7, np.uint8)
+    cp[:, :W] = chroma.reshape(H // 2, W)
+    inp = cj.Nv12HostInput(lp.ctypes.data, cp.ctypes.data, W + pad, W + pad)
+    out = cj.Output(enc.out.ctypes.data, 0, enc.capacity)
+    assert enc.lib.cujpegxl_encoder_encode_host(enc.handle, C.byref(inp), C.byref(out)) == cj.OK
+    assert enc.out[: out.size].tobytes() == want
+    # Page-locked planes take the direct-DMA branch.
+    pin = cj.PinnedBuffer(nv12.size)
+    pin.array[:] = nv12
+    inp = cj.Nv12HostInput(pin.ptr, pin.ptr + W * H, W, W)
+    out = cj.Output(enc.out.ctypes.data, 0, enc.capacity)
+    assert enc.lib.cujpegxl_encoder_encode_host(enc.handle, C.byref(inp), C.byref(out)) == cj.OK
+    assert enc.out[: out.size].tobytes() == want
+    # Argument checks of the new entry point.
+    short = cj.Nv12HostInput(lp.ctypes.data, cp.ctypes.data, W - 8, W + pad)
+    assert enc.lib.cujpegxl_encoder_encode_host(enc.handle, C.byref(short), C.byref(out)) == \
+        cj.INVALID_ARGUMENT
+    pin.close()
+    enc.close()
+
+
+def test_pinned_output_matches_oracle(frame):
+    _luma, _chroma, nv12, want = frame
+    say("encode into a page-locked output buffer, twice, then one too small")
+    enc = cj.Encoder(W, H, D)
+    dev = cj.DeviceBuffer.from_array(nv12)
+    pin = cj.PinnedBuffer(enc.capacity)
+    for _ in range(2):
+        pin.array[:64] = 0xAA
+        n = enc.encode_into(dev.ptr, dev.ptr + W * H, W, W, pin.ptr, enc.capacity)
+        assert pin.array[:n].tobytes() == want
+    inp = cj.Nv12Input(dev.ptr, dev.ptr + W * H, W, W)
+    out = cj.Output(pin.ptr, 0, 100)
+    assert enc.lib.cujpegxl_encoder_encode(enc.handle, C.byref(inp), C.byref(out)) == cj.BUFFER_TOO_SMALL
+    assert out.size == len(want)
+    pin.close()
+    dev.close()
+    enc.close()
+
+
+def test_async_encode_then_finish(frame):
+    _luma, _chroma, nv12, want = frame
+    say("encode_async on the null stream, pending rules, BUFFER_TOO_SMALL from finish")
+    enc = cj.Encoder(W, H, D)
+    lib = enc.lib
+    dev = cj.DeviceBuffer.from_array(nv12)
+    out = cj.Output(enc.out.ctypes.data, 0, enc.capacity)
+    assert lib.cujpegxl_encoder_finish(enc.handle, C.byref(out)) == cj.INVALID_ARGUMENT
+    assert enc.encode_async(dev.ptr, dev.ptr + W * H, W, W, 0) == cj.OK
+    assert enc.encode_async(dev.ptr, dev.ptr + W * H, W, W, 0) == cj.INVALID_ARGUMENT
+    host = np.ascontiguousarray(nv12)
+    hin = cj.Nv12HostInput(host.ctypes.data, host.ctypes.data + W * H, W, W)
+    assert lib.cujpegxl_encoder_encode_host(enc.handle, C.byref(hin), C.byref(out)) == cj.INVALID_ARGUMENT
+    assert enc.finish() == want
+    assert enc.encode_async(dev.ptr, dev.ptr + W * H, W, W, 0) == cj.OK
+    small = cj.Output(enc.out.ctypes.data, 0, 32)
+    assert lib.cujpegxl_encoder_finish(enc.handle, C.byref(small)) == cj.BUFFER_TOO_SMALL
+    assert small.size == len(want)
+    assert lib.cujpegxl_encoder_finish(enc.handle, C.byref(out)) == cj.INVALID_ARGUMENT
+    # The synchronous entry point still works after the asynchronous ones.
+    assert enc.encode_device(dev.ptr, dev.ptr + W * H, W, W) == want
+    dev.close()
+    enc.close()
+
+
+def test_async_encode_is_ordered_on_the_caller_stream(frame):
+    _luma, _chroma, nv12, want = frame
+    say("encode_async on a caller stream: planes written before, cleared after")
+    enc = cj.Encoder(W, H, D)
+    L = cj.stages_lib()
+    src = cj.DeviceBuffer.from_array(nv12)
+    planes = cj.DeviceBuffer(nv12.size)
+    s = L.cjx_stream_create()
+    assert s != 0
+    try:
+        for _ in range(2):
+            assert L.cjx_fill_async(planes.ptr, 0, nv12.size, s) == cj.OK
+            assert L.cjx_copy_async(planes.ptr, src.ptr, nv12.size, s) == cj.OK
+            assert enc.encode_async(planes.ptr, planes.ptr + W * H, W, W, s) == cj.OK
+            assert L.cjx_fill_async(planes.ptr, 0, nv12.size, s) == cj.OK
+            assert enc.finish() == want
+        assert L.cjx_stream_synchronize(s) == cj.OK
+    finally:
+        L.cjx_stream_destroy(s)
+    planes.close()
+    src.close()
+    enc.close()

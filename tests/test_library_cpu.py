@@ -29,7 +29,9 @@ def exported(lib: pathlib.Path) -> set[str]:
 
 def test_public_library_exports_exactly_the_abi():
     want = declared("cujpegxl.h", "cujpegxl_")
-    assert len(want) == 9  # cujpegxl.h:66-82
+    # cujpegxl.h:66-82, plus the host-input and asynchronous entry points of
+    # API version 4.
+    assert len(want) == 12
     assert exported(cj.PUBLIC_LIB) == want
     lib = cj.public_lib()
     for name in want:
@@ -49,7 +51,7 @@ def test_status_and_backend_strings():
     lib = cj.public_lib()
     assert [lib.cujpegxl_backend_string(i).decode() for i in range(3)] == ["unknown", "fp32-simt",
                                                                             "fp16-tensor"]
-    assert cj.api_version() == 3
+    assert cj.api_version() == 4
 
 
 def test_max_output_size_and_validation():
