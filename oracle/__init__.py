@@ -133,6 +133,10 @@ def lib() -> C.CDLL:
         L.jo_encode_nv12.argtypes = [_P, _SZ, _P, _SZ, _U32, _U32, _F, _P, _SZ]
         L.jo_encode_dct8_uniform.argtypes = [_P, _SZ, _P, _SZ, _U32, _U32, _F, _P, _SZ]
         L.jo_max_output_size.argtypes = [_U32, _U32]
+        L.jd_probe.argtypes = [_P, _SZ, C.POINTER(_U32), C.POINTER(_U32)]
+        L.jd_decode.argtypes = [_P, _SZ, _U32, _U32, C.POINTER(_U32), C.POINTER(_U32), _P, _P, _P, _P,
+                                _P, _P]
+        L.jd_error.restype = C.c_char_p
         w8, w16, w32 = (np.ascontiguousarray(w, dtype=np.float32) for w in dequant_weights())
         L.jo_set_dequant(ptr(w8), ptr(w16), ptr(w32))
         _lib = L
@@ -308,3 +312,30 @@ def encode_dct8_uniform(luma, chroma, w, h, distance) -> bytes:
     out = np.zeros(cap, np.uint8)
     n = lib().jo_encode_dct8_uniform(ptr(luma), w, ptr(chroma), w, w, h, distance, ptr(out), cap)
     return out[:n].tobytes()
+
+
+class DecodeError(ValueError):
+    """The file is not a codestream of the subset the encoder writes."""
+
+
+def decode(data: bytes) -> dict:
+    """Subset decoder (jxl_decode.c): `.jxl` bytes -> the quantized frame
+    {width, height, global_scale, quant_dc, acs, qf, dc, ac, ytox, ytob} with
+    the encoder's layouts (qf at first blocks, zero elsewhere)."""
+    buf = np.frombuffer(data, np.uint8)
+    w, h = _U32(), _U32()
+    if lib().jd_probe(ptr(buf), buf.size, C.byref(w), C.byref(h)) != 0:
+        raise DecodeError(lib().jd_error().decode())
+    w, h = w.value, h.value
+    nblk = (w // 8) * (h // 8)
+    tiles = ((w // 8 + 7) // 8) * ((h // 8 + 7) // 8)
+    out = {"acs": np.zeros(nblk, np.int8), "qf": np.zeros(nblk, np.int32),
+           "dc": np.zeros(3 * nblk, np.int32), "ac": np.zeros(3 * nblk * 64, np.int16),
+           "ytox": np.zeros(tiles, np.int8), "ytob": np.zeros(tiles, np.int8)}
+    gs, qdc = _U32(), _U32()
+    if lib().jd_decode(ptr(buf), buf.size, w, h, C.byref(gs), C.byref(qdc), ptr(out["acs"]),
+                       ptr(out["qf"]), ptr(out["dc"]), ptr(out["ac"]), ptr(out["ytox"]),
+                       ptr(out["ytob"])) != 0:
+        raise DecodeError(lib().jd_error().decode())
+    out.update(width=w, height=h, global_scale=gs.value, quant_dc=qdc.value)
+    return out

@@ -113,6 +113,18 @@ size_t jo_encode_dct8_uniform(const uint8_t* luma, size_t luma_pitch, const uint
                               uint8_t* out, size_t cap);
 size_t jo_max_output_size(uint32_t w, uint32_t h);
 
+/* ---- subset decoder (jxl_decode.c) --------------------------------------- */
+/* Frame size of a `.jxl` file (container or bare codestream); 0 or -1. */
+int jd_probe(const uint8_t* file, size_t n, uint32_t* width, uint32_t* height);
+/* Decodes the file back to the quantized frame: transform map (8/16/32 at
+ * first blocks, 0 covered), quant field at first blocks, DC [3][blocks]
+ * (X, Y, B), AC [3][blocks*64] in the covered-block layout, YtoX / YtoB
+ * [tiles].  0 on success, -1 with jd_error() naming the first violation. */
+int jd_decode(const uint8_t* file, size_t n, uint32_t width, uint32_t height, uint32_t* global_scale,
+              uint32_t* quant_dc, int8_t* acs, int32_t* qf, int32_t* dc, int16_t* ac, int8_t* ytox,
+              int8_t* ytob);
+const char* jd_error(void);
+
 #ifdef __cplusplus
 }
 #endif

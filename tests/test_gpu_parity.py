@@ -18,6 +18,7 @@ import pytest
 import cujpegxl_amd as cj
 import patterns
 from cujpegxl_amd.synth import synthetic_nv12
+from decoded import check_decoded
 
 pytestmark = pytest.mark.gpu
 
@@ -275,3 +276,16 @@ def test_single_symbol_histogram_has_zero_depths(jo):
         d, b, out, nb = _huff(jo, np.ascontiguousarray(h[i, :a]))
         assert not got[i]["depth"].any() and not d.any()
         assert got[i]["header_bits"] == nb and got[i]["header"] == out[: (nb + 7) // 8].tobytes()
+
+
+@pytest.mark.parametrize("w,h,d,seed", [(512, 264, 1.0, 1), (1024, 576, 3.0, 3), (3840, 2160, 1.0, 0)])
+def test_device_files_decode_to_device_buffers(jo, golden, w, h, d, seed):
+    """The subset decoder reads every device file back to the device's own
+    quantized frame (every section ends exactly at its TOC size)."""
+    say(f"decode device file {w}x{h} d={d} seed={seed}")
+    luma, chroma = synthetic_nv12(w, h, seed)
+    enc, data, _buf = device_encode(w, h, d, luma, chroma)
+    cal = jo.calibrate(d)
+    check_decoded(golden, jo.decode(data), w, h, cal.global_scale, cal.quant_dc, enc.fetch("acs", np.int8),
+                  enc.fetch("quant_field", np.int32), enc.fetch("dc", np.int32), enc.fetch("ac", np.int16),
+                  enc.fetch("ytox", np.int8), enc.fetch("ytob", np.int8))
