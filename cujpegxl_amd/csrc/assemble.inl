@@ -12,6 +12,7 @@ constexpr int kHdrWords = 64;      // 2048 bits: bound on one code header
 constexpr int kBlobWords = 128;    // per DC-group header blob
 constexpr int kAcGlobalWords = 8192;
 constexpr int kHeadWords = 1024;
+constexpr int kMaxAcGroups = 32 * 64;  // at most 32 DC groups of 64 AC groups
 
 struct Constants {
     // Bit strings precomputed on the host (see encoder.cc).
@@ -48,12 +49,15 @@ struct Layout {
     std::uint32_t framing_words[12];  // container framing, 40 or 48 bytes
 };
 
-// One 16-wave workgroup per histogram (histograms first, first + 1, ...):
+// One 4-wave workgroup per histogram (histograms first, first + 1, ...):
 // create_huffman_tree's count floors 1, 2, 4, ... are independent trials, so
-// wave w tries floor 2^w and the smallest floor whose tree fits 15 bits wins
-// (the serial builder stops at that same floor; floors above 2^15 continue on
-// wave 0).  Wave 0 then builds the codes and the header.
-constexpr int kTrials = 16;
+// in round r wave w tries floor 2^(4r + w), and the smallest floor whose tree
+// fits 15 bits wins (the serial builder stops at that same floor); most
+// histograms finish in the first round.  Floors above 2^15 continue on wave
+// 0.  Wave 0 then builds the codes and the header.  The small workgroup fits
+// beside the AC tokenizer, which the DC chain's codes run alongside.
+constexpr int kTrials = 4;
+constexpr int kTrialRounds = 4;
 
 struct HuffmanShared {
     WaveHuffScratch trial[kTrials];
@@ -69,24 +73,34 @@ struct HuffmanShared {
 template <int R>
 __device__ void huffman_depths(HuffmanShared& S, int alpha) {
     const int w{static_cast<int>(threadIdx.x) >> 6};
+    const int lane{static_cast<int>(threadIdx.x) & 63};
     const Leaves<R> L{wave_leaves<R>(S.hist, alpha)};
     if (L.n < 2) {
         return;  // one symbol or none: the simple header, all depths stay 0
     }
-    const std::uint32_t maxd{tree_trial<R>(L, 1u << w, S.trial_depth[w], S.trial[w])};
-    if ((threadIdx.x & 63) == 0) {
-        S.trial_max[w] = maxd;
-    }
-    __syncthreads();
     int win{-1};
-    for (int i{0}; i < kTrials; ++i) {
-        if (S.trial_max[i] <= 15u) {
-            win = i;
-            break;
+    for (int round{0}; round < kTrialRounds && win < 0; ++round) {
+        if (round > 0) {
+            for (int i{lane}; i < kHistStride; i += 64) {
+                S.trial_depth[w][i] = 0;
+            }
+            wave_sync();
         }
+        const std::uint32_t maxd{tree_trial<R>(L, 1u << (round * kTrials + w), S.trial_depth[w], S.trial[w])};
+        if (lane == 0) {
+            S.trial_max[w] = maxd;
+        }
+        __syncthreads();
+        for (int i{0}; i < kTrials; ++i) {
+            if (S.trial_max[i] <= 15u) {
+                win = i;
+                break;
+            }
+        }
+        __syncthreads();
     }
     if (win < 0 && w == 0) {
-        for (std::uint32_t count_limit{1u << kTrials};; count_limit *= 2) {
+        for (std::uint32_t count_limit{1u << (kTrials * kTrialRounds)};; count_limit *= 2) {
             for (int i{static_cast<int>(threadIdx.x)}; i < kHistStride; i += 64) {
                 S.trial_depth[0][i] = 0;
             }
@@ -239,17 +253,29 @@ __device__ __forceinline__ void u32_code(const Distr* d, std::uint32_t value, st
 // one section without padding in between.
 __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constants* __restrict__ k,
                                                      const CodeSet* __restrict__ cs,
-                                                     const std::uint32_t* __restrict__ ac_group_bytes,
+                                                     const std::uint64_t* __restrict__ ac_part_bits,
                                                      const std::uint32_t* __restrict__ dc_group_bytes,
-                                                     const std::uint64_t* __restrict__ ac_single_bits,
                                                      const std::uint64_t* __restrict__ dc_single_bits,
+                                                     std::uint32_t* __restrict__ ac_group_bytes,
+                                                     std::uint64_t* __restrict__ ac_group_bits,
                                                      const DcBlobs* __restrict__ blobs,
                                                      Layout* __restrict__ lay, std::uint64_t* __restrict__ ac_base,
                                                      std::uint64_t* __restrict__ dc_base) {
     __shared__ std::uint32_t agw[kAcGlobalWords];
     __shared__ std::uint32_t hw[kHeadWords];
+    __shared__ std::uint64_t agbits[kMaxAcGroups];
     __shared__ std::uint64_t sh[17];
     const int t{static_cast<int>(threadIdx.x)};
+    // AC group sizes: the sums of their parts' bits.
+    for (int g{t}; g < static_cast<int>(geo.num_ac); g += 256) {
+        std::uint64_t sum{0};
+        for (int p{0}; p < kGroupParts; ++p) {
+            sum += ac_part_bits[g * kGroupParts + p];
+        }
+        agbits[g] = sum;
+        ac_group_bits[g] = sum;
+        ac_group_bytes[g] = static_cast<std::uint32_t>((sum + 7) / 8);
+    }
     for (int i{t}; i < kAcGlobalWords; i += 256) {
         agw[i] = 0;
     }
@@ -313,7 +339,7 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
         std::uint32_t size{0}, code{0}, clen{0};
         if (e < entries) {
             if (single) {
-                const std::uint64_t bits{k->dc_global_bits + dc_single_bits[0] + abits + ac_single_bits[0]};
+                const std::uint64_t bits{k->dc_global_bits + dc_single_bits[0] + abits + agbits[0]};
                 size = static_cast<std::uint32_t>((bits + 7) / 8);
             } else if (e == 0) {
                 size = dcg;
@@ -322,7 +348,7 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
             } else if (e == geo.num_dc + 1) {
                 size = acg;
             } else {
-                size = ac_group_bytes[e - geo.num_dc - 2];
+                size = static_cast<std::uint32_t>((agbits[e - geo.num_dc - 2] + 7) / 8);
             }
             u32_code(toc, size, code, clen);
         }
@@ -365,7 +391,7 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
         std::uint64_t run{ac_global_bit + static_cast<std::uint64_t>(acg) * 8};
         for (std::uint32_t g0{0}; g0 < geo.num_ac; g0 += 256) {
             const std::uint32_t ga{g0 + static_cast<std::uint32_t>(t)};
-            const std::uint64_t asz{ga < geo.num_ac ? static_cast<std::uint64_t>(ac_group_bytes[ga]) * 8 : 0ull};
+            const std::uint64_t asz{ga < geo.num_ac ? (agbits[ga] + 7) / 8 * 8 : 0ull};
             std::uint64_t chunk{0};
             const std::uint64_t apre{block_excl_scan(asz, sh, chunk)};
             if (ga < geo.num_ac) {
@@ -430,43 +456,78 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
 struct Pieces {
     const std::uint32_t* dc_staging;
     std::uint64_t dc_slot_words;
-    const std::uint32_t* ac_staging;
-    std::uint64_t ac_slot_words;
+    const std::uint32_t* ac_staging;  // kGroupParts slots of kPartSlotWords per group
     const std::uint64_t* dc_bits;
     const std::uint64_t* ac_bits;
+    const std::uint64_t* ac_part_bits;
     const std::uint64_t* dc_base;
     const std::uint64_t* ac_base;
 };
 
-constexpr int kMaxPieces = 4 + 32 + 32 * 64;  // at most 32 DC groups of 64 AC groups
+constexpr int kMaxPieces = 4 + 32 + kMaxAcGroups;
 
+// A piece's bits are one run at `src`, or, for an AC group (`parts` set),
+// the concatenation of its parts' runs.
 struct Piece {
     std::uint64_t dst;
     std::uint64_t nbits;
     const std::uint32_t* src;
+    const std::uint64_t* parts;
 };
+
+// `len` (1..32) bits of a run from bit `o`.
+__device__ __forceinline__ std::uint32_t run_bits(const std::uint32_t* src, std::uint64_t o, std::uint32_t len) {
+    const std::uint32_t sh{static_cast<std::uint32_t>(o & 31u)};
+    std::uint32_t x{src[o >> 5] >> sh};
+    if (sh != 0 && len > 32 - sh) {
+        x |= src[(o >> 5) + 1] << (32 - sh);
+    }
+    return len < 32 ? x & ((1u << len) - 1u) : x;
+}
+
+__device__ __forceinline__ std::uint32_t piece_bits(const Piece& pc, std::uint64_t o, std::uint32_t len) {
+    if (pc.parts == nullptr) {
+        return run_bits(pc.src, o, len);
+    }
+    std::uint32_t v{0}, got{0};
+    std::uint64_t start{0};
+    for (int p{0}; p < kGroupParts && got < len; ++p) {
+        const std::uint64_t pb{pc.parts[p]};
+        if (o < start + pb) {
+            const std::uint64_t off{o - start};
+            const std::uint32_t take{static_cast<std::uint32_t>(
+                pb - off < static_cast<std::uint64_t>(len - got) ? pb - off : static_cast<std::uint64_t>(len - got))};
+            v |= run_bits(pc.src + static_cast<std::size_t>(p) * kPartSlotWords, off, take) << got;
+            got += take;
+            o += take;
+        }
+        start += pb;
+    }
+    return v;
+}
 
 __device__ __forceinline__ Piece piece_at(const Geometry& geo, const Constants* k, const Layout* lay,
                                           const Pieces& P, int i) {
     const int nd{static_cast<int>(geo.num_dc)};
     if (i == 0) {
-        return {0, static_cast<std::uint64_t>(lay->framing_bytes) * 8, lay->framing_words};
+        return {0, static_cast<std::uint64_t>(lay->framing_bytes) * 8, lay->framing_words, nullptr};
     }
     if (i == 1) {
-        return {static_cast<std::uint64_t>(lay->framing_bytes) * 8, lay->head_total_bits, lay->head_words};
+        return {static_cast<std::uint64_t>(lay->framing_bytes) * 8, lay->head_total_bits, lay->head_words, nullptr};
     }
     if (i == 2) {
-        return {lay->dc_global_bit, k->dc_global_bits, k->dc_global_words};
+        return {lay->dc_global_bit, k->dc_global_bits, k->dc_global_words, nullptr};
     }
     if (i < 3 + nd) {
         const int g{i - 3};
-        return {P.dc_base[g], P.dc_bits[g], P.dc_staging + static_cast<std::size_t>(g) * P.dc_slot_words};
+        return {P.dc_base[g], P.dc_bits[g], P.dc_staging + static_cast<std::size_t>(g) * P.dc_slot_words, nullptr};
     }
     if (i == 3 + nd) {
-        return {lay->ac_global_bit, lay->ac_global_bits, lay->ac_global_words};
+        return {lay->ac_global_bit, lay->ac_global_bits, lay->ac_global_words, nullptr};
     }
     const int g{i - 4 - nd};
-    return {P.ac_base[g], P.ac_bits[g], P.ac_staging + static_cast<std::size_t>(g) * P.ac_slot_words};
+    return {P.ac_base[g], P.ac_bits[g], P.ac_staging + static_cast<std::size_t>(g) * kGroupParts * kPartSlotWords,
+            P.ac_part_bits + static_cast<std::size_t>(g) * kGroupParts};
 }
 
 __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Constants* __restrict__ k,
@@ -508,17 +569,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Const
             if (s >= e) {
                 continue;
             }
-            const std::uint64_t o{s - pc.dst};
-            const std::uint32_t len{static_cast<std::uint32_t>(e - s)};
-            const std::uint32_t sh{static_cast<std::uint32_t>(o & 31u)};
-            std::uint32_t x{pc.src[o >> 5] >> sh};
-            if (sh != 0 && len > 32 - sh) {
-                x |= pc.src[(o >> 5) + 1] << (32 - sh);
-            }
-            if (len < 32) {
-                x &= (1u << len) - 1u;
-            }
-            v |= x << static_cast<std::uint32_t>(s - b0);
+            v |= piece_bits(pc, s - pc.dst, static_cast<std::uint32_t>(e - s)) << static_cast<std::uint32_t>(s - b0);
         }
         out_words[w] = v;
     }

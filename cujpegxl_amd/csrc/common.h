@@ -22,6 +22,14 @@ constexpr int kHistStride = 256;  // hybrid-uint token alphabet bound
 constexpr int kNumClusters = 32;  // fixed AC context clustering
 constexpr int kNumAcContexts = 15 * (37 + 458);
 
+// Buffers the entropy stage accumulates into, zeroed by the kernel that
+// precedes it (the quantizer, or the non-zero counter for host-supplied
+// coefficients) instead of by fill launches of their own.
+struct Clears {
+    std::uint32_t* ptr[4];
+    std::uint32_t words[4];
+};
+
 struct Geometry {
     std::uint32_t width, height;
     std::uint32_t bw, bh;        // 8x8 blocks

@@ -28,7 +28,8 @@ bool init_device_tables() {
 // first-block and the normalized grid over its covered positions, for frames
 // whose coefficients are supplied directly.
 __global__ void nz_kernel(Geometry geo, const std::int16_t* __restrict__ ac, const std::int8_t* __restrict__ acs,
-                          std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count) {
+                          std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr) {
+    clear_words(clr);
     const std::size_t i{blockIdx.x * 256ull + threadIdx.x};
     const std::size_t nblk{geo.nblocks};
     if (i >= 3 * nblk) {
@@ -117,14 +118,14 @@ void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, c
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
-                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count) {
+                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr) {
     tile_quantize_kernel<<<geo.cmw * geo.cmh, 256, 0, s>>>(geo, prm, coeffs, acs, qf, cfl_part, ytox, ytob,
-                                                           ac, dc, nz_grid, nz_count);
+                                                           ac, dc, nz_grid, nz_count, clr);
 }
 
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
-               std::uint8_t* nz_grid, std::uint16_t* nz_count) {
-    nz_kernel<<<grid_for(3ull * geo.nblocks, 256), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count);
+               std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr) {
+    nz_kernel<<<grid_for(3ull * geo.nblocks, 256), 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, clr);
 }
 
 void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v) {
@@ -208,14 +209,18 @@ void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, con
 }
 
 std::size_t ac_group_slot_words() {
-    return kGroupSlotWords;
+    return static_cast<std::size_t>(kGroupParts) * kPartSlotWords;
+}
+
+std::size_t ac_group_parts() {
+    return kGroupParts;
 }
 
 void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
                           const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
-                          std::uint32_t* staging, std::uint32_t* group_bytes, std::uint64_t* group_bits) {
-    ac_group_emit_kernel<<<geo.num_ac, kGroupThreads, 0, s>>>(geo, acs, item_tokens, tokens, &cs->depth[0][0],
-                                                              &cs->code[0][0], staging, group_bytes, group_bits);
+                          std::uint32_t* staging, std::uint64_t* part_bits) {
+    ac_group_emit_kernel<<<geo.num_ac * kGroupParts, kPartThreads, 0, s>>>(
+        geo, acs, item_tokens, tokens, &cs->depth[0][0], &cs->code[0][0], staging, part_bits);
 }
 
 std::size_t dc_group_slot_words() {
@@ -236,18 +241,18 @@ void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 }
 
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
-                   const std::uint32_t* ac_group_bytes, const std::uint32_t* dc_group_bytes,
-                   const std::uint64_t* ac_group_bits, const std::uint64_t* dc_group_bits, const DcBlobs* blobs,
-                   Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base) {
-    layout_kernel<<<1, 256, 0, s>>>(geo, k, cs, ac_group_bytes, dc_group_bytes, ac_group_bits, dc_group_bits,
-                                   blobs, lay, ac_base, dc_base);
+                   const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
+                   const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,
+                   const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base) {
+    layout_kernel<<<1, 256, 0, s>>>(geo, k, cs, ac_part_bits, dc_group_bytes, dc_group_bits, ac_group_bytes,
+                                   ac_group_bits, blobs, lay, ac_base, dc_base);
 }
 
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
-                     const std::uint64_t* ac_bits, const std::uint64_t* dc_base, const std::uint64_t* ac_base,
-                     std::uint32_t* out_words, std::uint64_t cap_words) {
-    const Pieces P{dc_staging, dc_group_slot_words(), ac_staging, kGroupSlotWords, dc_bits, ac_bits, dc_base, ac_base};
+                     const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
+                     const std::uint64_t* ac_base, std::uint32_t* out_words, std::uint64_t cap_words) {
+    const Pieces P{dc_staging, dc_group_slot_words(), ac_staging, dc_bits, ac_bits, ac_part_bits, dc_base, ac_base};
     assemble_kernel<<<2048, 256, 0, s>>>(geo, k, lay, P, out_words, cap_words);
 }
 

@@ -353,6 +353,7 @@ bool Encoder::alloc() {
     get(hist_partial_, ac_partial_words() * sizeof(std::uint32_t));
     get(ac_staging_, static_cast<std::size_t>(geo_.num_ac) * ac_group_slot_words() * sizeof(std::uint32_t));
     get(dc_staging_, static_cast<std::size_t>(geo_.num_dc) * dc_group_slot_words() * sizeof(std::uint32_t));
+    get(ac_part_bits_, geo_.num_ac * ac_group_parts() * sizeof(std::uint64_t));
     get(ac_group_bytes_, geo_.num_ac * sizeof(std::uint32_t));
     get(ac_group_bits_, geo_.num_ac * sizeof(std::uint64_t));
     get(chunk_bits_, static_cast<std::size_t>(geo_.num_dc) * dc_chunks_per_group() * sizeof(std::uint64_t));
@@ -577,7 +578,7 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     }
     mark(Stage::kQuantize);
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
-                         nz_count_);
+                         nz_count_, entropy_clears());
     return run_entropy() && queue_readback(speculative);
 }
 
@@ -695,13 +696,26 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     }
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
-    launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_);
+    launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
     if (!run_entropy() || !queue_readback(!host_pinned(out))) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
     }
     return finish(out, cap, size);
+}
+
+// The histograms (AC clusters, then DC and AcMetadata per DC group), the
+// first-block counts and the DC header-blob table start every frame at zero.
+Clears Encoder::entropy_clears() const {
+    Clears c{};
+    c.ptr[0] = hists_;
+    c.words[0] = static_cast<std::uint32_t>((kNumClusters + 2 * geo_.num_dc) * kHistStride);
+    c.ptr[1] = fb_count_;
+    c.words[1] = 64;
+    c.ptr[2] = static_cast<std::uint32_t*>(blobs_);
+    c.words[2] = static_cast<std::uint32_t>(sizeof_dc_blobs() / 4);
+    return c;
 }
 
 bool Encoder::run_entropy() {
@@ -717,9 +731,6 @@ bool Encoder::run_entropy() {
     if (hipEventRecord(ev_fork_, stream_) != hipSuccess || hipStreamWaitEvent(dc_stream_, ev_fork_, 0) != hipSuccess) {
         return false;
     }
-    launch_fill_u32(dc_stream_, hists_ + kNumClusters * kHistStride, static_cast<std::size_t>(ndc) * kHistStride, 0u);
-    launch_fill_u32(dc_stream_, fb_count_, 64, 0u);
-    launch_fill_u32(dc_stream_, static_cast<std::uint32_t*>(blobs_), sizeof_dc_blobs() / 4, 0u);
     launch_dc_hist(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
                    hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
     launch_huffman(dc_stream_, kNumClusters, ndc, hists_, codes);
@@ -731,21 +742,19 @@ bool Encoder::run_entropy() {
     if (hipEventRecord(ev_dc_done_, dc_stream_) != hipSuccess) {
         return false;
     }
-    launch_fill_u32(stream_, hists_, static_cast<std::size_t>(kNumClusters) * kHistStride, 0u);
     launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_);
     mark(Stage::kCodes);
     launch_huffman(stream_, 0, kNumClusters, hists_, codes);
     mark(Stage::kSizes);
-    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_group_bytes_,
-                         ac_group_bits_);
+    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_part_bits_);
     if (hipStreamWaitEvent(stream_, ev_dc_done_, 0) != hipSuccess) {
         return false;
     }
-    launch_layout(stream_, geo_, k, codes, ac_group_bytes_, dc_group_bytes_, ac_group_bits_, dc_group_bits_, blobs,
-                  lay, ac_base_, dc_base_);
+    launch_layout(stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
+                  ac_group_bits_, blobs, lay, ac_base_, dc_base_);
     mark(Stage::kEmit);
-    launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, dc_base_,
-                    ac_base_, out_, out_cap_ / 4);
+    launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
+                    dc_base_, ac_base_, out_, out_cap_ / 4);
     return hipGetLastError() == hipSuccess;
 }
 

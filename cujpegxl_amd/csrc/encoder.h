@@ -82,6 +82,7 @@ private:
     void release();
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
     bool run_entropy();
+    Clears entropy_clears() const;
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch, hipStream_t caller, bool speculative);
     bool queue_readback(bool speculative);
@@ -131,6 +132,7 @@ private:
     std::uint32_t* hist_partial_{nullptr}; // per-workgroup AC histograms
     std::uint32_t* ac_staging_{nullptr};   // each AC group's stream at group-relative bits
     std::uint32_t* dc_staging_{nullptr};   // each DC group's section at group-relative bits
+    std::uint64_t* ac_part_bits_{nullptr};   // bits of each part (row band) of each AC group
     std::uint32_t* ac_group_bytes_{nullptr};
     std::uint64_t* ac_group_bits_{nullptr};
     std::uint64_t* chunk_bits_{nullptr};

@@ -461,53 +461,64 @@ __device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std
     return r;
 }
 
-// ---- AC groups emitted whole -------------------------------------------------
+// ---- AC groups emitted in parts -----------------------------------------------
 //
-// One 1024-thread workgroup per AC group writes the group's token stream at
-// group-relative bit positions into its staging slot: the group's items in
-// decoder order (block rows, blocks left to right, channels Y, X, B) and
-// their token counts give a flat token numbering (token f belongs to the last
-// item whose prefix is <= f); block scans of the token lengths give the
-// positions; the stream is assembled in LDS and stored once.  The group's
-// bit and byte sizes fall out of the same pass.  A stream longer than the LDS
-// copy is emitted again with atomicOr into the zeroed slot.
-constexpr int kGroupThreads = 1024;
-constexpr int kGroupItems = 3 * kAcGroupBlocks * kAcGroupBlocks;
-constexpr int kGroupWords = 16384;
+// Each AC group is cut into kGroupParts parts of kPartRows block rows (each a
+// contiguous run of the group's stream), and one 512-thread workgroup writes
+// a part's token stream at part-relative bit positions into the part's
+// staging slot: the part's items in decoder order (block rows, blocks left to
+// right, channels Y, X, B) and their token counts give a flat token numbering
+// (token f belongs to the last item whose prefix is <= f); block scans of the
+// token lengths give the positions; the stream is assembled in LDS and stored
+// once.  The part's bit count falls out of the same pass (a group's size is
+// the sum of its parts').  A stream longer than the LDS copy is emitted again
+// with atomicOr into the zeroed slot.
+constexpr int kGroupParts = 4;
+constexpr int kPartRows = kAcGroupBlocks / kGroupParts;
+constexpr int kPartThreads = 512;
+constexpr int kPartItems = 3 * kAcGroupBlocks * kPartRows;
+constexpr int kPartWords = 4096;
 constexpr int kGroupPerThread = 8;
-// Bound on one group's stream: 28 bits (15-bit code + 13 extra bits) per
-// coefficient slot of its 1024 blocks in three channels.
-constexpr std::size_t kGroupSlotWords = 3ull * kAcGroupBlocks * kAcGroupBlocks * 64 * 28 / 32 + 64;
+// Bound on one part's stream: 28 bits (15-bit code + 13 extra bits) per
+// coefficient slot of its blocks in three channels.
+constexpr std::size_t kPartSlotWords = 3ull * kAcGroupBlocks * kPartRows * 64 * 28 / 32 + 64;
 
 struct GroupShared {
     std::uint32_t lut[kNumClusters * kAcSymbols];  // depth | code << 16
-    std::uint32_t prefix[kGroupItems + 1];
-    std::uint32_t slot0[kGroupItems];  // token slot of the item's first token
-    std::int8_t m[kGroupItems];
-    std::uint32_t buf[kGroupWords];
+    std::uint32_t prefix[kPartItems + 1];
+    std::uint32_t slot0[kPartItems];  // token slot of the item's first token
+    std::int8_t m[kPartItems];
+    std::uint32_t buf[kPartWords];
     std::uint64_t sh[17];
     std::uint32_t overflow;
 };
 
-__global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
+__global__ __launch_bounds__(kPartThreads) void ac_group_emit_kernel(
     Geometry geo, const std::int8_t* __restrict__ acs, const std::uint16_t* __restrict__ item_tokens,
     const std::uint32_t* __restrict__ tokens, const std::uint8_t* __restrict__ depth,
     const std::uint16_t* __restrict__ codes, std::uint32_t* __restrict__ staging,
-    std::uint32_t* __restrict__ group_bytes, std::uint64_t* __restrict__ group_bits) {
+    std::uint64_t* __restrict__ part_bits) {
     __shared__ GroupShared S;
     const int tid{static_cast<int>(threadIdx.x)};
-    const int g{static_cast<int>(blockIdx.x)};
+    const int g{static_cast<int>(blockIdx.x) / kGroupParts};
+    const int part{static_cast<int>(blockIdx.x) % kGroupParts};
     const int bx0{(g % static_cast<int>(geo.xg)) * kAcGroupBlocks};
-    const int by0{(g / static_cast<int>(geo.xg)) * kAcGroupBlocks};
+    const int by0{(g / static_cast<int>(geo.xg)) * kAcGroupBlocks + part * kPartRows};
     const int gbw{min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0)};
-    const int gbh{min(kAcGroupBlocks, static_cast<int>(geo.bh) - by0)};
+    const int gbh{min(kPartRows, static_cast<int>(geo.bh) - by0)};
+    if (gbh <= 0) {
+        if (tid == 0) {
+            part_bits[blockIdx.x] = 0;
+        }
+        return;
+    }
     const int row_items{3 * gbw};
     const int items{row_items * gbh};
-    std::uint32_t* const slot{staging + static_cast<std::size_t>(g) * kGroupSlotWords};
+    std::uint32_t* const slot{staging + static_cast<std::size_t>(blockIdx.x) * kPartSlotWords};
 
-    // 1. Token prefix over the group's items.
+    // 1. Token prefix over the part's items.
     std::uint32_t carry{0};
-    for (int base{0}; base < items; base += kGroupThreads) {
+    for (int base{0}; base < items; base += kPartThreads) {
         const int i{base + tid};
         std::uint32_t cnt{0};
         if (i < items) {
@@ -531,11 +542,11 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
         S.prefix[items] = carry;
         S.overflow = 0;
     }
-    for (int i{tid}; i < kNumClusters * kAcSymbols; i += kGroupThreads) {
+    for (int i{tid}; i < kNumClusters * kAcSymbols; i += kPartThreads) {
         const int at{(i / kAcSymbols) * kHistStride + i % kAcSymbols};
         S.lut[i] = static_cast<std::uint32_t>(depth[at]) | (static_cast<std::uint32_t>(codes[at]) << 16);
     }
-    for (int i{tid}; i < kGroupWords; i += kGroupThreads) {
+    for (int i{tid}; i < kPartWords; i += kPartThreads) {
         S.buf[i] = 0;
     }
     __syncthreads();
@@ -551,13 +562,13 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
                 break;
             }
             const std::uint64_t nw{(run + 31) / 32};
-            for (std::uint64_t i{static_cast<std::uint64_t>(tid)}; i < nw && i < kGroupSlotWords; i += kGroupThreads) {
+            for (std::uint64_t i{static_cast<std::uint64_t>(tid)}; i < nw && i < kPartSlotWords; i += kPartThreads) {
                 slot[i] = 0;
             }
             __syncthreads();
             run = 0;
         }
-        for (std::uint32_t f0{0}; f0 < carry; f0 += kGroupThreads * kGroupPerThread) {
+        for (std::uint32_t f0{0}; f0 < carry; f0 += kPartThreads * kGroupPerThread) {
             const std::uint32_t fb{f0 + static_cast<std::uint32_t>(tid) * kGroupPerThread};
             std::uint32_t rec[kGroupPerThread];
             int lo{0};
@@ -610,13 +621,13 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
                     const std::uint32_t b{static_cast<std::uint32_t>(at & 31u)};
                     const bool two{b + len[j] > 32};
                     if (global) {
-                        if (wi < kGroupSlotWords) {
+                        if (wi < kPartSlotWords) {
                             atomicOr(&slot[wi], val[j] << b);
                         }
-                        if (two && wi + 1 < kGroupSlotWords) {
+                        if (two && wi + 1 < kPartSlotWords) {
                             atomicOr(&slot[wi + 1], val[j] >> (32 - b));
                         }
-                    } else if (wi + (two ? 1 : 0) < static_cast<std::uint64_t>(kGroupWords)) {
+                    } else if (wi + (two ? 1 : 0) < static_cast<std::uint64_t>(kPartWords)) {
                         atomicOr(&S.buf[wi], val[j] << b);
                         if (two) {
                             atomicOr(&S.buf[wi + 1], val[j] >> (32 - b));
@@ -633,13 +644,12 @@ __global__ __launch_bounds__(kGroupThreads) void ac_group_emit_kernel(
     }
     if (S.overflow == 0) {
         const std::uint64_t nw{(run + 31) / 32};
-        for (std::uint64_t i{static_cast<std::uint64_t>(tid)}; i < nw; i += kGroupThreads) {
+        for (std::uint64_t i{static_cast<std::uint64_t>(tid)}; i < nw; i += kPartThreads) {
             slot[i] = S.buf[i];
         }
     }
     if (tid == 0) {
-        group_bits[g] = run;
-        group_bytes[g] = static_cast<std::uint32_t>((run + 7) / 8);
+        part_bits[blockIdx.x] = run;
     }
 }
 

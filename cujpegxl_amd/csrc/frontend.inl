@@ -23,6 +23,18 @@ namespace cjx {
 constexpr int kHalo = 2;
 constexpr int kTileW = 32 + 2 * kHalo;  // 36
 
+// Grid-strided zeroing of the Clears buffers by every thread of a launch.
+__device__ __forceinline__ void clear_words(const Clears& c) {
+    const std::uint32_t stride{gridDim.x * blockDim.x};
+    const std::uint32_t t{blockIdx.x * blockDim.x + threadIdx.x};
+#pragma unroll
+    for (int k{0}; k < 4; ++k) {
+        for (std::uint32_t i{t}; i < c.words[k]; i += stride) {
+            c.ptr[k][i] = 0;
+        }
+    }
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int off{32}; off > 0; off >>= 1) {
@@ -845,8 +857,9 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
     const float* __restrict__ cfl_part, std::int8_t* __restrict__ ytox_out,
     std::int8_t* __restrict__ ytob_out, std::int16_t* __restrict__ ac, std::int32_t* __restrict__ dc,
-    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count) {
+    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr) {
     __shared__ TileShared S;
+    clear_words(clr);
     const int tid{static_cast<int>(threadIdx.x)};
     const int w{tid >> 6};
     const int lane{tid & 63};

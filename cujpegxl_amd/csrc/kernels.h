@@ -37,9 +37,9 @@ void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, c
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
-                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count);
+                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
-               std::uint8_t* nz_grid, std::uint16_t* nz_count);
+               std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
 void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v);
 void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                       const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* tokens,
@@ -57,9 +57,11 @@ void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, con
                      const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words);
 // Words of one AC group's staging slot (bound on its token stream).
 std::size_t ac_group_slot_words();
+// AC groups are emitted in this many parts (row bands).
+std::size_t ac_group_parts();
 void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
                           const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
-                          std::uint32_t* staging, std::uint32_t* group_bytes, std::uint64_t* group_bits);
+                          std::uint32_t* staging, std::uint64_t* part_bits);
 // Words of one DC group's staging slot.
 std::size_t dc_group_slot_words();
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
@@ -67,13 +69,13 @@ void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
                     const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
                     std::uint32_t* group_bytes, std::uint64_t* group_bits, std::uint32_t* staging);
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
-                   const std::uint32_t* ac_group_bytes, const std::uint32_t* dc_group_bytes,
-                   const std::uint64_t* ac_group_bits, const std::uint64_t* dc_group_bits, const DcBlobs* blobs,
-                   Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base);
+                   const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
+                   const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,
+                   const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base);
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
-                     const std::uint64_t* ac_bits, const std::uint64_t* dc_base, const std::uint64_t* ac_base,
-                     std::uint32_t* out_words, std::uint64_t cap_words);
+                     const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
+                     const std::uint64_t* ac_base, std::uint32_t* out_words, std::uint64_t cap_words);
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
