@@ -260,18 +260,44 @@ constexpr int kLines16 = kLines32 + RowBuf<32>::BS;
 constexpr int kLines8 = kLines16 + 4 * RowBuf<16>::BS;
 constexpr int kScratch = kLines8 + 16 * RowBuf<8>::BS;
 
+// Element arrays are written in transposed (column) order by the transforms
+// and read in element order: one pad slot per 32 elements keeps the
+// column-order writes of 16 lanes in distinct banks.
+constexpr int kCoefPadded = 1024 + 2 * 32;  // FP16 arrays: two halfs (one bank) per 32
+__device__ __forceinline__ int cpad(int e) {
+    return e + 2 * (e >> 5);
+}
+constexpr int kTermPadded = 1024 + 32;  // float arrays: one float per 32
+__device__ __forceinline__ int tpad(int i) {
+    return i + (i >> 5);
+}
+
+// Step 5 uses the scratch lines below this float offset; the B channel's
+// coefficients live past it.
+constexpr int kCoefBOffset = 4 * 2 * 4 * RowBuf<8>::BS;
+static_assert(kCoefBOffset >= 2 * RowBuf<32>::BS, "step-5 DCT32 lines overlap the B coefficients");
+static_assert(kCoefBOffset * 4 + kCoefPadded * 2 <= kScratch * 4, "B coefficients exceed the scratch");
+
+// Arrays whose lifetimes do not overlap share storage, which keeps the
+// workgroup under 40 KB of LDS (four resident workgroups per CU).
 struct RegionShared {
     union {
         float xyb[3][kTileW * kTileW];  // NV12 -> XYB tile with halo (steps 1-3)
-        float scratch[kScratch];        // row-pass lines and selection costs (steps 4-5)
+        float scratch[kScratch];        // row-pass lines, selection terms, B coefficients (steps 4-5)
     } u;
     alignas(16) float sharp[3][32 * kSP];
-    _Float16 coefy[3][1024];   // Y coefficients of each candidate size, element order
-    _Float16 coefxb[2][1024];  // X and B coefficients of the chosen transforms
-    float mask[1024];
-    float cells[64];
-    float ero[64];         // fuzzy erosion of each cell
-    float blk[16][3];      // per block: HF sum, gamma, blue sum
+    union {
+        float mask[1024];                  // AQ pre-erosion terms (step 3)
+        _Float16 coefy[3][kCoefPadded];    // Y coefficients of each candidate size, element order (cpad)
+    } m;
+    union {
+        struct {
+            float cells[64];
+            float ero[64];      // fuzzy erosion of each cell
+            float blk[16][3];   // per block: HF sum, gamma, blue sum
+        } aq;                   // step 3
+        _Float16 coefx[kCoefPadded];  // X coefficients of the chosen transforms (cpad), step 5
+    } x;
     float part[4][4][2];   // [wave][quad][d|r]
     float cost8[16];       // quad-major
     float cost16[4];
@@ -383,7 +409,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             float diff{ratio_cbrt_gamma(yv + 0.019f, false) * (yv - base)};
             diff *= diff;
             diff = fminf(diff, 0.2f);
-            S.mask[p] = masking_sqrt(diff);
+            S.m.mask[p] = masking_sqrt(diff);
         }
     }
     __syncthreads();
@@ -392,10 +418,10 @@ __global__ __launch_bounds__(256) void region_kernel(
         float sum{0.0f};
         for (int dy{0}; dy < 4; ++dy) {
             for (int dx{0}; dx < 4; ++dx) {
-                sum += S.mask[(cy * 4 + dy) * 32 + cx * 4 + dx];
+                sum += S.m.mask[(cy * 4 + dy) * 32 + cx * 4 + dx];
             }
         }
-        S.cells[tid] = 0.25f * sum;
+        S.x.aq.cells[tid] = 0.25f * sum;
     }
     __syncthreads();
 
@@ -405,7 +431,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     //     (a block's four erosion terms sum as (e0 + e2) + (e1 + e3), which is
     //     the wave tree over four non-zero lanes).
     if (tid < 64) {
-        S.ero[tid] = erosion_cell(S.cells, tid & 7, tid >> 3, prm);
+        S.x.aq.ero[tid] = erosion_cell(S.x.aq.cells, tid & 7, tid >> 3, prm);
     }
     for (int j{0}; j < 4; ++j) {
         const int lb{j * 4 + w};
@@ -428,9 +454,9 @@ __global__ __launch_bounds__(256) void region_kernel(
         const float gamma_overall{wave_sum(gm) * (0.5f / 64.0f)};
         const float blue_sum{wave_sum(bl)};
         if (lane == 0) {
-            S.blk[lb][0] = hf_sum;
-            S.blk[lb][1] = gamma_overall;
-            S.blk[lb][2] = blue_sum;
+            S.x.aq.blk[lb][0] = hf_sum;
+            S.x.aq.blk[lb][1] = gamma_overall;
+            S.x.aq.blk[lb][2] = blue_sum;
         }
     }
     __syncthreads();
@@ -438,10 +464,10 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int lb{tid};
         const int lbx{lb & 3}, lby{lb >> 2};
         if (bx0 + lbx < bw && by0 + lby < bh) {
-            const float e0{S.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.ero[(2 * lby) * 8 + 2 * lbx + 1]};
-            const float e2{S.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
+            const float e0{S.x.aq.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.x.aq.ero[(2 * lby) * 8 + 2 * lbx + 1]};
+            const float e2{S.x.aq.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.x.aq.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
             const float aq{(e0 + e2) + (e1 + e3)};
-            float blue_sum{S.blk[lb][2]};
+            float blue_sum{S.x.aq.blk[lb][2]};
             constexpr float k_limit{0.027121074570634722f};
             if (blue_sum >= 32.0f * k_limit) {
                 blue_sum = 64.0f * k_limit - blue_sum;
@@ -451,8 +477,8 @@ __global__ __launch_bounds__(256) void region_kernel(
                 blue_sum = k_max_limit * k_limit;
             }
             float ov{compute_mask(aq)};
-            ov += S.blk[lb][0] * -0.38f + 0.42f;
-            ov += 0.1005613337192697f * log2_det(S.blk[lb][1]);
+            ov += S.x.aq.blk[lb][0] * -0.38f + 0.42f;
+            ov += 0.1005613337192697f * log2_det(S.x.aq.blk[lb][1]);
             ov += blue_sum * 0.14207000358439159f;
             const float qval{exp_det(ov) * prm.mul_pb + prm.add_pb};
             int qi{static_cast<int>(qval * prm.inv_global_scale + 0.5f)};
@@ -470,7 +496,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
     float* const lines{S.u.scratch};
     float* const dsc{S.u.scratch};
-    float* const rsc{S.u.scratch + 1024};
+    float* const rsc{S.u.scratch + kTermPadded};
     const int grp{lane >> 4}, t16{lane & 15};
     if (wu == 0) {
         // DCT32 as four 16x16 quadrants: block (fh, rh) = (grp >> 1, grp & 1).
@@ -522,7 +548,7 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const float v{acc[4 * b + u]};
                 selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, prm, td[4 * b + u], tr[4 * b + u]);
                 ti[4 * b + u] = e;
-                S.coefy[2][e] = static_cast<_Float16>(v);
+                S.m.coefy[2][cpad(e)] = static_cast<_Float16>(v);
             }
         }
     } else if (wu == 1) {
@@ -537,7 +563,7 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const float v{acc[4 * q + u]};
                 selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, prm, td[4 * q + u], tr[4 * q + u]);
                 ti[4 * q + u] = 256 * q + e;
-                S.coefy[1][256 * q + e] = static_cast<_Float16>(v);
+                S.m.coefy[1][cpad(256 * q + e)] = static_cast<_Float16>(v);
             }
         }
     } else {
@@ -552,7 +578,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             const int e{fx * 8 + f};
             selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, prm, td[f], tr[f]);
             ti[f] = base + e;
-            S.coefy[0][base + e] = static_cast<_Float16>(o[f]);
+            S.m.coefy[0][cpad(base + e)] = static_cast<_Float16>(o[f]);
         }
     }
     __syncthreads();  // row-pass lines are dead; dsc / rsc reuse the space
@@ -564,8 +590,8 @@ __global__ __launch_bounds__(256) void region_kernel(
 #pragma unroll
             for (int i{0}; i < 16; ++i) {
                 if (i < cnt) {
-                    dsc[ti[i]] = td[i];
-                    rsc[ti[i]] = tr[i];
+                    dsc[tpad(ti[i])] = td[i];
+                    rsc[tpad(ti[i])] = tr[i];
                 }
             }
         }
@@ -573,7 +599,7 @@ __global__ __launch_bounds__(256) void region_kernel(
         float d[4], rt[4];
 #pragma unroll
         for (int k{0}; k < 4; ++k) {
-            const int i{256 * k + 64 * w + lane};
+            const int i{tpad(256 * k + 64 * w + lane)};
             d[k] = dsc[i];
             rt[k] = rsc[i];
         }
@@ -684,8 +710,10 @@ __global__ __launch_bounds__(256) void region_kernel(
     }
 
     // 5. Chosen DCTs of X and B (Y reuses the selection coefficients) into
-    //    coefxb in element order, then FP16 out and the CfL partials
+    //    the X and B coefficient arrays in element order, then FP16 out and the CfL partials
     //    (frontend_quantize.cu:49-117 on this region).
+    _Float16* const coefb{reinterpret_cast<_Float16*>(S.u.scratch + kCoefBOffset)};
+    const auto cxb = [&](int ch) { return ch == 0 ? S.x.coefx : coefb; };
     const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
     if (acs0 == 32) {
         // Waves 0 and 1 transform X and B on the matrix cores (DCT32 as four
@@ -712,7 +740,7 @@ __global__ __launch_bounds__(256) void region_kernel(
 #pragma unroll
                 for (int u{0}; u < 4; ++u) {
                     const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
-                    S.coefxb[ch][fx * 32 + fy] = static_cast<_Float16>(acc[4 * b + u]);
+                    cxb(ch)[cpad(fx * 32 + fy)] = static_cast<_Float16>(acc[4 * b + u]);
                 }
             }
         }
@@ -753,7 +781,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             for (int b{0}; b < 2; ++b) {
 #pragma unroll
                 for (int u{0}; u < 4; ++u) {
-                    S.coefxb[b][256 * q + t16 * 16 + 4 * grp + u] = static_cast<_Float16>(acc[4 * b + u]);
+                    cxb(b)[cpad(256 * q + t16 * 16 + 4 * grp + u)] = static_cast<_Float16>(acc[4 * b + u]);
                 }
             }
         } else {
@@ -762,7 +790,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             dct_line<8, 8>(&ql[(ch * 4 + j) * RowBuf<8>::BS + fx * RowBuf<8>::RS], 0, o);
 #pragma unroll
             for (int f{0}; f < 8; ++f) {
-                S.coefxb[ch][256 * q + 64 * j + fx * 8 + f] = static_cast<_Float16>(o[f]);
+                cxb(ch)[cpad(256 * q + 64 * j + fx * 8 + f)] = static_cast<_Float16>(o[f]);
             }
         }
     }
@@ -787,9 +815,9 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int e{qc * 64 + lane};
         const int idx{n == 32 ? e : 256 * k + 64 * w + lane};
         const int sz{n == 32 ? 2 : (n == 16 ? 1 : 0)};
-        const _Float16 hx{S.coefxb[0][idx]};
-        const _Float16 hy{S.coefy[sz][idx]};
-        const _Float16 hb{S.coefxb[1][idx]};
+        const _Float16 hx{cxb(0)[cpad(idx)]};
+        const _Float16 hy{S.m.coefy[sz][cpad(idx)]};
+        const _Float16 hb{cxb(1)[cpad(idx)]};
         float v[3]{0.0f, 0.0f, 0.0f};
         if (bx0 + bxl < bw && by0 + byl < bh) {
             const std::size_t slot{(static_cast<std::size_t>(by0 + byl) * bw + bx0 + bxl) * 64 + lane};
