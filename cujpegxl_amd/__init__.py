@@ -102,6 +102,7 @@ def stages_lib() -> C.CDLL:
         L.cjx_stage_encoder_create_any.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
         L.cjx_stage_encode_nv12.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
+        L.cjx_stage_encode_dct8_uniform.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_timings.argtypes = [P, C.POINTER(C.c_float), C.c_int]
         L.cjx_device_alloc.restype = P
         L.cjx_stage_build_codes.argtypes = [C.c_int32, P, C.c_uint32, P, P, P, P, P]
@@ -277,6 +278,17 @@ class Encoder:
         if s != OK:
             raise RuntimeError(f"encode failed: {status_string(s)}")
         return out.size
+
+    def encode_dct8_uniform(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int,
+                            chroma_pitch: int) -> bytes:
+        """Config-1 semantics on the device (stage library): DCT8 everywhere,
+        uniform quant field, no CfL maps."""
+        inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
+        out = Output(self.out.ctypes.data, 0, self.capacity)
+        s = stages_lib().cjx_stage_encode_dct8_uniform(self.handle, C.byref(inp), C.byref(out))
+        if s != OK:
+            raise RuntimeError(f"encode_dct8_uniform failed: {status_string(s)}")
+        return self.out[: out.size].tobytes()
 
     def encode_host(self, nv12: np.ndarray) -> bytes:
         """Host NV12 array (luma rows, then interleaved CbCr rows) -> bytes,

@@ -115,6 +115,15 @@ void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, c
                                               qf, cfl_part);
 }
 
+void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
+                         std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
+                         std::int16_t* ac, std::int32_t* dc, std::int8_t* acs, std::int32_t* qf, std::int8_t* ytox,
+                         std::int8_t* ytob) {
+    const unsigned int per_xcd{(geo.rw * geo.rh + 7) / 8};
+    dct8_uniform_kernel<<<8 * per_xcd, 256, 0, s>>>(geo, prm, luma, luma_pitch, chroma, chroma_pitch, ac, dc, acs,
+                                                    qf, ytox, ytob);
+}
+
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,

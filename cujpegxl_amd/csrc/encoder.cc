@@ -593,6 +593,24 @@ bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     return finish(out, cap, size);
 }
 
+bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                                  std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
+    if (pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+        return false;
+    }
+    mark(Stage::kRegion);
+    launch_dct8_uniform(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, ac_, dc_, acs_, qf_, ytox_,
+                        ytob_);
+    mark(Stage::kQuantize);
+    launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
+    if (!run_entropy() || !queue_readback(!host_pinned(out))) {
+        pending_ = false;
+        (void)hipStreamSynchronize(stream_);
+        return false;
+    }
+    return finish(out, cap, size);
+}
+
 bool Encoder::encode_async(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                            std::size_t chroma_pitch, hipStream_t caller) {
     caller_set_ = true;

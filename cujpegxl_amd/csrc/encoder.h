@@ -59,6 +59,11 @@ public:
     bool encode_host(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size);
 
+    // Config-1 semantics (the reference's dct.cu + quant.cu test path): DCT8
+    // everywhere, uniform quant field, no CfL maps; same entropy coding.
+    bool encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                             std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size);
+
     // Entropy coding and assembly only, for coefficients supplied from the host
     // (same layouts as the front end's outputs).
     bool encode_coefficients(const std::int16_t* ac, const std::int32_t* dc, const std::int8_t* acs,

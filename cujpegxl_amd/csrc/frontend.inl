@@ -325,6 +325,61 @@ __device__ __forceinline__ void selection_terms(float s, int e, int n, const flo
     rt = aq < 256.0f ? g_tables.rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
 }
 
+// Step 1 of a region: NV12 -> XYB over the 36x36 tile (2-pixel halo,
+// clamped coordinates), exact 4:2:0 bilinear chroma.
+__device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* luma, std::size_t luma_pitch,
+                                         const std::uint8_t* chroma, std::size_t chroma_pitch, int px0, int py0,
+                                         int W, int H, int tid) {
+    for (int i{tid}; i < kTileW * kTileW; i += 256) {
+        const int lx{i % kTileW}, ly{i / kTileW};
+        int gx{px0 + lx - kHalo}, gy{py0 + ly - kHalo};
+        gx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+        gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+        int x0, x1, y0, y1;
+        float wx0, wx1, wy0, wy1;
+        chroma_taps(gx, W / 2, x0, x1, wx0, wx1);
+        chroma_taps(gy, H / 2, y0, y1, wy0, wy1);
+        const std::uint8_t* r0{chroma + static_cast<std::size_t>(y0) * chroma_pitch};
+        const std::uint8_t* r1{chroma + static_cast<std::size_t>(y1) * chroma_pitch};
+        const float b0{static_cast<float>(r0[2 * x0]) * wx0 + static_cast<float>(r0[2 * x1]) * wx1};
+        const float b1{static_cast<float>(r1[2 * x0]) * wx0 + static_cast<float>(r1[2 * x1]) * wx1};
+        const float c0{static_cast<float>(r0[2 * x0 + 1]) * wx0 +
+                       static_cast<float>(r0[2 * x1 + 1]) * wx1};
+        const float c1{static_cast<float>(r1[2 * x0 + 1]) * wx0 +
+                       static_cast<float>(r1[2 * x1 + 1]) * wx1};
+        const float cbu{b0 * wy0 + b1 * wy1};
+        const float cru{c0 * wy0 + c1 * wy1};
+        const float yv{static_cast<float>(luma[static_cast<std::size_t>(gy) * luma_pitch + gx]) *
+                       (1.0f / 255.0f)};
+        const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
+        const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
+        pixel_to_xyb(yv, cb, cr, S.u.xyb[0][i], S.u.xyb[1][i], S.u.xyb[2][i]);
+    }
+}
+
+// Step 2: inverse gaborish of the three channels into the sharpened planes
+// (gaborish.h:16-28; tap order frontend_transform.cu:30-48).
+__device__ __forceinline__ void gaborish_tile(RegionShared& S, int tid) {
+    constexpr float k0{-0.09495815671340026f}, k1{-0.041031725066768575f};
+    constexpr float k2{0.013710004822696948f}, k3{0.006510206083837737f};
+    constexpr float k4{-0.0014789063378272242f};
+    constexpr float nm{1.0f / (1.0f + 4.0f * (k0 + k1 + k2 + k4 + 2.0f * k3))};
+    constexpr float wc{nm}, wr{nm * k0}, wr2{nm * k2}, wd{nm * k1}, wl{nm * k3}, wd2{nm * k4};
+    for (int i{tid}; i < 3 * 1024; i += 256) {
+        const int c{i >> 10}, p{i & 1023};
+        const float* s{&S.u.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+#define AT(dx, dy) s[(dy) * kTileW + (dx)]
+        const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
+        const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
+        const float dg{AT(-1, -1) + AT(1, -1) + AT(-1, 1) + AT(1, 1)};
+        const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
+                       AT(-1, 2) + AT(1, 2)};
+        const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
+        S.sharp[c][(p >> 5) * kSP + (p & 31)] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
+#undef AT
+    }
+}
+
 __global__ __launch_bounds__(256) void region_kernel(
     Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
@@ -351,31 +406,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     const int bx0{rx * 4}, by0{ry * 4};
 
     // 1. NV12 -> XYB over the 36x36 tile (2-pixel halo, clamped coordinates).
-    for (int i{tid}; i < kTileW * kTileW; i += 256) {
-        const int lx{i % kTileW}, ly{i / kTileW};
-        int gx{px0 + lx - kHalo}, gy{py0 + ly - kHalo};
-        gx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
-        gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
-        int x0, x1, y0, y1;
-        float wx0, wx1, wy0, wy1;
-        chroma_taps(gx, W / 2, x0, x1, wx0, wx1);
-        chroma_taps(gy, H / 2, y0, y1, wy0, wy1);
-        const std::uint8_t* r0{chroma + static_cast<std::size_t>(y0) * chroma_pitch};
-        const std::uint8_t* r1{chroma + static_cast<std::size_t>(y1) * chroma_pitch};
-        const float b0{static_cast<float>(r0[2 * x0]) * wx0 + static_cast<float>(r0[2 * x1]) * wx1};
-        const float b1{static_cast<float>(r1[2 * x0]) * wx0 + static_cast<float>(r1[2 * x1]) * wx1};
-        const float c0{static_cast<float>(r0[2 * x0 + 1]) * wx0 +
-                       static_cast<float>(r0[2 * x1 + 1]) * wx1};
-        const float c1{static_cast<float>(r1[2 * x0 + 1]) * wx0 +
-                       static_cast<float>(r1[2 * x1 + 1]) * wx1};
-        const float cbu{b0 * wy0 + b1 * wy1};
-        const float cru{c0 * wy0 + c1 * wy1};
-        const float yv{static_cast<float>(luma[static_cast<std::size_t>(gy) * luma_pitch + gx]) *
-                       (1.0f / 255.0f)};
-        const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
-        const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
-        pixel_to_xyb(yv, cb, cr, S.u.xyb[0][i], S.u.xyb[1][i], S.u.xyb[2][i]);
-    }
+    xyb_tile(S, luma, luma_pitch, chroma, chroma_pitch, px0, py0, W, H, tid);
     if (tid < 16) {
         S.acs[tid] = 8;
     }
@@ -383,24 +414,7 @@ __global__ __launch_bounds__(256) void region_kernel(
 
     // 2. Inverse gaborish (gaborish.h:16-28; tap order frontend_transform.cu:30-48).
     {
-        constexpr float k0{-0.09495815671340026f}, k1{-0.041031725066768575f};
-        constexpr float k2{0.013710004822696948f}, k3{0.006510206083837737f};
-        constexpr float k4{-0.0014789063378272242f};
-        constexpr float nm{1.0f / (1.0f + 4.0f * (k0 + k1 + k2 + k4 + 2.0f * k3))};
-        constexpr float wc{nm}, wr{nm * k0}, wr2{nm * k2}, wd{nm * k1}, wl{nm * k3}, wd2{nm * k4};
-        for (int i{tid}; i < 3 * 1024; i += 256) {
-            const int c{i >> 10}, p{i & 1023};
-            const float* s{&S.u.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
-#define AT(dx, dy) s[(dy) * kTileW + (dx)]
-            const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
-            const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
-            const float dg{AT(-1, -1) + AT(1, -1) + AT(-1, 1) + AT(1, 1)};
-            const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
-                           AT(-1, 2) + AT(1, 2)};
-            const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
-            S.sharp[c][(p >> 5) * kSP + (p & 31)] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
-#undef AT
-        }
+        gaborish_tile(S, tid);
         // 3a. AQ pre-erosion terms per interior pixel (frontend_fused.cu:118-137).
         for (int p{tid}; p < 1024; p += 256) {
             const float* s{&S.u.xyb[1][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
@@ -1027,6 +1041,104 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
                 nz_count[c * nblk + b] = op == p ? static_cast<std::uint16_t>(cnt) : 0;
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Config-1 semantics (the reference's test path: dct.cu:43-101 forward DCT8
+// over the whole frame, quant.cu:30-97 quantize_dct8): DCT8 everywhere, the
+// uniform quant field, no colour-correlation maps; coefficient k of channel c
+// is rintf(v * f) with f = ac_scale / w8[c][k] (ac_scale = raw quant field x
+// global scale) or DC_INV_QUANT[c] x dc_scale at k = 0, and B is taken
+// against Y's round trip qy / f_y.  Note the operation order: a product by a
+// rounded quotient, unlike the VarDCT quantizer's quotient of a product.
+// One workgroup per 32x32 region: XYB, gaborish, then per channel (Y first,
+// whose quantized values B needs) a DCT8 row pass into LDS lines and a column
+// pass that quantizes and stores.
+__global__ __launch_bounds__(256) void dct8_uniform_kernel(
+    Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
+    const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch, std::int16_t* __restrict__ ac,
+    std::int32_t* __restrict__ dc, std::int8_t* __restrict__ acs_out, std::int32_t* __restrict__ qf_out,
+    std::int8_t* __restrict__ ytox, std::int8_t* __restrict__ ytob) {
+    __shared__ RegionShared S;
+    const int tid{static_cast<int>(threadIdx.x)};
+    const int nreg{static_cast<int>(geo.rw * geo.rh)};
+    const int per_xcd{(nreg + 7) / 8};
+    const int region{static_cast<int>(blockIdx.x & 7u) * per_xcd + static_cast<int>(blockIdx.x >> 3)};
+    if (region >= nreg) {
+        return;
+    }
+    const int rx{region % static_cast<int>(geo.rw)};
+    const int ry{region / static_cast<int>(geo.rw)};
+    const int W{static_cast<int>(geo.width)}, H{static_cast<int>(geo.height)};
+    const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
+    const int bx0{rx * 4}, by0{ry * 4};
+    const std::size_t nblk{geo.nblocks};
+    const std::size_t cplane{nblk * 64};
+
+    xyb_tile(S, luma, luma_pitch, chroma, chroma_pitch, rx * 32, ry * 32, W, H, tid);
+    __syncthreads();
+    gaborish_tile(S, tid);
+    __syncthreads();
+
+    float* const lines{S.u.scratch};                                     // 16 blocks x RowBuf<8>
+    std::int32_t* const qy{reinterpret_cast<std::int32_t*>(S.u.scratch + 2048)};  // Y's quantized values
+    const float ac_scale{prm.qgsf_select};
+    const float dcq[3]{4096.0f * prm.dc_scale, 512.0f * prm.dc_scale, 256.0f * prm.dc_scale};
+    const int b{tid >> 3}, sub{tid & 7};
+    const int bx{bx0 + (b & 3)}, by{by0 + (b >> 2)};
+    const bool inside{tid < 128 && bx < bw && by < bh};
+    const std::size_t blk{inside ? static_cast<std::size_t>(by) * bw + bx : 0};
+    for (int ci{0}; ci < 3; ++ci) {
+        const int c{ci == 0 ? 1 : (ci == 1 ? 0 : 2)};
+        if (tid < 128) {
+            float o[8];
+            dct_line<8, 8>(&S.sharp[c][((b >> 2) * 8 + sub) * kSP + (b & 3) * 8], 0, o);
+#pragma unroll
+            for (int f{0}; f < 8; ++f) {
+                lines[b * RowBuf<8>::BS + f * RowBuf<8>::RS + sub] = o[f];
+            }
+        }
+        __syncthreads();
+        if (tid < 128) {
+            float o[8];
+            dct_line<8, 8>(&lines[b * RowBuf<8>::BS + sub * RowBuf<8>::RS], 0, o);
+#pragma unroll
+            for (int f{0}; f < 8; ++f) {
+                const int e{sub * 8 + f};
+                const float fc{e == 0 ? dcq[c] : ac_scale / g_tables.dequant8[c][e]};
+                float v{o[f]};
+                if (c == 2) {
+                    const float fy{e == 0 ? dcq[1] : ac_scale / g_tables.dequant8[1][e]};
+                    v = v - 1.0f * (static_cast<float>(qy[b * 64 + e]) / fy);
+                }
+                const int q{static_cast<int>(__builtin_rintf(v * fc))};
+                if (c == 1) {
+                    qy[b * 64 + e] = q;
+                }
+                if (inside) {
+                    if (e == 0) {
+                        dc[c * nblk + blk] = q;
+                        ac[c * cplane + blk * 64] = 0;
+                    } else {
+                        ac[c * cplane + blk * 64 + e] = static_cast<std::int16_t>(q);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (tid < 16) {
+        const int x{bx0 + (tid & 3)}, y{by0 + (tid >> 2)};
+        if (x < bw && y < bh) {
+            acs_out[static_cast<std::size_t>(y) * bw + x] = 8;
+            qf_out[static_cast<std::size_t>(y) * bw + x] = static_cast<std::int32_t>(prm.raw_quant_field);
+        }
+    }
+    if (tid == 0 && (rx & 1) == 0 && (ry & 1) == 0) {
+        const std::size_t t{static_cast<std::size_t>(ry / 2) * geo.cmw + rx / 2};
+        ytox[t] = 0;
+        ytob[t] = 0;
     }
 }
 

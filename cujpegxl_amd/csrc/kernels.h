@@ -34,6 +34,11 @@ std::uint64_t layout_total_offset();
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part);
+// Config-1 semantics: DCT8 everywhere, uniform quant field, no CfL maps.
+void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
+                         std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
+                         std::int16_t* ac, std::int32_t* dc, std::int8_t* acs, std::int32_t* qf, std::int8_t* ytox,
+                         std::int8_t* ytob);
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,

@@ -79,6 +79,21 @@ int cjx_stage_encode_nv12(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cu
     return size > out->capacity ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
 }
 
+int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cujpegxl_output* out) {
+    if (e == nullptr || in == nullptr || out == nullptr || out->data == nullptr || in->luma == 0 ||
+        in->chroma == 0 || in->luma_pitch < e->config.width || in->chroma_pitch < e->config.width) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    std::size_t size{0};
+    if (!e->engine.encode_dct8_uniform(reinterpret_cast<const std::uint8_t*>(in->luma), in->luma_pitch,
+                                       reinterpret_cast<const std::uint8_t*>(in->chroma), in->chroma_pitch,
+                                       out->data, out->capacity, &size)) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    out->size = size;
+    return size > out->capacity ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
+}
+
 int cjx_stage_set_profiling(cujpegxl_encoder* e, int on) {
     if (e == nullptr) {
         return CUJPEGXL_INVALID_ARGUMENT;

@@ -51,6 +51,13 @@ int cjx_stage_encoder_create_any(const cujpegxl_config* config, cujpegxl_encoder
 int cjx_stage_encode_nv12(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
                           cujpegxl_output* output);
 
+/* Config-1 semantics on the device (the reference's dct.cu:43-101 forward
+ * DCT8 and quant.cu:30-97 quantize_dct8, a test path there): DCT8 everywhere,
+ * the uniform quant field, no colour-correlation maps, then the same entropy
+ * coding and assembly; any multiple-of-8 size. */
+int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
+                                  cujpegxl_output* output);
+
 /* Per-stage device time of the following encodes (HIP events on the
  * encoder's stream): region, quantize, histograms, codes, sizes, emit,
  * readback.  `ms` receives 7 floats for the last encode. */

@@ -289,3 +289,22 @@ def test_device_files_decode_to_device_buffers(jo, golden, w, h, d, seed):
     check_decoded(golden, jo.decode(data), w, h, cal.global_scale, cal.quant_dc, enc.fetch("acs", np.int8),
                   enc.fetch("quant_field", np.int32), enc.fetch("dc", np.int32), enc.fetch("ac", np.int16),
                   enc.fetch("ytox", np.int8), enc.fetch("ytob", np.int8))
+
+
+@pytest.mark.parametrize("w,h,seed", [(256, 256, 5), (640, 384, 6), (1024, 576, 7)])
+def test_dct8_uniform_mode_matches_oracle(jo, w, h, seed):
+    """Config-1 semantics on the device (DCT8 everywhere, uniform quant field,
+    quant.cu's quantizer): the file equals the oracle's and decodes to a
+    DCT8-only, uniform-field frame."""
+    say(f"dct8 uniform {w}x{h} seed={seed}")
+    luma, chroma = synthetic_nv12(w, h, seed)
+    enc = cj.Encoder(w, h, 1.0, any_size=True)
+    nv12 = cj.DeviceBuffer.from_array(np.concatenate([luma.ravel(), chroma.ravel()]))
+    data = enc.encode_dct8_uniform(nv12.ptr, nv12.ptr + w * h, w, w)
+    want = jo.encode_dct8_uniform(luma, chroma, w, h, 1.0)
+    assert data == want, f"file differs: {len(data)} vs {len(want)} bytes"
+    dec = jo.decode(data)
+    assert (dec["acs"] == 8).all()
+    assert (dec["qf"] == jo.calibrate(1.0).raw_quant_field).all()
+    nv12.close()
+    enc.close()
