@@ -267,10 +267,6 @@ constexpr int kCoefPadded = 1024 + 2 * 32;  // FP16 arrays: two halfs (one bank)
 __device__ __forceinline__ int cpad(int e) {
     return e + 2 * (e >> 5);
 }
-constexpr int kTermPadded = 1024 + 32;  // float arrays: one float per 32
-__device__ __forceinline__ int tpad(int i) {
-    return i + (i >> 5);
-}
 
 // Step 5 uses the scratch lines below this float offset; the B channel's
 // coefficients live past it.
@@ -298,7 +294,6 @@ struct RegionShared {
         } aq;                   // step 3
         _Float16 coefx[kCoefPadded];  // X coefficients of the chosen transforms (cpad), step 5
     } x;
-    float part[4][4][2];   // [wave][quad][d|r]
     float cost8[16];       // quad-major
     float cost16[4];
     float cost32;
@@ -504,13 +499,10 @@ __global__ __launch_bounds__(256) void region_kernel(
     // 4. RD transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
     //    The three sizes run side by side: wave 0 the DCT32 and wave 1 the four
     //    DCT16s on the matrix cores, waves 2-3 the sixteen DCT8s on the VALU.
-    //    Each wave keeps its coefficients' selection terms in registers; per
-    //    size, the terms are scattered to element order and summed over the
-    //    element layout as before.
+    //    Each wave keeps its coefficients' selection terms in registers and
+    //    sums them there (per lane in register order, then a lane tree).
     __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
     float* const lines{S.u.scratch};
-    float* const dsc{S.u.scratch};
-    float* const rsc{S.u.scratch + kTermPadded};
     const int grp{lane >> 4}, t16{lane & 15};
     if (wu == 0) {
         // DCT32 as four 16x16 quadrants: block (fh, rh) = (grp >> 1, grp & 1).
@@ -549,7 +541,6 @@ __global__ __launch_bounds__(256) void region_kernel(
     __syncthreads();
     // Column passes: 16 terms per lane on waves 0-1, 8 on waves 2-3.
     float td[16], tr[16];
-    int ti[16];
     if (wu == 0) {
         const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
                                       &lines[kLines32 + (16 * (grp & 1) + t16) * RowBuf<32>::RS])};
@@ -561,7 +552,6 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const int e{fx * 32 + fy};
                 const float v{acc[4 * b + u]};
                 selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, prm, td[4 * b + u], tr[4 * b + u]);
-                ti[4 * b + u] = e;
                 S.m.coefy[2][cpad(e)] = static_cast<_Float16>(v);
             }
         }
@@ -576,7 +566,6 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const int e{fx * 16 + fy};
                 const float v{acc[4 * q + u]};
                 selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, prm, td[4 * q + u], tr[4 * q + u]);
-                ti[4 * q + u] = 256 * q + e;
                 S.m.coefy[1][cpad(256 * q + e)] = static_cast<_Float16>(v);
             }
         }
@@ -591,81 +580,57 @@ __global__ __launch_bounds__(256) void region_kernel(
         for (int f{0}; f < 8; ++f) {
             const int e{fx * 8 + f};
             selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, prm, td[f], tr[f]);
-            ti[f] = base + e;
             S.m.coefy[0][cpad(base + e)] = static_cast<_Float16>(o[f]);
         }
     }
-    __syncthreads();  // row-pass lines are dead; dsc / rsc reuse the space
-    for (int sz{0}; sz < 3; ++sz) {
-        const int n{8 << sz};
-        const bool owner{sz == 0 ? wu >= 2 : (sz == 1 ? wu == 1 : wu == 0)};
-        if (owner) {
-            const int cnt{sz == 0 ? 8 : 16};
+    // Costs straight from the registers: per lane the sum of its terms in
+    // register order, then the wave tree (DCT32; DCT16 per block) or an
+    // 8-lane tree (DCT8: a block's 64 terms sit in 8 consecutive lanes, one
+    // column each).
+    if (wu == 0) {
+        float ad{td[0]}, ar{tr[0]};
 #pragma unroll
-            for (int i{0}; i < 16; ++i) {
-                if (i < cnt) {
-                    dsc[tpad(ti[i])] = td[i];
-                    rsc[tpad(ti[i])] = tr[i];
-                }
-            }
+        for (int i{1}; i < 16; ++i) {
+            ad += td[i];
+            ar += tr[i];
         }
-        __syncthreads();
-        float d[4], rt[4];
-#pragma unroll
-        for (int k{0}; k < 4; ++k) {
-            const int i{tpad(256 * k + 64 * w + lane)};
-            d[k] = dsc[i];
-            rt[k] = rsc[i];
+        const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
+        if (lane == 0) {
+            S.cost32 = 1024.0f * ds + 6.0e-4f * rs;
         }
-        if (n == 8) {
+    } else if (wu == 1) {
 #pragma unroll
-            for (int k{0}; k < 4; ++k) {
-                const float ds{wave_sum(d[k])}, rs{wave_sum(rt[k])};
-                if (lane == 0) {
-                    S.cost8[4 * k + w] = 64.0f * ds + 6.0e-4f * rs;
-                }
-            }
-        } else if (n == 16) {
+        for (int q{0}; q < 4; ++q) {
+            float ad{td[4 * q]}, ar{tr[4 * q]};
 #pragma unroll
-            for (int k{0}; k < 4; ++k) {
-                const float ds{wave_sum(d[k])}, rs{wave_sum(rt[k])};
-                if (lane == 0) {
-                    S.part[w][k][0] = ds;
-                    S.part[w][k][1] = rs;
-                }
+            for (int u{1}; u < 4; ++u) {
+                ad += td[4 * q + u];
+                ar += tr[4 * q + u];
             }
-        } else {
-            float ad{d[0]}, ar{rt[0]};
-            ad += d[1];
-            ar += rt[1];
-            ad += d[2];
-            ar += rt[2];
-            ad += d[3];
-            ar += rt[3];
             const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
             if (lane == 0) {
-                S.part[w][0][0] = ds;
-                S.part[w][0][1] = rs;
+                S.cost16[q] = 256.0f * ds + 6.0e-4f * rs;
             }
         }
-        __syncthreads();
-        if (tid == 0) {
-            if (n == 16) {
-                for (int q{0}; q < 4; ++q) {
-                    const float ds{(S.part[0][q][0] + S.part[1][q][0]) +
-                                   (S.part[2][q][0] + S.part[3][q][0])};
-                    const float rs{(S.part[0][q][1] + S.part[1][q][1]) +
-                                   (S.part[2][q][1] + S.part[3][q][1])};
-                    S.cost16[q] = 256.0f * ds + 6.0e-4f * rs;
-                }
-            } else if (n == 32) {
-                const float ds{(S.part[0][0][0] + S.part[1][0][0]) + (S.part[2][0][0] + S.part[3][0][0])};
-                const float rs{(S.part[0][0][1] + S.part[1][0][1]) + (S.part[2][0][1] + S.part[3][0][1])};
-                S.cost32 = 1024.0f * ds + 6.0e-4f * rs;
-            }
+    } else {
+        float ad{td[0]}, ar{tr[0]};
+#pragma unroll
+        for (int f{1}; f < 8; ++f) {
+            ad += td[f];
+            ar += tr[f];
         }
-        __syncthreads();
+#pragma unroll
+        for (int off{4}; off > 0; off >>= 1) {
+            ad += __shfl_down(ad, off, 8);
+            ar += __shfl_down(ar, off, 8);
+        }
+        if ((lane & 7) == 0) {
+            const int b{(((wu - 2) << 6) | lane) >> 3};
+            const int bx{b & 3}, by{b >> 2};
+            S.cost8[((by >> 1) * 2 + (bx >> 1)) * 4 + (by & 1) * 2 + (bx & 1)] = 64.0f * ad + 6.0e-4f * ar;
+        }
     }
+    __syncthreads();
 
     if (tid == 0) {
         constexpr float inf{1.0e30f}, eps{1.0e-6f};

@@ -754,10 +754,17 @@ static void element_cost(int n, int e, float coef, float qgsf, const float* wsel
     *r = q != 0.0f ? 1.0f + jo_log2(1.0f + fabsf(q)) : 0.0f;
 }
 
-/* Region cost of an n x n block in the device layout: 256 threads = 4 waves
- * (w) x 64 lanes (L), element e of a DCT8 block is lane L = e; a DCT16 quad
- * element e = 64w + L; a DCT32 element e = 256k + 64w + L summed over k first.
- * Per wave tree64, then (P0 + P1) + (P2 + P3). */
+/* Region cost of an n x n block, summed in the device's register layout.
+ * DCT8: lane fx holds column fx (elements fx*8 + f, f = 0..7), summed in f
+ * order; the block's 8 lanes combine by a shuffle-down tree over offsets 4, 2,
+ * 1.  DCT16: lane L (t = L & 15, g = L >> 4) holds elements t*16 + 4g + u,
+ * u = 0..3, summed in u order, then tree64.  DCT32: lane L holds, for
+ * b = 0..3 and u = 0..3 in that order, element (16(b&1) + t)*32 + 16(b>>1) +
+ * 4g + u, then tree64. */
+static float tree8(const float* s) {
+    return ((s[0] + s[4]) + (s[2] + s[6])) + ((s[1] + s[5]) + (s[3] + s[7]));
+}
+
 static float block_cost(const float* y, size_t pitch, int n, float qgsf) {
     const float* wsel = wsel_for(n);
     float coef[1024], d[1024], r[1024];
@@ -767,31 +774,39 @@ static float block_cost(const float* y, size_t pitch, int n, float qgsf) {
     }
     float dist, rate;
     if (n == 8) {
-        dist = tree64(d);
-        rate = tree64(r);
+        float sd[8], sr[8];
+        for (int fx = 0; fx < 8; ++fx) {
+            sd[fx] = d[fx * 8];
+            sr[fx] = r[fx * 8];
+            for (int f = 1; f < 8; ++f) {
+                sd[fx] += d[fx * 8 + f];
+                sr[fx] += r[fx * 8 + f];
+            }
+        }
+        dist = tree8(sd);
+        rate = tree8(sr);
     } else {
-        float pd[4], pr[4];
-        for (int wv = 0; wv < 4; ++wv) {
-            float ld[64], lr[64];
-            for (int l = 0; l < 64; ++l) {
-                if (n == 16) {
-                    ld[l] = d[64 * wv + l];
-                    lr[l] = r[64 * wv + l];
-                } else {
-                    float ad = d[64 * wv + l], ar = r[64 * wv + l];
-                    for (int k = 1; k < 4; ++k) {
-                        ad += d[256 * k + 64 * wv + l];
-                        ar += r[256 * k + 64 * wv + l];
+        float ld[64], lr[64];
+        for (int l = 0; l < 64; ++l) {
+            const int t = l & 15, g = l >> 4;
+            int first = 1;
+            for (int b = 0; b < (n == 32 ? 4 : 1); ++b) {
+                for (int u = 0; u < 4; ++u) {
+                    const int e = n == 32 ? (16 * (b & 1) + t) * 32 + 16 * (b >> 1) + 4 * g + u
+                                          : t * 16 + 4 * g + u;
+                    if (first) {
+                        ld[l] = d[e];
+                        lr[l] = r[e];
+                        first = 0;
+                    } else {
+                        ld[l] += d[e];
+                        lr[l] += r[e];
                     }
-                    ld[l] = ad;
-                    lr[l] = ar;
                 }
             }
-            pd[wv] = tree64(ld);
-            pr[wv] = tree64(lr);
         }
-        dist = (pd[0] + pd[1]) + (pd[2] + pd[3]);
-        rate = (pr[0] + pr[1]) + (pr[2] + pr[3]);
+        dist = tree64(ld);
+        rate = tree64(lr);
     }
     return (float)(n * n) * dist + 6.0e-4f * rate;
 }
