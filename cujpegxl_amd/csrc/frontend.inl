@@ -299,13 +299,14 @@ struct RegionShared {
     float cost32;
     float cfl[4][3];
     int acs[16];           // region-local raster
+    float rate[256];       // selection rate proxy 1 + log2(1 + k), staged from the tables
 };
 
 // Selection terms of one Y coefficient (transform_select_impl.cuh:49-108):
 // squared quantization error and rate proxy; LLF coefficients contribute 0.
 // The two quotients are the correctly rounded ones, by reciprocals.
 __device__ __forceinline__ void selection_terms(float s, int e, int n, const float* wsel, const float* wsel_inv,
-                                                const FrameParams& prm, float& d, float& rt) {
+                                                const float* rate, const FrameParams& prm, float& d, float& rt) {
     const int m{n / 8};
     if (e / n < m && e % n < m) {
         d = 0.0f;
@@ -317,7 +318,7 @@ __device__ __forceinline__ void selection_terms(float s, int e, int n, const flo
     const float err{div_by_recip(q * wq, prm.qgsf_select, prm.inv_qgsf_select) - s};
     d = err * err;
     const float aq{fabsf(q)};
-    rt = aq < 256.0f ? g_tables.rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
+    rt = aq < 256.0f ? rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
 }
 
 // Step 1 of a region: NV12 -> XYB over the 36x36 tile (2-pixel halo,
@@ -400,6 +401,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
     const int bx0{rx * 4}, by0{ry * 4};
 
+    S.rate[tid] = g_tables.rate[tid];
     // 1. NV12 -> XYB over the 36x36 tile (2-pixel halo, clamped coordinates).
     xyb_tile(S, luma, luma_pitch, chroma, chroma_pitch, px0, py0, W, H, tid);
     if (tid < 16) {
@@ -551,7 +553,7 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
                 const int e{fx * 32 + fy};
                 const float v{acc[4 * b + u]};
-                selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, prm, td[4 * b + u], tr[4 * b + u]);
+                selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, S.rate, prm, td[4 * b + u], tr[4 * b + u]);
                 S.m.coefy[2][cpad(e)] = static_cast<_Float16>(v);
             }
         }
@@ -565,7 +567,7 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const int fy{4 * grp + u}, fx{t16};
                 const int e{fx * 16 + fy};
                 const float v{acc[4 * q + u]};
-                selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, prm, td[4 * q + u], tr[4 * q + u]);
+                selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, S.rate, prm, td[4 * q + u], tr[4 * q + u]);
                 S.m.coefy[1][cpad(256 * q + e)] = static_cast<_Float16>(v);
             }
         }
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(256) void region_kernel(
 #pragma unroll
         for (int f{0}; f < 8; ++f) {
             const int e{fx * 8 + f};
-            selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, prm, td[f], tr[f]);
+            selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, S.rate, prm, td[f], tr[f]);
             S.m.coefy[0][cpad(base + e)] = static_cast<_Float16>(o[f]);
         }
     }
