@@ -398,3 +398,35 @@ def test_dct8_uniform_quantizer_is_rint_of_coeff_over_step(jo):
         check((v[2, :, 0] - qdy * dstep[1]) / dstep[2], dq[2])
         assert np.all(a[:, :, 0] == 0)
         assert checked > 3 * nblk * 60 and ties < checked // 100
+
+
+def test_dct_error_report_per_size(jo):
+    """Error of the path's DCT against an FP64 DCT of the same sharpened
+    planes, per transform size, on a synthetic frame with every block forced
+    to that size: the FP32 transform (what the device computes, bit for bit,
+    on the VALU and the matrix cores) and the FP16 values it stores.  The FP32
+    error stays within 1e-6 of the largest coefficient magnitude; the FP16
+    value is the FP32 one rounded to nearest (within half an FP16 ulp,
+    2^-11 relative, or 2^-25 absolute in the subnormal range)."""
+    w = h = 256
+    luma, chroma = synthetic_nv12(w, h, 11)
+    sharp = jo.gaborish(jo.nv12_to_xyb(luma, chroma, w, h))
+    report = {}
+    for n in (8, 16, 32):
+        acs = np.zeros((h // 8) * (w // 8), np.int8)
+        s = n // 8
+        for by in range(0, h // 8, s):
+            for bx in range(0, w // 8, s):
+                acs[by * (w // 8) + bx] = n
+        c32 = jo.forward_dct(sharp, acs).astype(np.float64)
+        c64 = jo.forward_dct_f64(sharp, acs)
+        c16 = jo.to_f16(c32.astype(np.float32)).view(np.float16).astype(np.float64)
+        scale = np.abs(c64).max()
+        e32, e16 = c32 - c64, c16 - c64
+        report[n] = (np.abs(e32).max() / scale, np.sqrt(np.mean(e32 ** 2)) / scale,
+                     np.abs(e16).max() / scale, np.sqrt(np.mean(e16 ** 2)) / scale)
+        assert report[n][0] < 1e-6, n
+        assert (np.abs(c16 - c32) <= np.abs(c32) * 2.0 ** -11 + 2.0 ** -25).all(), n
+    print("DCT error vs FP64 (relative to max |coef|): " + "; ".join(
+        f"N={n}: fp32 max {r[0]:.2e} rms {r[1]:.2e}, fp16 max {r[2]:.2e} rms {r[3]:.2e}"
+        for n, r in report.items()))
