@@ -253,6 +253,13 @@ def forward_dct_f64(planes: np.ndarray, acs: np.ndarray | None) -> np.ndarray:
     return out
 
 
+def dct_basis(n: int) -> np.ndarray:
+    """A[k][t] = g_k cos(pi (t + 1/2) k / n) as the path uses it (float32)."""
+    out = np.zeros(n * n, np.float32)
+    lib().jo_dct_basis(n, ptr(out))
+    return out
+
+
 def to_f16(a: np.ndarray) -> np.ndarray:
     """Binary16 bits, round to nearest even with subnormals (numpy's cast, which
     tests pin against the oracle's own jo_f32_to_f16)."""

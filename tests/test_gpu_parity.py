@@ -18,7 +18,7 @@ import pytest
 import cujpegxl_amd as cj
 import patterns
 from cujpegxl_amd.synth import synthetic_nv12
-from decoded import check_decoded
+from decoded import check_decoded, psnr, reconstruct_xyb
 
 pytestmark = pytest.mark.gpu
 
@@ -308,3 +308,16 @@ def test_dct8_uniform_mode_matches_oracle(jo, w, h, seed):
     assert (dec["qf"] == jo.calibrate(1.0).raw_quant_field).all()
     nv12.close()
     enc.close()
+
+
+def test_device_4k_file_decodes_to_the_source(jo):
+    """The 4K device file, decoded and rebuilt to pixels (no EPF), against
+    the source frame's XYB."""
+    say("decode 4K device file to pixels")
+    w, h = 3840, 2160
+    luma, chroma = synthetic_nv12(w, h, 0)
+    _enc, data, _buf = device_encode(w, h, 1.0, luma, chroma)
+    rec = reconstruct_xyb(jo, jo.decode(data))
+    xyb = jo.nv12_to_xyb(luma, chroma, w, h).astype(np.float64)
+    got = [psnr(rec[c], xyb[c]) for c in range(3)]
+    assert all(g >= f for g, f in zip(got, (38.0, 34.0, 35.0))), got

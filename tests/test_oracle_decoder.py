@@ -5,7 +5,8 @@ reference tests' frames, whose SHA-256 equal the reference host writer's
 golden hashes -- back to those frames; it round-trips the oracle's full
 encodes, multi-group, single-group and DCT8-only; and it rejects damaged
 files (truncation, a trailing byte the TOC does not account for, a bad
-signature, flipped bits)."""
+signature, flipped bits).  Pixels rebuilt from decoded files match the source
+frame to the PSNR the distance implies."""
 
 from __future__ import annotations
 
@@ -16,7 +17,7 @@ import pytest
 
 import patterns
 from cujpegxl_amd.synth import synthetic_nv12
-from decoded import check_decoded
+from decoded import check_decoded, psnr, reconstruct_xyb
 
 
 @pytest.mark.parametrize("name,maker", [
@@ -92,3 +93,21 @@ def test_rejects_damaged_files(jo):
         except jo.DecodeError:
             continue
         assert any(not np.array_equal(dec[k], good[k]) for k in ("ac", "dc", "acs", "qf", "ytox", "ytob"))
+
+
+@pytest.mark.parametrize("w,h,d,seed,floor", [(512, 264, 1.0, 1, (38.0, 34.0, 35.0)),
+                                              (256, 256, 1.0, 2, (38.0, 34.0, 35.0)),
+                                              (512, 264, 3.0, 1, (33.0, 30.0, 31.0))])
+def test_decoded_pixels_match_the_source(jo, w, h, d, seed, floor):
+    """Pixels rebuilt from the decoded file (dequantization, colour
+    correlation, DC-derived lowest frequencies, inverse DCT, the decoder's
+    gaborish; no EPF) against the source frame's XYB: per-channel PSNR floors,
+    and the decoder's gaborish undoes the encoder's sharpening (it raises the
+    PSNR of Y)."""
+    luma, chroma = synthetic_nv12(w, h, seed)
+    dec = jo.decode(jo.encode_nv12(luma, chroma, w, h, d))
+    xyb = jo.nv12_to_xyb(luma, chroma, w, h).astype(np.float64)
+    rec = reconstruct_xyb(jo, dec)
+    got = [psnr(rec[c], xyb[c]) for c in range(3)]
+    assert all(g >= f for g, f in zip(got, floor)), got
+    assert psnr(reconstruct_xyb(jo, dec, gaborish=False)[1], xyb[1]) < got[1]
