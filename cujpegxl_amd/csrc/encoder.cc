@@ -476,19 +476,32 @@ bool host_pinned(const void* p) {
 
 }  // namespace
 
-// The readback's first half, queued behind the frame: the total size, and,
-// when the output will go through the pinned staging buffer, the first two
-// chunks of the file before the size is known (bytes past the end are never
-// used), so the host's first wait returns with data already staged.
-bool Encoder::queue_readback(bool speculative) {
+// The readback's first half, queued behind the frame: the total size, and
+// part of the file before the size is known.  A page-locked output `dst`
+// receives a predicted length (the previous frame's size plus 1/16,
+// within `cap`) by direct DMA, so one host wait usually delivers the whole
+// file; bytes of `dst` past the file's end may be overwritten.  Any other
+// output (or none yet, for encode_async) gets the first two chunks staged in
+// the pinned buffer (bytes past the end are never used).
+bool Encoder::queue_readback(std::uint8_t* dst, std::size_t cap) {
     mark(Stage::kReadback);
     staged_ = 0;
+    direct_len_ = 0;
     if (hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),
                        sizeof(std::uint64_t), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
         hipEventRecord(ev_size_, stream_) != hipSuccess) {
         return false;
     }
-    if (speculative) {
+    if (dst != nullptr && host_pinned(dst)) {
+        const std::size_t guess{last_size_ != 0 ? last_size_ + last_size_ / 16 + 4096
+                                                : static_cast<std::size_t>(geo_.width) * geo_.height * 3 / 8};
+        direct_len_ = std::min({guess, cap, out_cap_});
+        if (direct_len_ != 0 &&
+            hipMemcpyAsync(dst, out_, direct_len_, hipMemcpyDeviceToHost, stream_) != hipSuccess) {
+            return false;
+        }
+        direct_dst_ = dst;
+    } else {
         for (int slot{0}; slot < 2 && staged_ < out_cap_; ++slot) {
             if (!stage_chunk(slot, std::min(kStageChunk, out_cap_ - staged_))) {
                 return false;
@@ -523,7 +536,12 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
     *size = static_cast<std::size_t>(total);
     bool ok{total <= out_cap_};
     if (ok && total <= cap && dst != nullptr) {
-        if (host_pinned(dst)) {
+        if (direct_len_ != 0 && dst == direct_dst_) {
+            if (total > direct_len_) {
+                ok = hipMemcpyAsync(dst + direct_len_, reinterpret_cast<const std::uint8_t*>(out_) + direct_len_,
+                                    total - direct_len_, hipMemcpyDeviceToHost, stream_) == hipSuccess;
+            }
+        } else if (host_pinned(dst)) {
             ok = hipMemcpyAsync(dst, out_, total, hipMemcpyDeviceToHost, stream_) == hipSuccess;
         } else {
             std::size_t done{0};
@@ -545,6 +563,11 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
         }
     }
     stage_len_[0] = stage_len_[1] = 0;
+    direct_len_ = 0;
+    direct_dst_ = nullptr;
+    if (ok) {
+        last_size_ = static_cast<std::size_t>(total);
+    }
     mark(Stage::kCount);
     if (hipStreamSynchronize(stream_) != hipSuccess) {
         return false;
@@ -560,7 +583,7 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
 }
 
 bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
-                          std::size_t chroma_pitch, hipStream_t caller, bool speculative) {
+                          std::size_t chroma_pitch, hipStream_t caller, std::uint8_t* dst, std::size_t cap) {
     if (pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
         return false;
     }
@@ -579,13 +602,13 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     mark(Stage::kQuantize);
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
                          nz_count_, entropy_clears());
-    return run_entropy() && queue_readback(speculative);
+    return run_entropy() && queue_readback(dst, cap);
 }
 
 bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                           std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
     caller_set_ = false;
-    if (!submit_nv12(luma, luma_pitch, chroma, chroma_pitch, nullptr, !host_pinned(out))) {
+    if (!submit_nv12(luma, luma_pitch, chroma, chroma_pitch, nullptr, out, cap)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -603,7 +626,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
                         ytob_);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy() || !queue_readback(!host_pinned(out))) {
+    if (!run_entropy() || !queue_readback(out, cap)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -614,7 +637,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
 bool Encoder::encode_async(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                            std::size_t chroma_pitch, hipStream_t caller) {
     caller_set_ = true;
-    const bool ok{submit_nv12(luma, luma_pitch, chroma, chroma_pitch, caller, true)};
+    const bool ok{submit_nv12(luma, luma_pitch, chroma, chroma_pitch, caller, nullptr, 0)};
     caller_set_ = false;
     if (!ok && pending_) {
         pending_ = false;
@@ -715,7 +738,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy() || !queue_readback(!host_pinned(out))) {
+    if (!run_entropy() || !queue_readback(out, cap)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;

@@ -89,8 +89,8 @@ private:
     bool run_entropy();
     Clears entropy_clears() const;
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
-                     std::size_t chroma_pitch, hipStream_t caller, bool speculative);
-    bool queue_readback(bool speculative);
+                     std::size_t chroma_pitch, hipStream_t caller, std::uint8_t* dst, std::size_t cap);
+    bool queue_readback(std::uint8_t* dst, std::size_t cap);
     bool stage_chunk(int slot, std::size_t n);
     bool upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch);
@@ -109,6 +109,9 @@ private:
     bool pending_{false};            // a frame's readback is queued, finish() not yet called
     std::size_t staged_{0};          // output bytes queued into the staging chunks
     std::size_t stage_len_[2]{0, 0};
+    std::uint8_t* direct_dst_{nullptr};  // page-locked output the predicted DMA went to
+    std::size_t direct_len_{0};          // bytes of that DMA
+    std::size_t last_size_{0};           // the previous frame's file size (the prediction)
     std::uint8_t* input_{nullptr};   // device NV12 frame for host input (allocated on first use)
     bool profiling_{false};
     std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
