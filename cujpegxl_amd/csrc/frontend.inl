@@ -302,6 +302,15 @@ struct RegionShared {
     float rate[256];       // selection rate proxy 1 + log2(1 + k), staged from the tables
 };
 
+// True for the lowest (n/8)^2 frequencies (fx, fy < n/8) of element
+// e = fx * n + fy of an n x n transform, n = 8, 16 or 32 (shifts, not a
+// division by a runtime n).
+__device__ __forceinline__ bool is_llf(int e, int n) {
+    const int l2{n == 32 ? 5 : (n == 16 ? 4 : 3)};
+    const int m{n >> 3};
+    return (e >> l2) < m && (e & (n - 1)) < m;
+}
+
 // Selection terms of one Y coefficient (transform_select_impl.cuh:49-108):
 // squared quantization error and rate proxy; LLF coefficients contribute 0.
 // The two quotients are the correctly rounded ones, by reciprocals.
@@ -361,6 +370,7 @@ __device__ __forceinline__ void gaborish_tile(RegionShared& S, int tid) {
     constexpr float k4{-0.0014789063378272242f};
     constexpr float nm{1.0f / (1.0f + 4.0f * (k0 + k1 + k2 + k4 + 2.0f * k3))};
     constexpr float wc{nm}, wr{nm * k0}, wr2{nm * k2}, wd{nm * k1}, wl{nm * k3}, wd2{nm * k4};
+#pragma unroll 4
     for (int i{tid}; i < 3 * 1024; i += 256) {
         const int c{i >> 10}, p{i & 1023};
         const float* s{&S.u.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
@@ -413,6 +423,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     {
         gaborish_tile(S, tid);
         // 3a. AQ pre-erosion terms per interior pixel (frontend_fused.cu:118-137).
+#pragma unroll
         for (int p{tid}; p < 1024; p += 256) {
             const float* s{&S.u.xyb[1][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
             const float yv{s[0]};
@@ -444,6 +455,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     if (tid < 64) {
         S.x.aq.ero[tid] = erosion_cell(S.x.aq.cells, tid & 7, tid >> 3, prm);
     }
+#pragma unroll
     for (int j{0}; j < 4; ++j) {
         const int lb{j * 4 + w};
         const int lbx{lb & 3}, lby{lb >> 2};
@@ -805,8 +817,7 @@ __global__ __launch_bounds__(256) void region_kernel(
             coeffs[slot] = __builtin_bit_cast(std::uint16_t, hx);
             coeffs[cplane + slot] = __builtin_bit_cast(std::uint16_t, hy);
             coeffs[2 * cplane + slot] = __builtin_bit_cast(std::uint16_t, hb);
-            const int m{n / 8};
-            if (!(e / n < m && e % n < m)) {
+            if (!is_llf(e, n)) {
                 const float xr{static_cast<float>(hx)}, yr{static_cast<float>(hy)};
                 const float br{static_cast<float>(hb)};
                 v[0] = xr * yr;
@@ -931,10 +942,11 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
         const float xr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[slot]))};
         const float yr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[cplane + slot]))};
         const float br{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[2 * cplane + slot]))};
-        const bool llf{e / n < m && e % n < m};
+        const bool llf{is_llf(e, n)};
         std::int16_t q0{0}, q1{0}, q2{0};
         if (llf) {
-            const int lp{(e / n) * m + e % n};
+            const int l2{n == 32 ? 5 : (n == 16 ? 4 : 3)};
+            const int lp{(e >> l2) * m + (e & (n - 1))};
             S.llf[op][0][lp] = xr;
             S.llf[op][1][lp] = yr;
             S.llf[op][2][lp] = br;
