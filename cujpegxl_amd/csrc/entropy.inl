@@ -818,7 +818,7 @@ __device__ __forceinline__ int dc_cursor_token(const Geometry& geo, const DcGrou
     return 1;
 }
 
-constexpr int kDcChunk = 4096;               // items per workgroup in the DC passes
+constexpr int kDcChunk = 1024;               // items per workgroup in the DC passes
 constexpr int kDcPerThread = kDcChunk / 256;  // consecutive items per thread
 
 // DC and AcMetadata token histograms per group, plus the first-block count.
