@@ -473,11 +473,11 @@ __device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std
 // once.  The part's bit count falls out of the same pass (a group's size is
 // the sum of its parts').  A stream longer than the LDS copy is emitted again
 // with atomicOr into the zeroed slot.
-constexpr int kGroupParts = 4;
+constexpr int kGroupParts = 8;
 constexpr int kPartRows = kAcGroupBlocks / kGroupParts;
-constexpr int kPartThreads = 512;
+constexpr int kPartThreads = 256;
 constexpr int kPartItems = 3 * kAcGroupBlocks * kPartRows;
-constexpr int kPartWords = 4096;
+constexpr int kPartWords = 2048;
 constexpr int kGroupPerThread = 8;
 // Bound on one part's stream: 28 bits (15-bit code + 13 extra bits) per
 // coefficient slot of its blocks in three channels.
