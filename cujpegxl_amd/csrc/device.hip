@@ -163,7 +163,7 @@ void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac
     const unsigned int waves{(3 * geo.nblocks + 3) / 4};
     const unsigned int grid{waves < resident ? waves : resident};
     ac_tokens_kernel<<<grid, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, tokens, item_tokens, partial);
-    ac_hist_reduce_kernel<<<dim3(kNumClusters * kAcSymbols / 256, 32), 256, 0, s>>>(partial, static_cast<int>(grid),
+    ac_hist_reduce_kernel<<<dim3(kNumClusters * kAcSymbols / 256, 80), 256, 0, s>>>(partial, static_cast<int>(grid),
                                                                                     hist);
 }
 
