@@ -344,8 +344,16 @@ class Encoder:
             raise RuntimeError(f"encode_coefficients failed: {status_string(s)}")
         return self.out[: size.value].tobytes()
 
-    def set_profiling(self, on: bool) -> None:
-        self.lib.cjx_stage_set_profiling(self.handle, 1 if on else 0)
+    def set_profiling(self, on: bool, region_only: bool = False) -> None:
+        """HIP-event timing of the following encodes: every stage, or with
+        `region_only` the region kernel alone (fewer events on the stream)."""
+        self.lib.cjx_stage_set_profiling(self.handle, (2 if region_only else 1) if on else 0)
+
+    def region_ms(self) -> float:
+        """The region kernel's device time in the last encode (profiling on)."""
+        ms = C.c_float(0.0)
+        self.lib.cjx_stage_timings(self.handle, C.byref(ms), 1)
+        return float(ms.value)
 
     def timings(self) -> dict[str, float]:
         ms = (C.c_float * len(STAGE_NAMES))()

@@ -260,7 +260,8 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
                                                      std::uint64_t* __restrict__ ac_group_bits,
                                                      const DcBlobs* __restrict__ blobs,
                                                      Layout* __restrict__ lay, std::uint64_t* __restrict__ ac_base,
-                                                     std::uint64_t* __restrict__ dc_base) {
+                                                     std::uint64_t* __restrict__ dc_base,
+                                                     std::uint64_t* __restrict__ total_out) {
     __shared__ std::uint32_t agw[kAcGlobalWords];
     __shared__ std::uint32_t hw[kHeadWords];
     __shared__ std::uint64_t agbits[kMaxAcGroups];
@@ -441,6 +442,7 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
         lay->framing_bytes = framing;
         lay->codestream_bytes = cs_bytes;
         lay->total_bytes = overflow ? ~0ull : framing + cs_bytes;
+        *total_out = lay->total_bytes;
         lay->overflow = overflow;
         lay->dc_global_bit = dcg_bit;
         lay->ac_global_bit = ac_global_bit;
@@ -533,7 +535,7 @@ __device__ __forceinline__ Piece piece_at(const Geometry& geo, const Constants* 
 __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Constants* __restrict__ k,
                                                        const Layout* __restrict__ lay, Pieces P,
                                                        std::uint32_t* __restrict__ out_words,
-                                                       std::uint64_t cap_words) {
+                                                       std::uint64_t cap_bytes) {
     __shared__ std::uint64_t end_bit[kMaxPieces];
     if (lay->overflow != 0) {
         return;
@@ -544,7 +546,10 @@ __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Const
         end_bit[i] = pc.dst + pc.nbits;
     }
     __syncthreads();
+    // Whole words up to the capacity, then the bytes of a partial last word
+    // that still fit (the output may be the caller's mapped host buffer).
     std::uint64_t words{(lay->total_bytes + 3) / 4};
+    const std::uint64_t cap_words{(cap_bytes + 3) / 4};
     words = words < cap_words ? words : cap_words;
     for (std::uint64_t w{blockIdx.x * 256ull + threadIdx.x}; w < words; w += gridDim.x * 256ull) {
         const std::uint64_t b0{w * 32};
@@ -571,7 +576,14 @@ __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Const
             }
             v |= piece_bits(pc, s - pc.dst, static_cast<std::uint32_t>(e - s)) << static_cast<std::uint32_t>(s - b0);
         }
-        out_words[w] = v;
+        if ((w + 1) * 4 <= cap_bytes) {
+            out_words[w] = v;
+        } else {
+            auto* ob{reinterpret_cast<std::uint8_t*>(out_words)};
+            for (std::uint64_t b{w * 4}; b < cap_bytes; ++b) {
+                ob[b] = static_cast<std::uint8_t>(v >> (8 * (b - w * 4)));
+            }
+        }
     }
 }
 

@@ -1,5 +1,6 @@
 // The encoder's device code in one translation unit (the kernels share the
 // __constant__ tables), plus the host-callable launchers.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -74,7 +75,6 @@ std::size_t sizeof_layout() { return sizeof(Layout); }
 std::size_t sizeof_dc_blobs() { return sizeof(DcBlobs); }
 std::uint32_t dc_chunks_per_group() { return (2 + 6 * 65536 + 2 * 1024 + kDcChunk - 1) / kDcChunk; }
 std::uint32_t blob_words_per_group() { return 2 * kBlobWords; }
-std::uint64_t layout_total_offset() { return offsetof(Layout, total_bytes); }
 
 bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::uint32_t dc_pre_bits,
                    std::uint32_t dc_mid_bits) {
@@ -107,12 +107,15 @@ static unsigned int grid_for(std::size_t n, std::size_t per) {
 
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
-                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part) {
+                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part, hipEvent_t start,
+                   hipEvent_t stop) {
     // Eight equal runs of regions, one per XCD (the kernel maps workgroup i to
-    // run i mod 8); workgroups past the last region exit at once.
+    // run i mod 8); workgroups past the last region exit at once.  The events,
+    // when given, take the kernel's own start and end times (no marker
+    // packets between it and its neighbours).
     const unsigned int per_xcd{(geo.rw * geo.rh + 7) / 8};
-    region_kernel<<<8 * per_xcd, 256, 0, s>>>(geo, prm, luma, luma_pitch, chroma, chroma_pitch, coeffs, acs,
-                                              qf, cfl_part);
+    hipExtLaunchKernelGGL(region_kernel, dim3(8 * per_xcd), dim3(256), 0, s, start, stop, 0, geo, prm, luma,
+                          luma_pitch, chroma, chroma_pitch, coeffs, acs, qf, cfl_part);
 }
 
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
@@ -252,17 +255,18 @@ void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                    const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,
-                   const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base) {
+                   const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base,
+                   std::uint64_t* total_out) {
     layout_kernel<<<1, 256, 0, s>>>(geo, k, cs, ac_part_bits, dc_group_bytes, dc_group_bits, ac_group_bytes,
-                                   ac_group_bits, blobs, lay, ac_base, dc_base);
+                                   ac_group_bits, blobs, lay, ac_base, dc_base, total_out);
 }
 
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
                      const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
-                     const std::uint64_t* ac_base, std::uint32_t* out_words, std::uint64_t cap_words) {
+                     const std::uint64_t* ac_base, std::uint8_t* out, std::uint64_t cap_bytes) {
     const Pieces P{dc_staging, dc_group_slot_words(), ac_staging, dc_bits, ac_bits, ac_part_bits, dc_base, ac_base};
-    assemble_kernel<<<2048, 256, 0, s>>>(geo, k, lay, P, out_words, cap_words);
+    assemble_kernel<<<2048, 256, 0, s>>>(geo, k, lay, P, reinterpret_cast<std::uint32_t*>(out), cap_bytes);
 }
 
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,

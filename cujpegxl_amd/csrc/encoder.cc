@@ -250,7 +250,7 @@ void Encoder::release() {
             e = nullptr;
         }
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
         if (*e != nullptr) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -294,7 +294,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess) {
         return false;
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             return false;
         }
@@ -305,6 +305,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         }
     }
     if (hipHostMalloc(reinterpret_cast<void**>(&total_host_), sizeof(std::uint64_t)) != hipSuccess ||
+        hipHostGetDevicePointer(reinterpret_cast<void**>(&total_dev_), total_host_, 0) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&stage_host_), 2 * kStageChunk) != hipSuccess) {
         return false;
     }
@@ -453,7 +454,7 @@ bool Encoder::build_constants(std::uint32_t global_scale, std::uint32_t quant_dc
 }
 
 void Encoder::mark(Stage s) {
-    if (profiling_) {
+    if (profiling_ == 1 || (profiling_ == 2 && (s == Stage::kRegion || s == Stage::kQuantize))) {
         (void)hipEventRecord(events_[static_cast<int>(s)], stream_);
     }
 }
@@ -476,32 +477,36 @@ bool host_pinned(const void* p) {
 
 }  // namespace
 
-// The readback's first half, queued behind the frame: the total size, and
-// part of the file before the size is known.  A page-locked output `dst`
-// receives a predicted length (the previous frame's size plus 1/16,
-// within `cap`) by direct DMA, so one host wait usually delivers the whole
-// file; bytes of `dst` past the file's end may be overwritten.  Any other
-// output (or none yet, for encode_async) gets the first two chunks staged in
-// the pinned buffer (bytes past the end are never used).
-bool Encoder::queue_readback(std::uint8_t* dst, std::size_t cap) {
+// The caller's output as a device address when it is page-locked host memory
+// mapped for the device: the assembly kernel then writes the file there
+// itself (over the host link, as a DMA would) and no copy follows it.
+std::uint8_t* Encoder::mapped_output(std::uint8_t* dst) const {
+    hipPointerAttribute_t a{};
+    if (dst == nullptr || hipPointerGetAttributes(&a, dst) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (a.type != hipMemoryTypeHost || a.devicePointer == nullptr || a.hostPointer == nullptr) {
+        return nullptr;
+    }
+    // Base-relative: correct whether the runtime reports the allocation's
+    // base or the queried address in both fields.  The kernel stores words,
+    // so an unaligned output takes the copy path instead.
+    std::uint8_t* const dev{static_cast<std::uint8_t*>(a.devicePointer) +
+                            (dst - static_cast<std::uint8_t*>(a.hostPointer))};
+    return reinterpret_cast<std::uintptr_t>(dev) % 4 == 0 ? dev : nullptr;
+}
+
+// The readback's first half, queued behind the frame.  The layout kernel
+// has already written the size into the pinned word (`ev_size_`).  A mapped
+// page-locked output `direct` holds the file once the assembly has finished;
+// any other output (or none yet, for encode_async) gets the first two chunks
+// staged in the pinned buffer (bytes past the end are never used).
+bool Encoder::queue_readback(std::uint8_t* direct) {
     mark(Stage::kReadback);
     staged_ = 0;
-    direct_len_ = 0;
-    if (hipMemcpyAsync(total_host_, reinterpret_cast<const std::uint8_t*>(layout_) + layout_total_offset(),
-                       sizeof(std::uint64_t), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
-        hipEventRecord(ev_size_, stream_) != hipSuccess) {
-        return false;
-    }
-    if (dst != nullptr && host_pinned(dst)) {
-        const std::size_t guess{last_size_ != 0 ? last_size_ + last_size_ / 16 + 4096
-                                                : static_cast<std::size_t>(geo_.width) * geo_.height * 3 / 8};
-        direct_len_ = std::min({guess, cap, out_cap_});
-        if (direct_len_ != 0 &&
-            hipMemcpyAsync(dst, out_, direct_len_, hipMemcpyDeviceToHost, stream_) != hipSuccess) {
-            return false;
-        }
-        direct_dst_ = dst;
-    } else {
+    direct_dst_ = direct;
+    if (direct == nullptr) {
         for (int slot{0}; slot < 2 && staged_ < out_cap_; ++slot) {
             if (!stage_chunk(slot, std::min(kStageChunk, out_cap_ - staged_))) {
                 return false;
@@ -521,6 +526,26 @@ bool Encoder::stage_chunk(int slot, std::size_t n) {
     return ok;
 }
 
+// The end of a frame.  The host has just woken from the size event, so only
+// the assembly's tail and the last copy remain on the stream (tens of
+// microseconds): polling the completion event here saves the wake-up latency
+// of a blocking wait, which is of the same order.
+bool Encoder::wait_done() {
+    if (hipEventRecord(ev_done_, stream_) != hipSuccess) {
+        return false;
+    }
+    for (;;) {
+        const hipError_t r{hipEventQuery(ev_done_)};
+        if (r == hipSuccess) {
+            return true;
+        }
+        if (r != hipErrorNotReady) {
+            return false;
+        }
+        __builtin_ia32_pause();
+    }
+}
+
 // The readback's second half: waits for the size, then moves the bytes into
 // `dst` -- one DMA when `dst` is pinned, else through the two staging chunks,
 // each host copy overlapping the next chunk's DMA.
@@ -536,11 +561,9 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
     *size = static_cast<std::size_t>(total);
     bool ok{total <= out_cap_};
     if (ok && total <= cap && dst != nullptr) {
-        if (direct_len_ != 0 && dst == direct_dst_) {
-            if (total > direct_len_) {
-                ok = hipMemcpyAsync(dst + direct_len_, reinterpret_cast<const std::uint8_t*>(out_) + direct_len_,
-                                    total - direct_len_, hipMemcpyDeviceToHost, stream_) == hipSuccess;
-            }
+        if (direct_dst_ != nullptr && dst == direct_dst_) {
+            // The assembly kernel writes the file into `dst`; wait_done() below
+            // waits for it.
         } else if (host_pinned(dst)) {
             ok = hipMemcpyAsync(dst, out_, total, hipMemcpyDeviceToHost, stream_) == hipSuccess;
         } else {
@@ -563,19 +586,17 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
         }
     }
     stage_len_[0] = stage_len_[1] = 0;
-    direct_len_ = 0;
     direct_dst_ = nullptr;
-    if (ok) {
-        last_size_ = static_cast<std::size_t>(total);
-    }
     mark(Stage::kCount);
-    if (hipStreamSynchronize(stream_) != hipSuccess) {
+    if (!wait_done()) {
         return false;
     }
-    if (profiling_) {
+    if (profiling_ != 0) {
         for (int i{0}; i < static_cast<int>(Stage::kCount); ++i) {
             float ms{0.0f};
-            (void)hipEventElapsedTime(&ms, events_[i], events_[i + 1]);
+            if (profiling_ == 1 || i == static_cast<int>(Stage::kRegion)) {
+                (void)hipEventElapsedTime(&ms, events_[i], events_[i + 1]);
+            }
             stage_ms_[i] = ms;
         }
     }
@@ -591,18 +612,25 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
                         hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess)) {
         return false;
     }
-    mark(Stage::kRegion);
-    launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_);
+    // With profiling on, the region kernel's own start and end times open the
+    // region and quantize stages.
+    const bool timed{profiling_ != 0};
+    launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_,
+                  timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr,
+                  timed ? events_[static_cast<int>(Stage::kQuantize)] : nullptr);
     // The planes are read by the region kernel only: later work on the
     // caller's stream may overwrite them once it has finished.
     if (caller_set_ && (hipEventRecord(ev_input_free_, stream_) != hipSuccess ||
                         hipStreamWaitEvent(caller, ev_input_free_, 0) != hipSuccess)) {
         return false;
     }
-    mark(Stage::kQuantize);
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
                          nz_count_, entropy_clears());
-    return run_entropy() && queue_readback(dst, cap);
+    // A mapped page-locked output receives the file from the assembly kernel.
+    std::uint8_t* const direct{mapped_output(dst)};
+    return run_entropy(direct != nullptr ? direct : reinterpret_cast<std::uint8_t*>(out_),
+                       direct != nullptr ? cap : out_cap_) &&
+           queue_readback(direct != nullptr ? dst : nullptr);
 }
 
 bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
@@ -626,7 +654,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
                         ytob_);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy() || !queue_readback(out, cap)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -738,7 +766,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy() || !queue_readback(out, cap)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -759,7 +787,7 @@ Clears Encoder::entropy_clears() const {
     return c;
 }
 
-bool Encoder::run_entropy() {
+bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap) {
     auto* codes{static_cast<CodeSet*>(codes_)};
     auto* k{static_cast<const Constants*>(constants_)};
     auto* blobs{static_cast<DcBlobs*>(blobs_)};
@@ -792,10 +820,13 @@ bool Encoder::run_entropy() {
         return false;
     }
     launch_layout(stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
-                  ac_group_bits_, blobs, lay, ac_base_, dc_base_);
+                  ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_);
+    if (hipEventRecord(ev_size_, stream_) != hipSuccess) {
+        return false;
+    }
     mark(Stage::kEmit);
     launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
-                    dc_base_, ac_base_, out_, out_cap_ / 4);
+                    dc_base_, ac_base_, out, cap);
     return hipGetLastError() == hipSuccess;
 }
 

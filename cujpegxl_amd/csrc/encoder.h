@@ -74,8 +74,12 @@ public:
     bool fetch(Buffer which, void* host, std::size_t bytes);
     std::size_t buffer_bytes(Buffer which) const;
 
-    void set_profiling(bool on) { profiling_ = on; }
-    // Milliseconds per stage of the last encode (profiling on).
+    // 0: no events; 1: an event pair around every stage; 2: the region
+    // kernel's pair only (each timed event on the stream costs a few
+    // microseconds of idle device time between the kernels it separates).
+    void set_profiling(int level) { profiling_ = level; }
+    // Milliseconds per stage of the last encode (profiling on; with level 2
+    // only the region entry is filled).
     const std::array<float, static_cast<int>(Stage::kCount)>& stage_ms() const { return stage_ms_; }
     hipStream_t stream() const { return stream_; }
     const Geometry& geometry() const { return geo_; }
@@ -86,15 +90,17 @@ private:
     bool alloc();
     void release();
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
-    bool run_entropy();
+    bool run_entropy(std::uint8_t* out, std::size_t cap);
+    std::uint8_t* mapped_output(std::uint8_t* dst) const;
     Clears entropy_clears() const;
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch, hipStream_t caller, std::uint8_t* dst, std::size_t cap);
-    bool queue_readback(std::uint8_t* dst, std::size_t cap);
+    bool queue_readback(std::uint8_t* direct);
     bool stage_chunk(int slot, std::size_t n);
     bool upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch);
     void mark(Stage s);
+    bool wait_done();
 
     int device_{-1};
     float distance_{1.0f};
@@ -104,16 +110,15 @@ private:
     hipStream_t dc_stream_{nullptr};  // DC entropy chain, joined by events
     hipEvent_t ev_fork_{nullptr}, ev_dc_done_{nullptr};  // fork onto / join from the DC stream
     hipEvent_t ev_size_{nullptr};                        // the total size has reached the host
+    hipEvent_t ev_done_{nullptr};                        // the frame's last copy has finished
     hipEvent_t ev_caller_{nullptr}, ev_input_free_{nullptr};  // ordering against a caller's stream
     bool caller_set_{false};
     bool pending_{false};            // a frame's readback is queued, finish() not yet called
     std::size_t staged_{0};          // output bytes queued into the staging chunks
     std::size_t stage_len_[2]{0, 0};
-    std::uint8_t* direct_dst_{nullptr};  // page-locked output the predicted DMA went to
-    std::size_t direct_len_{0};          // bytes of that DMA
-    std::size_t last_size_{0};           // the previous frame's file size (the prediction)
+    std::uint8_t* direct_dst_{nullptr};  // page-locked output the assembly wrote the file into
     std::uint8_t* input_{nullptr};   // device NV12 frame for host input (allocated on first use)
-    bool profiling_{false};
+    int profiling_{0};
     std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
     std::array<float, static_cast<int>(Stage::kCount)> stage_ms_{};
     std::uint32_t const_gs_{0}, const_qdc_{0};
@@ -151,7 +156,8 @@ private:
     std::uint64_t* dc_base_{nullptr};
     std::uint32_t* out_{nullptr};
     std::size_t out_cap_{0};
-    std::uint64_t* total_host_{nullptr};  // pinned
+    std::uint64_t* total_host_{nullptr};  // pinned; the layout kernel writes the file size here
+    std::uint64_t* total_dev_{nullptr};   // the same word as a device address
     // Readback: the file is copied in chunks through two pinned slots, the
     // device-to-host copy of one chunk overlapping the host copy of the last.
     static constexpr std::size_t kStageChunk = 512 * 1024;

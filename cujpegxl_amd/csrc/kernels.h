@@ -29,11 +29,11 @@ void fill_constants(void* dst, const std::uint32_t* head, std::uint32_t head_bit
                     const std::uint32_t* dc_mid, std::uint32_t dc_mid_bits);
 bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::uint32_t dc_pre_bits,
                    std::uint32_t dc_mid_bits);
-std::uint64_t layout_total_offset();
 
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
-                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part);
+                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
+                   hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // Config-1 semantics: DCT8 everywhere, uniform quant field, no CfL maps.
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                          std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
@@ -76,11 +76,12 @@ void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                    const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,
-                   const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base);
+                   const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base,
+                   std::uint64_t* total_out);
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
                      const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
-                     const std::uint64_t* ac_base, std::uint32_t* out_words, std::uint64_t cap_words);
+                     const std::uint64_t* ac_base, std::uint8_t* out, std::uint64_t cap_bytes);
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,

@@ -95,10 +95,10 @@ int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* e, const cujpegxl_nv12_input
 }
 
 int cjx_stage_set_profiling(cujpegxl_encoder* e, int on) {
-    if (e == nullptr) {
+    if (e == nullptr || on < 0 || on > 2) {
         return CUJPEGXL_INVALID_ARGUMENT;
     }
-    e->engine.set_profiling(on != 0);
+    e->engine.set_profiling(on);
     return CUJPEGXL_OK;
 }
 

@@ -60,7 +60,9 @@ int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* encoder, const cujpegxl_nv12
 
 /* Per-stage device time of the following encodes (HIP events on the
  * encoder's stream): region, quantize, histograms, codes, sizes, emit,
- * readback.  `ms` receives 7 floats for the last encode. */
+ * readback.  `ms` receives 7 floats for the last encode.  `on`: 0 off,
+ * 1 every stage, 2 the region kernel only (one event pair; the others read
+ * 0).  Each timed event adds a few microseconds of idle device time. */
 int cjx_stage_set_profiling(cujpegxl_encoder* encoder, int on);
 int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
 

@@ -75,6 +75,20 @@ def test_pinned_output_matches_oracle(frame):
     out = cj.Output(pin.ptr, 0, 100)
     assert enc.lib.cujpegxl_encoder_encode(enc.handle, C.byref(inp), C.byref(out)) == cj.BUFFER_TOO_SMALL
     assert out.size == len(want)
+    # Word-aligned interior addresses (the device writes the file straight
+    # into them) and an unaligned one (copied by DMA), with room for exactly
+    # the file or for a prefix that ends inside a word: nothing outside
+    # [off, off + cap) changes.
+    for off, cap in ((8, len(want)), (1, len(want)), (8, 101), (4, len(want) + 3)):
+        pin.array[:] = 0x5C
+        out = cj.Output(pin.ptr + off, 0, cap)
+        status = enc.lib.cujpegxl_encoder_encode(enc.handle, C.byref(inp), C.byref(out))
+        assert out.size == len(want)
+        if cap >= len(want):
+            assert status == cj.OK and pin.array[off: off + len(want)].tobytes() == want
+        else:
+            assert status == cj.BUFFER_TOO_SMALL
+        assert (pin.array[:off] == 0x5C).all() and (pin.array[off + cap: off + cap + 64] == 0x5C).all()
     pin.close()
     dev.close()
     enc.close()
