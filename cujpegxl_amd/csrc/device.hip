@@ -130,9 +130,10 @@ void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& 
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
-                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr) {
-    tile_quantize_kernel<<<geo.cmw * geo.cmh, 256, 0, s>>>(geo, prm, coeffs, acs, qf, cfl_part, ytox, ytob,
-                                                           ac, dc, nz_grid, nz_count, clr);
+                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr,
+                          hipEvent_t done) {
+    hipExtLaunchKernelGGL(tile_quantize_kernel, dim3(geo.cmw * geo.cmh), dim3(256), 0, s, nullptr, done, 0, geo, prm,
+                          coeffs, acs, qf, cfl_part, ytox, ytob, ac, dc, nz_grid, nz_count, clr);
 }
 
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
@@ -256,9 +257,10 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
                    const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,
                    const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base,
-                   std::uint64_t* total_out) {
-    layout_kernel<<<1, 256, 0, s>>>(geo, k, cs, ac_part_bits, dc_group_bytes, dc_group_bits, ac_group_bytes,
-                                   ac_group_bits, blobs, lay, ac_base, dc_base, total_out);
+                   std::uint64_t* total_out, hipEvent_t done) {
+    hipExtLaunchKernelGGL(layout_kernel, dim3(1), dim3(256), 0, s, nullptr, done, 0, geo, k, cs, ac_part_bits,
+                          dc_group_bytes, dc_group_bits, ac_group_bytes, ac_group_bits, blobs, lay, ac_base, dc_base,
+                          total_out);
 }
 
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,

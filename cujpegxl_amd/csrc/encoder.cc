@@ -294,8 +294,14 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess) {
         return false;
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_dc_done_, &ev_done_, &ev_caller_, &ev_input_free_}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+            return false;
+        }
+    }
+    // Bound to kernels' completion at launch (timing-capable events).
+    for (hipEvent_t* e : {&ev_fork_, &ev_size_}) {
+        if (hipEventCreate(e) != hipSuccess) {
             return false;
         }
     }
@@ -498,7 +504,7 @@ std::uint8_t* Encoder::mapped_output(std::uint8_t* dst) const {
 }
 
 // The readback's first half, queued behind the frame.  The layout kernel
-// has already written the size into the pinned word (`ev_size_`).  A mapped
+// writes the size into the pinned word (its completion is `ev_size_`).  A mapped
 // page-locked output `direct` holds the file once the assembly has finished;
 // any other output (or none yet, for encode_async) gets the first two chunks
 // staged in the pinned buffer (bytes past the end are never used).
@@ -624,12 +630,13 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
                         hipStreamWaitEvent(caller, ev_input_free_, 0) != hipSuccess)) {
         return false;
     }
+    // The quantizer's completion is the fork point of the DC chain.
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
-                         nz_count_, entropy_clears());
+                         nz_count_, entropy_clears(), ev_fork_);
     // A mapped page-locked output receives the file from the assembly kernel.
     std::uint8_t* const direct{mapped_output(dst)};
     return run_entropy(direct != nullptr ? direct : reinterpret_cast<std::uint8_t*>(out_),
-                       direct != nullptr ? cap : out_cap_) &&
+                       direct != nullptr ? cap : out_cap_, true) &&
            queue_readback(direct != nullptr ? dst : nullptr);
 }
 
@@ -654,7 +661,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
                         ytob_);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_) || !queue_readback(nullptr)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -766,7 +773,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_) || !queue_readback(nullptr)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -787,7 +794,7 @@ Clears Encoder::entropy_clears() const {
     return c;
 }
 
-bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap) {
+bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound) {
     auto* codes{static_cast<CodeSet*>(codes_)};
     auto* k{static_cast<const Constants*>(constants_)};
     auto* blobs{static_cast<DcBlobs*>(blobs_)};
@@ -797,7 +804,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap) {
     // the layout; DC emission runs beside AC emission.
     mark(Stage::kHistograms);
     const int ndc{static_cast<int>(2 * geo_.num_dc)};
-    if (hipEventRecord(ev_fork_, stream_) != hipSuccess || hipStreamWaitEvent(dc_stream_, ev_fork_, 0) != hipSuccess) {
+    if ((!fork_bound && hipEventRecord(ev_fork_, stream_) != hipSuccess) ||
+        hipStreamWaitEvent(dc_stream_, ev_fork_, 0) != hipSuccess) {
         return false;
     }
     launch_dc_hist(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
@@ -820,10 +828,7 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap) {
         return false;
     }
     launch_layout(stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
-                  ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_);
-    if (hipEventRecord(ev_size_, stream_) != hipSuccess) {
-        return false;
-    }
+                  ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_, ev_size_);
     mark(Stage::kEmit);
     launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
                     dc_base_, ac_base_, out, cap);

@@ -90,7 +90,8 @@ private:
     bool alloc();
     void release();
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
-    bool run_entropy(std::uint8_t* out, std::size_t cap);
+    // `fork_bound`: the quantizer's launch has bound `ev_fork_` already.
+    bool run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound);
     std::uint8_t* mapped_output(std::uint8_t* dst) const;
     Clears entropy_clears() const;
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,

@@ -30,6 +30,9 @@ void fill_constants(void* dst, const std::uint32_t* head, std::uint32_t head_bit
 bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::uint32_t dc_pre_bits,
                    std::uint32_t dc_mid_bits);
 
+// A launcher's `start`/`stop`/`done` events, when given, are bound to the
+// kernel itself (its start or completion), with no marker packet on the
+// stream between it and its neighbours.
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
@@ -42,7 +45,8 @@ void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& 
 void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams& prm,
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
-                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
+                          std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr,
+                          hipEvent_t done = nullptr);
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
 void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v);
@@ -77,7 +81,7 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
                    const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,
                    const DcBlobs* blobs, Layout* lay, std::uint64_t* ac_base, std::uint64_t* dc_base,
-                   std::uint64_t* total_out);
+                   std::uint64_t* total_out, hipEvent_t done = nullptr);
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
                      const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
