@@ -532,16 +532,10 @@ bool Encoder::stage_chunk(int slot, std::size_t n) {
     return ok;
 }
 
-// The end of a frame.  The host has just woken from the size event, so only
-// the assembly's tail and the last copy remain on the stream (tens of
-// microseconds): polling the completion event here saves the wake-up latency
-// of a blocking wait, which is of the same order.
-bool Encoder::wait_done() {
-    if (hipEventRecord(ev_done_, stream_) != hipSuccess) {
-        return false;
-    }
+// Polls `e` until it has completed (false on an error).
+bool Encoder::spin_on(hipEvent_t e) {
     for (;;) {
-        const hipError_t r{hipEventQuery(ev_done_)};
+        const hipError_t r{hipEventQuery(e)};
         if (r == hipSuccess) {
             return true;
         }
@@ -552,6 +546,14 @@ bool Encoder::wait_done() {
     }
 }
 
+// The end of a frame.  The host already polls (it has the size), and only
+// the assembly's tail or the last copy remains on the stream (tens of
+// microseconds): polling the completion event saves the wake-up latency of
+// a blocking wait, which is of the same order.
+bool Encoder::wait_done() {
+    return hipEventRecord(ev_done_, stream_) == hipSuccess && spin_on(ev_done_);
+}
+
 // The readback's second half: waits for the size, then moves the bytes into
 // `dst` -- one DMA when `dst` is pinned, else through the two staging chunks,
 // each host copy overlapping the next chunk's DMA.
@@ -560,7 +562,11 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
         return false;
     }
     pending_ = false;
-    if (hipEventSynchronize(ev_size_) != hipSuccess) {
+    // Block until the DC chain (off the critical path, done tens of
+    // microseconds before the layout), then poll for the size: a blocking
+    // wait on the layout's own event would add its wake-up latency, and
+    // the event's interrupt, to the frame.
+    if (hipEventSynchronize(ev_dc_done_) != hipSuccess || !spin_on(ev_size_)) {
         return false;
     }
     const std::uint64_t total{*total_host_};

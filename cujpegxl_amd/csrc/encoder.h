@@ -102,6 +102,7 @@ private:
                      std::size_t chroma_pitch);
     void mark(Stage s);
     bool wait_done();
+    static bool spin_on(hipEvent_t e);
 
     int device_{-1};
     float distance_{1.0f};
