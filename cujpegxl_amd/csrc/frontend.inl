@@ -22,6 +22,9 @@ namespace cjx {
 
 constexpr int kHalo = 2;
 constexpr int kTileW = 32 + 2 * kHalo;  // 36
+// Chroma samples a tile's bilinear taps can reach: (32 + 2 halo) / 2 + 1 on
+// each side.
+constexpr int kChromaW = kTileW / 2 + 2;  // 20
 
 // Grid-strided zeroing of the Clears buffers by every thread of a launch.
 __device__ __forceinline__ void clear_words(const Clears& c) {
@@ -47,9 +50,28 @@ __device__ __forceinline__ float clamp01(float v) {
     return fminf(fmaxf(v, 0.0f), 1.0f);
 }
 
+// x^2.4 as pow24_det (exp2_det(2.4 log2_det x)) for x in [0.05, 1], the sRGB
+// curve's upper branch: 2.4 log2 x lies in [-10.4, 0], where exp2_det's
+// clamps to [-126, 127] change nothing, so they are left out here; every
+// other operation is exp2_det's, in the same order.
+__device__ __forceinline__ float pow24_unit(float x) {
+    const float t{2.4f * log2_det(x)};
+    const float n{__builtin_rintf(t)};
+    const float g{(t - n) * 0.693147181f};
+    float p{1.98412698e-4f};
+    p = __builtin_fmaf(p, g, 1.38888889e-3f);
+    p = __builtin_fmaf(p, g, 8.33333333e-3f);
+    p = __builtin_fmaf(p, g, 4.16666667e-2f);
+    p = __builtin_fmaf(p, g, 0.166666667f);
+    p = __builtin_fmaf(p, g, 0.5f);
+    p = __builtin_fmaf(p, g, 1.0f);
+    p = __builtin_fmaf(p, g, 1.0f);
+    return p * u2f(static_cast<std::uint32_t>(static_cast<int>(n) + 127) << 23);
+}
+
 // xyb_impl.cuh:37-70 on the shared numeric primitives.
 __device__ __forceinline__ float srgb_to_linear(float e) {
-    return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_det((e + 0.055f) * (1.0f / 1.055f));
+    return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_unit((e + 0.055f) * (1.0f / 1.055f));
 }
 
 __device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float& ox, float& oy,
@@ -78,17 +100,11 @@ __device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float
 // 1/4 and 3/4; the result is exact in float.
 __device__ __forceinline__ void chroma_taps(int x, int cw, int& i0, int& i1, float& w0, float& w1) {
     const int k{x >> 1};
-    if (x & 1) {
-        i0 = k;
-        i1 = k + 1 < cw ? k + 1 : cw - 1;
-        w0 = 0.75f;
-        w1 = 0.25f;
-    } else {
-        i0 = k > 0 ? k - 1 : 0;
-        i1 = k;
-        w0 = 0.25f;
-        w1 = 0.75f;
-    }
+    const bool odd{(x & 1) != 0};
+    i0 = odd ? k : (k > 0 ? k - 1 : 0);
+    i1 = odd ? (k + 1 < cw ? k + 1 : cw - 1) : k;
+    w0 = odd ? 0.75f : 0.25f;
+    w1 = odd ? 0.25f : 0.75f;
 }
 
 // adaptive_quant_impl.cuh:29-61
@@ -283,6 +299,7 @@ struct RegionShared {
     } u;
     alignas(16) float sharp[3][32 * kSP];
     union {
+        std::uint16_t chroma[kChromaW * kChromaW];  // the tile's CbCr window (step 1)
         float mask[1024];                  // AQ pre-erosion terms (step 3)
         _Float16 coefy[3][kCoefPadded];    // Y coefficients of each candidate size, element order (cpad)
     } m;
@@ -335,6 +352,20 @@ __device__ __forceinline__ void selection_terms(float s, int e, int n, const flo
 __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* luma, std::size_t luma_pitch,
                                          const std::uint8_t* chroma, std::size_t chroma_pitch, int px0, int py0,
                                          int W, int H, int tid) {
+    // The tile's chroma window first: samples (cx0 + j, cy0 + i), clamped to
+    // the plane, Cb in the low byte.  Every tap of the tile's pixels (their
+    // coordinates clamped to the frame, the taps to the plane) lands in it.
+    const int cw{W / 2}, ch{H / 2};
+    const int cx0{(px0 >> 1) - 2}, cy0{(py0 >> 1) - 2};
+    for (int i{tid}; i < kChromaW * kChromaW; i += 256) {
+        int x{cx0 + i % kChromaW}, y{cy0 + i / kChromaW};
+        x = x < 0 ? 0 : (x >= cw ? cw - 1 : x);
+        y = y < 0 ? 0 : (y >= ch ? ch - 1 : y);
+        const std::uint8_t* p{chroma + static_cast<std::size_t>(static_cast<std::uint32_t>(y)) * chroma_pitch +
+                              2u * static_cast<std::uint32_t>(x)};
+        S.m.chroma[i] = static_cast<std::uint16_t>(p[0] | (p[1] << 8));
+    }
+    __syncthreads();
     for (int i{tid}; i < kTileW * kTileW; i += 256) {
         const int lx{i % kTileW}, ly{i / kTileW};
         int gx{px0 + lx - kHalo}, gy{py0 + ly - kHalo};
@@ -342,20 +373,19 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
         int x0, x1, y0, y1;
         float wx0, wx1, wy0, wy1;
-        chroma_taps(gx, W / 2, x0, x1, wx0, wx1);
-        chroma_taps(gy, H / 2, y0, y1, wy0, wy1);
-        const std::uint8_t* r0{chroma + static_cast<std::size_t>(y0) * chroma_pitch};
-        const std::uint8_t* r1{chroma + static_cast<std::size_t>(y1) * chroma_pitch};
-        const float b0{static_cast<float>(r0[2 * x0]) * wx0 + static_cast<float>(r0[2 * x1]) * wx1};
-        const float b1{static_cast<float>(r1[2 * x0]) * wx0 + static_cast<float>(r1[2 * x1]) * wx1};
-        const float c0{static_cast<float>(r0[2 * x0 + 1]) * wx0 +
-                       static_cast<float>(r0[2 * x1 + 1]) * wx1};
-        const float c1{static_cast<float>(r1[2 * x0 + 1]) * wx0 +
-                       static_cast<float>(r1[2 * x1 + 1]) * wx1};
+        chroma_taps(gx, cw, x0, x1, wx0, wx1);
+        chroma_taps(gy, ch, y0, y1, wy0, wy1);
+        const int r0{(y0 - cy0) * kChromaW}, r1{(y1 - cy0) * kChromaW};
+        const std::uint32_t s00{S.m.chroma[r0 + x0 - cx0]}, s01{S.m.chroma[r0 + x1 - cx0]};
+        const std::uint32_t s10{S.m.chroma[r1 + x0 - cx0]}, s11{S.m.chroma[r1 + x1 - cx0]};
+        const float b0{static_cast<float>(s00 & 255u) * wx0 + static_cast<float>(s01 & 255u) * wx1};
+        const float b1{static_cast<float>(s10 & 255u) * wx0 + static_cast<float>(s11 & 255u) * wx1};
+        const float c0{static_cast<float>(s00 >> 8) * wx0 + static_cast<float>(s01 >> 8) * wx1};
+        const float c1{static_cast<float>(s10 >> 8) * wx0 + static_cast<float>(s11 >> 8) * wx1};
         const float cbu{b0 * wy0 + b1 * wy1};
         const float cru{c0 * wy0 + c1 * wy1};
-        const float yv{static_cast<float>(luma[static_cast<std::size_t>(gy) * luma_pitch + gx]) *
-                       (1.0f / 255.0f)};
+        const std::uint8_t* lrow{luma + static_cast<std::size_t>(static_cast<std::uint32_t>(gy)) * luma_pitch};
+        const float yv{static_cast<float>(lrow[static_cast<std::uint32_t>(gx)]) * (1.0f / 255.0f)};
         const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
         const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
         pixel_to_xyb(yv, cb, cr, S.u.xyb[0][i], S.u.xyb[1][i], S.u.xyb[2][i]);
