@@ -957,6 +957,7 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     __syncthreads();
 
     const float yx{S.maps[0]}, yb{S.maps[1]};
+#pragma unroll 4
     for (int j{0}; j < 16; ++j) {
         const int p{j * 4 + w};
         const int bx{tx * 8 + (p & 7)}, by{ty * 8 + (p >> 3)};
