@@ -72,11 +72,94 @@ struct WaveHuffScratch {
     std::uint32_t sorted_symbol[256];
 };
 
+// Value of x in lane `src` (wave-uniform) of the wave.
+__device__ __forceinline__ std::uint32_t lane_value(std::uint32_t x, int src) {
+    return static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), src));
+}
+
+// The merge and depths below for n <= 64 leaves.  Each queue is one VGPR
+// (lane = queue index) and a merge step has no branch: the two picks are
+// selects on wave-uniform values.  The depths come from the parent links by
+// pointer jumping: every node holds (distance d, ancestor p), initially
+// (1, parent) and (0, root) at the root, and six rounds of d += d[p],
+// p = p[p] leave each node's depth in d (depths are below 64).  Ancestors
+// are internal nodes, so every lookup reads the internal nodes' registers.
+__device__ std::uint32_t merge_depths_small(WaveHuffScratch& s, int n, std::uint8_t* depth) {
+    const int lane{static_cast<int>(threadIdx.x) & 63};
+    const std::uint32_t leaf{lane < n ? s.sorted_total[lane] : kQueueEnd};
+    const std::uint32_t sym{lane < n ? s.sorted_symbol[lane] : 0u};
+    // Two-queue merge: leaves (tree index i < n) and internal nodes
+    // (internal k = tree index n + 1 + k), smaller total first, leaf on ties;
+    // internal nodes not yet made read as kQueueEnd.
+    std::uint32_t inner{kQueueEnd}, kid_l{0}, kid_r{0};
+    int i{0}, j{0};
+    for (int k{0}; k < n - 1; ++k) {
+        const std::uint32_t a0{i < n ? lane_value(leaf, i) : kQueueEnd};
+        const std::uint32_t b0{lane_value(inner, j)};
+        const bool leaf0{a0 <= b0};
+        const std::uint32_t t0{leaf0 ? a0 : b0};
+        const int c0{leaf0 ? i : n + 1 + j};
+        i += leaf0 ? 1 : 0;
+        j += leaf0 ? 0 : 1;
+        const std::uint32_t a1{i < n ? lane_value(leaf, i) : kQueueEnd};
+        const std::uint32_t b1{lane_value(inner, j)};
+        const bool leaf1{a1 <= b1};
+        const std::uint32_t t1{leaf1 ? a1 : b1};
+        const int c1{leaf1 ? i : n + 1 + j};
+        i += leaf1 ? 1 : 0;
+        j += leaf1 ? 0 : 1;
+        if (lane == k) {
+            inner = t0 + t1;
+            kid_l = static_cast<std::uint32_t>(c0);
+            kid_r = static_cast<std::uint32_t>(c1);
+        }
+    }
+    // Parent links in LDS (leaf_total is dead after the sort): node ids are
+    // leaf i -> i, internal k -> 64 + k; the root (internal n - 2) is its own
+    // parent.
+    std::uint32_t* const par{s.leaf_total};
+    const std::uint32_t root{64u + static_cast<std::uint32_t>(n - 2)};
+    if (lane < n - 1) {
+        const std::uint32_t me{64u + static_cast<std::uint32_t>(lane)};
+        const int nn{n};
+        par[kid_l < static_cast<std::uint32_t>(nn) ? kid_l : kid_l - static_cast<std::uint32_t>(nn) - 1u + 64u] = me;
+        par[kid_r < static_cast<std::uint32_t>(nn) ? kid_r : kid_r - static_cast<std::uint32_t>(nn) - 1u + 64u] = me;
+    }
+    if (lane == 0) {
+        par[root] = root;
+    }
+    wave_sync();
+    std::uint32_t pa{lane < n ? par[lane] : root};
+    std::uint32_t pb{lane < n - 1 ? par[64 + lane] : root};
+    std::uint32_t da{lane < n ? 1u : 0u};
+    std::uint32_t db{lane < n - 2 ? 1u : 0u};
+    wave_sync();
+#pragma unroll
+    for (int r{0}; r < 6; ++r) {
+        const int ia{static_cast<int>(pa - 64u)}, ib{static_cast<int>(pb - 64u)};
+        const std::uint32_t d_a{static_cast<std::uint32_t>(__shfl(static_cast<int>(db), ia, 64))};
+        const std::uint32_t p_a{static_cast<std::uint32_t>(__shfl(static_cast<int>(pb), ia, 64))};
+        const std::uint32_t d_b{static_cast<std::uint32_t>(__shfl(static_cast<int>(db), ib, 64))};
+        const std::uint32_t p_b{static_cast<std::uint32_t>(__shfl(static_cast<int>(pb), ib, 64))};
+        da += d_a;
+        pa = p_a;
+        db += d_b;
+        pb = p_b;
+    }
+    if (lane < n) {
+        depth[sym] = static_cast<std::uint8_t>(da);
+    }
+    return lane < n ? da : 0u;
+}
+
 // Two-queue merge and top-down depths of the n (>= 2) sorted leaves in
 // s.sorted_total / s.sorted_symbol; writes depth[symbol] and returns this
 // lane's largest leaf depth.  Q * 64 >= n.
 template <int Q>
-__device__ std::uint32_t merge_depths(const WaveHuffScratch& s, int n, std::uint8_t* depth) {
+__device__ std::uint32_t merge_depths(WaveHuffScratch& s, int n, std::uint8_t* depth) {
+    if constexpr (Q == 1) {
+        return merge_depths_small(s, n, depth);
+    }
     const int lane{static_cast<int>(threadIdx.x) & 63};
     LaneArray<Q> leaf{}, sym{};
 #pragma unroll
@@ -346,35 +429,41 @@ __device__ std::uint32_t wave_store_tree(const std::uint8_t* depth, int num, std
     wave_sync();
     wave_depths_to_codes<1>(s.cl_depth, 18, s.cl_bits);
     wave_sync();
-    if (lane == 0) {
-        constexpr std::uint8_t order[18]{1, 2, 3, 4, 0, 5, 17, 6, 16, 7, 8, 9, 10, 11, 12, 13, 14, 15};
-        constexpr std::uint8_t sym[6]{0, 7, 3, 2, 1, 15};
-        constexpr std::uint8_t len[6]{2, 4, 3, 2, 2, 4};
-        int to_store{18};
-        if (num_codes > 1) {
-            for (; to_store > 0; --to_store) {
-                if (s.cl_depth[order[to_store - 1]] != 0) {
-                    break;
-                }
+    {
+        // The code-length code's depths in the format's order (1, 2, 3, 4,
+        // 0, 5, 17, 6, 16, 7, ..., 15), lane i taking entry i: the stored run
+        // [skip, to_store) goes out as one prefix scan of the entries'
+        // lengths (the fixed code of depths 0..5: lengths 2,4,3,2,2,4,
+        // values 0,7,3,2,1,15, packed in nibbles).
+        const int ord{lane < 4    ? lane + 1
+                      : lane == 4 ? 0
+                      : lane == 5 ? 5
+                      : lane == 6 ? 17
+                      : lane == 7 ? 6
+                      : lane == 8 ? 16
+                                  : lane - 2};
+        const std::uint32_t l{lane < 18 ? static_cast<std::uint32_t>(s.cl_depth[ord]) : 0u};
+        const std::uint64_t nonzero{__ballot(lane < 18 && l != 0)};
+        const int to_store{num_codes > 1 ? (nonzero != 0 ? 64 - static_cast<int>(__builtin_clzll(nonzero)) : 0) : 18};
+        const int skip{(nonzero & 3ull) != 0 ? 0 : ((nonzero & 4ull) != 0 ? 2 : 3)};
+        const bool stored{lane >= skip && lane < to_store};
+        const std::uint32_t nb{stored ? (0x422342u >> (4 * l)) & 15u : 0u};
+        const std::uint32_t val{stored ? (0xf12370u >> (4 * l)) & 15u : 0u};
+        const std::uint32_t pre{static_cast<std::uint32_t>(wave_excl_scan(static_cast<int>(nb), lane))};
+        const std::uint32_t total{2u + static_cast<std::uint32_t>(wave_sum_int(static_cast<int>(nb)))};
+        wave_sync();  // every lane has read cl_depth
+        if (total <= cap * 32) {
+            if (lane == 0) {
+                lds_or_bits(hdr, cap, 0, static_cast<std::uint32_t>(skip), 2);
             }
+            lds_or_bits(hdr, cap, 2 + pre, val, nb);
         }
-        int skip{0};
-        if (s.cl_depth[order[0]] == 0 && s.cl_depth[order[1]] == 0) {
-            skip = 2;
-            if (s.cl_depth[order[2]] == 0) {
-                skip = 3;
+        if (lane == 0) {
+            if (num_codes == 1) {
+                s.cl_depth[code] = 0;
             }
+            s.pos = total <= cap * 32 ? total : cap * 32 + 1;
         }
-        BitSink w{hdr, 0, cap * 32};
-        w.write(2, static_cast<std::uint32_t>(skip));
-        for (int i{skip}; i < to_store; ++i) {
-            const std::uint8_t l{s.cl_depth[order[i]]};
-            w.write(len[l], sym[l]);
-        }
-        if (num_codes == 1) {
-            s.cl_depth[code] = 0;
-        }
-        s.pos = w.pos;
     }
     wave_sync();
     std::uint32_t at{s.pos};
