@@ -214,8 +214,9 @@ class DeviceBuffer:
 
 class PinnedBuffer:
     """Page-locked host memory from the runtime (hipHostMalloc), viewable as a
-    numpy array.  The library moves data into and out of such buffers by
-    direct DMA, skipping its own staging copy."""
+    numpy array.  The library reads inputs from and writes files into such
+    buffers directly (DMA, or the assembly kernel's own stores), skipping its
+    staging copy."""
 
     def __init__(self, nbytes: int):
         self.nbytes = nbytes
@@ -245,6 +246,7 @@ class Encoder:
                  stages: bool = False, any_size: bool = False):
         self.lib = stages_lib() if (stages or any_size) else public_lib()
         self.width, self.height, self.distance = width, height, distance
+        self.device_ordinal = device_ordinal
         h = C.c_void_p()
         cfg = Config(width, height, distance, device_ordinal)
         if any_size:
@@ -258,6 +260,15 @@ class Encoder:
         bw, bh = width // 8, height // 8
         self.capacity = 3 * bw * bh * 384 + 48 * bw * bh + (1 << 20)
         self.out = np.empty(self.capacity, np.uint8)
+
+    def configure(self, distance: float) -> None:
+        """A new distance for the same frame size (cujpegxl_encoder_configure:
+        no reallocation)."""
+        cfg = Config(self.width, self.height, distance, self.device_ordinal)
+        s = self.lib.cujpegxl_encoder_configure(self.handle, C.byref(cfg))
+        if s != OK:
+            raise RuntimeError(f"encoder_configure failed: {status_string(s)}")
+        self.distance = distance
 
     def encode_device(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int,
                       chroma_pitch: int) -> bytes:

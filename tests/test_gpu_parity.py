@@ -321,3 +321,24 @@ def test_device_4k_file_decodes_to_the_source(jo):
     xyb = jo.nv12_to_xyb(luma, chroma, w, h).astype(np.float64)
     got = [psnr(rec[c], xyb[c]) for c in range(3)]
     assert all(g >= f for g, f in zip(got, (38.0, 34.0, 35.0))), got
+
+
+def test_mixed_resolution_frames_match_oracle(jo):
+    """Frames of the mixed-resolution batch's sizes (640x360 ... 2560x1440,
+    distances in [0.8, 3.0]): heights that are not multiples of 32 or 64,
+    partial regions, tiles and groups.  Each encoder takes two seeds in a row
+    (its buffers are reused); every file equals the oracle's and decodes."""
+    cases = [(640, 360, 0.8), (1280, 720, 2.2), (1920, 1080, 3.0), (2560, 1440, 1.4)]
+    for w, h, d in cases:
+        enc = cj.Encoder(w, h, d)
+        for seed in (11, 12):
+            say(f"mixed batch {w}x{h} d={d} seed={seed}")
+            luma, chroma = synthetic_nv12(w, h, seed)
+            buf = cj.DeviceBuffer.from_array(np.concatenate([luma.ravel(), chroma.ravel()]))
+            data = enc.encode_device(buf.ptr, buf.ptr + w * h, w, w)
+            buf.close()
+            want = jo.encode_nv12(luma, chroma, w, h, d)
+            assert data == want, (w, h, d, seed, len(data), len(want))
+            dec = jo.decode(data)
+            assert dec["acs"].size == (w // 8) * (h // 8)
+        enc.close()
