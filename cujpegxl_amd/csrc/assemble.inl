@@ -49,15 +49,14 @@ struct Layout {
     std::uint32_t framing_words[12];  // container framing, 40 or 48 bytes
 };
 
-// One 4-wave workgroup per histogram (histograms first, first + 1, ...):
+// One 8-wave workgroup per histogram (histograms first, first + 1, ...):
 // create_huffman_tree's count floors 1, 2, 4, ... are independent trials, so
-// in round r wave w tries floor 2^(4r + w), and the smallest floor whose tree
-// fits 15 bits wins (the serial builder stops at that same floor); most
-// histograms finish in the first round.  Floors above 2^15 continue on wave
-// 0.  Wave 0 then builds the codes and the header.  The small workgroup fits
-// beside the AC tokenizer, which the DC chain's codes run alongside.
-constexpr int kTrials = 4;
-constexpr int kTrialRounds = 4;
+// in round r wave w tries floor 2^(8r + w), and the smallest floor whose tree
+// fits 15 bits wins (the serial builder stops at that same floor); floors up
+// to 2^7 take one round.  Floors above 2^15 continue on wave 0.  Wave 0 then
+// builds the codes and the header.
+constexpr int kTrials = 8;
+constexpr int kTrialRounds = 2;
 
 struct HuffmanShared {
     WaveHuffScratch trial[kTrials];
