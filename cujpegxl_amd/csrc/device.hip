@@ -107,15 +107,15 @@ static unsigned int grid_for(std::size_t n, std::size_t per) {
 
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
-                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part, hipEvent_t start,
-                   hipEvent_t stop) {
+                   std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
+                   std::uint32_t* list_count, hipEvent_t start, hipEvent_t stop) {
     // Eight equal runs of regions, one per XCD (the kernel maps workgroup i to
     // run i mod 8); workgroups past the last region exit at once.  The events,
     // when given, take the kernel's own start and end times (no marker
     // packets between it and its neighbours).
     const unsigned int per_xcd{(geo.rw * geo.rh + 7) / 8};
     hipExtLaunchKernelGGL(region_kernel, dim3(8 * per_xcd), dim3(256), 0, s, start, stop, 0, geo, prm, luma,
-                          luma_pitch, chroma, chroma_pitch, coeffs, acs, qf, cfl_part);
+                          luma_pitch, chroma, chroma_pitch, coeffs, acs, qf, cfl_part, list_count);
 }
 
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
@@ -131,9 +131,9 @@ void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams&
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
                           std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr,
-                          hipEvent_t done) {
+                          const ItemList& list, std::uint16_t* item_tokens, hipEvent_t done) {
     hipExtLaunchKernelGGL(tile_quantize_kernel, dim3(geo.cmw * geo.cmh), dim3(256), 0, s, nullptr, done, 0, geo, prm,
-                          coeffs, acs, qf, cfl_part, ytox, ytob, ac, dc, nz_grid, nz_count, clr);
+                          coeffs, acs, qf, cfl_part, ytox, ytob, ac, dc, nz_grid, nz_count, clr, list, item_tokens);
 }
 
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
@@ -149,7 +149,8 @@ void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32
 
 void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                       const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* tokens,
-                      std::uint16_t* item_tokens, std::uint32_t* partial, std::uint32_t* hist) {
+                      std::uint16_t* item_tokens, std::uint32_t* partial, std::uint32_t* hist,
+                      const ItemList& list) {
     // One resident round of workgroups (occupancy x CUs of the current
     // device, at most kTokenGrid): a second partial round would double the
     // kernel's time for a few waves.
@@ -166,7 +167,8 @@ void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac
     }()};
     const unsigned int waves{(3 * geo.nblocks + 3) / 4};
     const unsigned int grid{waves < resident ? waves : resident};
-    ac_tokens_kernel<<<grid, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, tokens, item_tokens, partial);
+    ac_tokens_kernel<<<grid, 256, 0, s>>>(geo, ac, acs, nz_grid, nz_count, tokens, item_tokens, partial, list.items,
+                                          list.count);
     ac_hist_reduce_kernel<<<dim3(kNumClusters * kAcSymbols / 256, 80), 256, 0, s>>>(partial, static_cast<int>(grid),
                                                                                     hist);
 }

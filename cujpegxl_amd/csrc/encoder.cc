@@ -357,6 +357,8 @@ bool Encoder::alloc() {
     get(blob_words_, static_cast<std::size_t>(geo_.num_dc) * blob_words_per_group() * sizeof(std::uint32_t));
     get(tokens_, 3 * nblk * 64 * sizeof(std::uint32_t));
     get(item_tokens_, 3 * nblk * sizeof(std::uint16_t));
+    get(item_list_, 3 * nblk * sizeof(std::uint32_t));
+    get(item_count_, 2 * sizeof(std::uint32_t));
     get(hist_partial_, ac_partial_words() * sizeof(std::uint32_t));
     get(ac_staging_, static_cast<std::size_t>(geo_.num_ac) * ac_group_slot_words() * sizeof(std::uint32_t));
     get(dc_staging_, static_cast<std::size_t>(geo_.num_dc) * dc_group_slot_words() * sizeof(std::uint32_t));
@@ -628,7 +630,7 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     // region and quantize stages.
     const bool timed{profiling_ != 0};
     launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_,
-                  timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr,
+                  item_count_, timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr,
                   timed ? events_[static_cast<int>(Stage::kQuantize)] : nullptr);
     // The planes are read by the region kernel only: later work on the
     // caller's stream may overwrite them once it has finished.
@@ -637,12 +639,13 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
         return false;
     }
     // The quantizer's completion is the fork point of the DC chain.
+    // The quantizer lists the non-empty items for the tokenizer.
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
-                         nz_count_, entropy_clears(), ev_fork_);
+                         nz_count_, entropy_clears(), ItemList{item_list_, item_count_}, item_tokens_, ev_fork_);
     // A mapped page-locked output receives the file from the assembly kernel.
     std::uint8_t* const direct{mapped_output(dst)};
     return run_entropy(direct != nullptr ? direct : reinterpret_cast<std::uint8_t*>(out_),
-                       direct != nullptr ? cap : out_cap_, true) &&
+                       direct != nullptr ? cap : out_cap_, true, true) &&
            queue_readback(direct != nullptr ? dst : nullptr);
 }
 
@@ -667,7 +670,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
                         ytob_);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false) || !queue_readback(nullptr)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -779,7 +782,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false) || !queue_readback(nullptr)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         return false;
@@ -800,7 +803,7 @@ Clears Encoder::entropy_clears() const {
     return c;
 }
 
-bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound) {
+bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, bool listed) {
     auto* codes{static_cast<CodeSet*>(codes_)};
     auto* k{static_cast<const Constants*>(constants_)};
     auto* blobs{static_cast<DcBlobs*>(blobs_)};
@@ -825,7 +828,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound) {
     if (hipEventRecord(ev_dc_done_, dc_stream_) != hipSuccess) {
         return false;
     }
-    launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_);
+    launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_,
+                     listed ? ItemList{item_list_, item_count_} : ItemList{nullptr, nullptr});
     mark(Stage::kCodes);
     launch_huffman(stream_, 0, kNumClusters, hists_, codes);
     mark(Stage::kSizes);

@@ -90,8 +90,9 @@ private:
     bool alloc();
     void release();
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
-    // `fork_bound`: the quantizer's launch has bound `ev_fork_` already.
-    bool run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound);
+    // `fork_bound`: the quantizer's launch has bound `ev_fork_` already;
+    // `listed`: it also listed the non-empty items for the tokenizer.
+    bool run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, bool listed);
     std::uint8_t* mapped_output(std::uint8_t* dst) const;
     Clears entropy_clears() const;
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
@@ -144,6 +145,8 @@ private:
     std::uint32_t* blob_words_{nullptr};
     std::uint32_t* tokens_{nullptr};       // token records in coefficient-slot layout
     std::uint16_t* item_tokens_{nullptr};  // tokens per (block, channel) item
+    std::uint32_t* item_list_{nullptr};    // first-block items (quantizer -> tokenizer), ItemList
+    std::uint32_t* item_count_{nullptr};   // entries at its front and back
     std::uint32_t* hist_partial_{nullptr}; // per-workgroup AC histograms
     std::uint32_t* ac_staging_{nullptr};   // each AC group's stream at group-relative bits
     std::uint32_t* dc_staging_{nullptr};   // each DC group's section at group-relative bits

@@ -342,7 +342,9 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
                                                         const std::uint16_t* __restrict__ nz_count,
                                                         std::uint32_t* __restrict__ tokens,
                                                         std::uint16_t* __restrict__ item_tokens,
-                                                        std::uint32_t* __restrict__ partial) {
+                                                        std::uint32_t* __restrict__ partial,
+                                                        const std::uint32_t* __restrict__ list,
+                                                        const std::uint32_t* __restrict__ list_count) {
     __shared__ std::uint32_t h[kNumClusters * kAcSymbols];
     __shared__ TokenTables T;
     __shared__ __align__(16) std::int16_t stage[4][2][1024];
@@ -356,11 +358,14 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
     std::int16_t* const coef{stage[threadIdx.x >> 6][half]};
     const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
     const std::uint32_t stride{2 * gridDim.x * 4};
-    const std::uint32_t items{geo.nblocks * 3};
+    // Walk the quantizer's list of items with non-zeros when there is one
+    // (the others follow below; covered blocks' items already have 0
+    // tokens), else every item.
+    const std::uint32_t items{list != nullptr ? list_count[0] : geo.nblocks * 3};
     for (std::uint32_t base{2 * wave}; base < items; base += stride * kTokenBatch) {
         const std::uint32_t li{base + stride * static_cast<std::uint32_t>(lane >> 1) +
                                static_cast<std::uint32_t>(lane & 1)};
-        const AcItem d{li < items ? ac_item(geo, acs, nz_grid, nz_count, li) : AcItem{}};
+        const AcItem d{li < items ? ac_item(geo, acs, nz_grid, nz_count, list != nullptr ? list[li] : li) : AcItem{}};
         const int npairs{static_cast<int>(min(static_cast<std::uint32_t>(kTokenBatch),
                                               (items - base + stride - 1) / stride))};
         AcItem cur{shfl_item(d, half)};
@@ -400,9 +405,28 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
                                    });
             }
             if (sub == 0 && item < items) {
-                item_tokens[item] = static_cast<std::uint16_t>(count);
+                // The item's own index: block * 3 + (0: Y, 1: X, 2: B).
+                const std::uint32_t own{(static_cast<std::uint32_t>(cur.by) * geo.bw + static_cast<std::uint32_t>(cur.bx)) * 3u +
+                                        (cur.c == 1 ? 0u : (cur.c == 0 ? 1u : 2u))};
+                item_tokens[own] = static_cast<std::uint16_t>(count);
             }
             cur = nx;
+        }
+    }
+    // First blocks without non-zero coefficients, one per thread: their one
+    // token is the zero count, in the count's context.
+    if (list != nullptr) {
+        const std::uint32_t zeros{list_count[1]};
+        const std::uint32_t last{3u * geo.nblocks - 1u};
+        for (std::uint32_t i{blockIdx.x * 256u + threadIdx.x}; i < zeros; i += gridDim.x * 256u) {
+            const AcItem it{ac_item(geo, acs, nz_grid, nz_count, list[last - i])};
+            std::uint32_t sym{0}, nb{0}, bits{0};
+            hybrid420(0u, sym, nb, bits);
+            const int cl{ac_cluster(it.nz_ctx)};
+            atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
+            tokens[token_slot(geo, it.c, it.bx, it.by, 1, 0)] = token_record(cl, sym, nb, bits);
+            item_tokens[(static_cast<std::uint32_t>(it.by) * geo.bw + static_cast<std::uint32_t>(it.bx)) * 3u +
+                        (it.c == 1 ? 0u : (it.c == 0 ? 1u : 2u))] = 1;
         }
     }
     __syncthreads();

@@ -420,9 +420,12 @@ __global__ __launch_bounds__(256) void region_kernel(
     Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
     std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
-    std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part) {
+    std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part, std::uint32_t* __restrict__ list_count) {
     __shared__ RegionShared S;
     const int tid{static_cast<int>(threadIdx.x)};
+    if (blockIdx.x == 0 && tid < 2 && list_count != nullptr) {
+        list_count[tid] = 0;  // the quantizer's item lists, appended after this kernel
+    }
     const int w{tid >> 6};
     const int wu{__builtin_amdgcn_readfirstlane(w)};
     const int lane{tid & 63};
@@ -881,6 +884,8 @@ struct TileShared {
     unsigned int nz[64][3];
     int own[64];              // packed owner: local position | (n << 8)
     float maps[2];
+    std::uint32_t list[3 * 64];  // the tile's items: with non-zeros from the front, without from the back
+    unsigned int list_n[2], list_base[2];
 };
 
 __device__ __forceinline__ int owner_in_tile(const std::int8_t* acs, int bw, int bx, int by, int& ox,
@@ -907,7 +912,8 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
     const float* __restrict__ cfl_part, std::int8_t* __restrict__ ytox_out,
     std::int8_t* __restrict__ ytob_out, std::int16_t* __restrict__ ac, std::int32_t* __restrict__ dc,
-    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr) {
+    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr, ItemList list,
+    std::uint16_t* __restrict__ item_tokens) {
     __shared__ TileShared S;
     clear_words(clr);
     const int tid{static_cast<int>(threadIdx.x)};
@@ -919,6 +925,9 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     const std::size_t nblk{geo.nblocks};
     const std::size_t cplane{nblk * 64};
 
+    if (tid == 0) {
+        S.list_n[0] = S.list_n[1] = 0;
+    }
     if (tid < 64) {
         S.nz[tid][0] = S.nz[tid][1] = S.nz[tid][2] = 0u;
         const int bx{tx * 8 + (tid & 7)}, by{ty * 8 + (tid >> 3)};
@@ -1049,8 +1058,34 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
                 const unsigned int cnt{S.nz[op][c]};
                 nz_grid[c * nblk + b] = static_cast<std::uint8_t>((cnt + m * m - 1) >> l2);
                 nz_count[c * nblk + b] = op == p ? static_cast<std::uint16_t>(cnt) : 0;
+                // Item order within a block: Y, X, B.
+                const std::uint32_t item{static_cast<std::uint32_t>(b) * 3u + (c == 1 ? 0u : (c == 0 ? 1u : 2u))};
+                if (op != p) {
+                    if (list.items != nullptr) {
+                        item_tokens[item] = 0;
+                    }
+                } else if (cnt != 0) {
+                    S.list[atomicAdd(&S.list_n[0], 1u)] = item;
+                } else {
+                    S.list[3 * 64 - 1 - atomicAdd(&S.list_n[1], 1u)] = item;
+                }
             }
         }
+    }
+    if (list.items == nullptr) {
+        return;
+    }
+    __syncthreads();
+    if (tid < 2) {
+        S.list_base[tid] = atomicAdd(&list.count[tid], S.list_n[tid]);
+    }
+    __syncthreads();
+    const std::uint32_t cap{3u * geo.nblocks};
+    for (unsigned int i{static_cast<unsigned int>(tid)}; i < S.list_n[0]; i += 256) {
+        list.items[S.list_base[0] + i] = S.list[i];
+    }
+    for (unsigned int i{static_cast<unsigned int>(tid)}; i < S.list_n[1]; i += 256) {
+        list.items[cap - 1 - (S.list_base[1] + i)] = S.list[3 * 64 - 1 - i];
     }
 }
 
