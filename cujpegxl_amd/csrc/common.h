@@ -33,8 +33,8 @@ struct Clears {
 // The AC items (block, channel) of first blocks, in no particular order,
 // listed by the quantizer (one reservation per workgroup and list): those
 // with non-zero coefficients from the front of `items`, those without (whose
-// one token is the zero count) from the back; covered blocks' items get 0
-// tokens from the quantizer.  The tokenizer walks the two lists only.
+// one token is the zero count) from the back.  Covered blocks' items have no
+// tokens.  The tokenizer walks the two lists only.
 // `count[0]` and `count[1]` (front, back) are zeroed by the region kernel.
 struct ItemList {
     std::uint32_t* items;  // item = block * 3 + (0: Y, 1: X, 2: B); 3 * blocks entries

@@ -131,9 +131,9 @@ void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams&
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
                           std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr,
-                          const ItemList& list, std::uint16_t* item_tokens, hipEvent_t done) {
+                          const ItemList& list, hipEvent_t done) {
     hipExtLaunchKernelGGL(tile_quantize_kernel, dim3(geo.cmw * geo.cmh), dim3(256), 0, s, nullptr, done, 0, geo, prm,
-                          coeffs, acs, qf, cfl_part, ytox, ytob, ac, dc, nz_grid, nz_count, clr, list, item_tokens);
+                          coeffs, acs, qf, cfl_part, ytox, ytob, ac, dc, nz_grid, nz_count, clr, list);
 }
 
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,

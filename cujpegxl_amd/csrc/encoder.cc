@@ -641,7 +641,7 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     // The quantizer's completion is the fork point of the DC chain.
     // The quantizer lists the non-empty items for the tokenizer.
     launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
-                         nz_count_, entropy_clears(), ItemList{item_list_, item_count_}, item_tokens_, ev_fork_);
+                         nz_count_, entropy_clears(), ItemList{item_list_, item_count_}, ev_fork_);
     // A mapped page-locked output receives the file from the assembly kernel.
     std::uint8_t* const direct{mapped_output(dst)};
     return run_entropy(direct != nullptr ? direct : reinterpret_cast<std::uint8_t*>(out_),

@@ -912,8 +912,7 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
     const float* __restrict__ cfl_part, std::int8_t* __restrict__ ytox_out,
     std::int8_t* __restrict__ ytob_out, std::int16_t* __restrict__ ac, std::int32_t* __restrict__ dc,
-    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr, ItemList list,
-    std::uint16_t* __restrict__ item_tokens) {
+    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr, ItemList list) {
     __shared__ TileShared S;
     clear_words(clr);
     const int tid{static_cast<int>(threadIdx.x)};
@@ -1060,14 +1059,14 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
                 nz_count[c * nblk + b] = op == p ? static_cast<std::uint16_t>(cnt) : 0;
                 // Item order within a block: Y, X, B.
                 const std::uint32_t item{static_cast<std::uint32_t>(b) * 3u + (c == 1 ? 0u : (c == 0 ? 1u : 2u))};
-                if (op != p) {
-                    if (list.items != nullptr) {
-                        item_tokens[item] = 0;
+                // Covered blocks' items have no tokens (emission reads no
+                // count for them).
+                if (op == p) {
+                    if (cnt != 0) {
+                        S.list[atomicAdd(&S.list_n[0], 1u)] = item;
+                    } else {
+                        S.list[3 * 64 - 1 - atomicAdd(&S.list_n[1], 1u)] = item;
                     }
-                } else if (cnt != 0) {
-                    S.list[atomicAdd(&S.list_n[0], 1u)] = item;
-                } else {
-                    S.list[3 * 64 - 1 - atomicAdd(&S.list_n[1], 1u)] = item;
                 }
             }
         }

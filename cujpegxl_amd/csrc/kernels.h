@@ -46,7 +46,7 @@ void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams&
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int8_t* ytox, std::int8_t* ytob, std::int16_t* ac,
                           std::int32_t* dc, std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr,
-                          const ItemList& list, std::uint16_t* item_tokens, hipEvent_t done = nullptr);
+                          const ItemList& list, hipEvent_t done = nullptr);
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
 void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v);
