@@ -884,8 +884,6 @@ struct TileShared {
     unsigned int nz[64][3];
     int own[64];              // packed owner: local position | (n << 8)
     float maps[2];
-    std::uint32_t list[3 * 64];  // the tile's items: with non-zeros from the front, without from the back
-    unsigned int list_n[2], list_base[2];
 };
 
 __device__ __forceinline__ int owner_in_tile(const std::int8_t* acs, int bw, int bx, int by, int& ox,
@@ -924,9 +922,6 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     const std::size_t nblk{geo.nblocks};
     const std::size_t cplane{nblk * 64};
 
-    if (tid == 0) {
-        S.list_n[0] = S.list_n[1] = 0;
-    }
     if (tid < 64) {
         S.nz[tid][0] = S.nz[tid][1] = S.nz[tid][2] = 0u;
         const int bx{tx * 8 + (tid & 7)}, by{ty * 8 + (tid >> 3)};
@@ -1022,6 +1017,9 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     if (tid < 64) {
         const int p{tid};
         const int bx{tx * 8 + (p & 7)}, by{ty * 8 + (p >> 3)};
+        // This block's items for the tokenizer's lists, in item order Y, X,
+        // B: with non-zeros (dense) or without (zero count).
+        bool dense[3]{false, false, false}, zero[3]{false, false, false};
         if (bx < bw && by < bh) {
             const int own{S.own[p]};
             const int n{own >> 8}, op{own & 255};
@@ -1057,34 +1055,50 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
                 const unsigned int cnt{S.nz[op][c]};
                 nz_grid[c * nblk + b] = static_cast<std::uint8_t>((cnt + m * m - 1) >> l2);
                 nz_count[c * nblk + b] = op == p ? static_cast<std::uint16_t>(cnt) : 0;
-                // Item order within a block: Y, X, B.
-                const std::uint32_t item{static_cast<std::uint32_t>(b) * 3u + (c == 1 ? 0u : (c == 0 ? 1u : 2u))};
-                // Covered blocks' items have no tokens (emission reads no
-                // count for them).
-                if (op == p) {
-                    if (cnt != 0) {
-                        S.list[atomicAdd(&S.list_n[0], 1u)] = item;
-                    } else {
-                        S.list[3 * 64 - 1 - atomicAdd(&S.list_n[1], 1u)] = item;
-                    }
+            }
+            // Covered blocks' items have no tokens (emission reads no count
+            // for them).
+            if (op == p) {
+#pragma unroll
+                for (int j{0}; j < 3; ++j) {
+                    const bool any{S.nz[op][j == 0 ? 1 : (j == 1 ? 0 : 2)] != 0};
+                    dense[j] = any;
+                    zero[j] = !any;
                 }
             }
         }
-    }
-    if (list.items == nullptr) {
-        return;
-    }
-    __syncthreads();
-    if (tid < 2) {
-        S.list_base[tid] = atomicAdd(&list.count[tid], S.list_n[tid]);
-    }
-    __syncthreads();
-    const std::uint32_t cap{3u * geo.nblocks};
-    for (unsigned int i{static_cast<unsigned int>(tid)}; i < S.list_n[0]; i += 256) {
-        list.items[S.list_base[0] + i] = S.list[i];
-    }
-    for (unsigned int i{static_cast<unsigned int>(tid)}; i < S.list_n[1]; i += 256) {
-        list.items[cap - 1 - (S.list_base[1] + i)] = S.list[3 * 64 - 1 - i];
+        // Wave 0 appends the tile's items: prefix sums of the per-lane counts
+        // (0..3, two ballots each), one reservation per list.
+        if (list.items != nullptr) {
+            const int nd{(dense[0] ? 1 : 0) + (dense[1] ? 1 : 0) + (dense[2] ? 1 : 0)};
+            const int nzc{(zero[0] ? 1 : 0) + (zero[1] ? 1 : 0) + (zero[2] ? 1 : 0)};
+            const std::uint64_t below{(1ull << p) - 1ull};
+            const std::uint64_t d0{__ballot(nd & 1)}, d1{__ballot(nd & 2)};
+            const std::uint64_t z0{__ballot(nzc & 1)}, z1{__ballot(nzc & 2)};
+            const std::uint32_t dpre{static_cast<std::uint32_t>(__popcll(d0 & below) + 2 * __popcll(d1 & below))};
+            const std::uint32_t zpre{static_cast<std::uint32_t>(__popcll(z0 & below) + 2 * __popcll(z1 & below))};
+            const std::uint32_t dtot{static_cast<std::uint32_t>(__popcll(d0) + 2 * __popcll(d1))};
+            const std::uint32_t ztot{static_cast<std::uint32_t>(__popcll(z0) + 2 * __popcll(z1))};
+            std::uint32_t base{0};
+            if (p < 2) {
+                base = atomicAdd(&list.count[p], p == 0 ? dtot : ztot);
+            }
+            const std::uint32_t dbase{static_cast<std::uint32_t>(__shfl(static_cast<int>(base), 0, 64))};
+            const std::uint32_t zbase{static_cast<std::uint32_t>(__shfl(static_cast<int>(base), 1, 64))};
+            const std::uint32_t last{3u * geo.nblocks - 1u};
+            const std::uint32_t b3{(static_cast<std::uint32_t>(by) * static_cast<std::uint32_t>(bw) +
+                                    static_cast<std::uint32_t>(bx)) * 3u};
+            std::uint32_t kd{dbase + dpre}, kz{zbase + zpre};
+#pragma unroll
+            for (int j{0}; j < 3; ++j) {
+                if (dense[j]) {
+                    list.items[kd++] = b3 + static_cast<std::uint32_t>(j);
+                }
+                if (zero[j]) {
+                    list.items[last - kz++] = b3 + static_cast<std::uint32_t>(j);
+                }
+            }
+        }
     }
 }
 
