@@ -120,6 +120,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed-batch", action="store_true",
+                    help="skip the mixed-resolution batch (4K frames only, e.g. under a profiler)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -233,7 +235,8 @@ def main() -> int:
             "traffic": region_traffic_bytes(),
         },
     }
-    out["mixed_batch"] = mixed_batch(cj, synthetic_nv12)
+    if not args.no_mixed_batch:
+        out["mixed_batch"] = mixed_batch(cj, synthetic_nv12)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(luma, chroma)
     pinned.close()
