@@ -276,13 +276,12 @@ void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, con
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
-                    const std::uint64_t* chunk_off, std::uint32_t* staging) {
+                    const std::uint64_t* chunk_off, std::uint32_t* staging, hipEvent_t done) {
     const std::uint32_t cpg{dc_chunks_per_group()};
-    const dim3 grid{cpg, geo.num_dc};
-    dc_emit_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
-                                        &cs->code[kNumClusters][0], &cs->depth[kNumClusters + geo.num_dc][0],
-                                        &cs->code[kNumClusters + geo.num_dc][0], blobs, blob_words, chunk_off,
-                                        cpg, staging, dc_group_slot_words());
+    hipExtLaunchKernelGGL(dc_emit_kernel, dim3(cpg, geo.num_dc), dim3(256), 0, s, nullptr, done, 0, geo, dc, acs, qf,
+                          ytox, ytob, &cs->depth[kNumClusters][0], &cs->code[kNumClusters][0],
+                          &cs->depth[kNumClusters + geo.num_dc][0], &cs->code[kNumClusters + geo.num_dc][0], blobs,
+                          blob_words, chunk_off, cpg, staging, dc_group_slot_words());
 }
 
 }  // namespace cjx

@@ -294,13 +294,13 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess) {
         return false;
     }
-    for (hipEvent_t* e : {&ev_dc_done_, &ev_done_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_done_, &ev_caller_, &ev_input_free_}) {
         if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
             return false;
         }
     }
     // Bound to kernels' completion at launch (timing-capable events).
-    for (hipEvent_t* e : {&ev_fork_, &ev_size_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_}) {
         if (hipEventCreate(e) != hipSuccess) {
             return false;
         }
@@ -823,11 +823,10 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, b
     launch_dc_blobs(dc_stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
     launch_dc_bits(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
                    dc_group_bits_, dc_staging_);
+    // The DC chain's end is the emission kernel's own completion (the join
+    // below and the host's first wait use it).
     launch_dc_emit(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_,
-                   dc_staging_);
-    if (hipEventRecord(ev_dc_done_, dc_stream_) != hipSuccess) {
-        return false;
-    }
+                   dc_staging_, ev_dc_done_);
     launch_ac_tokens(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_,
                      listed ? ItemList{item_list_, item_count_} : ItemList{nullptr, nullptr});
     mark(Stage::kCodes);

@@ -90,6 +90,6 @@ void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, con
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
-                    const std::uint64_t* chunk_off, std::uint32_t* staging);
+                    const std::uint64_t* chunk_off, std::uint32_t* staging, hipEvent_t done = nullptr);
 
 }  // namespace cjx
