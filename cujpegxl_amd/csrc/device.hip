@@ -233,9 +233,10 @@ std::size_t ac_group_parts() {
 
 void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
                           const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
-                          std::uint32_t* staging, std::uint64_t* part_bits) {
-    ac_group_emit_kernel<<<geo.num_ac * kGroupParts, kPartThreads, 0, s>>>(
-        geo, acs, item_tokens, tokens, &cs->depth[0][0], &cs->code[0][0], staging, part_bits);
+                          std::uint32_t* staging, std::uint64_t* part_bits, hipEvent_t done) {
+    hipExtLaunchKernelGGL(ac_group_emit_kernel, dim3(geo.num_ac * kGroupParts), dim3(kPartThreads), 0, s, nullptr,
+                          done, 0, geo, acs, item_tokens, tokens, &cs->depth[0][0], &cs->code[0][0], staging,
+                          part_bits);
 }
 
 std::size_t dc_group_slot_words() {

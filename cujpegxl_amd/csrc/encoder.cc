@@ -250,7 +250,7 @@ void Encoder::release() {
             e = nullptr;
         }
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_ac_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
         if (*e != nullptr) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -300,7 +300,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         }
     }
     // Bound to kernels' completion at launch (timing-capable events).
-    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_, &ev_ac_done_}) {
         if (hipEventCreate(e) != hipSuccess) {
             return false;
         }
@@ -461,9 +461,9 @@ bool Encoder::build_constants(std::uint32_t global_scale, std::uint32_t quant_dc
     return true;
 }
 
-void Encoder::mark(Stage s) {
+void Encoder::mark(Stage s, hipStream_t on) {
     if (profiling_ == 1 || (profiling_ == 2 && (s == Stage::kRegion || s == Stage::kQuantize))) {
-        (void)hipEventRecord(events_[static_cast<int>(s)], stream_);
+        (void)hipEventRecord(events_[static_cast<int>(s)], on != nullptr ? on : stream_);
     }
 }
 
@@ -511,7 +511,7 @@ std::uint8_t* Encoder::mapped_output(std::uint8_t* dst) const {
 // any other output (or none yet, for encode_async) gets the first two chunks
 // staged in the pinned buffer (bytes past the end are never used).
 bool Encoder::queue_readback(std::uint8_t* direct) {
-    mark(Stage::kReadback);
+    mark(Stage::kReadback, tail_);
     staged_ = 0;
     direct_dst_ = direct;
     if (direct == nullptr) {
@@ -528,8 +528,8 @@ bool Encoder::queue_readback(std::uint8_t* direct) {
 bool Encoder::stage_chunk(int slot, std::size_t n) {
     stage_len_[slot] = n;
     const bool ok{hipMemcpyAsync(stage_host_ + slot * kStageChunk, reinterpret_cast<const std::uint8_t*>(out_) + staged_,
-                                 n, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
-                  hipEventRecord(stage_ready_[slot], stream_) == hipSuccess};
+                                 n, hipMemcpyDeviceToHost, tail_) == hipSuccess &&
+                  hipEventRecord(stage_ready_[slot], tail_) == hipSuccess};
     staged_ += n;
     return ok;
 }
@@ -553,7 +553,7 @@ bool Encoder::spin_on(hipEvent_t e) {
 // microseconds): polling the completion event saves the wake-up latency of
 // a blocking wait, which is of the same order.
 bool Encoder::wait_done() {
-    return hipEventRecord(ev_done_, stream_) == hipSuccess && spin_on(ev_done_);
+    return hipEventRecord(ev_done_, tail_) == hipSuccess && spin_on(ev_done_);
 }
 
 // The readback's second half: waits for the size, then moves the bytes into
@@ -579,7 +579,7 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
             // The assembly kernel writes the file into `dst`; wait_done() below
             // waits for it.
         } else if (host_pinned(dst)) {
-            ok = hipMemcpyAsync(dst, out_, total, hipMemcpyDeviceToHost, stream_) == hipSuccess;
+            ok = hipMemcpyAsync(dst, out_, total, hipMemcpyDeviceToHost, tail_) == hipSuccess;
         } else {
             std::size_t done{0};
             for (int slot{0}; slot < 2 && staged_ < total && stage_len_[slot] == 0; ++slot) {
@@ -601,7 +601,7 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
     }
     stage_len_[0] = stage_len_[1] = 0;
     direct_dst_ = nullptr;
-    mark(Stage::kCount);
+    mark(Stage::kCount, tail_);
     if (!wait_done()) {
         return false;
     }
@@ -655,6 +655,7 @@ bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     if (!submit_nv12(luma, luma_pitch, chroma, chroma_pitch, nullptr, out, cap)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
+        (void)hipStreamSynchronize(dc_stream_);
         return false;
     }
     return finish(out, cap, size);
@@ -673,6 +674,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
     if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
+        (void)hipStreamSynchronize(dc_stream_);
         return false;
     }
     return finish(out, cap, size);
@@ -686,6 +688,7 @@ bool Encoder::encode_async(const std::uint8_t* luma, std::size_t luma_pitch, con
     if (!ok && pending_) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
+        (void)hipStreamSynchronize(dc_stream_);
     }
     return ok;
 }
@@ -785,6 +788,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
+        (void)hipStreamSynchronize(dc_stream_);
         return false;
     }
     return finish(out, cap, size);
@@ -832,14 +836,18 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, b
     mark(Stage::kCodes);
     launch_huffman(stream_, 0, kNumClusters, hists_, codes);
     mark(Stage::kSizes);
-    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_part_bits_);
-    if (hipStreamWaitEvent(stream_, ev_dc_done_, 0) != hipSuccess) {
+    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_part_bits_, ev_ac_done_);
+    // Join: the DC stream waits for the AC chain's end and runs the layout,
+    // the assembly and the readback (the two chains end close together; the
+    // wait is the cheaper one when its event is already done).
+    if (hipStreamWaitEvent(dc_stream_, ev_ac_done_, 0) != hipSuccess) {
         return false;
     }
-    launch_layout(stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
+    tail_ = dc_stream_;
+    launch_layout(dc_stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
                   ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_, ev_size_);
-    mark(Stage::kEmit);
-    launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
+    mark(Stage::kEmit, tail_);
+    launch_assemble(dc_stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
                     dc_base_, ac_base_, out, cap);
     return hipGetLastError() == hipSuccess;
 }

@@ -101,7 +101,7 @@ private:
     bool stage_chunk(int slot, std::size_t n);
     bool upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                      std::size_t chroma_pitch);
-    void mark(Stage s);
+    void mark(Stage s, hipStream_t on = nullptr);
     bool wait_done();
     static bool spin_on(hipEvent_t e);
 
@@ -114,6 +114,8 @@ private:
     hipEvent_t ev_fork_{nullptr}, ev_dc_done_{nullptr};  // fork onto / join from the DC stream
     hipEvent_t ev_size_{nullptr};                        // the total size has reached the host
     hipEvent_t ev_done_{nullptr};                        // the frame's last copy has finished
+    hipEvent_t ev_ac_done_{nullptr};                     // the AC chain's end (AC group emission)
+    hipStream_t tail_{nullptr};  // layout, assembly and readback: the DC stream, after both chains
     hipEvent_t ev_caller_{nullptr}, ev_input_free_{nullptr};  // ordering against a caller's stream
     bool caller_set_{false};
     bool pending_{false};            // a frame's readback is queued, finish() not yet called

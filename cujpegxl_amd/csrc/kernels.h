@@ -71,7 +71,7 @@ std::size_t ac_group_slot_words();
 std::size_t ac_group_parts();
 void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t* acs,
                           const std::uint16_t* item_tokens, const std::uint32_t* tokens, const CodeSet* cs,
-                          std::uint32_t* staging, std::uint64_t* part_bits);
+                          std::uint32_t* staging, std::uint64_t* part_bits, hipEvent_t done = nullptr);
 // Words of one DC group's staging slot.
 std::size_t dc_group_slot_words();
 void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
