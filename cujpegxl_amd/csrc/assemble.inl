@@ -170,16 +170,15 @@ __global__ __launch_bounds__(64 * kTrials) void huffman_kernel(const std::uint32
 
 // Per DC group: blob_pre = constant prefix + data alphabet + code header;
 // blob_mid = (count - 1) + constant prefix + alphabet + header.
-__global__ __launch_bounds__(64) void dc_blobs_kernel(Geometry geo, const Constants* __restrict__ k,
-                                                      const CodeSet* __restrict__ cs,
-                                                      const std::uint32_t* __restrict__ fb_count,
-                                                      DcBlobs* __restrict__ blobs,
-                                                      std::uint32_t* __restrict__ blob_words) {
-    const int g{static_cast<int>(blockIdx.x)};
+// One DC group's header blobs (group g, threads of the workgroup take the
+// clearing; thread 0 writes the bits).
+__device__ void dc_blobs_body(const Geometry& geo, const Constants* __restrict__ k, const CodeSet* __restrict__ cs,
+                              const std::uint32_t* __restrict__ fb_count, DcBlobs* __restrict__ blobs,
+                              std::uint32_t* __restrict__ blob_words, int g) {
     const int t{static_cast<int>(threadIdx.x)};
     std::uint32_t* pre{blob_words + static_cast<std::size_t>(2 * g) * kBlobWords};
     std::uint32_t* mid{pre + kBlobWords};
-    for (int i{t}; i < kBlobWords; i += 64) {
+    for (int i{t}; i < kBlobWords; i += static_cast<int>(blockDim.x)) {
         pre[i] = 0;
         mid[i] = 0;
     }
@@ -206,6 +205,23 @@ __global__ __launch_bounds__(64) void dc_blobs_kernel(Geometry geo, const Consta
             atomicOr(&blobs->overflow, 1u);
         }
         blobs->mid_off[g] = static_cast<std::uint32_t>(2 * g + 1) * kBlobWords;
+    }
+}
+
+// The DC chain's sizing pass: workgroup columns 0 .. chunks_per_group - 1 of
+// group row g size the group's item chunks, column chunks_per_group builds
+// its two header blobs (independent work, one launch).
+__global__ __launch_bounds__(256) void dc_blobs_bits_kernel(
+    Geometry geo, const Constants* __restrict__ k, const CodeSet* __restrict__ cs,
+    const std::uint32_t* __restrict__ fb_count, DcBlobs* __restrict__ blobs, std::uint32_t* __restrict__ blob_words,
+    const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
+    const std::int8_t* __restrict__ ytox, const std::int8_t* __restrict__ ytob,
+    std::uint64_t* __restrict__ chunk_bits, std::uint32_t chunks_per_group) {
+    if (blockIdx.x == chunks_per_group) {
+        dc_blobs_body(geo, k, cs, fb_count, blobs, blob_words, static_cast<int>(blockIdx.y));
+    } else {
+        dc_chunk_bits_body(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
+                           &cs->depth[kNumClusters + geo.num_dc][0], chunk_bits, chunks_per_group);
     }
 }
 

@@ -218,10 +218,6 @@ bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint
     return ok;
 }
 
-void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
-                     const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words) {
-    dc_blobs_kernel<<<geo.num_dc, 64, 0, s>>>(geo, k, cs, fb_count, blobs, blob_words);
-}
 
 std::size_t ac_group_slot_words() {
     return static_cast<std::size_t>(kGroupParts) * kPartSlotWords;
@@ -244,15 +240,15 @@ std::size_t dc_group_slot_words() {
     return static_cast<std::size_t>(dc_chunks_per_group()) * kDcChunk + 2 * kBlobWords + 64;
 }
 
-void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
-                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
-                    const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
-                    std::uint32_t* group_bytes, std::uint64_t* group_bits, std::uint32_t* staging) {
+void launch_dc_sizes(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
+                     const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words, const std::int32_t* dc,
+                     const std::int8_t* acs, const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                     std::uint64_t* chunk_bits, std::uint32_t* group_bytes, std::uint64_t* group_bits,
+                     std::uint32_t* staging) {
     const std::uint32_t cpg{dc_chunks_per_group()};
-    const dim3 grid{cpg, geo.num_dc};
-    dc_chunk_bits_kernel<<<grid, 256, 0, s>>>(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
-                                              &cs->depth[kNumClusters + geo.num_dc][0], blobs, chunk_bits, cpg);
-    dc_chunk_scan_kernel<<<geo.num_dc, 1024, 0, s>>>(geo, chunk_bits, cpg, group_bytes, group_bits, staging,
+    dc_blobs_bits_kernel<<<dim3(cpg + 1, geo.num_dc), 256, 0, s>>>(geo, k, cs, fb_count, blobs, blob_words, dc, acs,
+                                                                   qf, ytox, ytob, chunk_bits, cpg);
+    dc_chunk_scan_kernel<<<geo.num_dc, 1024, 0, s>>>(geo, chunk_bits, cpg, blobs, group_bytes, group_bits, staging,
                                                      dc_group_slot_words());
 }
 

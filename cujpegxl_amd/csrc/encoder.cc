@@ -824,9 +824,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, b
     launch_dc_hist(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
                    hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
     launch_huffman(dc_stream_, kNumClusters, ndc, hists_, codes);
-    launch_dc_blobs(dc_stream_, geo_, k, codes, fb_count_, blobs, blob_words_);
-    launch_dc_bits(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, chunk_bits_, dc_group_bytes_,
-                   dc_group_bits_, dc_staging_);
+    launch_dc_sizes(dc_stream_, geo_, k, codes, fb_count_, blobs, blob_words_, dc_, acs_, qf_, ytox_, ytob_,
+                    chunk_bits_, dc_group_bytes_, dc_group_bits_, dc_staging_);
     // The DC chain's end is the emission kernel's own completion (the join
     // below and the host's first wait use it).
     launch_dc_emit(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_,

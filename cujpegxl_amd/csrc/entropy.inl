@@ -927,12 +927,14 @@ __device__ __forceinline__ std::uint32_t dc_cursor_code(const Geometry& geo, con
 }
 
 // Chunk bit totals of every DC group (grid: chunks x groups).
-__global__ __launch_bounds__(256) void dc_chunk_bits_kernel(
-    Geometry geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
+// Bits of one chunk of a DC group's items, the two header blobs not
+// counted (the scan adds them: their sizes are built beside this pass).
+__device__ void dc_chunk_bits_body(
+    const Geometry& geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
     const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
     const std::int8_t* __restrict__ ytob, const std::uint8_t* __restrict__ dc_depth,
-    const std::uint8_t* __restrict__ meta_depth, const DcBlobs* __restrict__ blobs,
-    std::uint64_t* __restrict__ chunk_bits, std::uint32_t chunks_per_group) {
+    const std::uint8_t* __restrict__ meta_depth, std::uint64_t* __restrict__ chunk_bits,
+    std::uint32_t chunks_per_group) {
     __shared__ std::uint64_t sh[5];
     const int g{static_cast<int>(blockIdx.y)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
@@ -943,7 +945,6 @@ __global__ __launch_bounds__(256) void dc_chunk_bits_kernel(
         DcCursor c{dc_cursor_at(d, k0)};
         for (std::uint32_t k{k0}; k < k1; ++k, dc_cursor_next(d, c)) {
             if (c.sec == 0 || c.sec == 2) {
-                run += c.sec == 0 ? blobs->pre_bits[g] : blobs->mid_bits[g];
                 continue;
             }
             std::uint32_t val;
@@ -963,6 +964,7 @@ __global__ __launch_bounds__(256) void dc_chunk_bits_kernel(
 // OR into.
 __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::uint64_t* __restrict__ chunk_bits,
                                                              std::uint32_t chunks_per_group,
+                                                             const DcBlobs* __restrict__ blobs,
                                                              std::uint32_t* __restrict__ group_bytes,
                                                              std::uint64_t* __restrict__ group_bits,
                                                              std::uint32_t* __restrict__ staging,
@@ -975,6 +977,9 @@ __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::
     std::uint64_t v{0};
     if (t < used) {
         v = chunk_bits[static_cast<std::size_t>(g) * chunks_per_group + t];
+        // The blobs are items 0 and 1 + 3 n of the group.
+        v += t == 0 ? blobs->pre_bits[g] : 0u;
+        v += t == (1u + 3u * static_cast<std::uint32_t>(d.n)) / kDcChunk ? blobs->mid_bits[g] : 0u;
     }
     std::uint64_t total{0};
     const std::uint64_t ex{block_excl_scan(v, sh, total)};

@@ -63,8 +63,6 @@ void launch_dc_hist(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 bool build_codes_host(const std::uint32_t* hists, std::uint32_t count, std::uint32_t* alpha, std::uint8_t* depth,
                       std::uint16_t* bits, std::uint32_t* hdr_words, std::uint32_t* hdr_bits);
 void launch_huffman(hipStream_t s, int first, int count, const std::uint32_t* hists, CodeSet* cs);
-void launch_dc_blobs(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
-                     const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words);
 // Words of one AC group's staging slot (bound on its token stream).
 std::size_t ac_group_slot_words();
 // AC groups are emitted in this many parts (row bands).
@@ -74,10 +72,13 @@ void launch_ac_group_emit(hipStream_t s, const Geometry& geo, const std::int8_t*
                           std::uint32_t* staging, std::uint64_t* part_bits, hipEvent_t done = nullptr);
 // Words of one DC group's staging slot.
 std::size_t dc_group_slot_words();
-void launch_dc_bits(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
-                    const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
-                    const CodeSet* cs, const DcBlobs* blobs, std::uint64_t* chunk_bits,
-                    std::uint32_t* group_bytes, std::uint64_t* group_bits, std::uint32_t* staging);
+// The DC chain's sizing: header blobs and chunk bits in one launch, then
+// the per-group scan of chunk offsets (which adds the blobs' bits).
+void launch_dc_sizes(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
+                     const std::uint32_t* fb_count, DcBlobs* blobs, std::uint32_t* blob_words, const std::int32_t* dc,
+                     const std::int8_t* acs, const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
+                     std::uint64_t* chunk_bits, std::uint32_t* group_bytes, std::uint64_t* group_bits,
+                     std::uint32_t* staging);
 void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const CodeSet* cs,
                    const std::uint64_t* ac_part_bits, const std::uint32_t* dc_group_bytes,
                    const std::uint64_t* dc_group_bits, std::uint32_t* ac_group_bytes, std::uint64_t* ac_group_bits,

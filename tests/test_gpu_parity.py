@@ -342,3 +342,25 @@ def test_mixed_resolution_frames_match_oracle(jo):
             dec = jo.decode(data)
             assert dec["acs"].size == (w // 8) * (h // 8)
         enc.close()
+
+
+@pytest.mark.parametrize("kind", ["flat", "noise"])
+def test_extreme_content_matches_oracle(jo, kind):
+    """A flat frame (every block's AC is zero: the tokenizer's list of items
+    with non-zeros is empty and every first block sits in the zero-count
+    list) and a full-range noise frame (nearly every item has non-zeros)."""
+    w, h, d = 512, 264, 1.0
+    rng = np.random.default_rng(21)
+    if kind == "flat":
+        luma = np.full((h, w), 117, np.uint8)
+        chroma = np.full((h // 2, w), 131, np.uint8)
+    else:
+        luma = rng.integers(0, 256, (h, w), dtype=np.uint8)
+        chroma = rng.integers(0, 256, (h // 2, w), dtype=np.uint8)
+    say(f"extreme content {kind} {w}x{h}")
+    enc, data, _buf = device_encode(w, h, d, luma, chroma)
+    want = jo.encode_nv12(luma, chroma, w, h, d)
+    assert data == want, (kind, len(data), len(want))
+    dec = jo.decode(data)
+    if kind == "flat":
+        assert not dec["ac"].any()
