@@ -200,12 +200,16 @@ __device__ __forceinline__ const float* basis_n() {
     }
 }
 
+// Columns col0..col0 + N - 1 of the row at `src`, whose 16-byte groups are
+// permuted by `swz` (group g stored at g ^ swz; 0 for the unpermuted row-pass
+// lines).
 template <int N, int F>
-__device__ __forceinline__ void dct_line(const float* __restrict__ src, int k0, float (&out)[F]) {
+__device__ __forceinline__ void dct_line(const float* __restrict__ src, int k0, float (&out)[F], int col0 = 0,
+                                         int swz = 0) {
     float p[N];
 #pragma unroll
     for (int c{0}; c < N; c += 4) {
-        const float4 v{*reinterpret_cast<const float4*>(src + c)};
+        const float4 v{*reinterpret_cast<const float4*>(src + ((((col0 + c) >> 2) ^ swz) << 2))};
         p[c] = v.x;
         p[c + 1] = v.y;
         p[c + 2] = v.z;
@@ -229,11 +233,13 @@ __device__ __forceinline__ void dct_line(const float* __restrict__ src, int k0, 
 // every K=1 step equals fmaf(a, b, d) bit for bit), so N steps over c
 // reproduce dct_line's fmaf chain exactly.  Lanes 16b..16b+15 supply block b:
 // lane 16b + t gives A[b][t] = a_row[c] and B[b][t] = b_row[c].  On return
-// lane l holds D[b][4 (l / 16) + u][l % 16] in element 4 b + u.
+// lane l holds D[b][4 (l / 16) + u][l % 16] in element 4 b + u.  The B row
+// is read like dct_line's source (columns from col0, groups permuted by swz).
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 template <int N>
-__device__ __forceinline__ v16f mfma_lines(const float* __restrict__ a_row, const float* __restrict__ b_row) {
+__device__ __forceinline__ v16f mfma_lines(const float* __restrict__ a_row, const float* __restrict__ b_row,
+                                           int col0 = 0, int swz = 0) {
     v16f acc;
 #pragma unroll
     for (int i{0}; i < 16; ++i) {
@@ -242,7 +248,7 @@ __device__ __forceinline__ v16f mfma_lines(const float* __restrict__ a_row, cons
 #pragma unroll
     for (int c{0}; c < N; c += 4) {
         const float4 a{*reinterpret_cast<const float4*>(a_row + c)};
-        const float4 b{*reinterpret_cast<const float4*>(b_row + c)};
+        const float4 b{*reinterpret_cast<const float4*>(b_row + ((((col0 + c) >> 2) ^ swz) << 2))};
         acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.x, b.x, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.y, b.y, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.z, b.z, acc, 0, 0, 0);
@@ -269,7 +275,17 @@ struct RowBuf<32> {
     static constexpr int RS = 36, BS = 1152;
 };
 
-constexpr int kSP = 36;  // sharpened-plane row stride (floats)
+// Sharpened planes: rows of 32 floats without padding, the row's eight
+// 16-byte groups permuted by (row / 2) mod 8, so that the 16 consecutive rows
+// a matrix-core row pass reads at one column fall into 16 different 4-bank
+// groups.  Planes are stored in the order X, B, Y.
+constexpr int kSP = 32;
+__device__ __forceinline__ int sp_swz(int row) {
+    return (row >> 1) & 7;
+}
+__device__ __forceinline__ int sp_index(int row, int col) {
+    return row * kSP + (((col >> 2) ^ sp_swz(row)) << 2) + (col & 3);
+}
 // Row-pass lines of the three selection sizes side by side.
 constexpr int kLines32 = 0;
 constexpr int kLines16 = kLines32 + RowBuf<32>::BS;
@@ -284,40 +300,61 @@ __device__ __forceinline__ int cpad(int e) {
     return e + 2 * (e >> 5);
 }
 
-// Step 5 uses the scratch lines below this float offset; the B channel's
-// coefficients live past it.
-constexpr int kCoefBOffset = 4 * 2 * 4 * RowBuf<8>::BS;
-static_assert(kCoefBOffset >= 2 * RowBuf<32>::BS, "step-5 DCT32 lines overlap the B coefficients");
-static_assert(kCoefBOffset * 4 + kCoefPadded * 2 <= kScratch * 4, "B coefficients exceed the scratch");
-
 // Arrays whose lifetimes do not overlap share storage, which keeps the
-// workgroup under 40 KB of LDS (four resident workgroups per CU).
+// workgroup under 32 KB of LDS (five resident workgroups per CU):
+// - steps 1-3: the XYB tile, then the chroma window (step 1) or the AQ
+//   terms (step 3) past its end; steps 4-5: the row-pass lines;
+// - the sharpened planes, from step 2 until their last row pass (Y in
+//   step 4, X and B in step 5); each plane's space then holds coefficients:
+//   X's and B's the chosen transforms (step 5 on), Y's (the last plane, and
+//   256 bytes past it) the DCT16 and DCT32 candidates (step 4 on).  The DCT8
+//   candidates of Y have their own array.
+constexpr int kPlaneHalfs = 2 * 32 * kSP;
+static_assert(kCoefPadded <= kPlaneHalfs, "a coefficient array per plane");
+
 struct RegionShared {
     union {
-        float xyb[3][kTileW * kTileW];  // NV12 -> XYB tile with halo (steps 1-3)
-        float scratch[kScratch];        // row-pass lines, selection terms, B coefficients (steps 4-5)
-    } u;
-    alignas(16) float sharp[3][32 * kSP];
-    union {
-        std::uint16_t chroma[kChromaW * kChromaW];  // the tile's CbCr window (step 1)
-        float mask[1024];                  // AQ pre-erosion terms (step 3)
-        _Float16 coefy[3][kCoefPadded];    // Y coefficients of each candidate size, element order (cpad)
-    } m;
-    union {
         struct {
-            float cells[64];
-            float ero[64];      // fuzzy erosion of each cell
-            float blk[16][3];   // per block: HF sum, gamma, blue sum
-        } aq;                   // step 3
-        _Float16 coefx[kCoefPadded];  // X coefficients of the chosen transforms (cpad), step 5
-    } x;
+            float xyb[3][kTileW * kTileW];  // NV12 -> XYB tile with halo (steps 1-3)
+            union {
+                std::uint16_t chroma[kChromaW * kChromaW];  // the tile's CbCr window (step 1)
+                struct {
+                    float cells[64];
+                    float ero[64];     // fuzzy erosion of each cell
+                    float blk[16][3];  // per block: HF sum, gamma, blue sum
+                } aq;                  // step 3
+            } t;
+        } f;
+        float scratch[kScratch];  // row-pass lines (steps 4-5)
+    } u;
+    union {
+        alignas(16) float sharp[3][32 * kSP];
+        struct {
+            _Float16 x[kPlaneHalfs];      // X coefficients of the chosen transforms (cpad)
+            _Float16 b[kPlaneHalfs];      // B coefficients of the chosen transforms
+            _Float16 y[2 * kCoefPadded];  // Y coefficients of the DCT16 candidates, then the DCT32 candidate
+        } coef;
+    } p;
+    _Float16 coefy8[kCoefPadded];  // Y coefficients of the DCT8 candidates
     float cost8[16];       // quad-major
     float cost16[4];
     float cost32;
     float cfl[4][3];
     int acs[16];           // region-local raster
-    float rate[256];       // selection rate proxy 1 + log2(1 + k), staged from the tables
 };
+// LDS is allocated in 1,280-byte blocks: five workgroups per CU (160 KB)
+// need at most 25 blocks each.
+static_assert(sizeof(RegionShared) <= 25 * 1280, "five workgroups per CU");
+
+// Sharpened plane of channel c (0 X, 1 Y, 2 B).
+__device__ __forceinline__ float* plane_of(RegionShared& S, int c) {
+    return S.p.sharp[c == 0 ? 0 : (c == 1 ? 2 : 1)];
+}
+
+// Y coefficients of candidate size 8 << sz, element order (cpad).
+__device__ __forceinline__ _Float16* coefy_of(RegionShared& S, int sz) {
+    return sz == 0 ? S.coefy8 : S.p.coef.y + (sz - 1) * kCoefPadded;
+}
 
 // True for the lowest (n/8)^2 frequencies (fx, fy < n/8) of element
 // e = fx * n + fy of an n x n transform, n = 8, 16 or 32 (shifts, not a
@@ -363,7 +400,7 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         y = y < 0 ? 0 : (y >= ch ? ch - 1 : y);
         const std::uint8_t* p{chroma + static_cast<std::size_t>(static_cast<std::uint32_t>(y)) * chroma_pitch +
                               2u * static_cast<std::uint32_t>(x)};
-        S.m.chroma[i] = static_cast<std::uint16_t>(p[0] | (p[1] << 8));
+        S.u.f.t.chroma[i] = static_cast<std::uint16_t>(p[0] | (p[1] << 8));
     }
     __syncthreads();
     for (int i{tid}; i < kTileW * kTileW; i += 256) {
@@ -376,8 +413,8 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         chroma_taps(gx, cw, x0, x1, wx0, wx1);
         chroma_taps(gy, ch, y0, y1, wy0, wy1);
         const int r0{(y0 - cy0) * kChromaW}, r1{(y1 - cy0) * kChromaW};
-        const std::uint32_t s00{S.m.chroma[r0 + x0 - cx0]}, s01{S.m.chroma[r0 + x1 - cx0]};
-        const std::uint32_t s10{S.m.chroma[r1 + x0 - cx0]}, s11{S.m.chroma[r1 + x1 - cx0]};
+        const std::uint32_t s00{S.u.f.t.chroma[r0 + x0 - cx0]}, s01{S.u.f.t.chroma[r0 + x1 - cx0]};
+        const std::uint32_t s10{S.u.f.t.chroma[r1 + x0 - cx0]}, s11{S.u.f.t.chroma[r1 + x1 - cx0]};
         const float b0{static_cast<float>(s00 & 255u) * wx0 + static_cast<float>(s01 & 255u) * wx1};
         const float b1{static_cast<float>(s10 & 255u) * wx0 + static_cast<float>(s11 & 255u) * wx1};
         const float c0{static_cast<float>(s00 >> 8) * wx0 + static_cast<float>(s01 >> 8) * wx1};
@@ -388,7 +425,7 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         const float yv{static_cast<float>(lrow[static_cast<std::uint32_t>(gx)]) * (1.0f / 255.0f)};
         const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
         const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
-        pixel_to_xyb(yv, cb, cr, S.u.xyb[0][i], S.u.xyb[1][i], S.u.xyb[2][i]);
+        pixel_to_xyb(yv, cb, cr, S.u.f.xyb[0][i], S.u.f.xyb[1][i], S.u.f.xyb[2][i]);
     }
 }
 
@@ -403,7 +440,7 @@ __device__ __forceinline__ void gaborish_tile(RegionShared& S, int tid) {
 #pragma unroll 4
     for (int i{tid}; i < 3 * 1024; i += 256) {
         const int c{i >> 10}, p{i & 1023};
-        const float* s{&S.u.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+        const float* s{&S.u.f.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
 #define AT(dx, dy) s[(dy) * kTileW + (dx)]
         const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
         const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
@@ -411,12 +448,12 @@ __device__ __forceinline__ void gaborish_tile(RegionShared& S, int tid) {
         const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
                        AT(-1, 2) + AT(1, 2)};
         const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
-        S.sharp[c][(p >> 5) * kSP + (p & 31)] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
+        plane_of(S, c)[sp_index(p >> 5, p & 31)] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
 #undef AT
     }
 }
 
-__global__ __launch_bounds__(256) void region_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void region_kernel(
     Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
     std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
@@ -444,7 +481,6 @@ __global__ __launch_bounds__(256) void region_kernel(
     const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
     const int bx0{rx * 4}, by0{ry * 4};
 
-    S.rate[tid] = g_tables.rate[tid];
     // 1. NV12 -> XYB over the 36x36 tile (2-pixel halo, clamped coordinates).
     xyb_tile(S, luma, luma_pitch, chroma, chroma_pitch, px0, py0, W, H, tid);
     if (tid < 16) {
@@ -455,28 +491,38 @@ __global__ __launch_bounds__(256) void region_kernel(
     // 2. Inverse gaborish (gaborish.h:16-28; tap order frontend_transform.cu:30-48).
     {
         gaborish_tile(S, tid);
-        // 3a. AQ pre-erosion terms per interior pixel (frontend_fused.cu:118-137).
+        // 3a. AQ pre-erosion terms per interior pixel (frontend_fused.cu:118-137)
+        //     and their 4x4 cell sums in row order: four lanes per cell, one
+        //     per row; row dy continues the running sum of row dy - 1.
+        const int cell{tid >> 2}, dy{tid & 3};
+        const int row{(cell >> 3) * 4 + dy}, col0{(cell & 7) * 4};
+        float m[4];
 #pragma unroll
-        for (int p{tid}; p < 1024; p += 256) {
-            const float* s{&S.u.xyb[1][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
+        for (int dx{0}; dx < 4; ++dx) {
+            const float* s{&S.u.f.xyb[1][(row + kHalo) * kTileW + col0 + dx + kHalo]};
             const float yv{s[0]};
             const float base{0.25f * (s[kTileW] + s[-kTileW] + s[-1] + s[1])};
             float diff{ratio_cbrt_gamma(yv + 0.019f, false) * (yv - base)};
             diff *= diff;
             diff = fminf(diff, 0.2f);
-            S.m.mask[p] = masking_sqrt(diff);
+            m[dx] = masking_sqrt(diff);
         }
-    }
-    __syncthreads();
-    if (tid < 64) {
-        const int cx{tid & 7}, cy{tid >> 3};
         float sum{0.0f};
-        for (int dy{0}; dy < 4; ++dy) {
-            for (int dx{0}; dx < 4; ++dx) {
-                sum += S.m.mask[(cy * 4 + dy) * 32 + cx * 4 + dx];
+#pragma unroll
+        for (int k{0}; k < 4; ++k) {
+            const float prev{__shfl_up(sum, 1, 4)};
+            if (dy == k) {
+                float acc{k == 0 ? 0.0f : prev};
+                acc += m[0];
+                acc += m[1];
+                acc += m[2];
+                acc += m[3];
+                sum = acc;
             }
         }
-        S.x.aq.cells[tid] = 0.25f * sum;
+        if (dy == 3) {
+            S.u.f.t.aq.cells[cell] = 0.25f * sum;
+        }
     }
     __syncthreads();
 
@@ -486,7 +532,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     //     (a block's four erosion terms sum as (e0 + e2) + (e1 + e3), which is
     //     the wave tree over four non-zero lanes).
     if (tid < 64) {
-        S.x.aq.ero[tid] = erosion_cell(S.x.aq.cells, tid & 7, tid >> 3, prm);
+        S.u.f.t.aq.ero[tid] = erosion_cell(S.u.f.t.aq.cells, tid & 7, tid >> 3, prm);
     }
 #pragma unroll
     for (int j{0}; j < 4; ++j) {
@@ -494,13 +540,13 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int lbx{lb & 3}, lby{lb >> 2};
         const int fx{lane & 7}, fy{lane >> 3};
         const int idx{(lby * 8 + fy + kHalo) * kTileW + lbx * 8 + fx + kHalo};
-        const float yv{S.u.xyb[1][idx]}, xv{S.u.xyb[0][idx]}, bv{S.u.xyb[2][idx]};
+        const float yv{S.u.f.xyb[1][idx]}, xv{S.u.f.xyb[0][idx]}, bv{S.u.f.xyb[2][idx]};
         float hf{0.0f};
         if (fx < 7) {
-            hf += fminf(0.0206f, fabsf(yv - S.u.xyb[1][idx + 1]));
+            hf += fminf(0.0206f, fabsf(yv - S.u.f.xyb[1][idx + 1]));
         }
         if (fy < 7) {
-            hf += fminf(0.0206f, fabsf(yv - S.u.xyb[1][idx + kTileW]));
+            hf += fminf(0.0206f, fabsf(yv - S.u.f.xyb[1][idx + kTileW]));
         }
         const float iny{yv + 0.16f};
         const float gm{ratio_cbrt_gamma(iny - xv, true) + ratio_cbrt_gamma(iny + xv, true)};
@@ -510,9 +556,9 @@ __global__ __launch_bounds__(256) void region_kernel(
         const float gamma_overall{wave_sum(gm) * (0.5f / 64.0f)};
         const float blue_sum{wave_sum(bl)};
         if (lane == 0) {
-            S.x.aq.blk[lb][0] = hf_sum;
-            S.x.aq.blk[lb][1] = gamma_overall;
-            S.x.aq.blk[lb][2] = blue_sum;
+            S.u.f.t.aq.blk[lb][0] = hf_sum;
+            S.u.f.t.aq.blk[lb][1] = gamma_overall;
+            S.u.f.t.aq.blk[lb][2] = blue_sum;
         }
     }
     __syncthreads();
@@ -520,10 +566,10 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int lb{tid};
         const int lbx{lb & 3}, lby{lb >> 2};
         if (bx0 + lbx < bw && by0 + lby < bh) {
-            const float e0{S.x.aq.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.x.aq.ero[(2 * lby) * 8 + 2 * lbx + 1]};
-            const float e2{S.x.aq.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.x.aq.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
+            const float e0{S.u.f.t.aq.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.u.f.t.aq.ero[(2 * lby) * 8 + 2 * lbx + 1]};
+            const float e2{S.u.f.t.aq.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.u.f.t.aq.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
             const float aq{(e0 + e2) + (e1 + e3)};
-            float blue_sum{S.x.aq.blk[lb][2]};
+            float blue_sum{S.u.f.t.aq.blk[lb][2]};
             constexpr float k_limit{0.027121074570634722f};
             if (blue_sum >= 32.0f * k_limit) {
                 blue_sum = 64.0f * k_limit - blue_sum;
@@ -533,8 +579,8 @@ __global__ __launch_bounds__(256) void region_kernel(
                 blue_sum = k_max_limit * k_limit;
             }
             float ov{compute_mask(aq)};
-            ov += S.x.aq.blk[lb][0] * -0.38f + 0.42f;
-            ov += 0.1005613337192697f * log2_det(S.x.aq.blk[lb][1]);
+            ov += S.u.f.t.aq.blk[lb][0] * -0.38f + 0.42f;
+            ov += 0.1005613337192697f * log2_det(S.u.f.t.aq.blk[lb][1]);
             ov += blue_sum * 0.14207000358439159f;
             const float qval{exp_det(ov) * prm.mul_pb + prm.add_pb};
             int qi{static_cast<int>(qval * prm.inv_global_scale + 0.5f)};
@@ -551,10 +597,12 @@ __global__ __launch_bounds__(256) void region_kernel(
     __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
     float* const lines{S.u.scratch};
     const int grp{lane >> 4}, t16{lane & 15};
+    const float* const yp{plane_of(S, 1)};
     if (wu == 0) {
         // DCT32 as four 16x16 quadrants: block (fh, rh) = (grp >> 1, grp & 1).
-        const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
-                                      &S.sharp[1][(16 * (grp & 1) + t16) * kSP])};
+        const int row{16 * (grp & 1) + t16};
+        const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32, yp + row * kSP, 0,
+                                      sp_swz(row))};
 #pragma unroll
         for (int b{0}; b < 4; ++b) {
 #pragma unroll
@@ -565,8 +613,8 @@ __global__ __launch_bounds__(256) void region_kernel(
         }
     } else if (wu == 1) {
         // Block q = quad q.
-        const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
-                                      &S.sharp[1][((grp >> 1) * 16 + t16) * kSP + (grp & 1) * 16])};
+        const int row{(grp >> 1) * 16 + t16};
+        const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, yp + row * kSP, (grp & 1) * 16, sp_swz(row))};
 #pragma unroll
         for (int q{0}; q < 4; ++q) {
 #pragma unroll
@@ -579,7 +627,8 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int job{((wu - 2) << 6) | lane};
         const int b{job >> 3}, r{job & 7};
         float o[8];
-        dct_line<8, 8>(&S.sharp[1][((b >> 2) * 8 + r) * kSP + (b & 3) * 8], 0, o);
+        const int row{(b >> 2) * 8 + r};
+        dct_line<8, 8>(yp + row * kSP, 0, o, (b & 3) * 8, sp_swz(row));
 #pragma unroll
         for (int f{0}; f < 8; ++f) {
             lines[kLines8 + b * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
@@ -598,8 +647,8 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
                 const int e{fx * 32 + fy};
                 const float v{acc[4 * b + u]};
-                selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, S.rate, prm, td[4 * b + u], tr[4 * b + u]);
-                S.m.coefy[2][cpad(e)] = static_cast<_Float16>(v);
+                selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, g_tables.rate, prm, td[4 * b + u], tr[4 * b + u]);
+                coefy_of(S, 2)[cpad(e)] = static_cast<_Float16>(v);
             }
         }
     } else if (wu == 1) {
@@ -612,8 +661,8 @@ __global__ __launch_bounds__(256) void region_kernel(
                 const int fy{4 * grp + u}, fx{t16};
                 const int e{fx * 16 + fy};
                 const float v{acc[4 * q + u]};
-                selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, S.rate, prm, td[4 * q + u], tr[4 * q + u]);
-                S.m.coefy[1][cpad(256 * q + e)] = static_cast<_Float16>(v);
+                selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, g_tables.rate, prm, td[4 * q + u], tr[4 * q + u]);
+                coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(v);
             }
         }
     } else {
@@ -626,8 +675,8 @@ __global__ __launch_bounds__(256) void region_kernel(
 #pragma unroll
         for (int f{0}; f < 8; ++f) {
             const int e{fx * 8 + f};
-            selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, S.rate, prm, td[f], tr[f]);
-            S.m.coefy[0][cpad(base + e)] = static_cast<_Float16>(o[f]);
+            selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, g_tables.rate, prm, td[f], tr[f]);
+            coefy_of(S, 0)[cpad(base + e)] = static_cast<_Float16>(o[f]);
         }
     }
     // Costs straight from the registers: per lane the sum of its terms in
@@ -738,8 +787,7 @@ __global__ __launch_bounds__(256) void region_kernel(
     // 5. Chosen DCTs of X and B (Y reuses the selection coefficients) into
     //    the X and B coefficient arrays in element order, then FP16 out and the CfL partials
     //    (frontend_quantize.cu:49-117 on this region).
-    _Float16* const coefb{reinterpret_cast<_Float16*>(S.u.scratch + kCoefBOffset)};
-    const auto cxb = [&](int ch) { return ch == 0 ? S.x.coefx : coefb; };
+    const auto cxb = [&](int ch) { return ch == 0 ? S.p.coef.x : S.p.coef.b; };
     const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
     if (acs0 == 32) {
         // Waves 0 and 1 transform X and B on the matrix cores (DCT32 as four
@@ -747,8 +795,9 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int ch{wu};
         float* const cl{lines + ch * RowBuf<32>::BS};
         if (wu < 2) {
+            const int row{16 * (grp & 1) + t16};
             const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
-                                          &S.sharp[2 * ch][(16 * (grp & 1) + t16) * kSP])};
+                                          plane_of(S, 2 * ch) + row * kSP, 0, sp_swz(row))};
 #pragma unroll
             for (int b{0}; b < 4; ++b) {
 #pragma unroll
@@ -780,8 +829,9 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int n16{__builtin_amdgcn_readfirstlane(S.acs[(q >> 1) * 8 + (q & 1) * 2])};
         const int ch16{grp & 1};  // lanes 32-63 repeat blocks 0-1; their results are dropped
         if (n16 == 16) {
-            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
-                                          &S.sharp[2 * ch16][((q >> 1) * 16 + t16) * kSP + (q & 1) * 16])};
+            const int row{(q >> 1) * 16 + t16};
+            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, plane_of(S, 2 * ch16) + row * kSP,
+                                          (q & 1) * 16, sp_swz(row))};
 #pragma unroll
             for (int b{0}; b < 2; ++b) {
 #pragma unroll
@@ -793,7 +843,8 @@ __global__ __launch_bounds__(256) void region_kernel(
             const int ch{lane >> 5}, j{(lane >> 3) & 3}, r{lane & 7};
             const int bx{2 * (q & 1) + (j & 1)}, by{2 * (q >> 1) + (j >> 1)};
             float o[8];
-            dct_line<8, 8>(&S.sharp[2 * ch][(by * 8 + r) * kSP + bx * 8], 0, o);
+            const int row{by * 8 + r};
+            dct_line<8, 8>(plane_of(S, 2 * ch) + row * kSP, 0, o, bx * 8, sp_swz(row));
 #pragma unroll
             for (int f{0}; f < 8; ++f) {
                 ql[(ch * 4 + j) * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
@@ -842,7 +893,7 @@ __global__ __launch_bounds__(256) void region_kernel(
         const int idx{n == 32 ? e : 256 * k + 64 * w + lane};
         const int sz{n == 32 ? 2 : (n == 16 ? 1 : 0)};
         const _Float16 hx{cxb(0)[cpad(idx)]};
-        const _Float16 hy{S.m.coefy[sz][cpad(idx)]};
+        const _Float16 hy{coefy_of(S, sz)[cpad(idx)]};
         const _Float16 hb{cxb(1)[cpad(idx)]};
         float v[3]{0.0f, 0.0f, 0.0f};
         if (bx0 + bxl < bw && by0 + byl < bh) {
@@ -1151,7 +1202,8 @@ __global__ __launch_bounds__(256) void dct8_uniform_kernel(
         const int c{ci == 0 ? 1 : (ci == 1 ? 0 : 2)};
         if (tid < 128) {
             float o[8];
-            dct_line<8, 8>(&S.sharp[c][((b >> 2) * 8 + sub) * kSP + (b & 3) * 8], 0, o);
+            const int row{(b >> 2) * 8 + sub};
+            dct_line<8, 8>(plane_of(S, c) + row * kSP, 0, o, (b & 3) * 8, sp_swz(row));
 #pragma unroll
             for (int f{0}; f < 8; ++f) {
                 lines[b * RowBuf<8>::BS + f * RowBuf<8>::RS + sub] = o[f];
