@@ -250,7 +250,7 @@ void Encoder::release() {
             e = nullptr;
         }
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_ac_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
         if (*e != nullptr) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -300,7 +300,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         }
     }
     // Bound to kernels' completion at launch (timing-capable events).
-    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_, &ev_ac_done_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_}) {
         if (hipEventCreate(e) != hipSuccess) {
             return false;
         }
@@ -565,10 +565,12 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
     }
     pending_ = false;
     // Block until the DC chain (off the critical path, done tens of
-    // microseconds before the layout), then poll for the size: a blocking
-    // wait on the layout's own event would add its wake-up latency, and
-    // the event's interrupt, to the frame.
-    if (hipEventSynchronize(ev_dc_done_) != hipSuccess || !spin_on(ev_size_)) {
+    // microseconds before the layout), then poll for the size, or for the
+    // frame's end when no size event was bound (the assembly writes the
+    // file into `dst`): a blocking wait on a late event would add its
+    // wake-up latency, and the event's interrupt, to the frame.
+    const bool ended{!size_bound_};
+    if (hipEventSynchronize(ev_dc_done_) != hipSuccess || !(ended ? wait_done() : spin_on(ev_size_))) {
         return false;
     }
     const std::uint64_t total{*total_host_};
@@ -576,8 +578,8 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
     bool ok{total <= out_cap_};
     if (ok && total <= cap && dst != nullptr) {
         if (direct_dst_ != nullptr && dst == direct_dst_) {
-            // The assembly kernel writes the file into `dst`; wait_done() below
-            // waits for it.
+            // The assembly kernel writes the file into `dst`; the frame's end
+            // (waited for above without a size event, else below) completes it.
         } else if (host_pinned(dst)) {
             ok = hipMemcpyAsync(dst, out_, total, hipMemcpyDeviceToHost, tail_) == hipSuccess;
         } else {
@@ -599,11 +601,14 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
             }
         }
     }
+    const bool copied{direct_dst_ == nullptr || dst != direct_dst_};
     stage_len_[0] = stage_len_[1] = 0;
     direct_dst_ = nullptr;
-    mark(Stage::kCount, tail_);
-    if (!wait_done()) {
-        return false;
+    if (!ended || copied) {
+        mark(Stage::kCount, tail_);
+        if (!wait_done()) {
+            return false;
+        }
     }
     if (profiling_ != 0) {
         for (int i{0}; i < static_cast<int>(Stage::kCount); ++i) {
@@ -627,25 +632,28 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
         return false;
     }
     // With profiling on, the region kernel's own start and end times open the
-    // region and quantize stages.
+    // region and quantize stages.  Its end is the fork point of the DC chain
+    // (it leaves the DC values and the CfL partials the colour maps come
+    // from, and clears the entropy stage's accumulators).
     const bool timed{profiling_ != 0};
+    const hipEvent_t region_end{timed ? events_[static_cast<int>(Stage::kQuantize)] : ev_fork_};
     launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_,
-                  item_count_, timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr,
-                  timed ? events_[static_cast<int>(Stage::kQuantize)] : nullptr);
+                  item_count_, dc_, entropy_clears(),
+                  timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr, region_end);
     // The planes are read by the region kernel only: later work on the
     // caller's stream may overwrite them once it has finished.
     if (caller_set_ && (hipEventRecord(ev_input_free_, stream_) != hipSuccess ||
                         hipStreamWaitEvent(caller, ev_input_free_, 0) != hipSuccess)) {
         return false;
     }
-    // The quantizer's completion is the fork point of the DC chain.
     // The quantizer lists the non-empty items for the tokenizer.
-    launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ytox_, ytob_, ac_, dc_, nz_grid_,
-                         nz_count_, entropy_clears(), ItemList{item_list_, item_count_}, ev_fork_);
+    launch_tile_quantize(stream_, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ac_, nz_grid_, nz_count_,
+                         ItemList{item_list_, item_count_});
     // A mapped page-locked output receives the file from the assembly kernel.
     std::uint8_t* const direct{mapped_output(dst)};
     return run_entropy(direct != nullptr ? direct : reinterpret_cast<std::uint8_t*>(out_),
-                       direct != nullptr ? cap : out_cap_, true, true) &&
+                       direct != nullptr ? cap : out_cap_, region_end, true, true,
+                       direct == nullptr || profiling_ != 0) &&
            queue_readback(direct != nullptr ? dst : nullptr);
 }
 
@@ -671,7 +679,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
                         ytob_);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false, false) || !queue_readback(nullptr)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, nullptr, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamSynchronize(dc_stream_);
@@ -785,7 +793,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
     mark(Stage::kRegion);
     mark(Stage::kQuantize);
     launch_nz(stream_, geo_, ac_, acs_, nz_grid_, nz_count_, entropy_clears());
-    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, false, false) || !queue_readback(nullptr)) {
+    if (!run_entropy(reinterpret_cast<std::uint8_t*>(out_), out_cap_, nullptr, false) || !queue_readback(nullptr)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamSynchronize(dc_stream_);
@@ -807,19 +815,31 @@ Clears Encoder::entropy_clears() const {
     return c;
 }
 
-bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, bool listed) {
+bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, bool listed, bool maps,
+                          bool size_event) {
     auto* codes{static_cast<CodeSet*>(codes_)};
     auto* k{static_cast<const Constants*>(constants_)};
     auto* blobs{static_cast<DcBlobs*>(blobs_)};
     auto* lay{static_cast<Layout*>(layout_)};
-    // The DC chain (histograms, codes, header blobs, chunk sizes) depends on
-    // nothing of the AC chain: it runs on the second stream and joins before
-    // the layout; DC emission runs beside AC emission.
+    // The DC chain (histograms, codes, header blobs, chunk sizes, emission)
+    // depends on nothing of the AC chain: it runs on the second stream from
+    // `fork` (the stream's work so far when null) and joins before the layout.
+    // With `maps` it first derives the tiles' colour maps from the CfL
+    // partials.  `size_event` binds ev_size_ to the layout kernel (the host
+    // then reads the size before the frame's end).
     mark(Stage::kHistograms);
     const int ndc{static_cast<int>(2 * geo_.num_dc)};
-    if ((!fork_bound && hipEventRecord(ev_fork_, stream_) != hipSuccess) ||
-        hipStreamWaitEvent(dc_stream_, ev_fork_, 0) != hipSuccess) {
+    if (fork == nullptr) {
+        fork = ev_fork_;
+        if (hipEventRecord(fork, stream_) != hipSuccess) {
+            return false;
+        }
+    }
+    if (hipStreamWaitEvent(dc_stream_, fork, 0) != hipSuccess) {
         return false;
+    }
+    if (maps) {
+        launch_cfl_maps(dc_stream_, geo_, cfl_part_, ytox_, ytob_);
     }
     launch_dc_hist(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
                    hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
@@ -835,18 +855,20 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, b
     mark(Stage::kCodes);
     launch_huffman(stream_, 0, kNumClusters, hists_, codes);
     mark(Stage::kSizes);
-    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_part_bits_, ev_ac_done_);
-    // Join: the DC stream waits for the AC chain's end and runs the layout,
-    // the assembly and the readback (the two chains end close together; the
-    // wait is the cheaper one when its event is already done).
-    if (hipStreamWaitEvent(dc_stream_, ev_ac_done_, 0) != hipSuccess) {
+    launch_ac_group_emit(stream_, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_part_bits_);
+    // Join: the AC stream waits for the DC chain's end (which, forked at the
+    // region kernel's end, is done well before the AC emission: the wait on
+    // a completed event is the cheap one) and runs the layout, the assembly
+    // and the readback.
+    if (hipStreamWaitEvent(stream_, ev_dc_done_, 0) != hipSuccess) {
         return false;
     }
-    tail_ = dc_stream_;
-    launch_layout(dc_stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
-                  ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_, ev_size_);
+    tail_ = stream_;
+    size_bound_ = size_event;
+    launch_layout(stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
+                  ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_, size_event ? ev_size_ : nullptr);
     mark(Stage::kEmit, tail_);
-    launch_assemble(dc_stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
+    launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
                     dc_base_, ac_base_, out, cap);
     return hipGetLastError() == hipSuccess;
 }

@@ -92,7 +92,8 @@ private:
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
     // `fork_bound`: the quantizer's launch has bound `ev_fork_` already;
     // `listed`: it also listed the non-empty items for the tokenizer.
-    bool run_entropy(std::uint8_t* out, std::size_t cap, bool fork_bound, bool listed);
+    bool run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, bool listed, bool maps = false,
+                     bool size_event = true);
     std::uint8_t* mapped_output(std::uint8_t* dst) const;
     Clears entropy_clears() const;
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
@@ -114,7 +115,7 @@ private:
     hipEvent_t ev_fork_{nullptr}, ev_dc_done_{nullptr};  // fork onto / join from the DC stream
     hipEvent_t ev_size_{nullptr};                        // the total size has reached the host
     hipEvent_t ev_done_{nullptr};                        // the frame's last copy has finished
-    hipEvent_t ev_ac_done_{nullptr};                     // the AC chain's end (AC group emission)
+    bool size_bound_{true};  // ev_size_ is bound to this frame's layout kernel
     hipStream_t tail_{nullptr};  // layout, assembly and readback: the DC stream, after both chains
     hipEvent_t ev_caller_{nullptr}, ev_input_free_{nullptr};  // ordering against a caller's stream
     bool caller_set_{false};

@@ -5,12 +5,15 @@
 // once into LDS; everything downstream reads LDS: inverse gaborish, the
 // adaptive-quantization field, the RD transform selection over the 21 DCT8/16/32
 // candidates of the sharpened Y, the chosen DCTs of all three channels and the
-// CfL regression partials.  Only the FP16 coefficients, the transform map, the
-// quant field and three floats per region reach HBM.
+// CfL regression partials, then LLF -> DC of its blocks.  Only the FP16
+// coefficients, the transform map, the quant field, the quantized DC and
+// three floats per region reach HBM.  The DC chain of the entropy stage
+// starts after this kernel (with cfl_maps_kernel, the 64x64 tiles' colour
+// maps from the partials).
 //
-// tile_quantize_kernel: one workgroup per 64x64 CfL tile: colour maps from the
-// region partials, quantization of every coefficient with the CfL residual,
-// LLF -> DC, and the per-block non-zero counts the entropy coder predicts from.
+// tile_quantize_kernel: one workgroup per 64x64 CfL tile: the tile's colour
+// maps (the same function), quantization of every coefficient with the CfL
+// residual and the per-block non-zero counts the entropy coder predicts from.
 //
 // Element layout used throughout a region: thread (wave w, lane L) at
 // iteration k owns element i = 256k + 64w + L of [0, 1024).  Element ids nest:
@@ -341,6 +344,7 @@ struct RegionShared {
     float cost32;
     float cfl[4][3];
     int acs[16];           // region-local raster
+    double llf[20];        // the LLF resampling tables: 2x2, then 4x4
 };
 // LDS is allocated in 1,280-byte blocks: five workgroups per CU (160 KB)
 // need at most 25 blocks each.
@@ -382,6 +386,34 @@ __device__ __forceinline__ void selection_terms(float s, int e, int n, const flo
     d = err * err;
     const float aq{fabsf(q)};
     rt = aq < 256.0f ? rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
+}
+
+// Colour maps of a 64x64 tile from its regions' partial sums (cfl.h:58-64 on
+// (R00 + R10) + (R01 + R11); absent regions contribute zeros).
+__device__ __forceinline__ void cfl_tile_maps(const float (&r)[4][3], int& mx, int& mb) {
+    float s[3];
+    for (int c{0}; c < 3; ++c) {
+        s[c] = (r[0][c] + r[1][c]) + (r[2][c] + r[3][c]);
+    }
+    const double sxy{s[0]}, syy{s[1]}, sby{s[2]};
+    const float fx{syy > 0.0 ? static_cast<float>(sxy / syy) : 0.0f};
+    const float db{syy > 0.0 ? static_cast<float>(sby / syy) : 0.0f};
+    mx = static_cast<int>(__builtin_rintf(fx * 84));
+    mb = static_cast<int>(__builtin_rintf(db * 84));
+    mx = mx < -128 ? -128 : (mx > 127 ? 127 : mx);
+    mb = mb < -128 ? -128 : (mb > 127 ? 127 : mb);
+}
+
+// Partial sums of tile (tx, ty)'s regions, zeros for regions past the frame.
+__device__ __forceinline__ void tile_partials(const Geometry& geo, const float* cfl_part, int tx, int ty,
+                                              float (&r)[4][3]) {
+    for (int i{0}; i < 4; ++i) {
+        const int rx{tx * 2 + (i & 1)}, ry{ty * 2 + (i >> 1)};
+        const bool ok{rx < static_cast<int>(geo.rw) && ry < static_cast<int>(geo.rh)};
+        for (int c{0}; c < 3; ++c) {
+            r[i][c] = ok ? cfl_part[(static_cast<std::size_t>(ry) * geo.rw + rx) * 3 + c] : 0.0f;
+        }
+    }
 }
 
 // Step 1 of a region: NV12 -> XYB over the 36x36 tile (2-pixel halo,
@@ -457,8 +489,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
     std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
-    std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part, std::uint32_t* __restrict__ list_count) {
+    std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part, std::uint32_t* __restrict__ list_count,
+    std::int32_t* __restrict__ dc, Clears clr) {
     __shared__ RegionShared S;
+    clear_words(clr);  // every workgroup of the grid takes part
     const int tid{static_cast<int>(threadIdx.x)};
     if (blockIdx.x == 0 && tid < 2 && list_count != nullptr) {
         list_count[tid] = 0;  // the quantizer's item lists, appended after this kernel
@@ -481,6 +515,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
     const int bx0{rx * 4}, by0{ry * 4};
 
+    if (tid < 20) {
+        S.llf[tid] = tid < 4 ? g_tables.llf2[tid] : g_tables.llf4[tid - 4];
+    }
     // 1. NV12 -> XYB over the 36x36 tile (2-pixel halo, clamped coordinates).
     xyb_tile(S, luma, luma_pitch, chroma, chroma_pitch, px0, py0, W, H, tid);
     if (tid < 16) {
@@ -925,13 +962,79 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     if (tid < 3) {
         cfl_part[static_cast<std::size_t>(region) * 3 + tid] =
             (S.cfl[0][tid] + S.cfl[1][tid]) + (S.cfl[2][tid] + S.cfl[3][tid]);
+    } else if (w == 1 && lane < 48) {
+        // LLF -> DC (frontend_quantize.cu:181-199, dc_from_llf_strided): lane
+        // 16 c + lb takes channel c of block lb: the owner transform's (n/8)^2
+        // lowest frequencies, resampled to the block, from the FP16
+        // coefficients; then the DC quantization, B against Y's round trip.
+        const int lb{lane & 15}, c{lane >> 4};
+        const int bxl{lb & 3}, byl{lb >> 2};
+        int n{8}, ox{bxl}, oy{byl};
+        if (S.acs[0] == 32) {
+            n = 32;
+            ox = 0;
+            oy = 0;
+        } else if (S.acs[(byl & ~1) * 4 + (bxl & ~1)] == 16) {
+            n = 16;
+            ox = bxl & ~1;
+            oy = byl & ~1;
+        }
+        const int m{n / 8}, dx{bxl - ox}, dy{byl - oy};
+        const int quad{(byl >> 1) * 2 + (bxl >> 1)};
+        const int base{n == 32 ? 0 : (n == 16 ? 256 * quad : 256 * quad + 64 * ((byl & 1) * 2 + (bxl & 1)))};
+        const _Float16* const arr{c == 0 ? S.p.coef.x
+                                         : (c == 1 ? coefy_of(S, n == 32 ? 2 : (n == 16 ? 1 : 0)) : S.p.coef.b)};
+        float dcv;
+        if (m == 1) {
+            dcv = static_cast<float>(arr[cpad(base)]);
+        } else {
+            const double* t{m == 2 ? S.llf : S.llf + 4};
+            double sum{0.0};
+            for (int fx{0}; fx < m; ++fx) {
+                for (int fy{0}; fy < m; ++fy) {
+                    const float l{static_cast<float>(arr[cpad(base + fx * n + fy)])};
+                    sum += t[fx * m + dx] * t[fy * m + dy] * static_cast<double>(l);
+                }
+            }
+            dcv = static_cast<float>(static_cast<double>(m) * sum);
+        }
+        const float dcq_y{512.0f * prm.dc_scale};
+        const int qdy{__shfl(static_cast<int>(__builtin_rintf(dcv * dcq_y)), 16 + lb, 64)};
+        if (bx0 + bxl < bw && by0 + byl < bh) {
+            const std::size_t nblk{geo.nblocks};
+            const std::size_t b{static_cast<std::size_t>(by0 + byl) * bw + bx0 + bxl};
+            if (c == 0) {
+                dc[b] = static_cast<int>(__builtin_rintf(dcv * (4096.0f * prm.dc_scale)));
+            } else if (c == 1) {
+                dc[nblk + b] = qdy;
+            } else {
+                const float y_dc_round{static_cast<float>(qdy) / dcq_y};
+                dc[2 * nblk + b] =
+                    static_cast<int>(__builtin_rintf((dcv - 1.0f * y_dc_round) * (256.0f * prm.dc_scale)));
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 
+// The colour maps of every 64x64 tile (one thread each), for the DC chain's
+// AcMetadata; the quantizer derives its tile's maps with the same function.
+__global__ __launch_bounds__(256) void cfl_maps_kernel(Geometry geo, const float* __restrict__ cfl_part,
+                                                       std::int8_t* __restrict__ ytox, std::int8_t* __restrict__ ytob) {
+    const int tile{static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x)};
+    if (tile >= static_cast<int>(geo.cmw * geo.cmh)) {
+        return;
+    }
+    float r[4][3];
+    tile_partials(geo, cfl_part, tile % static_cast<int>(geo.cmw), tile / static_cast<int>(geo.cmw), r);
+    int mx, mb;
+    cfl_tile_maps(r, mx, mb);
+    ytox[tile] = static_cast<std::int8_t>(mx);
+    ytob[tile] = static_cast<std::int8_t>(mb);
+}
+
 struct TileShared {
-    float llf[64][3][16];     // by first-block position in the tile
     unsigned int nz[64][3];
     int own[64];              // packed owner: local position | (n << 8)
     float maps[2];
@@ -959,11 +1062,9 @@ __device__ __forceinline__ int owner_in_tile(const std::int8_t* acs, int bw, int
 __global__ __launch_bounds__(256) void tile_quantize_kernel(
     Geometry geo, FrameParams prm, const std::uint16_t* __restrict__ coeffs,
     const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
-    const float* __restrict__ cfl_part, std::int8_t* __restrict__ ytox_out,
-    std::int8_t* __restrict__ ytob_out, std::int16_t* __restrict__ ac, std::int32_t* __restrict__ dc,
-    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr, ItemList list) {
+    const float* __restrict__ cfl_part, std::int16_t* __restrict__ ac,
+    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, ItemList list) {
     __shared__ TileShared S;
-    clear_words(clr);
     const int tid{static_cast<int>(threadIdx.x)};
     const int w{tid >> 6};
     const int lane{tid & 63};
@@ -983,28 +1084,10 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
         S.own[tid] = ((oy - ty * 8) * 8 + (ox - tx * 8)) | (n << 8);
     }
     if (tid == 0) {
-        // cfl.h:58-64 on (R00 + R10) + (R01 + R11) of the tile's regions.
         float r[4][3];
-        for (int i{0}; i < 4; ++i) {
-            const int rx{tx * 2 + (i & 1)}, ry{ty * 2 + (i >> 1)};
-            const bool ok{rx < static_cast<int>(geo.rw) && ry < static_cast<int>(geo.rh)};
-            for (int c{0}; c < 3; ++c) {
-                r[i][c] = ok ? cfl_part[(static_cast<std::size_t>(ry) * geo.rw + rx) * 3 + c] : 0.0f;
-            }
-        }
-        float s[3];
-        for (int c{0}; c < 3; ++c) {
-            s[c] = (r[0][c] + r[1][c]) + (r[2][c] + r[3][c]);
-        }
-        const double sxy{s[0]}, syy{s[1]}, sby{s[2]};
-        const float fx{syy > 0.0 ? static_cast<float>(sxy / syy) : 0.0f};
-        const float db{syy > 0.0 ? static_cast<float>(sby / syy) : 0.0f};
-        int mx{static_cast<int>(__builtin_rintf(fx * 84))};
-        int mb{static_cast<int>(__builtin_rintf(db * 84))};
-        mx = mx < -128 ? -128 : (mx > 127 ? 127 : mx);
-        mb = mb < -128 ? -128 : (mb > 127 ? 127 : mb);
-        ytox_out[tile] = static_cast<std::int8_t>(mx);
-        ytob_out[tile] = static_cast<std::int8_t>(mb);
+        tile_partials(geo, cfl_part, tx, ty, r);
+        int mx, mb;
+        cfl_tile_maps(r, mx, mb);
         S.maps[0] = 0.0f + static_cast<float>(mx) * (1.0f / 84);
         S.maps[1] = 1.0f + static_cast<float>(mb) * (1.0f / 84);
     }
@@ -1027,15 +1110,8 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
         const float xr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[slot]))};
         const float yr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[cplane + slot]))};
         const float br{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[2 * cplane + slot]))};
-        const bool llf{is_llf(e, n)};
         std::int16_t q0{0}, q1{0}, q2{0};
-        if (llf) {
-            const int l2{n == 32 ? 5 : (n == 16 ? 4 : 3)};
-            const int lp{(e >> l2) * m + (e & (n - 1))};
-            S.llf[op][0][lp] = xr;
-            S.llf[op][1][lp] = yr;
-            S.llf[op][2][lp] = br;
-        } else {
+        if (!is_llf(e, n)) {
             const float qgsf{static_cast<float>(qf[static_cast<std::size_t>(oy) * bw + ox]) *
                              prm.global_scale_float};
             // Quotients by the table weights are the correctly rounded ones, by
@@ -1074,33 +1150,8 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
         if (bx < bw && by < bh) {
             const int own{S.own[p]};
             const int n{own >> 8}, op{own & 255};
-            const int ox{tx * 8 + (op & 7)}, oy{ty * 8 + (op >> 3)};
             const int m{n / 8};
-            const int dx{bx - ox}, dy{by - oy};
-            float dcv[3];
-            for (int c{0}; c < 3; ++c) {
-                const float* l{S.llf[op][c]};
-                if (m == 1) {
-                    dcv[c] = l[0];
-                } else {
-                    const double* t{m == 2 ? g_tables.llf2 : g_tables.llf4};
-                    double sum{0.0};
-                    for (int fx{0}; fx < m; ++fx) {
-                        for (int fy{0}; fy < m; ++fy) {
-                            sum += t[fx * m + dx] * t[fy * m + dy] * static_cast<double>(l[fx * m + fy]);
-                        }
-                    }
-                    dcv[c] = static_cast<float>(static_cast<double>(m) * sum);
-                }
-            }
-            const float dcq_x{4096.0f * prm.dc_scale}, dcq_y{512.0f * prm.dc_scale};
-            const float dcq_b{256.0f * prm.dc_scale};
             const std::size_t b{static_cast<std::size_t>(by) * bw + bx};
-            const int qdy{static_cast<int>(__builtin_rintf(dcv[1] * dcq_y))};
-            const float y_dc_round{static_cast<float>(qdy) / dcq_y};
-            dc[nblk + b] = qdy;
-            dc[b] = static_cast<int>(__builtin_rintf(dcv[0] * dcq_x));
-            dc[2 * nblk + b] = static_cast<int>(__builtin_rintf((dcv[2] - 1.0f * y_dc_round) * dcq_b));
             const unsigned int l2{n == 16 ? 2u : (n == 32 ? 4u : 0u)};
             for (int c{0}; c < 3; ++c) {
                 const unsigned int cnt{S.nz[op][c]};
