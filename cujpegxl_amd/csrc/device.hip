@@ -271,9 +271,10 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
                      const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
-                     const std::uint64_t* ac_base, std::uint8_t* out, std::uint64_t cap_bytes) {
+                     const std::uint64_t* ac_base, std::uint8_t* out, std::uint64_t cap_bytes, hipEvent_t done) {
     const Pieces P{dc_staging, dc_group_slot_words(), ac_staging, dc_bits, ac_bits, ac_part_bits, dc_base, ac_base};
-    assemble_kernel<<<2048, 256, 0, s>>>(geo, k, lay, P, reinterpret_cast<std::uint32_t*>(out), cap_bytes);
+    hipExtLaunchKernelGGL(assemble_kernel, dim3(2048), dim3(256), 0, s, nullptr, done, 0, geo, k, lay, P,
+                          reinterpret_cast<std::uint32_t*>(out), cap_bytes);
 }
 
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,

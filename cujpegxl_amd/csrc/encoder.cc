@@ -250,7 +250,7 @@ void Encoder::release() {
             e = nullptr;
         }
     }
-    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_done_, &ev_caller_, &ev_input_free_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_dc_done_, &ev_size_, &ev_end_, &ev_done_, &ev_caller_, &ev_input_free_}) {
         if (*e != nullptr) {
             (void)hipEventDestroy(*e);
             *e = nullptr;
@@ -300,7 +300,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
         }
     }
     // Bound to kernels' completion at launch (timing-capable events).
-    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_}) {
+    for (hipEvent_t* e : {&ev_fork_, &ev_size_, &ev_dc_done_, &ev_end_}) {
         if (hipEventCreate(e) != hipSuccess) {
             return false;
         }
@@ -548,11 +548,15 @@ bool Encoder::spin_on(hipEvent_t e) {
     }
 }
 
-// The end of a frame.  The host already polls (it has the size), and only
-// the assembly's tail or the last copy remains on the stream (tens of
-// microseconds): polling the completion event saves the wake-up latency of
-// a blocking wait, which is of the same order.
+// The end of a frame: the assembly kernel's own completion when it writes
+// the output and ends the frame, else an event after the last copy.  The
+// host polls (only tens of microseconds of work remain): polling saves the
+// wake-up latency of a blocking wait, which is of the same order.
 bool Encoder::wait_done() {
+    if (end_bound_) {
+        end_bound_ = false;
+        return spin_on(ev_end_);
+    }
     return hipEventRecord(ev_done_, tail_) == hipSuccess && spin_on(ev_done_);
 }
 
@@ -865,11 +869,14 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, b
     }
     tail_ = stream_;
     size_bound_ = size_event;
+    // Without a size event the output is written by the assembly kernel and
+    // nothing follows it: its own completion ends the frame.
+    end_bound_ = !size_event;
     launch_layout(stream_, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
                   ac_group_bits_, blobs, lay, ac_base_, dc_base_, total_dev_, size_event ? ev_size_ : nullptr);
     mark(Stage::kEmit, tail_);
     launch_assemble(stream_, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
-                    dc_base_, ac_base_, out, cap);
+                    dc_base_, ac_base_, out, cap, end_bound_ ? ev_end_ : nullptr);
     return hipGetLastError() == hipSuccess;
 }
 

@@ -115,6 +115,8 @@ private:
     hipEvent_t ev_fork_{nullptr}, ev_dc_done_{nullptr};  // fork onto / join from the DC stream
     hipEvent_t ev_size_{nullptr};                        // the total size has reached the host
     hipEvent_t ev_done_{nullptr};                        // the frame's last copy has finished
+    hipEvent_t ev_end_{nullptr};  // the assembly kernel's completion, when it ends the frame
+    bool end_bound_{false};       // ev_end_ is bound to this frame's assembly kernel
     bool size_bound_{true};  // ev_size_ is bound to this frame's layout kernel
     hipStream_t tail_{nullptr};  // layout, assembly and readback: the DC stream, after both chains
     hipEvent_t ev_caller_{nullptr}, ev_input_free_{nullptr};  // ordering against a caller's stream

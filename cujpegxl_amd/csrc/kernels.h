@@ -90,7 +90,8 @@ void launch_layout(hipStream_t s, const Geometry& geo, const Constants* k, const
 void launch_assemble(hipStream_t s, const Geometry& geo, const Constants* k, const Layout* lay,
                      const std::uint32_t* dc_staging, const std::uint32_t* ac_staging, const std::uint64_t* dc_bits,
                      const std::uint64_t* ac_bits, const std::uint64_t* ac_part_bits, const std::uint64_t* dc_base,
-                     const std::uint64_t* ac_base, std::uint8_t* out, std::uint64_t cap_bytes);
+                     const std::uint64_t* ac_base, std::uint8_t* out, std::uint64_t cap_bytes,
+                     hipEvent_t done = nullptr);
 void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, const std::int8_t* acs,
                     const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                     const CodeSet* cs, const DcBlobs* blobs, const std::uint32_t* blob_words,
