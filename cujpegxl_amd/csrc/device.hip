@@ -108,15 +108,14 @@ static unsigned int grid_for(std::size_t n, std::size_t per) {
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
-                   std::uint32_t* list_count, std::int32_t* dc, const Clears& clr, hipEvent_t start,
-                   hipEvent_t stop) {
+                   std::uint32_t* list_count, const Clears& clr, hipEvent_t start, hipEvent_t stop) {
     // Eight equal runs of regions, one per XCD (the kernel maps workgroup i to
     // run i mod 8); workgroups past the last region exit at once.  The events,
     // when given, take the kernel's own start and end times (no marker
     // packets between it and its neighbours).
     const unsigned int per_xcd{(geo.rw * geo.rh + 7) / 8};
     hipExtLaunchKernelGGL(region_kernel, dim3(8 * per_xcd), dim3(256), 0, s, start, stop, 0, geo, prm, luma,
-                          luma_pitch, chroma, chroma_pitch, coeffs, acs, qf, cfl_part, list_count, dc, clr);
+                          luma_pitch, chroma, chroma_pitch, coeffs, acs, qf, cfl_part, list_count, clr);
 }
 
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
@@ -136,10 +135,10 @@ void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams&
                                                            nz_count, list);
 }
 
-void launch_cfl_maps(hipStream_t s, const Geometry& geo, const float* cfl_part, std::int8_t* ytox,
-                     std::int8_t* ytob) {
-    cfl_maps_kernel<<<grid_for(static_cast<std::size_t>(geo.cmw) * geo.cmh, 256), 256, 0, s>>>(geo, cfl_part, ytox,
-                                                                                                ytob);
+void launch_dc_maps(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint16_t* coeffs,
+                    const std::int8_t* acs, const float* cfl_part, std::int32_t* dc, std::int8_t* ytox,
+                    std::int8_t* ytob) {
+    dc_maps_kernel<<<grid_for(geo.nblocks, 256), 256, 0, s>>>(geo, prm, coeffs, acs, cfl_part, dc, ytox, ytob);
 }
 
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,

@@ -637,12 +637,13 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     }
     // With profiling on, the region kernel's own start and end times open the
     // region and quantize stages.  Its end is the fork point of the DC chain
-    // (it leaves the DC values and the CfL partials the colour maps come
-    // from, and clears the entropy stage's accumulators).
+    // (it leaves the coefficients the DC values come from and the CfL
+    // partials the colour maps come from, and clears the entropy stage's
+    // accumulators).
     const bool timed{profiling_ != 0};
     const hipEvent_t region_end{timed ? events_[static_cast<int>(Stage::kQuantize)] : ev_fork_};
     launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_,
-                  item_count_, dc_, entropy_clears(),
+                  item_count_, entropy_clears(),
                   timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr, region_end);
     // The planes are read by the region kernel only: later work on the
     // caller's stream may overwrite them once it has finished.
@@ -828,8 +829,8 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, b
     // The DC chain (histograms, codes, header blobs, chunk sizes, emission)
     // depends on nothing of the AC chain: it runs on the second stream from
     // `fork` (the stream's work so far when null) and joins before the layout.
-    // With `maps` it first derives the tiles' colour maps from the CfL
-    // partials.  `size_event` binds ev_size_ to the layout kernel (the host
+    // With `maps` it first derives the DC values from the coefficients and
+    // the tiles' colour maps from the CfL partials.  `size_event` binds ev_size_ to the layout kernel (the host
     // then reads the size before the frame's end).
     mark(Stage::kHistograms);
     const int ndc{static_cast<int>(2 * geo_.num_dc)};
@@ -843,7 +844,7 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, b
         return false;
     }
     if (maps) {
-        launch_cfl_maps(dc_stream_, geo_, cfl_part_, ytox_, ytob_);
+        launch_dc_maps(dc_stream_, geo_, prm_, coeffs_, acs_, cfl_part_, dc_, ytox_, ytob_);
     }
     launch_dc_hist(dc_stream_, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
                    hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);

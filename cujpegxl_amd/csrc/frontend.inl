@@ -5,11 +5,10 @@
 // once into LDS; everything downstream reads LDS: inverse gaborish, the
 // adaptive-quantization field, the RD transform selection over the 21 DCT8/16/32
 // candidates of the sharpened Y, the chosen DCTs of all three channels and the
-// CfL regression partials, then LLF -> DC of its blocks.  Only the FP16
-// coefficients, the transform map, the quant field, the quantized DC and
-// three floats per region reach HBM.  The DC chain of the entropy stage
-// starts after this kernel (with cfl_maps_kernel, the 64x64 tiles' colour
-// maps from the partials).
+// CfL regression partials.  Only the FP16 coefficients, the transform map,
+// the quant field and three floats per region reach HBM.  The DC chain of the
+// entropy stage starts after this kernel (with dc_maps_kernel: LLF -> DC of
+// every block, and the 64x64 tiles' colour maps from the partials).
 //
 // tile_quantize_kernel: one workgroup per 64x64 CfL tile: the tile's colour
 // maps (the same function), quantization of every coefficient with the CfL
@@ -344,7 +343,6 @@ struct RegionShared {
     float cost32;
     float cfl[4][3];
     int acs[16];           // region-local raster
-    double llf[20];        // the LLF resampling tables: 2x2, then 4x4
 };
 // LDS is allocated in 1,280-byte blocks: five workgroups per CU (160 KB)
 // need at most 25 blocks each.
@@ -490,7 +488,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
     std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
     std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part, std::uint32_t* __restrict__ list_count,
-    std::int32_t* __restrict__ dc, Clears clr) {
+    Clears clr) {
     __shared__ RegionShared S;
     clear_words(clr);  // every workgroup of the grid takes part
     const int tid{static_cast<int>(threadIdx.x)};
@@ -515,9 +513,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
     const int bx0{rx * 4}, by0{ry * 4};
 
-    if (tid < 20) {
-        S.llf[tid] = tid < 4 ? g_tables.llf2[tid] : g_tables.llf4[tid - 4];
-    }
     // 1. NV12 -> XYB over the 36x36 tile (2-pixel halo, clamped coordinates).
     xyb_tile(S, luma, luma_pitch, chroma, chroma_pitch, px0, py0, W, H, tid);
     if (tid < 16) {
@@ -962,77 +957,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     if (tid < 3) {
         cfl_part[static_cast<std::size_t>(region) * 3 + tid] =
             (S.cfl[0][tid] + S.cfl[1][tid]) + (S.cfl[2][tid] + S.cfl[3][tid]);
-    } else if (w == 1 && lane < 48) {
-        // LLF -> DC (frontend_quantize.cu:181-199, dc_from_llf_strided): lane
-        // 16 c + lb takes channel c of block lb: the owner transform's (n/8)^2
-        // lowest frequencies, resampled to the block, from the FP16
-        // coefficients; then the DC quantization, B against Y's round trip.
-        const int lb{lane & 15}, c{lane >> 4};
-        const int bxl{lb & 3}, byl{lb >> 2};
-        int n{8}, ox{bxl}, oy{byl};
-        if (S.acs[0] == 32) {
-            n = 32;
-            ox = 0;
-            oy = 0;
-        } else if (S.acs[(byl & ~1) * 4 + (bxl & ~1)] == 16) {
-            n = 16;
-            ox = bxl & ~1;
-            oy = byl & ~1;
-        }
-        const int m{n / 8}, dx{bxl - ox}, dy{byl - oy};
-        const int quad{(byl >> 1) * 2 + (bxl >> 1)};
-        const int base{n == 32 ? 0 : (n == 16 ? 256 * quad : 256 * quad + 64 * ((byl & 1) * 2 + (bxl & 1)))};
-        const _Float16* const arr{c == 0 ? S.p.coef.x
-                                         : (c == 1 ? coefy_of(S, n == 32 ? 2 : (n == 16 ? 1 : 0)) : S.p.coef.b)};
-        float dcv;
-        if (m == 1) {
-            dcv = static_cast<float>(arr[cpad(base)]);
-        } else {
-            const double* t{m == 2 ? S.llf : S.llf + 4};
-            double sum{0.0};
-            for (int fx{0}; fx < m; ++fx) {
-                for (int fy{0}; fy < m; ++fy) {
-                    const float l{static_cast<float>(arr[cpad(base + fx * n + fy)])};
-                    sum += t[fx * m + dx] * t[fy * m + dy] * static_cast<double>(l);
-                }
-            }
-            dcv = static_cast<float>(static_cast<double>(m) * sum);
-        }
-        const float dcq_y{512.0f * prm.dc_scale};
-        const int qdy{__shfl(static_cast<int>(__builtin_rintf(dcv * dcq_y)), 16 + lb, 64)};
-        if (bx0 + bxl < bw && by0 + byl < bh) {
-            const std::size_t nblk{geo.nblocks};
-            const std::size_t b{static_cast<std::size_t>(by0 + byl) * bw + bx0 + bxl};
-            if (c == 0) {
-                dc[b] = static_cast<int>(__builtin_rintf(dcv * (4096.0f * prm.dc_scale)));
-            } else if (c == 1) {
-                dc[nblk + b] = qdy;
-            } else {
-                const float y_dc_round{static_cast<float>(qdy) / dcq_y};
-                dc[2 * nblk + b] =
-                    static_cast<int>(__builtin_rintf((dcv - 1.0f * y_dc_round) * (256.0f * prm.dc_scale)));
-            }
-        }
     }
 }
 
 // ---------------------------------------------------------------------------
-
-// The colour maps of every 64x64 tile (one thread each), for the DC chain's
-// AcMetadata; the quantizer derives its tile's maps with the same function.
-__global__ __launch_bounds__(256) void cfl_maps_kernel(Geometry geo, const float* __restrict__ cfl_part,
-                                                       std::int8_t* __restrict__ ytox, std::int8_t* __restrict__ ytob) {
-    const int tile{static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x)};
-    if (tile >= static_cast<int>(geo.cmw * geo.cmh)) {
-        return;
-    }
-    float r[4][3];
-    tile_partials(geo, cfl_part, tile % static_cast<int>(geo.cmw), tile / static_cast<int>(geo.cmw), r);
-    int mx, mb;
-    cfl_tile_maps(r, mx, mb);
-    ytox[tile] = static_cast<std::int8_t>(mx);
-    ytob[tile] = static_cast<std::int8_t>(mb);
-}
 
 struct TileShared {
     unsigned int nz[64][3];
@@ -1057,6 +985,67 @@ __device__ __forceinline__ int owner_in_tile(const std::int8_t* acs, int bw, int
     ox = bx;
     oy = by;
     return 8;
+}
+
+// The first kernel of the DC chain: thread i derives the DC of block i
+// (frontend_quantize.cu:181-199, dc_from_llf_strided: the owner transform's
+// (n/8)^2 lowest frequencies, resampled to the block, from the FP16
+// coefficients; then the DC quantization, B against Y's round trip) and,
+// for i below the tile count, the colour maps of tile i for the AcMetadata
+// (the quantizer derives its tile's maps with the same function).
+__global__ __launch_bounds__(256) void dc_maps_kernel(Geometry geo, FrameParams prm,
+                                                      const std::uint16_t* __restrict__ coeffs,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const float* __restrict__ cfl_part, std::int32_t* __restrict__ dc,
+                                                      std::int8_t* __restrict__ ytox, std::int8_t* __restrict__ ytob) {
+    const int i{static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x)};
+    if (i < static_cast<int>(geo.cmw * geo.cmh)) {
+        float r[4][3];
+        tile_partials(geo, cfl_part, i % static_cast<int>(geo.cmw), i / static_cast<int>(geo.cmw), r);
+        int mx, mb;
+        cfl_tile_maps(r, mx, mb);
+        ytox[i] = static_cast<std::int8_t>(mx);
+        ytob[i] = static_cast<std::int8_t>(mb);
+    }
+    if (i >= static_cast<int>(geo.nblocks)) {
+        return;
+    }
+    const int bw{static_cast<int>(geo.bw)};
+    const int bx{i % bw}, by{i / bw};
+    int ox, oy;
+    const int n{owner_in_tile(acs, bw, bx, by, ox, oy)};
+    const int m{n / 8}, dx{bx - ox}, dy{by - oy};
+    const std::size_t nblk{geo.nblocks};
+    const std::size_t cplane{nblk * 64};
+    float dcv[3];
+    for (int c{0}; c < 3; ++c) {
+        // Element e of the owner's transform sits in slot e % 64 of the
+        // covered block e / 64 (row-major over the m x m blocks).
+        const auto llf = [&](int e) {
+            const int qc{e >> 6};
+            const std::size_t blk{static_cast<std::size_t>(oy + qc / m) * bw + ox + qc % m};
+            return static_cast<float>(__builtin_bit_cast(_Float16, coeffs[c * cplane + blk * 64 + (e & 63)]));
+        };
+        if (m == 1) {
+            dcv[c] = llf(0);
+        } else {
+            const double* t{m == 2 ? g_tables.llf2 : g_tables.llf4};
+            double sum{0.0};
+            for (int fx{0}; fx < m; ++fx) {
+                for (int fy{0}; fy < m; ++fy) {
+                    sum += t[fx * m + dx] * t[fy * m + dy] * static_cast<double>(llf(fx * n + fy));
+                }
+            }
+            dcv[c] = static_cast<float>(static_cast<double>(m) * sum);
+        }
+    }
+    const float dcq_x{4096.0f * prm.dc_scale}, dcq_y{512.0f * prm.dc_scale};
+    const float dcq_b{256.0f * prm.dc_scale};
+    const int qdy{static_cast<int>(__builtin_rintf(dcv[1] * dcq_y))};
+    const float y_dc_round{static_cast<float>(qdy) / dcq_y};
+    dc[nblk + i] = qdy;
+    dc[i] = static_cast<int>(__builtin_rintf(dcv[0] * dcq_x));
+    dc[2 * nblk + i] = static_cast<int>(__builtin_rintf((dcv[2] - 1.0f * y_dc_round) * dcq_b));
 }
 
 __global__ __launch_bounds__(256) void tile_quantize_kernel(

@@ -36,7 +36,7 @@ bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::u
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
-                   std::uint32_t* list_count, std::int32_t* dc, const Clears& clr, hipEvent_t start = nullptr,
+                   std::uint32_t* list_count, const Clears& clr, hipEvent_t start = nullptr,
                    hipEvent_t stop = nullptr);
 // Config-1 semantics: DCT8 everywhere, uniform quant field, no CfL maps.
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
@@ -47,9 +47,11 @@ void launch_tile_quantize(hipStream_t s, const Geometry& geo, const FrameParams&
                           const std::uint16_t* coeffs, const std::int8_t* acs, const std::int32_t* qf,
                           const float* cfl_part, std::int16_t* ac, std::uint8_t* nz_grid,
                           std::uint16_t* nz_count, const ItemList& list);
-// The 64x64 tiles' colour maps from the regions' CfL partials.
-void launch_cfl_maps(hipStream_t s, const Geometry& geo, const float* cfl_part, std::int8_t* ytox,
-                     std::int8_t* ytob);
+// LLF -> DC of every block from the FP16 coefficients, and the 64x64
+// tiles' colour maps from the regions' CfL partials.
+void launch_dc_maps(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint16_t* coeffs,
+                    const std::int8_t* acs, const float* cfl_part, std::int32_t* dc, std::int8_t* ytox,
+                    std::int8_t* ytob);
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
 void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v);
