@@ -5,15 +5,18 @@ seed 0) resident in HBM, encoded to a `.jxl` file on the host by the plain HIP
 VarDCT path (RD selection over DCT8/16/32, CfL, adaptive quantization,
 32-cluster prefix coding).  One step is one synchronous
 cujpegxl_encoder_encode call: NV12 in HBM -> file bytes in a caller-owned
-host buffer.  The timed buffer is page-locked (allocated through the
-library), into which the assembly kernel writes the file over the host link;
-the rate with an ordinary pageable buffer, which goes through the library's
-pinned staging copy plus a Python bytes copy, is reported beside it as
-`pageable_output_fps`.
+host buffer.  The timed steps run back to back in native code (the stage
+library's cjx_stage_encode_repeat, as the reference's C++ benchmark drives
+the C API); the same steps from a Python loop, one ctypes call per frame,
+are reported as `python_loop_fps`.  The timed buffer is page-locked
+(allocated through the library), into which the assembly kernel writes the
+file over the host link; the rate with an ordinary pageable buffer, which
+goes through the library's pinned staging copy plus a Python bytes copy, is
+reported beside it as `pageable_output_fps`.
 
 Prints one JSON line.  `roofline` is the dominant kernel's achieved
 algorithmic HBM bandwidth (HIP events bound to the kernel on the encoder's
-stream, sampled every REGION_SAMPLE-th timed step); `cpu_baseline` is the C
+stream, over REGION_FRAMES separate frames); `cpu_baseline` is the C
 oracle (a scalar restatement of the reference path) encoding the same frame
 on one host core.
 """
@@ -33,7 +36,7 @@ sys.path.insert(0, HERE)
 
 WIDTH, HEIGHT, DISTANCE, SEED = 3840, 2160, 1.0, 0
 HBM_PEAK_GBS = 8000.0
-REGION_SAMPLE = 8  # timed frames per region-kernel event sample
+REGION_FRAMES = 40  # frames whose region kernel is timed by bound events
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_4k_d1.txt")
 
 
@@ -151,26 +154,34 @@ def main() -> int:
     luma_ptr, chroma_ptr = dev.ptr, dev.ptr + WIDTH * HEIGHT
     for _ in range(args.warmup):
         size = enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
-    # Timed steps.  The region kernel's duration (the roofline figure) comes
-    # from HIP events bound to the kernel's own start and end on the
-    # encoder's stream, on every REGION_SAMPLE-th timed frame: a start event
-    # adds a few microseconds of idle device time after the kernel.  The
-    # other stages are timed in separate untimed frames below.
-    region_sum, region_frames, sampling = 0.0, 0, False
+    # Timed steps: `steps` synchronous encodes back to back in native code
+    # (the reference's benchmark drives the C API from a C++ loop), no
+    # events.  The same steps from a Python loop (one ctypes call per
+    # frame) are reported beside them.
     lib.cjx_device_synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        sample = i % REGION_SAMPLE == 0
-        if sample != sampling:
-            enc.set_profiling(sample, region_only=True)
-            sampling = sample
-        size = enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
-        if sample:
-            region_sum += enc.region_ms()
-            region_frames += 1
+    _, size = enc.encode_repeat(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity, args.steps)
     lib.cjx_device_synchronize()
     elapsed = time.perf_counter() - t0
     data = pinned.array[:size].tobytes()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
+    lib.cjx_device_synchronize()
+    python_loop_fps = args.steps / (time.perf_counter() - t1)
+
+    # The region kernel's duration (the roofline figure), from HIP events
+    # bound to the kernel's own start and end on the encoder's stream, over
+    # separate frames (a start event adds a few microseconds of idle device
+    # time after the kernel).  The other stages are timed in separate
+    # frames below.
+    region_frames = min(args.steps, REGION_FRAMES)
+    region_sum = 0.0
+    enc.set_profiling(True, region_only=True)
+    for _ in range(region_frames):
+        enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
+        region_sum += enc.region_ms()
+    enc.set_profiling(False)
 
     enc.set_profiling(True)
     stage_frames = min(args.steps, 10)
@@ -217,6 +228,7 @@ def main() -> int:
             "path": "plain",
             "output": "caller-owned page-locked host buffer",
             "pageable_output_fps": round(pageable_fps, 3),
+            "python_loop_fps": round(python_loop_fps, 3),
             "bytes_per_frame": len(data),
             "bpp": round(8.0 * len(data) / (WIDTH * HEIGHT), 4),
             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},

@@ -102,6 +102,8 @@ def stages_lib() -> C.CDLL:
         L.cjx_stage_encoder_create_any.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
         L.cjx_stage_encode_nv12.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
+        L.cjx_stage_encode_repeat.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output), C.c_int,
+                                              C.POINTER(C.c_double)]
         L.cjx_stage_encode_dct8_uniform.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_timings.argtypes = [P, C.POINTER(C.c_float), C.c_int]
         L.cjx_device_alloc.restype = P
@@ -289,6 +291,19 @@ class Encoder:
         if s != OK:
             raise RuntimeError(f"encode failed: {status_string(s)}")
         return out.size
+
+    def encode_repeat(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
+                      out_ptr: int, capacity: int, count: int) -> tuple[float, int]:
+        """`count` synchronous encodes of one frame into caller memory, back
+        to back in native code (stage library); returns the loop's wall
+        seconds and the last file's size."""
+        inp = Nv12Input(luma_ptr, chroma_ptr, luma_pitch, chroma_pitch)
+        out = Output(out_ptr, 0, capacity)
+        sec = C.c_double(0.0)
+        s = stages_lib().cjx_stage_encode_repeat(self.handle, C.byref(inp), C.byref(out), count, C.byref(sec))
+        if s != OK:
+            raise RuntimeError(f"encode_repeat failed: {status_string(s)}")
+        return sec.value, out.size
 
     def encode_dct8_uniform(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int,
                             chroma_pitch: int) -> bytes:

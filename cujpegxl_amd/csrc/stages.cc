@@ -1,6 +1,7 @@
 // Stage harness (cujpegxl_stages.h).
 #include "cujpegxl/cujpegxl_stages.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -77,6 +78,22 @@ int cjx_stage_encode_nv12(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cu
     }
     out->size = size;
     return size > out->capacity ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
+}
+
+int cjx_stage_encode_repeat(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cujpegxl_output* out, int count,
+                            double* seconds) {
+    if (count <= 0 || seconds == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const auto t0{std::chrono::steady_clock::now()};
+    for (int i{0}; i < count; ++i) {
+        const int s{cujpegxl_encoder_encode(e, in, out)};
+        if (s != CUJPEGXL_OK) {
+            return s;
+        }
+    }
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return CUJPEGXL_OK;
 }
 
 int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cujpegxl_output* out) {

@@ -66,6 +66,16 @@ int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* encoder, const cujpegxl_nv12
 int cjx_stage_set_profiling(cujpegxl_encoder* encoder, int on);
 int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
 
+/* `count` synchronous cujpegxl_encoder_encode calls on the same input and
+ * output, back to back, timed on the host around the whole loop (wall
+ * seconds into *seconds).  No counterpart in the reference's library: the
+ * native-loop counterpart of its C++ benchmark driver
+ * (tools/benchmark/encode_bench.cc:462-470, 514-523), for callers that
+ * time the encoder without an interpreter's per-call overhead.  Returns the
+ * first failing call's status. */
+int cjx_stage_encode_repeat(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
+                            cujpegxl_output* output, int count, double* seconds);
+
 /* Prefix codes of `count` (<= 96) histograms of 256 uint32 bins, built on
  * device `device_ordinal` by the encoder's code kernel (prefix_code.cc:36-397
  * semantics): per histogram the alphabet size, 256 depths, 256 LSB-first

@@ -141,3 +141,27 @@ def test_async_encode_is_ordered_on_the_caller_stream(frame):
     planes.close()
     src.close()
     enc.close()
+
+
+def test_native_repeat_loop_matches_oracle(frame):
+    _luma, _chroma, nv12, want = frame
+    say("cjx_stage_encode_repeat: 5 encodes into a page-locked and a pageable output")
+    enc = cj.Encoder(W, H, D, stages=True)
+    dev = cj.DeviceBuffer.from_array(nv12)
+    pin = cj.PinnedBuffer(enc.capacity)
+    sec, size = enc.encode_repeat(dev.ptr, dev.ptr + W * H, W, W, pin.ptr, enc.capacity, 5)
+    assert sec > 0.0 and pin.array[:size].tobytes() == want
+    sec, size = enc.encode_repeat(dev.ptr, dev.ptr + W * H, W, W, enc.out.ctypes.data, enc.capacity, 5)
+    assert enc.out[:size].tobytes() == want
+    # A count below one and a too-small output are refused.
+    inp = cj.Nv12Input(dev.ptr, dev.ptr + W * H, W, W)
+    out = cj.Output(pin.ptr, 0, enc.capacity)
+    s = C.c_double(0.0)
+    assert enc.lib.cjx_stage_encode_repeat(enc.handle, C.byref(inp), C.byref(out), 0, C.byref(s)) == \
+        cj.INVALID_ARGUMENT
+    small = cj.Output(pin.ptr, 0, 64)
+    assert enc.lib.cjx_stage_encode_repeat(enc.handle, C.byref(inp), C.byref(small), 3, C.byref(s)) == \
+        cj.BUFFER_TOO_SMALL
+    pin.close()
+    dev.close()
+    enc.close()
