@@ -1,24 +1,28 @@
 """Benchmark: 4K NV12 frames/s at distance 1.0 on one MI355X.
 
-Workload: a 3840x2160 seeded synthetic NV12 frame (cujpegxl_amd/synth.py,
-seed 0) resident in HBM, encoded to a `.jxl` file on the host by the plain HIP
+Workload: 3840x2160 seeded synthetic NV12 frames (cujpegxl_amd/synth.py, seeds
+0-7) resident in HBM, encoded to `.jxl` files on the host by the plain HIP
 VarDCT path (RD selection over DCT8/16/32, CfL, adaptive quantization,
-32-cluster prefix coding).  One step is one synchronous
-cujpegxl_encoder_encode call: NV12 in HBM -> file bytes in a caller-owned
-host buffer.  The timed steps run back to back in native code (the stage
-library's cjx_stage_encode_repeat, as the reference's C++ benchmark drives
-the C API); the same steps from a Python loop, one ctypes call per frame,
-are reported as `python_loop_fps`.  The timed buffer is page-locked
-(allocated through the library), into which the assembly kernel writes the
-file over the host link; the rate with an ordinary pageable buffer, which
-goes through the library's pinned staging copy plus a Python bytes copy, is
-reported beside it as `pageable_output_fps`.
+32-cluster prefix coding).  One step is one synchronous cujpegxl_encoder_encode
+call: NV12 in HBM -> file bytes in a caller-owned host buffer.  The timed steps
+cycle through 24 device frames (the 8 seeds, 3 copies each: 299 MB, more than
+the 256 MB last-level cache), as the reference's benchmark alternates inputs
+(tools/benchmark/encode_bench.cc:512-518), back to back in native code (the
+stage library's cjx_stage_encode_cycle, as the reference's C++ benchmark
+drives the C API).  The same steps from a Python loop are reported as
+`python_loop_fps`, and one resident frame encoded repeatedly as
+`single_frame_fps`.  The timed buffer is page-locked (allocated through the
+library), into which the assembly kernel writes the file over the host link;
+the rate with an ordinary pageable buffer (the library's pinned staging copy
+plus a Python bytes copy) is `pageable_output_fps`.
 
 Prints one JSON line.  `roofline` is the dominant kernel's achieved
 algorithmic HBM bandwidth (HIP events bound to the kernel on the encoder's
-stream, over REGION_FRAMES separate frames); `cpu_baseline` is the C
-oracle (a scalar restatement of the reference path) encoding the same frame
-on one host core.
+stream, over REGION_FRAMES separate frames), with the instruction-issue
+floor from the committed SQ counter summary beside it; `cpu_baseline` is the
+C oracle (a restatement of the reference path) encoding the same frames on
+all host cores the process may use (up to 16), with its single-thread rate
+beside it.
 """
 
 from __future__ import annotations
@@ -35,9 +39,26 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 WIDTH, HEIGHT, DISTANCE, SEED = 3840, 2160, 1.0, 0
+SEEDS = 8    # distinct seeded frames in the timed cycle
+COPIES = 3   # device copies of each (8 x 3 x 12.4 MB = 299 MB)
 HBM_PEAK_GBS = 8000.0
-REGION_FRAMES = 40  # frames whose region kernel is timed by bound events
+REGION_FRAMES = 48  # frames whose region kernel is timed by bound events
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_4k_d1.txt")
+COUNTER_FILE = os.path.join(HERE, "profiles", "sq_counters_4k_d1.txt")
+CUS, SCLK_GHZ = 256, 2.4  # MI355X compute units and peak engine clock
+
+
+def region_valu_instructions() -> int | None:
+    """VALU instructions per region_kernel launch (SQ_INSTS_VALU summed over
+    the device), from the committed SQ counter summary."""
+    try:
+        with open(COUNTER_FILE) as f:
+            for line in f:
+                if line.startswith("region_kernel valu_insts="):
+                    return int(line.split("=", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def region_traffic_bytes() -> int | None:
@@ -101,20 +122,51 @@ def mixed_batch(cj, synthetic_nv12, frames: int = 64, seed: int = 4) -> dict:
             "launches": "one synchronous encode per frame (no graph, no batching across frames)"}
 
 
-def cpu_baseline(luma: np.ndarray, chroma: np.ndarray, budget_s: float = 12.0) -> dict:
+def host_info() -> dict:
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(frames: list, budget_s: float = 10.0) -> dict:
+    """The C oracle encoding the bench's 4K frames (cycling through them):
+    single-threaded, then with its row/tile/region/block loops on all the
+    cores this process may use (at most 16).  Each leg runs whole frames
+    until `budget_s` has passed (at least one, at most 8)."""
     import oracle
 
     oracle.lib()
-    frames, t0 = 0, time.perf_counter()
-    while True:
-        oracle.encode_nv12(luma, chroma, WIDTH, HEIGHT, DISTANCE)
-        frames += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or frames >= 4:
-            break
-    return {"value": frames / el, "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"{frames} full 3840x2160 frame(s), d=1.0, seed {SEED}, single thread, "
-                      f"{el:.1f} s"}
+
+    def leg(threads: int) -> tuple[float, int, float]:
+        oracle.set_threads(threads)
+        n, t0 = 0, time.perf_counter()
+        while True:
+            luma, chroma = frames[n % len(frames)]
+            oracle.encode_nv12(luma, chroma, WIDTH, HEIGHT, DISTANCE)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s or n >= 8:
+                break
+        oracle.set_threads(1)
+        return n / el, n, el
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    one = leg(1)
+    many = leg(cores) if cores > 1 else one
+    return {"value": round(many[0], 4), "unit": "frames/s", "cores": cores, "kind": "port",
+            "sample": f"{many[1]} full 3840x2160 frame(s) of the bench's seeds, d=1.0, {cores} threads "
+                      f"(OpenMP over rows, AQ tiles, selection regions and DCT blocks; entropy coding "
+                      f"serial), {many[2]:.1f} s",
+            "single_thread": {"value": round(one[0], 4), "cores": 1,
+                              "sample": f"{one[1]} frame(s), {one[2]:.1f} s"},
+            "host": host_info()}
 
 
 def main() -> int:
@@ -144,31 +196,41 @@ def main() -> int:
         print(json.dumps({"copy_bytes": args.copy_bytes, "copies": max(1, args.steps)}))
         return 0
 
-    luma, chroma = synthetic_nv12(WIDTH, HEIGHT, SEED)
-    nv12 = np.concatenate([luma.ravel(), chroma.ravel()])
-    dev = cj.DeviceBuffer.from_array(nv12)
+    host_frames = [synthetic_nv12(WIDTH, HEIGHT, SEED + k) for k in range(SEEDS)]
+    devs = []
+    for _ in range(COPIES):
+        for luma, chroma in host_frames:
+            devs.append(cj.DeviceBuffer.from_array(np.concatenate([luma.ravel(), chroma.ravel()])))
+    inputs = [(d.ptr, d.ptr + WIDTH * HEIGHT, WIDTH, WIDTH) for d in devs]
     enc = cj.Encoder(WIDTH, HEIGHT, DISTANCE, stages=True)
     lib = enc.lib
 
     pinned = cj.PinnedBuffer(enc.capacity)
-    luma_ptr, chroma_ptr = dev.ptr, dev.ptr + WIDTH * HEIGHT
-    for _ in range(args.warmup):
-        size = enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
+    for k in range(max(args.warmup, len(inputs))):
+        enc.encode_into(*inputs[k % len(inputs)], pinned.ptr, enc.capacity)
     # Timed steps: `steps` synchronous encodes back to back in native code
-    # (the reference's benchmark drives the C API from a C++ loop), no
-    # events.  The same steps from a Python loop (one ctypes call per
-    # frame) are reported beside them.
+    # (the reference's benchmark drives the C API from a C++ loop), cycling
+    # through the frames, no events.  The same steps from a Python loop (one
+    # ctypes call per frame) are reported beside them.
     lib.cjx_device_synchronize()
     t0 = time.perf_counter()
-    _, size = enc.encode_repeat(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity, args.steps)
+    _, size = enc.encode_cycle(inputs, pinned.ptr, enc.capacity, args.steps)
     lib.cjx_device_synchronize()
     elapsed = time.perf_counter() - t0
+    last = (args.steps - 1) % len(inputs)
     data = pinned.array[:size].tobytes()
+    sizes = []
+    for k in range(SEEDS):
+        sizes.append(enc.encode_into(*inputs[k], pinned.ptr, enc.capacity))
     t1 = time.perf_counter()
-    for _ in range(args.steps):
-        enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
+    for k in range(args.steps):
+        enc.encode_into(*inputs[k % len(inputs)], pinned.ptr, enc.capacity)
     lib.cjx_device_synchronize()
     python_loop_fps = args.steps / (time.perf_counter() - t1)
+    t1 = time.perf_counter()
+    enc.encode_repeat(*inputs[0], pinned.ptr, enc.capacity, args.steps)
+    lib.cjx_device_synchronize()
+    single_frame_fps = args.steps / (time.perf_counter() - t1)
 
     # The region kernel's duration (the roofline figure), from HIP events
     # bound to the kernel's own start and end on the encoder's stream, over
@@ -178,28 +240,28 @@ def main() -> int:
     region_frames = min(args.steps, REGION_FRAMES)
     region_sum = 0.0
     enc.set_profiling(True, region_only=True)
-    for _ in range(region_frames):
-        enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
+    for k in range(region_frames):
+        enc.encode_into(*inputs[k % len(inputs)], pinned.ptr, enc.capacity)
         region_sum += enc.region_ms()
     enc.set_profiling(False)
 
     enc.set_profiling(True)
-    stage_frames = min(args.steps, 10)
+    stage_frames = min(args.steps, 16)
     stage_sum = {k: 0.0 for k in cj.STAGE_NAMES}
-    for _ in range(stage_frames):
-        enc.encode_into(luma_ptr, chroma_ptr, WIDTH, WIDTH, pinned.ptr, enc.capacity)
-        for k, v in enc.timings().items():
-            stage_sum[k] += v
+    for k in range(stage_frames):
+        enc.encode_into(*inputs[k % len(inputs)], pinned.ptr, enc.capacity)
+        for name, v in enc.timings().items():
+            stage_sum[name] += v
     enc.set_profiling(False)
 
     # The same steps into a pageable numpy buffer, returned as Python bytes.
     lib.cjx_device_synchronize()
     t1 = time.perf_counter()
-    for _ in range(args.steps):
-        paged = enc.encode_device(luma_ptr, chroma_ptr, WIDTH, WIDTH)
+    for k in range(args.steps):
+        paged = enc.encode_device(*inputs[k % len(inputs)])
     lib.cjx_device_synchronize()
     pageable_fps = args.steps / (time.perf_counter() - t1)
-    if paged != data:
+    if enc.encode_device(*inputs[last]) != data:
         raise RuntimeError("pinned and pageable outputs differ")
 
     ms_per_step = 1e3 * elapsed / args.steps
@@ -208,6 +270,16 @@ def main() -> int:
     region_ms = region_sum / region_frames
     nv12_bytes = 1.5 * WIDTH * HEIGHT
     achieved = nv12_bytes / (region_ms * 1e-3) / 1e9
+    mean_bytes = float(np.mean(sizes))
+    valu = region_valu_instructions()
+    limiter = None
+    if valu is not None:
+        floor_us = valu / (CUS * SCLK_GHZ * 1e3)  # one wave64 VALU instruction per CU per clock
+        limiter = {"kind": "valu-issue", "valu_insts_per_launch": valu,
+                   "issue_floor_us": round(floor_us, 1),
+                   "frac": round(floor_us / (region_ms * 1e3), 4),
+                   "source": "profiles/sq_counters_4k_d1.txt (SQ_INSTS_VALU over the device; "
+                             "256 CUs x 2.4 GHz, one wave64 VALU instruction per CU per clock)"}
     out = {
         "metric": "4K NV12 frames/s at d=1.0 on 1 MI355X",
         "value": round(fps, 3),
@@ -222,18 +294,19 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "3840x2160 NV12 8-bit seeded synthetic frame, distance 1.0, plain HIP "
-                        "VarDCT (DCT8/16/32 RD selection, CfL, adaptive quant, 32-cluster prefix "
-                        "coding), NV12 resident in HBM, .jxl bytes to host",
+            "workload": "3840x2160 NV12 8-bit seeded synthetic frames (seeds 0-7, 3 device copies each, "
+                        "cycled), distance 1.0, plain HIP VarDCT (DCT8/16/32 RD selection, CfL, adaptive "
+                        "quant, 32-cluster prefix coding), NV12 resident in HBM, .jxl bytes to host",
             "path": "plain",
             "output": "caller-owned page-locked host buffer",
+            "single_frame_fps": round(single_frame_fps, 3),
             "pageable_output_fps": round(pageable_fps, 3),
             "python_loop_fps": round(python_loop_fps, 3),
-            "bytes_per_frame": len(data),
-            "bpp": round(8.0 * len(data) / (WIDTH * HEIGHT), 4),
+            "bytes_per_frame": round(mean_bytes, 1),
+            "bpp": round(8.0 * mean_bytes / (WIDTH * HEIGHT), 4),
             "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
             "stage_ms_frames": stage_frames,
-            "frame_alg_bytes_frac": round((nv12_bytes + len(data)) * fps / (HBM_PEAK_GBS * 1e9), 6),
+            "frame_alg_bytes_frac": round((nv12_bytes + mean_bytes) * fps / (HBM_PEAK_GBS * 1e9), 6),
         },
         "roofline": {
             "bound": "hbm",
@@ -245,14 +318,16 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 6),
             "traffic": region_traffic_bytes(),
+            "limiter": limiter,
         },
     }
     if not args.no_mixed_batch:
         out["mixed_batch"] = mixed_batch(cj, synthetic_nv12)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(luma, chroma)
+        out["cpu_baseline"] = cpu_baseline(host_frames)
     pinned.close()
-    dev.close()
+    for d in devs:
+        d.close()
     enc.close()
     print(json.dumps(out), flush=True)
     return 0

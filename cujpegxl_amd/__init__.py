@@ -21,7 +21,8 @@ LIB_DIR = PKG / "lib"
 PUBLIC_LIB = LIB_DIR / "libcujpegxl.so.1"
 STAGES_LIB = LIB_DIR / "libcujpegxl_stages.so"
 
-API_VERSION = 4
+API_VERSION = 3  # cujpegxl.h's CUJPEGXL_API_VERSION, the reference's value
+EXT_VERSION = 1  # CUJPEGXL_EXT_VERSION: host-input and asynchronous entry points
 
 OK, NOT_IMPLEMENTED, INVALID_ARGUMENT, UNSUPPORTED_RESOLUTION, NO_DEVICE, INTERNAL_ERROR, \
     BUFFER_TOO_SMALL = range(7)
@@ -59,6 +60,7 @@ _stages = None
 
 def _bind_public(L: C.CDLL) -> C.CDLL:
     L.cujpegxl_api_version.restype = C.c_uint32
+    L.cujpegxl_ext_version.restype = C.c_uint32
     L.cujpegxl_status_string.restype = C.c_char_p
     L.cujpegxl_status_string.argtypes = [C.c_int]
     L.cujpegxl_backend_string.restype = C.c_char_p
@@ -104,6 +106,8 @@ def stages_lib() -> C.CDLL:
         L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
         L.cjx_stage_encode_repeat.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output), C.c_int,
                                               C.POINTER(C.c_double)]
+        L.cjx_stage_encode_cycle.argtypes = [P, C.POINTER(Nv12Input), C.c_int, C.POINTER(Output), C.c_int,
+                                             C.POINTER(C.c_double)]
         L.cjx_stage_encode_dct8_uniform.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_timings.argtypes = [P, C.POINTER(C.c_float), C.c_int]
         L.cjx_device_alloc.restype = P
@@ -124,6 +128,8 @@ def stages_lib() -> C.CDLL:
         L.cjx_copy_to_device.argtypes = [P, P, C.c_size_t]
         L.cjx_copy_to_host.argtypes = [P, P, C.c_size_t]
         L.cjx_device_copy.argtypes = [P, P, C.c_size_t]
+        L.cjx_stage_fail_allocation_after.argtypes = [C.c_int]
+        L.cjx_stage_fail_allocation_after.restype = None
         L.cjx_host_table.restype = C.c_size_t
         L.cjx_host_table.argtypes = [C.c_int, P, C.c_size_t]
         _stages = L
@@ -136,6 +142,10 @@ def status_string(status: int) -> str:
 
 def api_version() -> int:
     return int(public_lib().cujpegxl_api_version())
+
+
+def ext_version() -> int:
+    return int(public_lib().cujpegxl_ext_version())
 
 
 def query_backend(device_ordinal: int = 0) -> str:
@@ -303,6 +313,19 @@ class Encoder:
         s = stages_lib().cjx_stage_encode_repeat(self.handle, C.byref(inp), C.byref(out), count, C.byref(sec))
         if s != OK:
             raise RuntimeError(f"encode_repeat failed: {status_string(s)}")
+        return sec.value, out.size
+
+    def encode_cycle(self, inputs: list[tuple[int, int, int, int]], out_ptr: int, capacity: int,
+                     count: int) -> tuple[float, int]:
+        """`count` synchronous encodes cycling through `inputs` (luma, chroma,
+        luma pitch, chroma pitch) in native code (stage library); returns the
+        loop's wall seconds and the last file's size."""
+        arr = (Nv12Input * len(inputs))(*(Nv12Input(*i) for i in inputs))
+        out = Output(out_ptr, 0, capacity)
+        sec = C.c_double(0.0)
+        s = stages_lib().cjx_stage_encode_cycle(self.handle, arr, len(inputs), C.byref(out), count, C.byref(sec))
+        if s != OK:
+            raise RuntimeError(f"encode_cycle failed: {status_string(s)}")
         return sec.value, out.size
 
     def encode_dct8_uniform(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int,

@@ -11,13 +11,19 @@ namespace cjx {
 constexpr int kHdrWords = 64;      // 2048 bits: bound on one code header
 constexpr int kBlobWords = 128;    // per DC-group header blob
 constexpr int kAcGlobalWords = 8192;
-constexpr int kHeadWords = 1024;
 constexpr int kMaxAcGroups = 32 * 64;  // at most 32 DC groups of 64 AC groups
+// Codestream head: the constant headers (under 32 words) and the TOC, whose
+// 2 + 32 + 2048 entries take at most 32 bits each at the largest accepted
+// frame (2 + 30 for sections of 4,211,712 bytes or more).
+constexpr int kHeadConstWords = 256;
+constexpr int kHeadWords = ((kHeadConstWords * 32 + (2 + 32 + kMaxAcGroups) * 32 + 8 + 8191) / 8192) * 256;
+static_assert(kHeadWords % 256 == 0 && kHeadWords * 32 >= kHeadConstWords * 32 + (2 + 32 + kMaxAcGroups) * 32 + 8,
+              "the head holds the worst-case TOC");
 
 struct Constants {
     // Bit strings precomputed on the host (see encoder.cc).
     std::uint32_t head_bits, dc_global_bits, ac_prefix_bits, dc_pre_bits, dc_mid_bits;
-    std::uint32_t head_words[kHeadWords / 4];
+    std::uint32_t head_words[kHeadConstWords];
     std::uint32_t dc_global_words[4];
     std::uint32_t ac_prefix_words[kAcGlobalWords / 2];
     std::uint32_t dc_pre_words[64];

@@ -34,6 +34,10 @@ uint32_t cujpegxl_api_version(void) {
     return CUJPEGXL_API_VERSION;
 }
 
+uint32_t cujpegxl_ext_version(void) {
+    return CUJPEGXL_EXT_VERSION;
+}
+
 const char* cujpegxl_status_string(cujpegxl_status status) {
     switch (status) {
         case CUJPEGXL_OK: return "ok";
@@ -106,7 +110,7 @@ cujpegxl_status cujpegxl_encoder_create(const cujpegxl_config* config, cujpegxl_
 }
 
 cujpegxl_status cujpegxl_encoder_configure(cujpegxl_encoder* encoder, const cujpegxl_config* config) {
-    if (encoder == nullptr || config == nullptr) {
+    if (encoder == nullptr || config == nullptr || encoder->engine.pending()) {
         return CUJPEGXL_INVALID_ARGUMENT;
     }
     const cujpegxl_status s{validate_config(*config)};
@@ -126,7 +130,7 @@ cujpegxl_status cujpegxl_encoder_configure(cujpegxl_encoder* encoder, const cujp
 cujpegxl_status cujpegxl_encoder_encode(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
                                         cujpegxl_output* output) {
     if (encoder == nullptr || input == nullptr || output == nullptr || output->data == nullptr ||
-        input->luma == 0 || input->chroma == 0) {
+        input->luma == 0 || input->chroma == 0 || encoder->engine.pending()) {
         return CUJPEGXL_INVALID_ARGUMENT;
     }
     const cujpegxl_status s{validate_config(encoder->config)};

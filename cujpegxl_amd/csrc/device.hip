@@ -78,7 +78,7 @@ std::uint32_t blob_words_per_group() { return 2 * kBlobWords; }
 
 bool constants_fit(std::uint32_t head_bits, std::uint32_t ac_prefix_bits, std::uint32_t dc_pre_bits,
                    std::uint32_t dc_mid_bits) {
-    return head_bits <= 32u * (kHeadWords / 4) && ac_prefix_bits <= 32u * (kAcGlobalWords / 2) &&
+    return head_bits <= 32u * kHeadConstWords && ac_prefix_bits <= 32u * (kAcGlobalWords / 2) &&
            dc_pre_bits <= 32u * 64 && dc_mid_bits <= 32u * 64;
 }
 

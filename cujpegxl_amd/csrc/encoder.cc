@@ -5,6 +5,7 @@
 #include "encoder.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -184,7 +185,25 @@ void modular_prefix(HostBits& w, int predictor) {
 std::once_flag g_tables_once[64];
 bool g_tables_ok[64];
 
+// Test hook: the allocation that brings the countdown from 1 to 0 fails as
+// if the device were out of memory (0: no failure pending).
+std::atomic<int> g_fail_alloc_countdown{0};
+
+bool injected_alloc_failure() {
+    int n{g_fail_alloc_countdown.load()};
+    while (n > 0) {
+        if (g_fail_alloc_countdown.compare_exchange_weak(n, n - 1)) {
+            return n == 1;
+        }
+    }
+    return false;
+}
+
 }  // namespace
+
+void fail_allocation_after(int n) {
+    g_fail_alloc_countdown.store(n < 0 ? 0 : n);
+}
 
 bool device_exists(int ordinal) {
     int count{0};
@@ -264,6 +283,34 @@ void Encoder::release() {
         (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
     }
+    null_buffers();
+    total_dev_ = nullptr;
+    configured_ = false;
+}
+
+// Every device buffer member back to null (their memory was freed with
+// allocations_), so nothing can reach freed memory after a failed configure.
+void Encoder::null_buffers() {
+    coeffs_ = nullptr;
+    acs_ = nullptr;
+    qf_ = nullptr;
+    cfl_part_ = nullptr;
+    ytox_ = ytob_ = nullptr;
+    ac_ = nullptr;
+    dc_ = nullptr;
+    nz_grid_ = nullptr;
+    nz_count_ = nullptr;
+    hists_ = fb_count_ = nullptr;
+    codes_ = constants_ = blobs_ = layout_ = nullptr;
+    blob_words_ = nullptr;
+    tokens_ = nullptr;
+    item_tokens_ = nullptr;
+    item_list_ = item_count_ = hist_partial_ = ac_staging_ = dc_staging_ = nullptr;
+    ac_part_bits_ = ac_group_bits_ = chunk_bits_ = dc_group_bits_ = ac_base_ = dc_base_ = nullptr;
+    ac_group_bytes_ = dc_group_bytes_ = nullptr;
+    out_ = nullptr;
+    out_cap_ = 0;
+    const_gs_ = const_qdc_ = 0;
 }
 
 bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distance, int device) {
@@ -274,7 +321,10 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
     if (geo.num_dc > 32) {
         return false;  // per-group tables are sized for up to 32 DC groups
     }
-    const bool same_shape{device == device_ && width == geo_.width && height == geo_.height && stream_ != nullptr};
+    const bool same_shape{configured_ && device == device_ && width == geo_.width && height == geo_.height};
+    if (pending_) {
+        return false;  // a frame queued by encode_async is still owed to finish()
+    }
     distance_ = distance;
     prm_ = make_frame_params(distance);
     if (same_shape) {
@@ -283,6 +333,20 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
     release();
     device_ = device;
     geo_ = geo;
+    if (!setup(device)) {
+        // Leave nothing half-built: no buffers, no shape a later configure
+        // could mistake for its own, and every encode refused until a
+        // configure succeeds.
+        release();
+        device_ = -1;
+        geo_ = Geometry{};
+        return false;
+    }
+    configured_ = true;
+    return true;
+}
+
+bool Encoder::setup(int device) {
     if (hipSetDevice(device) != hipSuccess) {
         return false;
     }
@@ -332,7 +396,7 @@ bool Encoder::alloc() {
     bool ok{true};
     auto get = [&](auto*& p, std::size_t bytes) {
         void* q{nullptr};
-        if (ok && hipMalloc(&q, bytes < 256 ? 256 : bytes) == hipSuccess) {
+        if (ok && !injected_alloc_failure() && hipMalloc(&q, bytes < 256 ? 256 : bytes) == hipSuccess) {
             allocations_.push_back(q);
             p = static_cast<std::remove_reference_t<decltype(p)>>(q);
         } else {
@@ -627,12 +691,16 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
 }
 
 bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
-                          std::size_t chroma_pitch, hipStream_t caller, std::uint8_t* dst, std::size_t cap) {
-    if (pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+                          std::size_t chroma_pitch, hipStream_t caller, bool release_caller, std::uint8_t* dst,
+                          std::size_t cap) {
+    if (!configured_ || pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
         return false;
     }
-    if (caller_set_ && (hipEventRecord(ev_caller_, caller) != hipSuccess ||
-                        hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess)) {
+    // The frame starts after the work already queued on `caller` (the null
+    // stream for the synchronous entry points: a caller that filled the
+    // planes there, as callers of the reference did, is ordered before the
+    // region kernel reads them).
+    if (hipEventRecord(ev_caller_, caller) != hipSuccess || hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess) {
         return false;
     }
     // With profiling on, the region kernel's own start and end times open the
@@ -647,8 +715,8 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
                   timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr, region_end);
     // The planes are read by the region kernel only: later work on the
     // caller's stream may overwrite them once it has finished.
-    if (caller_set_ && (hipEventRecord(ev_input_free_, stream_) != hipSuccess ||
-                        hipStreamWaitEvent(caller, ev_input_free_, 0) != hipSuccess)) {
+    if (release_caller && (hipEventRecord(ev_input_free_, stream_) != hipSuccess ||
+                           hipStreamWaitEvent(caller, ev_input_free_, 0) != hipSuccess)) {
         return false;
     }
     // The quantizer lists the non-empty items for the tokenizer.
@@ -664,8 +732,7 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
 
 bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                           std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
-    caller_set_ = false;
-    if (!submit_nv12(luma, luma_pitch, chroma, chroma_pitch, nullptr, out, cap)) {
+    if (!submit_nv12(luma, luma_pitch, chroma, chroma_pitch, nullptr, false, out, cap)) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
         (void)hipStreamSynchronize(dc_stream_);
@@ -676,7 +743,10 @@ bool Encoder::encode_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
 
 bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                                   std::size_t chroma_pitch, std::uint8_t* out, std::size_t cap, std::size_t* size) {
-    if (pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+    if (!configured_ || pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+        return false;
+    }
+    if (hipEventRecord(ev_caller_, nullptr) != hipSuccess || hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess) {
         return false;
     }
     mark(Stage::kRegion);
@@ -695,9 +765,7 @@ bool Encoder::encode_dct8_uniform(const std::uint8_t* luma, std::size_t luma_pit
 
 bool Encoder::encode_async(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                            std::size_t chroma_pitch, hipStream_t caller) {
-    caller_set_ = true;
-    const bool ok{submit_nv12(luma, luma_pitch, chroma, chroma_pitch, caller, nullptr, 0)};
-    caller_set_ = false;
+    const bool ok{submit_nv12(luma, luma_pitch, chroma, chroma_pitch, caller, true, nullptr, 0)};
     if (!ok && pending_) {
         pending_ = false;
         (void)hipStreamSynchronize(stream_);
@@ -711,7 +779,7 @@ bool Encoder::encode_async(const std::uint8_t* luma, std::size_t luma_pitch, con
 // DMA of the other.  Pinned planes are copied by one 2D DMA each.
 bool Encoder::upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                           std::size_t chroma_pitch) {
-    if (pending_ || hipSetDevice(device_) != hipSuccess) {
+    if (!configured_ || pending_ || hipSetDevice(device_) != hipSuccess) {
         return false;
     }
     const std::size_t w{geo_.width}, h{geo_.height};
@@ -776,7 +844,7 @@ bool Encoder::encode_coefficients(const std::int16_t* ac, const std::int32_t* dc
                                   const std::int32_t* qf, const std::int8_t* ytox, const std::int8_t* ytob,
                                   std::uint32_t global_scale, std::uint32_t quant_dc, std::uint8_t* out,
                                   std::size_t cap, std::size_t* size) {
-    if (hipSetDevice(device_) != hipSuccess || !build_constants(global_scale, quant_dc)) {
+    if (!configured_ || pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(global_scale, quant_dc)) {
         return false;
     }
     const std::size_t nblk{geo_.nblocks};
@@ -905,7 +973,7 @@ std::size_t Encoder::buffer_bytes(Buffer which) const {
 }
 
 bool Encoder::fetch(Buffer which, void* host, std::size_t bytes) {
-    if (bytes > buffer_bytes(which) || hipSetDevice(device_) != hipSuccess) {
+    if (!configured_ || bytes > buffer_bytes(which) || hipSetDevice(device_) != hipSuccess) {
         return false;
     }
     const void* src{nullptr};

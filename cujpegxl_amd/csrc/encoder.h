@@ -53,6 +53,8 @@ public:
                       std::size_t chroma_pitch, hipStream_t caller);
     bool finish(std::uint8_t* out, std::size_t cap, std::size_t* size);
     bool pending() const { return pending_; }
+    // True once a configure has succeeded (and no later one has failed).
+    bool configured() const { return configured_; }
 
     // NV12 host planes: uploaded into an encoder-owned device frame, then
     // encoded as encode_nv12 does.
@@ -87,8 +89,10 @@ public:
     static std::size_t max_output_size(std::uint32_t width, std::uint32_t height);
 
 private:
+    bool setup(int device);
     bool alloc();
     void release();
+    void null_buffers();
     bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
     // `fork_bound`: the quantizer's launch has bound `ev_fork_` already;
     // `listed`: it also listed the non-empty items for the tokenizer.
@@ -96,8 +100,11 @@ private:
                      bool size_event = true);
     std::uint8_t* mapped_output(std::uint8_t* dst) const;
     Clears entropy_clears() const;
+    // Queues a frame after the work on `caller` (null: the null stream);
+    // `release_caller`: later work on `caller` waits for the region kernel.
     bool submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
-                     std::size_t chroma_pitch, hipStream_t caller, std::uint8_t* dst, std::size_t cap);
+                     std::size_t chroma_pitch, hipStream_t caller, bool release_caller, std::uint8_t* dst,
+                     std::size_t cap);
     bool queue_readback(std::uint8_t* direct);
     bool stage_chunk(int slot, std::size_t n);
     bool upload_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
@@ -107,6 +114,7 @@ private:
     static bool spin_on(hipEvent_t e);
 
     int device_{-1};
+    bool configured_{false};
     float distance_{1.0f};
     Geometry geo_{};
     FrameParams prm_{};
@@ -120,7 +128,6 @@ private:
     bool size_bound_{true};  // ev_size_ is bound to this frame's layout kernel
     hipStream_t tail_{nullptr};  // layout, assembly and readback: the DC stream, after both chains
     hipEvent_t ev_caller_{nullptr}, ev_input_free_{nullptr};  // ordering against a caller's stream
-    bool caller_set_{false};
     bool pending_{false};            // a frame's readback is queued, finish() not yet called
     std::size_t staged_{0};          // output bytes queued into the staging chunks
     std::size_t stage_len_[2]{0, 0};
@@ -178,5 +185,7 @@ private:
 
 bool device_exists(int ordinal);
 bool device_has_matrix_cores(int ordinal);
+// Test hook: the n-th device allocation from now fails (0 cancels).
+void fail_allocation_after(int n);
 
 }  // namespace cjx

@@ -15,6 +15,7 @@ CUJPEGXL_1 {
 
 CUJPEGXL_2 {
     global:
+        cujpegxl_ext_version;
         cujpegxl_encoder_encode_host;
         cujpegxl_encoder_encode_async;
         cujpegxl_encoder_finish;

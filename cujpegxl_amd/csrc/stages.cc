@@ -96,6 +96,22 @@ int cjx_stage_encode_repeat(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, 
     return CUJPEGXL_OK;
 }
 
+int cjx_stage_encode_cycle(cujpegxl_encoder* e, const cujpegxl_nv12_input* inputs, int n_inputs,
+                           cujpegxl_output* out, int count, double* seconds) {
+    if (inputs == nullptr || n_inputs <= 0 || count <= 0 || seconds == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const auto t0{std::chrono::steady_clock::now()};
+    for (int i{0}; i < count; ++i) {
+        const int s{cujpegxl_encoder_encode(e, &inputs[i % n_inputs], out)};
+        if (s != CUJPEGXL_OK) {
+            return s;
+        }
+    }
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return CUJPEGXL_OK;
+}
+
 int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cujpegxl_output* out) {
     if (e == nullptr || in == nullptr || out == nullptr || out->data == nullptr || in->luma == 0 ||
         in->chroma == 0 || in->luma_pitch < e->config.width || in->chroma_pitch < e->config.width) {
@@ -141,6 +157,10 @@ int cjx_stage_build_codes(int32_t device_ordinal, const uint32_t* hists, uint32_
     }
     return cjx::build_codes_host(hists, count, alpha, depth, bits, hdr_words, hdr_bits) ? CUJPEGXL_OK
                                                                                        : CUJPEGXL_INTERNAL_ERROR;
+}
+
+void cjx_stage_fail_allocation_after(int n) {
+    cjx::fail_allocation_after(n);
 }
 
 void* cjx_device_alloc(size_t bytes) {

@@ -16,9 +16,11 @@
 extern "C" {
 #endif
 
-/* cujpegxl.h:24 defines version 3; version 4 is that ABI unchanged plus the
- * host-input and asynchronous entry points at the end of this header. */
-#define CUJPEGXL_API_VERSION 4u
+/* cujpegxl.h:24: the reference's ABI version, unchanged.  The host-input
+ * and asynchronous entry points at the end of this header are extensions,
+ * versioned on their own (CUJPEGXL_EXT_VERSION, cujpegxl_ext_version). */
+#define CUJPEGXL_API_VERSION 3u
+#define CUJPEGXL_EXT_VERSION 1u
 
 /* cujpegxl.h:26-34; values are ABI. */
 typedef enum cujpegxl_status {
@@ -86,7 +88,12 @@ cujpegxl_status cujpegxl_encoder_encode(cujpegxl_encoder* encoder, const cujpegx
 /* cujpegxl.h:82 */
 void cujpegxl_encoder_destroy(cujpegxl_encoder* encoder);
 
-/* ---- API version 4 additions (no counterpart in the reference) ---------- */
+/* ---- Extensions, version 1 (no counterpart in the reference) ------------ */
+
+/* CUJPEGXL_EXT_VERSION of the library: 1 for the three entry points below.
+ * Exported under the version node CUJPEGXL_2; a caller that finds this
+ * symbol may use them. */
+uint32_t cujpegxl_ext_version(void);
 
 /* NV12 (BT.709 full range) in host memory: plane addresses and pitches in
  * bytes (pitches >= width). */

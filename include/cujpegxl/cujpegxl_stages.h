@@ -76,12 +76,26 @@ int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
 int cjx_stage_encode_repeat(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* input,
                             cujpegxl_output* output, int count, double* seconds);
 
+/* `count` synchronous encodes cycling through `n_inputs` frames (encode i
+ * reads inputs[i % n_inputs]) into the same output, timed like
+ * cjx_stage_encode_repeat: the reference's benchmark alternates its inputs
+ * so that cache effects do not follow one frame
+ * (tools/benchmark/encode_bench.cc:512-518).  Returns the first failing
+ * call's status. */
+int cjx_stage_encode_cycle(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* inputs, int n_inputs,
+                           cujpegxl_output* output, int count, double* seconds);
+
 /* Prefix codes of `count` (<= 96) histograms of 256 uint32 bins, built on
  * device `device_ordinal` by the encoder's code kernel (prefix_code.cc:36-397
  * semantics): per histogram the alphabet size, 256 depths, 256 LSB-first
  * codes, the header (64 words, LSB-first) and its length in bits. */
 int cjx_stage_build_codes(int32_t device_ordinal, const uint32_t* hists, uint32_t count, uint32_t* alpha,
                           uint8_t* depth, uint16_t* bits, uint32_t* hdr_words, uint32_t* hdr_bits);
+
+/* Test hook for the failure paths: the n-th device allocation an encoder
+ * makes from now on (create, configure) fails as if the device were out of
+ * memory; 0 cancels.  No counterpart in the reference. */
+void cjx_stage_fail_allocation_after(int n);
 
 /* Device memory for harnesses that have no other allocator. */
 void* cjx_device_alloc(size_t bytes);

@@ -35,6 +35,11 @@ def build(with_ref: bool = False) -> None:
     subprocess.run(["make", "-s", "-C", str(HERE), *target], check=True)
 
 
+def set_threads(n: int) -> None:
+    """Threads for the oracle's front-end loops (1 by default)."""
+    lib().jo_set_threads(int(n))
+
+
 def ptr(a: np.ndarray | None) -> int | None:
     return None if a is None else a.ctypes.data
 
@@ -117,6 +122,8 @@ def lib() -> C.CDLL:
         L.jo_natural_order.argtypes = [_U32, _P]
         L.jo_dct_basis.argtypes = [_U32, _P]
         L.jo_set_dequant.argtypes = [_P, _P, _P]
+        L.jo_set_threads.argtypes = [C.c_int]
+        L.jo_set_threads.restype = None
         L.jo_huffman.argtypes = [_P, _U32, _P, _P, _P, _SZ, C.POINTER(_SZ)]
         L.jo_best_context_map.argtypes = [_P, _U32, _U32, _P, _SZ, C.POINTER(_SZ),
                                           C.POINTER(C.c_int)]

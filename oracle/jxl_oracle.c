@@ -24,6 +24,15 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Threads for the per-row, per-tile, per-region and per-block loops (their
+ * iterations write disjoint outputs, so any count gives the same bytes).
+ * Default 1; the CPU baseline's all-cores leg raises it. */
+static int g_threads = 1;
+
+void jo_set_threads(int n) {
+    g_threads = n < 1 ? 1 : n;
+}
+
 /* ======================================================================== */
 /* Numeric primitives (shared definition with the HIP kernels).              */
 /* ======================================================================== */
@@ -394,6 +403,7 @@ void jo_chroma_upsample(const uint8_t* chroma, size_t cp, uint32_t w, uint32_t h
 void jo_nv12_to_xyb(const uint8_t* luma, size_t lp, const uint8_t* chroma, size_t cp, uint32_t w,
                     uint32_t h, float* xyb) {
     const size_t plane = (size_t)w * h;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (uint32_t y = 0; y < h; ++y) {
         for (uint32_t x = 0; x < w; ++x) {
             float cbu, cru;
@@ -453,6 +463,7 @@ void jo_gaborish(const float* xyb, uint32_t w, uint32_t h, float* out) {
     const size_t plane = (size_t)w * h;
     for (int c = 0; c < 3; ++c) {
         const float* p = xyb + c * plane;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
         for (uint32_t y = 0; y < h; ++y) {
             for (uint32_t x = 0; x < w; ++x) {
 #define AT(dx, dy)                                                                            \
@@ -593,8 +604,9 @@ void jo_adaptive_quant(const float* xyb, uint32_t w, uint32_t h, float distance,
     float km[4], mul_pb, add_pb;
     aq_constants(distance, km, &mul_pb, &add_pb);
     enum { PAD = 34 };
-    float ty[PAD * PAD], tx_[PAD * PAD], tb[PAD * PAD], pre[64];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (uint32_t tby = 0; tby < (bh + 3) / 4; ++tby) {
+        float ty[PAD * PAD], tx_[PAD * PAD], tb[PAD * PAD], pre[64];
         for (uint32_t tbx = 0; tbx < (bw + 3) / 4; ++tbx) {
             const long px0 = (long)tbx * 32, py0 = (long)tby * 32;
             for (int ly = 0; ly < PAD; ++ly) {
@@ -826,6 +838,8 @@ void jo_select_transforms(const float* y, uint32_t w, uint32_t h, float distance
     jo_calibrate(distance, &cal);
     const float qgsf = (float)cal.raw_quant_field * cal.global_scale_float;
     const float inf = 1.0e30f, eps = 1.0e-6f;
+    ensure_basis();
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
     for (uint32_t rby = 0; rby < bh; rby += 4) {
         for (uint32_t rbx = 0; rbx < bw; rbx += 4) {
             float c8[16], c16[4], c32;
@@ -904,8 +918,10 @@ static inline size_t covered_slot(int n, uint32_t bx, uint32_t by, uint32_t bw, 
 void jo_forward_dct(const float* planes, uint32_t w, uint32_t h, const int8_t* acs, float* coef) {
     const uint32_t bw = w / 8, bh = h / 8;
     const size_t plane = (size_t)w * h, cplane = (size_t)bw * bh * 64;
-    float blk[1024];
+    ensure_basis();
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 4)
     for (uint32_t by = 0; by < bh; ++by) {
+        float blk[1024];
         for (uint32_t bx = 0; bx < bw; ++bx) {
             const int n = acs ? acs[(size_t)by * bw + bx] : 8;
             if (n == 0) {
@@ -2516,6 +2532,7 @@ size_t jo_encode_nv12(const uint8_t* luma, size_t lp, const uint8_t* chroma, siz
     jo_gaborish(xyb, w, h, sharp);
     jo_select_transforms(sharp + plane, w, h, distance, acs);
     jo_forward_dct(sharp, w, h, acs, coef);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (size_t i = 0; i < 3 * cplane; ++i) {
         c16[i] = jo_f32_to_f16(coef[i]);
     }

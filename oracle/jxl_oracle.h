@@ -43,6 +43,9 @@ float jo_f16_to_f32(uint16_t h);
 /* ---- tables -------------------------------------------------------------- */
 /* Dequant matrices [3][n*n] for n = 8, 16, 32 (libjxl 0.11.2 defaults). */
 void jo_set_dequant(const float* w8, const float* w16, const float* w32);
+/* Threads for the front end's per-row/tile/region/block loops (default 1);
+ * outputs are the same bytes for every count. */
+void jo_set_threads(int n);
 void jo_calibrate(float distance, jo_cal* out);
 void jo_natural_order(uint32_t n, uint32_t* out);
 void jo_dct_basis(uint32_t n, float* out);

@@ -195,6 +195,15 @@ def main() -> None:
     i = np.arange(88 * 72)
     y = (0.3 + 0.2 * ((i * 7) % 13) / 13.0).astype(np.float32)
     v["sel_edge_88x72"] = sel(y, 88, 72)
+    # ... and on the sharpened Y of seeded synthetic frames (the oracle's
+    # NV12 -> XYB and gaborish, themselves pinned by closed forms).
+    from cujpegxl_amd.synth import synthetic_nv12
+
+    for w, h, seed, d in ((264, 200, 3, 1.0), (264, 200, 3, 3.0), (776, 456, 21, 1.0), (776, 456, 22, 3.0),
+                          (1280, 720, 23, 1.0)):
+        luma, chroma = synthetic_nv12(w, h, seed)
+        yp = np.ascontiguousarray(oracle.gaborish(oracle.nv12_to_xyb(luma, chroma, w, h))[1])
+        v[f"sel_synth_{w}x{h}_s{seed}_d{d:g}"] = sel(yp, w, h, d)
 
     # CfL map quantization (cfl.h) and LLF -> DC (vardct_layout.h).
     sums = rng.normal(0, 1, (64, 3))

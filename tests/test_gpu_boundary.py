@@ -1,6 +1,7 @@
-"""GPU: the API version 4 entry points (host-memory input, asynchronous
-encode) and page-locked outputs deliver the same file as the synchronous
-device-input encode, whose bytes equal the oracle's."""
+"""GPU: the extension entry points (host-memory input, asynchronous encode)
+and page-locked outputs deliver the same file as the synchronous
+device-input encode, whose bytes equal the oracle's; the synchronous encode
+is ordered after the caller's work on the null stream."""
 
 from __future__ import annotations
 
@@ -107,6 +108,13 @@ def test_async_encode_then_finish(frame):
     host = np.ascontiguousarray(nv12)
     hin = cj.Nv12HostInput(host.ctypes.data, host.ctypes.data + W * H, W, W)
     assert lib.cujpegxl_encoder_encode_host(enc.handle, C.byref(hin), C.byref(out)) == cj.INVALID_ARGUMENT
+    # Neither the synchronous encode nor a reconfiguration may drop the
+    # pending frame: both are refused, and finish still delivers it.
+    din = cj.Nv12Input(dev.ptr, dev.ptr + W * H, W, W)
+    assert lib.cujpegxl_encoder_encode(enc.handle, C.byref(din), C.byref(out)) == cj.INVALID_ARGUMENT
+    for w2, h2 in ((W, H), (W * 2, H)):
+        cfg = cj.Config(w2, h2, D, 0)
+        assert lib.cujpegxl_encoder_configure(enc.handle, C.byref(cfg)) == cj.INVALID_ARGUMENT
     assert enc.finish() == want
     assert enc.encode_async(dev.ptr, dev.ptr + W * H, W, W, 0) == cj.OK
     small = cj.Output(enc.out.ctypes.data, 0, 32)
@@ -164,4 +172,29 @@ def test_native_repeat_loop_matches_oracle(frame):
         cj.BUFFER_TOO_SMALL
     pin.close()
     dev.close()
+    enc.close()
+
+
+def test_sync_encode_is_ordered_after_the_null_stream(jo, frame):
+    """A caller that clears and fills the planes on the null stream, as
+    callers of the reference's default-stream encoder did, then calls the
+    synchronous encode: the frame reads the filled planes.  Two frames
+    alternate so a read of stale planes would show."""
+    _luma, _chroma, nv12, want = frame
+    say("null-stream fill then cujpegxl_encoder_encode, two frames alternating")
+    luma2, chroma2 = synthetic_nv12(W, H, SEED + 1)
+    nv12b = np.concatenate([luma2.ravel(), chroma2.ravel()])
+    want2 = jo.encode_nv12(luma2, chroma2, W, H, D)
+    L = cj.stages_lib()
+    enc = cj.Encoder(W, H, D)
+    srcs = [cj.DeviceBuffer.from_array(nv12), cj.DeviceBuffer.from_array(nv12b)]
+    planes = cj.DeviceBuffer(nv12.size)
+    for i in range(4):
+        src, expect = srcs[i % 2], (want, want2)[i % 2]
+        assert L.cjx_fill_async(planes.ptr, 0, nv12.size, 0) == cj.OK
+        assert L.cjx_copy_async(planes.ptr, src.ptr, nv12.size, 0) == cj.OK
+        assert enc.encode_device(planes.ptr, planes.ptr + W * H, W, W) == expect, i
+    assert L.cjx_device_synchronize() == cj.OK
+    for b in srcs + [planes]:
+        b.close()
     enc.close()

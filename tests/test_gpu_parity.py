@@ -169,12 +169,15 @@ def test_c_abi_behaviour():
     assert enc.encode_device(dl.ptr, dcb.ptr, 640, 640) == a
 
 
-def test_4k_frame_matches_oracle(jo):
-    """The benchmark workload (3840x2160, d = 1.0) byte for byte."""
-    say("4K d=1.0")
-    luma, chroma = synthetic_nv12(3840, 2160, 0)
-    enc, data, _buf = device_encode(3840, 2160, 1.0, luma, chroma)
-    want = jo.encode_nv12(luma, chroma, 3840, 2160, 1.0)
+@pytest.mark.parametrize("d,seed", [(1.0, 0), (3.0, 1)])
+def test_4k_frame_matches_oracle(jo, d, seed):
+    """The plain 4K workloads (3840x2160 at d = 1.0 and 3.0): the file equals
+    the oracle's byte for byte, and a second encode of the same planes
+    through the same encoder gives the same file."""
+    say(f"4K d={d} seed={seed}")
+    luma, chroma = synthetic_nv12(3840, 2160, seed)
+    enc, data, _buf = device_encode(3840, 2160, d, luma, chroma)
+    want = jo.encode_nv12(luma, chroma, 3840, 2160, d)
     assert data == want, f"{len(data)} vs {len(want)}"
     again = enc.encode_device(_buf.ptr, _buf.ptr + 3840 * 2160, 3840, 3840)
     assert again == data

@@ -29,9 +29,9 @@ def exported(lib: pathlib.Path) -> set[str]:
 
 def test_public_library_exports_exactly_the_abi():
     want = declared("cujpegxl.h", "cujpegxl_")
-    # cujpegxl.h:66-82, plus the host-input and asynchronous entry points of
-    # API version 4.
-    assert len(want) == 12
+    # cujpegxl.h:66-82, plus the extension version query and the host-input
+    # and asynchronous entry points.
+    assert len(want) == 13
     assert exported(cj.PUBLIC_LIB) == want
     lib = cj.public_lib()
     for name in want:
@@ -51,7 +51,7 @@ def test_status_and_backend_strings():
     lib = cj.public_lib()
     assert [lib.cujpegxl_backend_string(i).decode() for i in range(3)] == ["unknown", "fp32-simt",
                                                                             "fp16-tensor"]
-    assert cj.api_version() == 4
+    assert cj.api_version() == 3 and cj.ext_version() == 1
 
 
 def test_max_output_size_and_validation():
@@ -95,3 +95,36 @@ def test_python_binding_argument_errors():
         cj.encode(np.zeros(10, np.uint8), 512, 512)
     with pytest.raises(ValueError):
         cj.encode(np.zeros((512 * 512 * 3 // 2) * 2, np.uint8)[::2], 512, 512)
+
+
+def test_symbol_versions_keep_the_reference_node():
+    """The reference's nine entry points stay under CUJPEGXL_1
+    (cujpegxl.lds:4-9); the extensions sit under CUJPEGXL_2."""
+    out = subprocess.run(["objdump", "-T", str(cj.PUBLIC_LIB)], capture_output=True, text=True,
+                         check=True).stdout
+    nodes = {}
+    for line in out.splitlines():
+        m = re.search(r"\s(CUJPEGXL_\d)\s+(cujpegxl_\w+)$", line)
+        if m:
+            nodes[m.group(2)] = m.group(1)
+    ext = {"cujpegxl_ext_version", "cujpegxl_encoder_encode_host", "cujpegxl_encoder_encode_async",
+           "cujpegxl_encoder_finish"}
+    assert len(nodes) == 13
+    for name, node in nodes.items():
+        assert node == ("CUJPEGXL_2" if name in ext else "CUJPEGXL_1"), name
+
+
+def test_pycujpegxl_module_as_the_reference_tests_it():
+    """python/pycujpegxl_test.py:10-27 of the reference, run against the
+    `pycujpegxl` module here (no device: query_backend raises)."""
+    import pycujpegxl
+
+    assert pycujpegxl.api_version() == 3
+    assert pycujpegxl.API_VERSION == 3
+    assert callable(pycujpegxl.query_backend)
+    assert callable(pycujpegxl.encode)
+    assert callable(pycujpegxl.encode_with_stats)
+    with pytest.raises(RuntimeError):
+        pycujpegxl.query_backend(9999)
+    with pytest.raises(ValueError):
+        pycujpegxl.encode(np.zeros(10, np.uint8), 512, 512)

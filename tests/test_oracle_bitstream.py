@@ -222,3 +222,20 @@ def test_oracle_against_live_reference_build(jo):
         n = R.ref_write_codestream(w, h, fc.gs, fc.qdc, fc.rqf, P(dc), P(ac), P(fc.acs),
                                    P(fc.ytox), P(fc.ytob), 1, P(buf), buf.size)
         assert ours == buf[:n].tobytes(), (w, h)
+
+
+def test_threaded_oracle_writes_the_same_file(jo):
+    """The oracle's front-end loops split rows, tiles, regions and blocks
+    over threads with disjoint outputs: any thread count gives the same file
+    (the CPU baseline's all-cores leg relies on it)."""
+    from cujpegxl_amd.synth import synthetic_nv12
+
+    w, h = 776, 456
+    luma, chroma = synthetic_nv12(w, h, 31)
+    one = jo.encode_nv12(luma, chroma, w, h, 1.0)
+    try:
+        jo.set_threads(8)
+        many = jo.encode_nv12(luma, chroma, w, h, 1.0)
+    finally:
+        jo.set_threads(1)
+    assert many == one

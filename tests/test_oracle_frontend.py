@@ -65,6 +65,110 @@ def test_xyb_against_fp64_and_flat_tile(jo):
     assert np.abs(jo.nv12_to_xyb(luma, chroma, 64, 64)[0]).max() < 1e-7
 
 
+def _xyb_constants():
+    import json
+    import pathlib
+
+    path = pathlib.Path(__file__).resolve().parent / "golden" / "xyb_constants.json"
+    consts = json.loads(path.read_text())["constants"]
+    return {k: float(np.float32(v["f32"])) for k, v in consts.items() if "f32" in v}
+
+
+def _bilinear_f64(plane, w, h):
+    """tests/reference.py:23-46: centred 4:2:0 bilinear with clamped taps."""
+    ch, cw = plane.shape
+    xs = (np.arange(w) + 0.5) / 2.0 - 0.5
+    ys = (np.arange(h) + 0.5) / 2.0 - 0.5
+    x0, y0 = np.floor(xs), np.floor(ys)
+    fx, fy = xs - x0, ys - y0
+    x0i, x1i = np.clip(x0.astype(int), 0, cw - 1), np.clip(x0.astype(int) + 1, 0, cw - 1)
+    y0i, y1i = np.clip(y0.astype(int), 0, ch - 1), np.clip(y0.astype(int) + 1, 0, ch - 1)
+    p = plane.astype(np.float64)
+    top = p[y0i][:, x0i] * (1 - fx) + p[y0i][:, x1i] * fx
+    bot = p[y1i][:, x0i] * (1 - fx) + p[y1i][:, x1i] * fx
+    return top * (1 - fy)[:, None] + bot * fy[:, None]
+
+
+def _xyb_f64_from_constants(luma, chroma, w, h, k):
+    """NV12 -> XYB in float64 with the reference's constants `k`
+    (xyb_impl.cuh:17-70): BT.709 full-range inverse, the sRGB EOTF, the opsin
+    matrix with bias, cube root minus cbrt(bias)."""
+    cb = _bilinear_f64(chroma[:, 0::2], w, h) / 255.0 - 128.0 / 255.0
+    cr = _bilinear_f64(chroma[:, 1::2], w, h) / 255.0 - 128.0 / 255.0
+    y = luma.astype(np.float64) / 255.0
+    rgb = (y + k["CR_TO_R"] * cr, y - k["CB_TO_G"] * cb - k["CR_TO_G"] * cr, y + k["CB_TO_B"] * cb)
+    lin = []
+    for e in rgb:
+        e = np.clip(e, 0.0, 1.0)
+        lin.append(np.clip(np.where(e <= 0.04045, e / 12.92, ((e + 0.055) / 1.055) ** 2.4), 0.0, 1.0))
+    rows = ((k["M00"], 1.0 - k["M02"] - k["M00"], k["M02"]), (k["M10"], 1.0 - k["M12"] - k["M10"], k["M12"]),
+            (k["M20"], k["M21"], 1.0 - k["M20"] - k["M21"]))
+    bias = k["OPSIN_BIAS"]
+    q = [np.cbrt(np.maximum(a * lin[0] + b * lin[1] + c * lin[2] + bias, 0.0)) - np.cbrt(bias) for a, b, c in rows]
+    return np.stack([0.5 * (q[0] - q[1]), 0.5 * (q[0] + q[1]), q[2]])
+
+
+def _xyb_error(got, want):
+    """Max error per channel relative to the channel's largest magnitude."""
+    return [float(np.abs(got[c] - want[c]).max() / np.abs(want[c]).max()) for c in range(3)]
+
+
+_XYB_BOUND = (1e-5, 1e-6, 1e-6)
+
+
+def _xyb_within(err):
+    return all(e <= b for e, b in zip(err, _XYB_BOUND))
+
+
+def test_xyb_against_reference_constants(jo):
+    """The oracle's NV12 -> XYB against a float64 recomputation from the
+    constants written in the reference's source (tests/golden/xyb_constants.json,
+    extracted from src/xyb_impl.cuh:17-35 by extract_constants.py), on a
+    synthetic frame and on uniformly random NV12 (saturated colours reach every
+    clamp).  Bounds, relative to each channel's largest magnitude: X 1e-5
+    (measured 5e-6 on the synthetic frame, where |X| < 0.01), Y and B 1e-6
+    (measured 2.5e-7).  Every constant matters at these bounds: perturbing any
+    one of them by 1e-4 relative in the float64 recomputation breaks the
+    agreement on the random frame."""
+    k = _xyb_constants()
+    rng = np.random.default_rng(0)
+    cases = [_nv12(128, 96, 7) + (128, 96)]
+    luma = rng.integers(0, 256, (48, 64), dtype=np.uint8)
+    chroma = rng.integers(0, 256, (24, 64), dtype=np.uint8)
+    cases.append((luma, chroma, 64, 48))
+    for luma, chroma, w, h in cases:
+        got = jo.nv12_to_xyb(luma, chroma, w, h).reshape(3, h, w).astype(np.float64)
+        want = _xyb_f64_from_constants(luma, chroma, w, h, k)
+        assert _xyb_within(_xyb_error(got, want)), _xyb_error(got, want)
+    luma, chroma, w, h = cases[1]
+    got = jo.nv12_to_xyb(luma, chroma, w, h).reshape(3, h, w).astype(np.float64)
+    for name in k:
+        bad = dict(k)
+        bad[name] *= 1.0 + 1e-4
+        assert not _xyb_within(_xyb_error(got, _xyb_f64_from_constants(luma, chroma, w, h, bad))), name
+
+
+def test_bt709_constants_from_kr_kb():
+    """The reference's BT.709 inverse coefficients (xyb_impl.cuh:27-33) are
+    the full-range inverse for Kr = 0.2126, Kb = 0.0722 (its comment):
+    R = Y + 2(1-Kr) Cr, B = Y + 2(1-Kb) Cb, G = Y - 2Kb(1-Kb)/Kg Cb - 2Kr(1-Kr)/Kg Cr."""
+    k = _xyb_constants()
+    kr, kb = 0.2126, 0.0722
+    kg = 1.0 - kr - kb
+    derived = {"CR_TO_R": 2 * (1 - kr), "CB_TO_B": 2 * (1 - kb), "CB_TO_G": 2 * kb * (1 - kb) / kg,
+               "CR_TO_G": 2 * kr * (1 - kr) / kg}
+    for name, v in derived.items():
+        assert k[name] == float(np.float32(v)), (name, k[name], v)
+    # Forward BT.709 then this inverse returns the RGB it started from.
+    rgb = np.random.default_rng(3).uniform(0, 1, (3, 100))
+    yy = kr * rgb[0] + kg * rgb[1] + kb * rgb[2]
+    cb = (rgb[2] - yy) / (2 * (1 - kb))
+    cr = (rgb[0] - yy) / (2 * (1 - kr))
+    back = np.stack([yy + derived["CR_TO_R"] * cr, yy - derived["CB_TO_G"] * cb - derived["CR_TO_G"] * cr,
+                     yy + derived["CB_TO_B"] * cb])
+    assert np.abs(back - rgb).max() < 1e-12
+
+
 def test_chroma_upsample_is_the_exact_bilinear(jo):
     """tests/reference.py:23-46: centred bilinear 4:2:0 -> 4:4:4, each tap
     clamped to the edge independently, evaluated here in float64."""
@@ -116,8 +220,11 @@ def test_gaborish_against_bruteforce(jo):
 
 
 def _aq_block_f64(xyb, bx, by, w, h, d):
-    """FP64 restatement of one block's AQ integer from frontend_fused.cu:54-204
-    (the tile-local erosion included), for the blocks of the tile at (0, 0)."""
+    """FP64 restatement of block (bx, by)'s AQ integer from
+    frontend_fused.cu:54-204: the block's 32x32 tile (4x4 blocks) with a
+    one-pixel halo taken from the frame (coordinates clamped to the frame
+    only), the tile's 8x8 pre-erosion cells, erosion over the 3x3 cell
+    neighbourhood clamped to the tile, then the block's modulation terms."""
     import math
 
     def ratio(v, inv):
@@ -128,7 +235,10 @@ def _aq_block_f64(xyb, bx, by, w, h, d):
         den = lg * mul * v ** 3 + 7.7825991679894591 * lg + 1e-2
         return num / den if inv else den / num
 
-    ty = np.pad(xyb[1].astype(np.float64), 1, mode="edge")
+    tx0, ty0 = (bx // 4) * 32, (by // 4) * 32
+    xs = np.clip(np.arange(tx0 - 1, tx0 + 33), 0, w - 1)
+    ys = np.clip(np.arange(ty0 - 1, ty0 + 33), 0, h - 1)
+    ty = xyb[1].astype(np.float64)[ys][:, xs]
     pre = np.zeros((8, 8))
     for cy in range(8):
         for cx in range(8):
@@ -145,8 +255,9 @@ def _aq_block_f64(xyb, bx, by, w, h, d):
     km = np.array([0.125, 0.10 - efm * 0.10, 0.09 - efm * 0.09, 0.06 - efm * 0.06])
     km *= 0.29959705784054957 / km.sum()
     er = 0.0
-    for cy in (2 * by, 2 * by + 1):
-        for cx in (2 * bx, 2 * bx + 1):
+    lbx, lby = bx % 4, by % 4
+    for cy in (2 * lby, 2 * lby + 1):
+        for cx in (2 * lbx, 2 * lbx + 1):
             nb = sorted(pre[min(max(cy + a, 0), 7), min(max(cx + b, 0), 7)]
                         for a in (-1, 0, 1) for b in (-1, 0, 1))
             er += float(np.dot(km, nb[:4]))
@@ -174,18 +285,26 @@ def _aq_block_f64(xyb, bx, by, w, h, d):
     return q
 
 
-@pytest.mark.parametrize("d", [1.0, 3.0])
-def test_adaptive_quant_against_fp64(jo, d):
-    luma, chroma = _nv12(32, 32, 9)
-    xyb = jo.nv12_to_xyb(luma, chroma, 32, 32)
-    qf = jo.adaptive_quant(xyb, d)
+@pytest.mark.parametrize("d,w,h,seed", [(1.0, 32, 32, 9), (3.0, 32, 32, 9), (1.0, 96, 64, 12),
+                                        (3.0, 96, 64, 13), (1.5, 80, 56, 14)])
+def test_adaptive_quant_against_fp64(jo, d, w, h, seed):
+    """Every block of the frame, so interior tiles (whose halo pixels come
+    from neighbouring tiles) and partial tiles at the right and bottom edges
+    (80x56: 10x7 blocks) are covered, besides the all-clamped single tile."""
+    luma, chroma = _nv12(w, h, seed)
+    xyb = jo.nv12_to_xyb(luma, chroma, w, h).reshape(3, h, w)
+    qf = jo.adaptive_quant(np.ascontiguousarray(xyb), d)
     inv = jo.calibrate(d).inv_global_scale
-    for by in range(4):
-        for bx in range(4):
-            want = _aq_block_f64(xyb, bx, by, 32, 32, d) * inv + 0.5
-            got = int(qf[by * 4 + bx])
+    bw, bh = w // 8, h // 8
+    checked = 0
+    for by in range(bh):
+        for bx in range(bw):
+            want = _aq_block_f64(xyb, bx, by, w, h, d) * inv + 0.5
+            got = int(qf[by * bw + bx])
             if abs(want - round(want)) > 1e-3:  # not at a rounding tie
                 assert got == min(256, max(1, int(want))), (bx, by, want)
+                checked += 1
+    assert checked >= bw * bh - 2
 
 
 def test_adaptive_quant_flat_and_range(jo):
@@ -232,21 +351,28 @@ def _valid_tiling(acs, bw, bh):
     assert (owner >= 0).all()
 
 
-def test_selection_on_synthetic_frames(jo):
-    """Real content: valid tilings, all three sizes used, and the decisions
-    agree with the reference's host decide_region where _ref is built (its
-    float reduction order differs, so only near-ties may flip)."""
-    luma, chroma = _nv12(264, 200, 3)
-    xyb = jo.gaborish(jo.nv12_to_xyb(luma, chroma, 264, 200))
-    acs = jo.select_transforms(xyb[1], 1.0)
-    _valid_tiling(acs, 33, 25)
-    assert {8, 16, 32} <= set(np.unique(acs).tolist())
+def test_selection_on_synthetic_frames(jo, golden):
+    """Real content: valid tilings, all three sizes used, and every decision
+    equal to the reference's host decide_region (transform_select_impl.cuh:130-201,
+    compiled from its sources into oracle/_ref) on the same sharpened Y plane:
+    the golden vectors hold its output for five seeded frames at d = 1 and 3,
+    and where _ref is built it is run again on the first frame."""
+    cases = ((264, 200, 3, 1.0), (264, 200, 3, 3.0), (776, 456, 21, 1.0), (776, 456, 22, 3.0),
+             (1280, 720, 23, 1.0))
+    for w, h, seed, d in cases:
+        luma, chroma = _nv12(w, h, seed)
+        y = np.ascontiguousarray(jo.gaborish(jo.nv12_to_xyb(luma, chroma, w, h))[1])
+        acs = jo.select_transforms(y, d)
+        _valid_tiling(acs, w // 8, h // 8)
+        assert {8, 16, 32} <= set(np.unique(acs).tolist())
+        np.testing.assert_array_equal(acs, golden[f"sel_synth_{w}x{h}_s{seed}_d{d:g}"])
     R = jo.ref()
     if R is not None:
-        want = np.full(acs.size, -1, np.int8)
-        y = np.ascontiguousarray(xyb[1])
+        luma, chroma = _nv12(264, 200, 3)
+        y = np.ascontiguousarray(jo.gaborish(jo.nv12_to_xyb(luma, chroma, 264, 200))[1])
+        want = np.full(y.size // 64, -1, np.int8)
         R.ref_select_transforms(jo.ptr(y), 264, 200, 1.0, jo.ptr(want))
-        assert np.mean(acs == want) > 0.99
+        np.testing.assert_array_equal(jo.select_transforms(y, 1.0), want)
 
 
 def test_dct_against_fp64(jo):
