@@ -2460,8 +2460,10 @@ size_t jo_write_codestream(const jo_frame* f, uint8_t* out, size_t cap) {
         sizes[1 + ndc] = (uint32_t)jo_ac_global(hist, nac, depth, bits, acg, 65536);
         uint8_t* acb = (uint8_t*)malloc(dcb_cap);
         const size_t act = jo_ac_groups(f, depth, bits, sizes + 2 + ndc, acb, dcb_cap);
-        uint8_t head[4096];
-        const size_t hn = jo_codestream_head(f->width, f->height, sizes, nsec, head, sizeof(head));
+        /* Headers under 1 KB plus at most 4 bytes per TOC entry. */
+        const size_t head_cap = 1024 + 4 * (size_t)nsec;
+        uint8_t* head = (uint8_t*)malloc(head_cap);
+        const size_t hn = jo_codestream_head(f->width, f->height, sizes, nsec, head, head_cap);
         total = hn + sizes[0] + dct + sizes[1 + ndc] + act;
         if (out && total <= cap) {
             size_t at = 0;
@@ -2475,6 +2477,7 @@ size_t jo_write_codestream(const jo_frame* f, uint8_t* out, size_t cap) {
             at += sizes[1 + ndc];
             memcpy(out + at, acb, act);
         }
+        free(head);
         free(sizes);
         free(dcb);
         free(acg);

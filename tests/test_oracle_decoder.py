@@ -111,3 +111,27 @@ def test_decoded_pixels_match_the_source(jo, w, h, d, seed, floor):
     got = [psnr(rec[c], xyb[c]) for c in range(3)]
     assert all(g >= f for g, f in zip(got, floor)), got
     assert psnr(reconstruct_xyb(jo, dec, gaborish=False)[1], xyb[1]) < got[1]
+
+
+def test_largest_frame_head_round_trips(jo):
+    """16384x8192 with one small non-zero per block and channel: 2,048 AC
+    groups of at least 1 KiB, so the TOC alone takes more than 4 KB; the
+    oracle's file decodes back to the frame and every section ends at its
+    TOC size."""
+    w, h = 16384, 8192
+    bw, bh = w // 8, h // 8
+    nblk = bw * bh
+    rng = np.random.default_rng(5)
+    acs = np.full(nblk, 8, np.int8)
+    ac = np.zeros((3, nblk, 64), np.int16)
+    pos = rng.integers(1, 64, size=(3, nblk))
+    val = rng.integers(1, 4, size=(3, nblk)) * rng.choice([-1, 1], size=(3, nblk))
+    np.put_along_axis(ac, pos[:, :, None], val[:, :, None].astype(np.int16), axis=2)
+    dc = rng.integers(-50, 50, size=3 * nblk).astype(np.int32)
+    qf = rng.integers(1, 64, size=nblk).astype(np.int32)
+    cs = jo.write_codestream(jo.make_frame(w, h, 2048, 300, dc, ac.ravel(), acs, qf, None, None))
+    dec = jo.decode(cs)
+    assert dec["width"] == w and dec["height"] == h
+    assert np.array_equal(dec["dc"].ravel(), dc)
+    assert np.array_equal(dec["ac"].ravel(), ac.ravel())
+    assert np.array_equal(dec["qf"].ravel(), qf)
