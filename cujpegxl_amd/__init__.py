@@ -75,6 +75,10 @@ def _bind_public(L: C.CDLL) -> C.CDLL:
     L.cujpegxl_encoder_encode_host.argtypes = [C.c_void_p, C.POINTER(Nv12HostInput), C.POINTER(Output)]
     L.cujpegxl_encoder_encode_async.argtypes = [C.c_void_p, C.POINTER(Nv12Input), C.c_size_t]
     L.cujpegxl_encoder_finish.argtypes = [C.c_void_p, C.POINTER(Output)]
+    L.cujpegxl_noalloc_workspace_bytes.restype = C.c_size_t
+    L.cujpegxl_noalloc_workspace_bytes.argtypes = [C.c_uint32, C.c_uint32]
+    L.cujpegxl_encode_noalloc.argtypes = [C.POINTER(Config), C.POINTER(Nv12Input), C.c_size_t, C.c_size_t,
+                                          C.c_size_t, C.c_size_t]
     return L
 
 

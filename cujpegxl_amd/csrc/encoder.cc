@@ -286,6 +286,7 @@ void Encoder::release() {
     null_buffers();
     total_dev_ = nullptr;
     configured_ = false;
+    workspace_ = false;
 }
 
 // Every device buffer member back to null (their memory was freed with
@@ -387,22 +388,12 @@ bool Encoder::setup(int device) {
     return alloc();
 }
 
-bool Encoder::alloc() {
+template <class Get>
+void Encoder::carve(Get& get) {
     const std::size_t nblk{geo_.nblocks};
     const std::size_t tiles{static_cast<std::size_t>(geo_.cmw) * geo_.cmh};
     const std::size_t regions{static_cast<std::size_t>(geo_.rw) * geo_.rh};
     const std::size_t nh{static_cast<std::size_t>(kNumClusters) + 2 * geo_.num_dc};
-    out_cap_ = (max_output_size(geo_.width, geo_.height) + 64 + 3) & ~static_cast<std::size_t>(3);
-    bool ok{true};
-    auto get = [&](auto*& p, std::size_t bytes) {
-        void* q{nullptr};
-        if (ok && !injected_alloc_failure() && hipMalloc(&q, bytes < 256 ? 256 : bytes) == hipSuccess) {
-            allocations_.push_back(q);
-            p = static_cast<std::remove_reference_t<decltype(p)>>(q);
-        } else {
-            ok = false;
-        }
-    };
     get(coeffs_, 3 * nblk * 64 * sizeof(std::uint16_t));
     get(acs_, nblk);
     get(qf_, nblk * sizeof(std::int32_t));
@@ -435,13 +426,123 @@ bool Encoder::alloc() {
     get(layout_, sizeof_layout());
     get(ac_base_, geo_.num_ac * sizeof(std::uint64_t));
     get(dc_base_, geo_.num_dc * sizeof(std::uint64_t));
+}
+
+bool Encoder::alloc() {
+    out_cap_ = (max_output_size(geo_.width, geo_.height) + 64 + 3) & ~static_cast<std::size_t>(3);
+    bool ok{true};
+    auto get = [&](auto*& p, std::size_t bytes) {
+        void* q{nullptr};
+        if (ok && !injected_alloc_failure() && hipMalloc(&q, bytes < 256 ? 256 : bytes) == hipSuccess) {
+            allocations_.push_back(q);
+            p = static_cast<std::remove_reference_t<decltype(p)>>(q);
+        } else {
+            ok = false;
+        }
+    };
+    carve(get);
     get(out_, out_cap_);
     const_gs_ = const_qdc_ = 0;
     return ok;
 }
 
+// Consecutive 256-byte aligned ranges from offset 0 of a workspace (`base`
+// null: sizing only).
+namespace {
+struct Bump {
+    std::uint8_t* base;
+    std::size_t used{0};
+    template <class T>
+    void operator()(T*& p, std::size_t bytes) {
+        p = base != nullptr ? reinterpret_cast<T*>(base + used) : nullptr;
+        used += (std::max<std::size_t>(bytes, 1) + 255) & ~static_cast<std::size_t>(255);
+    }
+};
+}  // namespace
+
+std::size_t Encoder::workspace_bytes(std::uint32_t width, std::uint32_t height) {
+    Encoder probe;
+    probe.geo_ = Geometry::of(width, height);
+    Bump b{nullptr};
+    probe.carve(b);
+    probe.null_buffers();
+    return b.used;
+}
+
+bool Encoder::bind_workspace(std::uint32_t width, std::uint32_t height, float distance, int device, void* base,
+                             std::size_t bytes) {
+    if (!allocations_.empty() || stream_ != nullptr || base == nullptr || width == 0 || height == 0 ||
+        width % 8 || height % 8) {
+        return false;  // an encoder that owns allocations is never rebound
+    }
+    const Geometry geo{Geometry::of(width, height)};
+    if (geo.num_dc > 32 || workspace_bytes(width, height) > bytes) {
+        return false;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        return false;
+    }
+    std::call_once(g_tables_once[device & 63], [device] { g_tables_ok[device & 63] = init_device_tables(); });
+    if (!g_tables_ok[device & 63]) {
+        return false;
+    }
+    const bool same{workspace_ && configured_ && device == device_ && geo.width == geo_.width &&
+                    geo.height == geo_.height &&
+                    static_cast<void*>(coeffs_) == base};
+    device_ = device;
+    distance_ = distance;
+    prm_ = make_frame_params(distance);
+    if (!same) {
+        geo_ = geo;
+        Bump b{static_cast<std::uint8_t*>(base)};
+        carve(b);
+        out_ = nullptr;
+        out_cap_ = 0;
+        const_gs_ = const_qdc_ = 0;  // the constants depend on the shape
+    }
+    workspace_ = true;
+    configured_ = true;
+    return true;
+}
+
+// One frame queued on `s` in dependency order (the two entropy chains one
+// after the other); nothing waits on the host unless the constant bit
+// strings change (a new shape or distance), which copies them in order on
+// `s` and waits for that copy.
+bool Encoder::encode_on(hipStream_t s, const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                        std::size_t chroma_pitch, std::uint8_t* out_dev, std::size_t cap, std::uint64_t* size_dev) {
+    if (!workspace_ || !configured_ || hipSetDevice(device_) != hipSuccess ||
+        !build_constants(prm_.global_scale, prm_.quant_dc, s)) {
+        return false;
+    }
+    auto* codes{static_cast<CodeSet*>(codes_)};
+    auto* k{static_cast<const Constants*>(constants_)};
+    auto* blobs{static_cast<DcBlobs*>(blobs_)};
+    auto* lay{static_cast<Layout*>(layout_)};
+    launch_region(s, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_, item_count_,
+                  entropy_clears());
+    launch_tile_quantize(s, geo_, prm_, coeffs_, acs_, qf_, cfl_part_, ac_, nz_grid_, nz_count_,
+                         ItemList{item_list_, item_count_});
+    launch_dc_maps(s, geo_, prm_, coeffs_, acs_, cfl_part_, dc_, ytox_, ytob_);
+    launch_dc_hist(s, geo_, dc_, acs_, qf_, ytox_, ytob_, hists_ + kNumClusters * kHistStride,
+                   hists_ + (kNumClusters + geo_.num_dc) * kHistStride, fb_count_);
+    launch_huffman(s, kNumClusters, static_cast<int>(2 * geo_.num_dc), hists_, codes);
+    launch_dc_sizes(s, geo_, k, codes, fb_count_, blobs, blob_words_, dc_, acs_, qf_, ytox_, ytob_, chunk_bits_,
+                    dc_group_bytes_, dc_group_bits_, dc_staging_);
+    launch_dc_emit(s, geo_, dc_, acs_, qf_, ytox_, ytob_, codes, blobs, blob_words_, chunk_bits_, dc_staging_);
+    launch_ac_tokens(s, geo_, ac_, acs_, nz_grid_, nz_count_, tokens_, item_tokens_, hist_partial_, hists_,
+                     ItemList{item_list_, item_count_});
+    launch_huffman(s, 0, kNumClusters, hists_, codes);
+    launch_ac_group_emit(s, geo_, acs_, item_tokens_, tokens_, codes, ac_staging_, ac_part_bits_);
+    launch_layout(s, geo_, k, codes, ac_part_bits_, dc_group_bytes_, dc_group_bits_, ac_group_bytes_,
+                  ac_group_bits_, blobs, lay, ac_base_, dc_base_, size_dev);
+    launch_assemble(s, geo_, k, lay, dc_staging_, ac_staging_, dc_group_bits_, ac_group_bits_, ac_part_bits_,
+                    dc_base_, ac_base_, out_dev, cap);
+    return hipGetLastError() == hipSuccess;
+}
+
 // Constant bit strings of a configuration (codestream.cc, frame_assembly.cc).
-bool Encoder::build_constants(std::uint32_t global_scale, std::uint32_t quant_dc) {
+bool Encoder::build_constants(std::uint32_t global_scale, std::uint32_t quant_dc, hipStream_t s) {
     if (global_scale == const_gs_ && quant_dc == const_qdc_) {
         return true;
     }
@@ -516,8 +617,9 @@ bool Encoder::build_constants(std::uint32_t global_scale, std::uint32_t quant_dc
     std::vector<std::uint8_t> host(sizeof_constants());
     fill_constants(host.data(), head.words.data(), head.pos, dcg.words.data(), dcg.pos, acp.words.data(),
                    acp.pos, pre.words.data(), pre.pos, mid.words.data(), mid.pos);
-    if (hipMemcpyAsync(constants_, host.data(), host.size(), hipMemcpyHostToDevice, stream_) != hipSuccess ||
-        hipStreamSynchronize(stream_) != hipSuccess) {
+    const hipStream_t on{s != nullptr ? s : stream_};
+    if (hipMemcpyAsync(constants_, host.data(), host.size(), hipMemcpyHostToDevice, on) != hipSuccess ||
+        hipStreamSynchronize(on) != hipSuccess) {
         return false;
     }
     const_gs_ = global_scale;
@@ -693,7 +795,8 @@ bool Encoder::finish(std::uint8_t* dst, std::size_t cap, std::size_t* size) {
 bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                           std::size_t chroma_pitch, hipStream_t caller, bool release_caller, std::uint8_t* dst,
                           std::size_t cap) {
-    if (!configured_ || pending_ || hipSetDevice(device_) != hipSuccess || !build_constants(prm_.global_scale, prm_.quant_dc)) {
+    if (!configured_ || workspace_ || pending_ || hipSetDevice(device_) != hipSuccess ||
+        !build_constants(prm_.global_scale, prm_.quant_dc)) {
         return false;
     }
     // The frame starts after the work already queued on `caller` (the null

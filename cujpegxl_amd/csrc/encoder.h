@@ -76,6 +76,18 @@ public:
     bool fetch(Buffer which, void* host, std::size_t bytes);
     std::size_t buffer_bytes(Buffer which) const;
 
+    // Workspace form, for callers that cannot allocate.  Every device buffer
+    // of the shape is a 256-byte aligned range of `base` (no allocation, no
+    // stream, event or pinned word is created), and encode_on() queues a
+    // whole frame on the caller's stream alone, the file into a device
+    // output and its size into a device word.  workspace_bytes() is what
+    // bind_workspace() needs for a shape.
+    static std::size_t workspace_bytes(std::uint32_t width, std::uint32_t height);
+    bool bind_workspace(std::uint32_t width, std::uint32_t height, float distance, int device, void* base,
+                        std::size_t bytes);
+    bool encode_on(hipStream_t s, const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                   std::size_t chroma_pitch, std::uint8_t* out_dev, std::size_t cap, std::uint64_t* size_dev);
+
     // 0: no events; 1: an event pair around every stage; 2: the region
     // kernel's pair only (each timed event on the stream costs a few
     // microseconds of idle device time between the kernels it separates).
@@ -91,9 +103,14 @@ public:
 private:
     bool setup(int device);
     bool alloc();
+    // Points every device buffer except the output at get(pointer, bytes).
+    template <class Get>
+    void carve(Get& get);
     void release();
     void null_buffers();
-    bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc);
+    // Uploads the constant bit strings on `s` (the encoder's stream when
+    // null) unless they are already those of (global_scale, quant_dc).
+    bool build_constants(std::uint32_t global_scale, std::uint32_t quant_dc, hipStream_t s = nullptr);
     // `fork_bound`: the quantizer's launch has bound `ev_fork_` already;
     // `listed`: it also listed the non-empty items for the tokenizer.
     bool run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, bool listed, bool maps = false,
@@ -115,6 +132,7 @@ private:
 
     int device_{-1};
     bool configured_{false};
+    bool workspace_{false};  // buffers bound to a caller's workspace (bind_workspace)
     float distance_{1.0f};
     Geometry geo_{};
     FrameParams prm_{};

@@ -20,3 +20,9 @@ CUJPEGXL_2 {
         cujpegxl_encoder_encode_async;
         cujpegxl_encoder_finish;
 } CUJPEGXL_1;
+
+CUJPEGXL_3 {
+    global:
+        cujpegxl_noalloc_workspace_bytes;
+        cujpegxl_encode_noalloc;
+} CUJPEGXL_2;

@@ -128,6 +128,44 @@ cujpegxl_status cujpegxl_encoder_encode_async(cujpegxl_encoder* encoder, const c
  * CUJPEGXL_INVALID_ARGUMENT when no frame is pending. */
 cujpegxl_status cujpegxl_encoder_finish(cujpegxl_encoder* encoder, cujpegxl_output* output);
 
+/* ---- Allocation-free encode (no counterpart in the reference) ----------
+ *
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Exported
+ * under the version node CUJPEGXL_3.  Every intermediate buffer lives in one
+ * device array of CUJPEGXL_NOALLOC_SCRATCH_BYTES that the library reserves on
+ * each device it is loaded on, until the process ends; nothing is allocated,
+ * and no stream or event is created, per call.  Frames up to
+ * CUJPEGXL_NOALLOC_MAX_WIDTH x CUJPEGXL_NOALLOC_MAX_HEIGHT (and at most
+ * CUJPEGXL_NOALLOC_MAX_PIXELS pixels); larger ones return
+ * CUJPEGXL_UNSUPPORTED_RESOLUTION. */
+#define CUJPEGXL_NOALLOC_MAX_WIDTH 4096u
+#define CUJPEGXL_NOALLOC_MAX_HEIGHT 4096u
+#define CUJPEGXL_NOALLOC_MAX_PIXELS (1920u * 1088u)
+#define CUJPEGXL_NOALLOC_SCRATCH_BYTES (104ull << 20)
+
+/* Device scratch bytes a frame of width x height uses (0 for shapes the
+ * encoder never accepts); at most CUJPEGXL_NOALLOC_SCRATCH_BYTES for every
+ * accepted frame. */
+size_t cujpegxl_noalloc_workspace_bytes(uint32_t width, uint32_t height);
+
+/* Encodes one device-resident NV12 frame, the work queued on `stream` (a
+ * hipStream_t as an integer; 0: the null stream) after the work already
+ * there.  The file goes into the device buffer `output` (4-byte aligned; the
+ * first `capacity` bytes of the file are written) and its size into the
+ * 8-byte device word at `size_out`, both once the stream's work has finished:
+ * a size above `capacity` means the file was cut short.  Validation and
+ * statuses as for cujpegxl_encoder_create and cujpegxl_encoder_encode, plus
+ * CUJPEGXL_UNSUPPORTED_RESOLUTION above the limits.  Calls share one scratch
+ * array per device, which keeps the last call's contents: calls whose work
+ * overlaps on one device (on different streams) race and must be serialised
+ * by the caller.  The first call on a device, and a call with a new shape or
+ * distance, also waits for `stream`, as it copies the frame's constant
+ * headers in order on it. */
+cujpegxl_status cujpegxl_encode_noalloc(const cujpegxl_config* config, const cujpegxl_nv12_input* input,
+                                        uintptr_t output, size_t capacity, uintptr_t size_out,
+                                        uintptr_t stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

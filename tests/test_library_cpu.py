@@ -29,9 +29,10 @@ def exported(lib: pathlib.Path) -> set[str]:
 
 def test_public_library_exports_exactly_the_abi():
     want = declared("cujpegxl.h", "cujpegxl_")
-    # cujpegxl.h:66-82, plus the extension version query and the host-input
-    # and asynchronous entry points.
-    assert len(want) == 13
+    # cujpegxl.h:66-82, plus the extension version query, the host-input
+    # and asynchronous entry points, and the allocation-free encode with its
+    # workspace query.
+    assert len(want) == 15
     assert exported(cj.PUBLIC_LIB) == want
     lib = cj.public_lib()
     for name in want:
@@ -99,7 +100,8 @@ def test_python_binding_argument_errors():
 
 def test_symbol_versions_keep_the_reference_node():
     """The reference's nine entry points stay under CUJPEGXL_1
-    (cujpegxl.lds:4-9); the extensions sit under CUJPEGXL_2."""
+    (cujpegxl.lds:4-9); the extensions sit under CUJPEGXL_2, the
+    allocation-free encode under CUJPEGXL_3."""
     out = subprocess.run(["objdump", "-T", str(cj.PUBLIC_LIB)], capture_output=True, text=True,
                          check=True).stdout
     nodes = {}
@@ -109,9 +111,11 @@ def test_symbol_versions_keep_the_reference_node():
             nodes[m.group(2)] = m.group(1)
     ext = {"cujpegxl_ext_version", "cujpegxl_encoder_encode_host", "cujpegxl_encoder_encode_async",
            "cujpegxl_encoder_finish"}
-    assert len(nodes) == 13
+    noalloc = {"cujpegxl_noalloc_workspace_bytes", "cujpegxl_encode_noalloc"}
+    assert len(nodes) == 15
     for name, node in nodes.items():
-        assert node == ("CUJPEGXL_2" if name in ext else "CUJPEGXL_1"), name
+        want = "CUJPEGXL_2" if name in ext else ("CUJPEGXL_3" if name in noalloc else "CUJPEGXL_1")
+        assert node == want, name
 
 
 def test_pycujpegxl_module_as_the_reference_tests_it():
@@ -128,3 +132,28 @@ def test_pycujpegxl_module_as_the_reference_tests_it():
         pycujpegxl.query_backend(9999)
     with pytest.raises(ValueError):
         pycujpegxl.encode(np.zeros(10, np.uint8), 512, 512)
+
+
+def test_noalloc_scratch_holds_every_accepted_shape():
+    """The allocation-free encode's scratch array (CUJPEGXL_NOALLOC_SCRATCH_BYTES)
+    holds the buffers of every shape it accepts: for each width, the tallest
+    accepted height (the workspace grows with the height)."""
+    text = (ROOT / "include" / "cujpegxl" / "cujpegxl.h").read_text()
+    scratch = int(re.search(r"NOALLOC_SCRATCH_BYTES \((\d+)ull << 20\)", text).group(1)) << 20
+    max_side = int(re.search(r"NOALLOC_MAX_WIDTH (\d+)u", text).group(1))
+    a, b = re.search(r"NOALLOC_MAX_PIXELS \((\d+)u \* (\d+)u\)", text).groups()
+    max_px = int(a) * int(b)
+    lib = cj.public_lib()
+    worst = 0
+    for w in range(8, max_side + 1, 8):
+        h = min(max_side, max_px // w // 8 * 8)
+        if h >= 8:
+            worst = max(worst, lib.cujpegxl_noalloc_workspace_bytes(w, h))
+            worst = max(worst, lib.cujpegxl_noalloc_workspace_bytes(h, w))
+    assert 0 < worst <= scratch
+    # 4K needs far more than the array: such frames are refused, not truncated.
+    assert lib.cujpegxl_noalloc_workspace_bytes(3840, 2160) > scratch
+    cfg = cj.Config(3840, 2160, 1.0, 0)
+    inp = cj.Nv12Input(1 << 20, 2 << 20, 3840, 3840)
+    assert lib.cujpegxl_encode_noalloc(C.byref(cfg), C.byref(inp), 1 << 24, 1 << 20, 1 << 12, 0) == \
+        cj.UNSUPPORTED_RESOLUTION
