@@ -46,6 +46,8 @@ REGION_FRAMES = 48  # frames whose region kernel is timed by bound events
 TRAFFIC_FILE = os.path.join(HERE, "profiles", "hbm_traffic_4k_d1.txt")
 COUNTER_FILE = os.path.join(HERE, "profiles", "sq_counters_4k_d1.txt")
 CUS, SCLK_GHZ = 256, 2.4  # MI355X compute units and peak engine clock
+# Forms of the path timed beside the plain one (encoder mode bits).
+FORMS = (("mfma16", 1), ("learned", 2), ("learned+mfma16", 3))
 
 
 def region_valu_instructions() -> int | None:
@@ -72,6 +74,25 @@ def region_traffic_bytes() -> int | None:
     except OSError:
         pass
     return None
+
+
+QUALITY_FILE = os.path.join(HERE, "profiles", "quality_4k.json")
+
+
+def quality_summary() -> dict | None:
+    """Rate at equal quality of each form against the plain path, from the
+    committed summary of tests/test_gpu_quality.py at 3840x2160 (decoded
+    files scored with the Butteraugli-structured distance of
+    tests/butteraugli.py, not libjxl's)."""
+    try:
+        with open(QUALITY_FILE) as f:
+            q = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return {"source": "profiles/quality_4k.json (tests/test_gpu_quality.py, CJX_QUALITY_FULL=1)",
+            "metric": q.get("metric"), "size": q.get("size"),
+            "bpp_increase_at_plain_score": q.get("mean_bpp_increase"),
+            "region_ms": q.get("mean_region_ms")}
 
 
 MIXED_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
@@ -232,6 +253,30 @@ def main() -> int:
     lib.cjx_device_synchronize()
     single_frame_fps = args.steps / (time.perf_counter() - t1)
 
+    # The other forms of the path on the same frames and loop: FP16-operand
+    # matrix-core DCTs, the learned transform selection, and both.
+    forms = {}
+    for name, mode in FORMS:
+        enc.set_mode(mode)
+        for k in range(len(inputs)):
+            enc.encode_into(*inputs[k], pinned.ptr, enc.capacity)
+        lib.cjx_device_synchronize()
+        t1 = time.perf_counter()
+        enc.encode_cycle(inputs, pinned.ptr, enc.capacity, args.steps)
+        lib.cjx_device_synchronize()
+        f_fps = args.steps / (time.perf_counter() - t1)
+        f_sizes = [enc.encode_into(*inputs[k], pinned.ptr, enc.capacity) for k in range(SEEDS)]
+        enc.set_profiling(True, region_only=True)
+        rsum = 0.0
+        nreg = min(args.steps, REGION_FRAMES // 2)
+        for k in range(nreg):
+            enc.encode_into(*inputs[k % len(inputs)], pinned.ptr, enc.capacity)
+            rsum += enc.region_ms()
+        enc.set_profiling(False)
+        forms[name] = {"fps": round(f_fps, 3), "bpp": round(8.0 * float(np.mean(f_sizes)) / (WIDTH * HEIGHT), 4),
+                       "region_ms": round(rsum / max(nreg, 1), 4)}
+    enc.set_mode(0)
+
     # The region kernel's duration (the roofline figure), from HIP events
     # bound to the kernel's own start and end on the encoder's stream, over
     # separate frames (a start event adds a few microseconds of idle device
@@ -321,6 +366,13 @@ def main() -> int:
             "limiter": limiter,
         },
     }
+    for name, f in forms.items():
+        f["vs_plain_fps"] = round(f["fps"] / fps, 4)
+        f["bpp_vs_plain"] = round(f["bpp"] / out["config"]["bpp"] - 1.0, 5)
+    out["forms"] = forms
+    quality = quality_summary()
+    if quality is not None:
+        out["quality"] = quality
     if not args.no_mixed_batch:
         out["mixed_batch"] = mixed_batch(cj, synthetic_nv12)
     if not args.no_cpu_baseline:

@@ -27,11 +27,15 @@ EXT_VERSION = 1  # CUJPEGXL_EXT_VERSION: host-input and asynchronous entry point
 OK, NOT_IMPLEMENTED, INVALID_ARGUMENT, UNSUPPORTED_RESOLUTION, NO_DEVICE, INTERNAL_ERROR, \
     BUFFER_TOO_SMALL = range(7)
 
+MODE_MFMA16 = 1  # CJX_MODE_MFMA16: FP16-operand matrix-core DCTs
+MODE_LEARNED = 2  # CJX_MODE_LEARNED: the selection network where it is confident
+MODE_FORCE8, MODE_FORCE16, MODE_FORCE32 = 0x10, 0x20, 0x30  # test modes: one transform size
+
 STAGE_NAMES = ("region", "quantize", "histograms", "codes", "sizes", "emit", "readback")
 BUFFERS = {
     "coeffs": 0, "acs": 1, "quant_field": 2, "cfl_partials": 3, "ytox": 4, "ytob": 5, "ac": 6,
     "dc": 7, "nz_grid": 8, "nz_count": 9, "histograms": 10, "ac_group_bytes": 11,
-    "dc_group_bytes": 12, "code_depth": 13, "code_bits": 14,
+    "dc_group_bytes": 12, "code_depth": 13, "code_bits": 14, "features": 15,
 }
 
 
@@ -108,6 +112,21 @@ def stages_lib() -> C.CDLL:
         L.cjx_stage_encoder_create_any.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
         L.cjx_stage_encode_nv12.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
         L.cjx_stage_set_profiling.argtypes = [P, C.c_int]
+        L.cjx_stage_set_mode.argtypes = [P, C.c_uint32]
+        L.cjx_stage_export_features.argtypes = [P, C.c_int]
+        L.cjx_mlp_set_weights.argtypes = [C.c_int32, P, P, P, P, C.c_float]
+        L.cjx_mlp_default_weights.argtypes = [C.c_int32]
+        L.cjx_mlp_partial_floats.restype = C.c_size_t
+        L.cjx_mlp_partial_floats.argtypes = [C.c_uint32]
+        L.cjx_mlp_step.argtypes = [P, P, C.c_uint32, C.c_float, P, P, P, P, P, P, C.c_uint32, C.c_float, C.c_size_t]
+        L.cjx_mlp_forward.argtypes = [P, P, C.c_uint32, P, C.c_size_t]
+        L.cjx_batch_create.argtypes = [C.c_int32, C.c_uint32, P, P, P, C.c_uint32, C.POINTER(C.c_void_p)]
+        L.cjx_batch_set_inputs.argtypes = [P, P]
+        L.cjx_batch_run.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
+        L.cjx_batch_result.argtypes = [P, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
+        L.cjx_batch_tune.argtypes = [P, C.c_int32, C.c_char_p, C.c_size_t]
+        L.cjx_batch_destroy.argtypes = [P]
+        L.cjx_batch_destroy.restype = None
         L.cjx_stage_encode_repeat.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output), C.c_int,
                                               C.POINTER(C.c_double)]
         L.cjx_stage_encode_cycle.argtypes = [P, C.POINTER(Nv12Input), C.c_int, C.POINTER(Output), C.c_int,
@@ -402,6 +421,18 @@ class Encoder:
         `region_only` the region kernel alone (fewer events on the stream)."""
         self.lib.cjx_stage_set_profiling(self.handle, (2 if region_only else 1) if on else 0)
 
+    def set_mode(self, mode: int) -> None:
+        """Encoder mode bits (MODE_MFMA16; 0 for the plain path), from the
+        next encode."""
+        if self.lib.cjx_stage_set_mode(self.handle, mode) != OK:
+            raise ValueError(f"bad mode {mode}")
+
+    def export_features(self, on: bool = True) -> None:
+        """From the next encode, the per-region network features and the
+        heuristic's labels ([regions][108] floats, fetch("features"))."""
+        if self.lib.cjx_stage_export_features(self.handle, int(on)) != OK:
+            raise RuntimeError("feature export failed")
+
     def region_ms(self) -> float:
         """The region kernel's device time in the last encode (profiling on)."""
         ms = C.c_float(0.0)
@@ -470,3 +501,61 @@ def encode_with_stats(nv12: np.ndarray, width: int, height: int, distance: float
         {"name": "entropy", "bytes_moved": len(data), "gpu_us": ent, "cpu_us": 0.0},
         {"name": "assembly", "bytes_moved": len(data), "gpu_us": asm, "cpu_us": 0.0},
     ]
+
+
+class Batch:
+    """Frames of uneven sizes and distances encoded together (stage library):
+    one launch per stage over all frames, captured once as a hipGraph."""
+
+    def __init__(self, sizes: list[tuple[int, int]], distances: list[float], mode: int = 0,
+                 device_ordinal: int = 0):
+        self.lib = stages_lib()
+        n = len(sizes)
+        w = np.array([s[0] for s in sizes], np.uint32)
+        h = np.array([s[1] for s in sizes], np.uint32)
+        d = np.array(distances, np.float32)
+        self.handle = C.c_void_p()
+        st = self.lib.cjx_batch_create(device_ordinal, n, w.ctypes.data, h.ctypes.data, d.ctypes.data, mode,
+                                       C.byref(self.handle))
+        if st != OK:
+            raise RuntimeError(f"batch create failed: {status_string(st)}")
+        self.sizes = list(sizes)
+
+    def set_inputs(self, inputs: list[tuple[int, int, int, int]]) -> None:
+        """Per frame (luma, chroma, luma pitch, chroma pitch) device planes."""
+        arr = (Nv12Input * len(inputs))(*(Nv12Input(*i) for i in inputs))
+        if self.lib.cjx_batch_set_inputs(self.handle, arr) != OK:
+            raise ValueError("bad batch inputs")
+
+    def run(self, count: int = 1) -> float:
+        sec = C.c_double(0.0)
+        st = self.lib.cjx_batch_run(self.handle, count, C.byref(sec))
+        if st != OK:
+            raise RuntimeError(f"batch run failed: {status_string(st)}")
+        return sec.value
+
+    def files(self) -> list[bytes]:
+        out = []
+        for i in range(len(self.sizes)):
+            p, n = C.c_void_p(), C.c_uint64()
+            if self.lib.cjx_batch_result(self.handle, i, C.byref(p), C.byref(n)) != OK:
+                raise RuntimeError("batch result failed")
+            out.append(C.string_at(p.value, n.value))
+        return out
+
+    def tune(self, reps: int = 5) -> str:
+        buf = C.create_string_buffer(8192)
+        if self.lib.cjx_batch_tune(self.handle, reps, buf, len(buf)) != OK:
+            raise RuntimeError("batch tune failed")
+        return buf.value.decode()
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.cjx_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,7 +24,7 @@ LIB = PKG / "lib"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CJX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["device.hip", "tables.cc", "encoder.cc", "capi.cc", "noalloc.cc", "stages.cc"]
+SOURCES = ["device.hip", "tables.cc", "encoder.cc", "capi.cc", "noalloc.cc", "batch.cc", "stages.cc"]
 
 CFLAGS = [
     "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -59,7 +59,7 @@ def build(jobs: int = 8) -> list[pathlib.Path]:
     outs = []
     for name, extra, script, soname in (
         ("libcujpegxl.so.1", [by_name["capi"], by_name["noalloc"]], "libcujpegxl.lds", "libcujpegxl.so.1"),
-        ("libcujpegxl_stages.so", [by_name["capi"], by_name["noalloc"], by_name["stages"]], "libcujpegxl_stages.lds",
+        ("libcujpegxl_stages.so", [by_name["capi"], by_name["noalloc"], by_name["batch"], by_name["stages"]], "libcujpegxl_stages.lds",
          "libcujpegxl_stages.so"),
     ):
         target = LIB / name

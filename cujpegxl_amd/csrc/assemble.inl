@@ -122,10 +122,10 @@ __device__ void huffman_depths(HuffmanShared& S, int alpha) {
     }
 }
 
-__global__ __launch_bounds__(64 * kTrials) void huffman_kernel(const std::uint32_t* __restrict__ hists,
+__device__ __attribute__((always_inline)) inline void huffman_kernel_body(const Wg& wg, const std::uint32_t* __restrict__ hists,
                                                                CodeSet* __restrict__ cs, int first) {
     __shared__ HuffmanShared S;
-    const int h{first + static_cast<int>(blockIdx.x)};
+    const int h{first + static_cast<int>(wg.x)};
     const int t{static_cast<int>(threadIdx.x)};
     const int w{t >> 6};
     for (int i{t}; i < kHistStride; i += 64 * kTrials) {
@@ -174,6 +174,11 @@ __global__ __launch_bounds__(64 * kTrials) void huffman_kernel(const std::uint32
     }
 }
 
+__global__ __launch_bounds__(64 * kTrials) void huffman_kernel(const std::uint32_t* __restrict__ hists,
+                                                               CodeSet* __restrict__ cs, int first) {
+    huffman_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, hists, cs, first);
+}
+
 // Per DC group: blob_pre = constant prefix + data alphabet + code header;
 // blob_mid = (count - 1) + constant prefix + alphabet + header.
 // One DC group's header blobs (group g, threads of the workgroup take the
@@ -217,18 +222,27 @@ __device__ void dc_blobs_body(const Geometry& geo, const Constants* __restrict__
 // The DC chain's sizing pass: workgroup columns 0 .. chunks_per_group - 1 of
 // group row g size the group's item chunks, column chunks_per_group builds
 // its two header blobs (independent work, one launch).
+__device__ __attribute__((always_inline)) inline void dc_blobs_bits_kernel_body(const Wg& wg, 
+    Geometry geo, const Constants* __restrict__ k, const CodeSet* __restrict__ cs,
+    const std::uint32_t* __restrict__ fb_count, DcBlobs* __restrict__ blobs, std::uint32_t* __restrict__ blob_words,
+    const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
+    const std::int8_t* __restrict__ ytox, const std::int8_t* __restrict__ ytob,
+    std::uint64_t* __restrict__ chunk_bits, std::uint32_t chunks_per_group) {
+    if (wg.x == chunks_per_group) {
+        dc_blobs_body(geo, k, cs, fb_count, blobs, blob_words, static_cast<int>(wg.y));
+    } else {
+        dc_chunk_bits_body(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
+                           &cs->depth[kNumClusters + geo.num_dc][0], chunk_bits, chunks_per_group);
+    }
+}
+
 __global__ __launch_bounds__(256) void dc_blobs_bits_kernel(
     Geometry geo, const Constants* __restrict__ k, const CodeSet* __restrict__ cs,
     const std::uint32_t* __restrict__ fb_count, DcBlobs* __restrict__ blobs, std::uint32_t* __restrict__ blob_words,
     const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
     const std::int8_t* __restrict__ ytox, const std::int8_t* __restrict__ ytob,
     std::uint64_t* __restrict__ chunk_bits, std::uint32_t chunks_per_group) {
-    if (blockIdx.x == chunks_per_group) {
-        dc_blobs_body(geo, k, cs, fb_count, blobs, blob_words, static_cast<int>(blockIdx.y));
-    } else {
-        dc_chunk_bits_body(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
-                           &cs->depth[kNumClusters + geo.num_dc][0], chunk_bits, chunks_per_group);
-    }
+    dc_blobs_bits_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, k, cs, fb_count, blobs, blob_words, dc, acs, qf, ytox, ytob, chunk_bits, chunks_per_group);
 }
 
 // U32 field coder choice (field_coder.h:46-107) as one code word: selector
@@ -272,7 +286,7 @@ __device__ __forceinline__ void u32_code(const Distr* d, std::uint32_t value, st
 // the section bases from scans of the section sizes.  `single` frames (one AC
 // group) concatenate DcGlobal, the DC group, AcGlobal and the AC tokens into
 // one section without padding in between.
-__global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constants* __restrict__ k,
+__device__ __attribute__((always_inline)) inline void layout_kernel_body(const Wg& wg, Geometry geo, const Constants* __restrict__ k,
                                                      const CodeSet* __restrict__ cs,
                                                      const std::uint64_t* __restrict__ ac_part_bits,
                                                      const std::uint32_t* __restrict__ dc_group_bytes,
@@ -470,6 +484,20 @@ __global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constan
     }
 }
 
+__global__ __launch_bounds__(256) void layout_kernel(Geometry geo, const Constants* __restrict__ k,
+                                                     const CodeSet* __restrict__ cs,
+                                                     const std::uint64_t* __restrict__ ac_part_bits,
+                                                     const std::uint32_t* __restrict__ dc_group_bytes,
+                                                     const std::uint64_t* __restrict__ dc_single_bits,
+                                                     std::uint32_t* __restrict__ ac_group_bytes,
+                                                     std::uint64_t* __restrict__ ac_group_bits,
+                                                     const DcBlobs* __restrict__ blobs,
+                                                     Layout* __restrict__ lay, std::uint64_t* __restrict__ ac_base,
+                                                     std::uint64_t* __restrict__ dc_base,
+                                                     std::uint64_t* __restrict__ total_out) {
+    layout_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, k, cs, ac_part_bits, dc_group_bytes, dc_single_bits, ac_group_bytes, ac_group_bits, blobs, lay, ac_base, dc_base, total_out);
+}
+
 // The file is a sequence of bit strings ("pieces") laid end to end, each at
 // the position the layout gave it: framing, head (headers and TOC), DcGlobal,
 // the DC groups, AcGlobal, the AC groups.  Gaps between pieces (section
@@ -553,7 +581,7 @@ __device__ __forceinline__ Piece piece_at(const Geometry& geo, const Constants* 
             P.ac_part_bits + static_cast<std::size_t>(g) * kGroupParts};
 }
 
-__global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Constants* __restrict__ k,
+__device__ __attribute__((always_inline)) inline void assemble_kernel_body(const Wg& wg, Geometry geo, const Constants* __restrict__ k,
                                                        const Layout* __restrict__ lay, Pieces P,
                                                        std::uint32_t* __restrict__ out_words,
                                                        std::uint64_t cap_bytes) {
@@ -572,7 +600,7 @@ __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Const
     std::uint64_t words{(lay->total_bytes + 3) / 4};
     const std::uint64_t cap_words{(cap_bytes + 3) / 4};
     words = words < cap_words ? words : cap_words;
-    for (std::uint64_t w{blockIdx.x * 256ull + threadIdx.x}; w < words; w += gridDim.x * 256ull) {
+    for (std::uint64_t w{wg.x * 256ull + threadIdx.x}; w < words; w += wg.nx * 256ull) {
         const std::uint64_t b0{w * 32};
         // First piece ending after b0 (pieces are ordered and disjoint).
         int lo{0}, hi{n};
@@ -606,6 +634,13 @@ __global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Const
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void assemble_kernel(Geometry geo, const Constants* __restrict__ k,
+                                                       const Layout* __restrict__ lay, Pieces P,
+                                                       std::uint32_t* __restrict__ out_words,
+                                                       std::uint64_t cap_bytes) {
+    assemble_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, k, lay, P, out_words, cap_bytes);
 }
 
 }  // namespace cjx

@@ -41,6 +41,12 @@ struct ItemList {
     std::uint32_t* count;
 };
 
+// A workgroup's coordinates in its launch: blockIdx and gridDim of a frame's
+// own launch, or the frame's share of a launch over a batch of frames.
+struct Wg {
+    std::uint32_t x, y, nx, ny;
+};
+
 struct Geometry {
     std::uint32_t width, height;
     std::uint32_t bw, bh;        // 8x8 blocks
@@ -85,7 +91,24 @@ struct FrameParams {
     float km[4];        // fuzzy-erosion weights
     float mul_pb, add_pb;
     float dc_scale;     // global_scale_float * quant_dc
+    std::uint32_t mode;  // kMode* bits of the encoder (0: the plain path)
 };
+
+// Encoder modes (FrameParams::mode).
+// kModeMfma16: the region kernel's DCTs as FP16-operand matrix products with
+// FP32 accumulation and a hi + lo split of every operand.
+constexpr std::uint32_t kModeMfma16 = 1u;
+// kModeForce*: every region takes the largest transform of that size that
+// fits instead of the RD choice (DCT error reports per transform size).
+constexpr std::uint32_t kModeForceShift = 4;
+constexpr std::uint32_t kModeForceMask = 3u << kModeForceShift;
+constexpr std::uint32_t kModeForce8 = 1u << kModeForceShift;
+constexpr std::uint32_t kModeForce16 = 2u << kModeForceShift;
+constexpr std::uint32_t kModeForce32 = 3u << kModeForceShift;
+// kModeLearned: the transform map from the selection network where it is
+// confident, the heuristic elsewhere.
+constexpr std::uint32_t kModeLearned = 2u;
+constexpr std::uint32_t kModeMask = kModeMfma16 | kModeLearned | kModeForceMask;
 
 FrameParams make_frame_params(float distance);
 
@@ -112,6 +135,13 @@ struct DeviceTables {
     double llf2[4];        // T[f][o] = ortho_2(f, o) * resample_scale_2[f]
     double llf4[16];
     float rate[256];       // selection rate term 1 + log2(1 + k) for |q| = k >= 1; 0 for k = 0
+    float xyb_tab[kXybTabSize];  // segments and powers of pow24_tab / cbrt_tab
+    // FP16 operands of the matrix-core DCT form: [0] = RN16(a), [1] =
+    // RN16(a - [0]) of every basis entry a (FP16 bit patterns, row-major):
+    // blockdiag(A8, A8) (a 16x16 quad of four DCT8s), A16 and A32.
+    std::uint16_t mf8[2][256];
+    std::uint16_t mf16[2][256];
+    std::uint16_t mf32[2][1024];
 };
 
 extern __constant__ __align__(16) DeviceTables g_tables;

@@ -17,12 +17,51 @@ __constant__ __align__(16) DeviceTables g_tables;
 #include "entropy.inl"
 #include "huffman.inl"
 #include "assemble.inl"
+#include "mlp.inl"
+#include "batch.h"
+#include "batch.inl"
 
 namespace cjx {
 
+namespace {
+#include "mlp_weights.inc"
+}  // namespace
+
 bool init_device_tables() {
     const DeviceTables& t{host_tables()};
-    return hipMemcpyToSymbol(HIP_SYMBOL(g_tables), &t, sizeof(t)) == hipSuccess;
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_tables), &t, sizeof(t)) == hipSuccess && set_default_mlp_weights();
+}
+
+bool set_default_mlp_weights() {
+    return set_mlp_weights(kTrainedW1, kTrainedB1, kTrainedW2, kTrainedB2, kTrainedThreshold);
+}
+
+bool set_mlp_weights(const std::uint16_t* w1, const float* b1, const std::uint16_t* w2, const float* b2,
+                     float threshold) {
+    MlpWeights m{};
+    std::memcpy(m.w1, w1, sizeof(m.w1));
+    std::memcpy(m.w2, w2, sizeof(m.w2));
+    std::memcpy(m.b1, b1, sizeof(m.b1));
+    std::memcpy(m.b2, b2, 7 * sizeof(float));
+    m.threshold = threshold;
+    return hipMemcpyToSymbol(HIP_SYMBOL(g_mlp), &m, sizeof(m)) == hipSuccess;
+}
+
+std::size_t mlp_partial_floats(std::uint32_t n) {
+    return static_cast<std::size_t>((n + kMlpTiles - 1) / kMlpTiles) * (kMlpParams + 1);
+}
+
+void launch_mlp_step(hipStream_t s, const float* x, const float* labels, std::uint32_t n, float lambda,
+                     float* partial, float* grad, float* loss, float* params, float* m, float* v, std::uint32_t step,
+                     float lr) {
+    const std::uint32_t rows{(n + kMlpTiles - 1) / kMlpTiles};
+    mlp_grad_kernel<<<rows, 256, 0, s>>>(params, x, labels, n, lambda, partial);
+    mlp_reduce_kernel<<<(kMlpParams + 1 + 255) / 256, 256, 0, s>>>(partial, rows, n, grad, loss, params, m, v, step,
+                                                                   lr);
+}
+
+void launch_mlp_forward(hipStream_t s, const float* params, const float* x, std::uint32_t n, float* z) {
+    mlp_forward_kernel<<<(n + 255) / 256, 256, 0, s>>>(params, x, n, z);
 }
 
 // One thread per (block position, channel): non-zero AC counts of a
@@ -30,7 +69,7 @@ bool init_device_tables() {
 // whose coefficients are supplied directly.
 __global__ void nz_kernel(Geometry geo, const std::int16_t* __restrict__ ac, const std::int8_t* __restrict__ acs,
                           std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, Clears clr) {
-    clear_words(clr);
+    clear_words(clr, Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y});
     const std::size_t i{blockIdx.x * 256ull + threadIdx.x};
     const std::size_t nblk{geo.nblocks};
     if (i >= 3 * nblk) {
@@ -108,14 +147,18 @@ static unsigned int grid_for(std::size_t n, std::size_t per) {
 void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
-                   std::uint32_t* list_count, const Clears& clr, hipEvent_t start, hipEvent_t stop) {
+                   std::uint32_t* list_count, const Clears& clr, hipEvent_t start, hipEvent_t stop,
+                   float* features) {
     // Eight equal runs of regions, one per XCD (the kernel maps workgroup i to
     // run i mod 8); workgroups past the last region exit at once.  The events,
     // when given, take the kernel's own start and end times (no marker
     // packets between it and its neighbours).
     const unsigned int per_xcd{(geo.rw * geo.rh + 7) / 8};
-    hipExtLaunchKernelGGL(region_kernel, dim3(8 * per_xcd), dim3(256), 0, s, start, stop, 0, geo, prm, luma,
-                          luma_pitch, chroma, chroma_pitch, coeffs, acs, qf, cfl_part, list_count, clr);
+    const bool mf{(prm.mode & kModeMfma16) != 0}, learned{(prm.mode & kModeLearned) != 0};
+    auto* const kernel{mf ? (learned ? region_kernel<true, true> : region_kernel<true, false>)
+                          : (learned ? region_kernel<false, true> : region_kernel<false, false>)};
+    hipExtLaunchKernelGGL(kernel, dim3(8 * per_xcd), dim3(256), 0, s, start, stop, 0, geo, prm, luma, luma_pitch,
+                          chroma, chroma_pitch, coeffs, acs, qf, cfl_part, list_count, clr, features);
 }
 
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,

@@ -201,6 +201,11 @@ bool injected_alloc_failure() {
 
 }  // namespace
 
+bool ensure_device_tables(int device) {
+    std::call_once(g_tables_once[device & 63], [device] { g_tables_ok[device & 63] = init_device_tables(); });
+    return g_tables_ok[device & 63];
+}
+
 void fail_allocation_after(int n) {
     g_fail_alloc_countdown.store(n < 0 ? 0 : n);
 }
@@ -311,6 +316,8 @@ void Encoder::null_buffers() {
     ac_group_bytes_ = dc_group_bytes_ = nullptr;
     out_ = nullptr;
     out_cap_ = 0;
+    features_ = nullptr;
+    export_ = false;
     const_gs_ = const_qdc_ = 0;
 }
 
@@ -328,6 +335,7 @@ bool Encoder::configure(std::uint32_t width, std::uint32_t height, float distanc
     }
     distance_ = distance;
     prm_ = make_frame_params(distance);
+    prm_.mode = mode_;
     if (same_shape) {
         return true;
     }
@@ -492,6 +500,7 @@ bool Encoder::bind_workspace(std::uint32_t width, std::uint32_t height, float di
     device_ = device;
     distance_ = distance;
     prm_ = make_frame_params(distance);
+    prm_.mode = mode_;
     if (!same) {
         geo_ = geo;
         Bump b{static_cast<std::uint8_t*>(base)};
@@ -502,6 +511,52 @@ bool Encoder::bind_workspace(std::uint32_t width, std::uint32_t height, float di
     }
     workspace_ = true;
     configured_ = true;
+    return true;
+}
+
+bool Encoder::fill_job(FrameJob& j, hipStream_t s, std::uint8_t* out_dev, std::size_t cap, std::uint64_t* size_dev) {
+    if (!workspace_ || !configured_ || hipSetDevice(device_) != hipSuccess ||
+        !build_constants(prm_.global_scale, prm_.quant_dc, s)) {
+        return false;
+    }
+    j.geo = geo_;
+    j.prm = prm_;
+    j.coeffs = coeffs_;
+    j.acs = acs_;
+    j.qf = qf_;
+    j.cfl_part = cfl_part_;
+    j.ytox = ytox_;
+    j.ytob = ytob_;
+    j.ac = ac_;
+    j.dc = dc_;
+    j.nz_grid = nz_grid_;
+    j.nz_count = nz_count_;
+    j.hists = hists_;
+    j.fb_count = fb_count_;
+    j.codes = static_cast<CodeSet*>(codes_);
+    j.k = static_cast<const Constants*>(constants_);
+    j.blobs = static_cast<DcBlobs*>(blobs_);
+    j.blob_words = blob_words_;
+    j.tokens = tokens_;
+    j.item_tokens = item_tokens_;
+    j.item_list = item_list_;
+    j.item_count = item_count_;
+    j.hist_partial = hist_partial_;
+    j.ac_staging = ac_staging_;
+    j.dc_staging = dc_staging_;
+    j.ac_part_bits = ac_part_bits_;
+    j.ac_group_bytes = ac_group_bytes_;
+    j.ac_group_bits = ac_group_bits_;
+    j.chunk_bits = chunk_bits_;
+    j.dc_group_bytes = dc_group_bytes_;
+    j.dc_group_bits = dc_group_bits_;
+    j.lay = static_cast<Layout*>(layout_);
+    j.ac_base = ac_base_;
+    j.dc_base = dc_base_;
+    j.out = reinterpret_cast<std::uint32_t*>(out_dev);
+    j.cap = cap;
+    j.size_out = size_dev;
+    j.clr = entropy_clears();
     return true;
 }
 
@@ -815,7 +870,8 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     const hipEvent_t region_end{timed ? events_[static_cast<int>(Stage::kQuantize)] : ev_fork_};
     launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_,
                   item_count_, entropy_clears(),
-                  timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr, region_end);
+                  timed ? events_[static_cast<int>(Stage::kRegion)] : nullptr, region_end,
+                  export_ ? features_ : nullptr);
     // The planes are read by the region kernel only: later work on the
     // caller's stream may overwrite them once it has finished.
     if (release_caller && (hipEventRecord(ev_input_free_, stream_) != hipSuccess ||
@@ -1052,6 +1108,22 @@ bool Encoder::run_entropy(std::uint8_t* out, std::size_t cap, hipEvent_t fork, b
     return hipGetLastError() == hipSuccess;
 }
 
+bool Encoder::export_features(bool on) {
+    if (!configured_ || workspace_ || hipSetDevice(device_) != hipSuccess) {
+        return false;
+    }
+    if (on && features_ == nullptr) {
+        void* p{nullptr};
+        if (hipMalloc(&p, static_cast<std::size_t>(geo_.rw) * geo_.rh * 108 * sizeof(float)) != hipSuccess) {
+            return false;
+        }
+        allocations_.push_back(p);
+        features_ = static_cast<float*>(p);
+    }
+    export_ = on;
+    return true;
+}
+
 std::size_t Encoder::buffer_bytes(Buffer which) const {
     const std::size_t nblk{geo_.nblocks};
     const std::size_t tiles{static_cast<std::size_t>(geo_.cmw) * geo_.cmh};
@@ -1071,6 +1143,7 @@ std::size_t Encoder::buffer_bytes(Buffer which) const {
         case Buffer::kDcGroupBytes: return geo_.num_dc * 4;
         case Buffer::kCodeDepth: return (kNumClusters + 2 * geo_.num_dc) * kHistStride;
         case Buffer::kCodeBits: return (kNumClusters + 2 * geo_.num_dc) * kHistStride * 2;
+        case Buffer::kFeatures: return features_ != nullptr ? static_cast<std::size_t>(geo_.rw) * geo_.rh * 108 * 4 : 0;
         default: return 0;
     }
 }
@@ -1094,6 +1167,7 @@ bool Encoder::fetch(Buffer which, void* host, std::size_t bytes) {
         case Buffer::kHistograms: src = hists_; break;
         case Buffer::kAcGroupBytes: src = ac_group_bytes_; break;
         case Buffer::kDcGroupBytes: src = dc_group_bytes_; break;
+        case Buffer::kFeatures: src = features_; break;
         case Buffer::kCodeDepth:
         case Buffer::kCodeBits: {
             // Rows of the code set, one per histogram.

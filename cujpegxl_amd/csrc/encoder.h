@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 
 namespace cjx {
@@ -27,7 +28,7 @@ enum class Stage : int {
 // Internal buffers a harness can read back after an encode.
 enum class Buffer : int {
     kCoeffs = 0, kAcs, kQuantField, kCflPartials, kYtoX, kYtoB, kAc, kDc, kNzGrid, kNzCount,
-    kHistograms, kAcGroupBytes, kDcGroupBytes, kCodeDepth, kCodeBits, kCount
+    kHistograms, kAcGroupBytes, kDcGroupBytes, kCodeDepth, kCodeBits, kFeatures, kCount
 };
 
 class Encoder {
@@ -87,11 +88,25 @@ public:
                         std::size_t bytes);
     bool encode_on(hipStream_t s, const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                    std::size_t chroma_pitch, std::uint8_t* out_dev, std::size_t cap, std::uint64_t* size_dev);
+    // A workspace-bound encoder's frame as a batch job: its constant bit
+    // strings uploaded (in order on `s`), its buffers, output and size word;
+    // the stage grids are the caller's.
+    bool fill_job(FrameJob& job, hipStream_t s, std::uint8_t* out_dev, std::size_t cap, std::uint64_t* size_dev);
 
     // 0: no events; 1: an event pair around every stage; 2: the region
     // kernel's pair only (each timed event on the stream costs a few
     // microseconds of idle device time between the kernels it separates).
     void set_profiling(int level) { profiling_ = level; }
+    // kMode* bits (0: the plain path); applies from the next encode.
+    void set_mode(std::uint32_t mode) {
+        mode_ = mode;
+        prm_.mode = mode;
+    }
+    std::uint32_t mode() const { return mode_; }
+    // The region kernel also writes each region's network features and the
+    // heuristic's labels (Buffer::kFeatures, 108 floats per region) from the
+    // next encode; the buffer is allocated on the first call.
+    bool export_features(bool on);
     // Milliseconds per stage of the last encode (profiling on; with level 2
     // only the region entry is filled).
     const std::array<float, static_cast<int>(Stage::kCount)>& stage_ms() const { return stage_ms_; }
@@ -152,6 +167,7 @@ private:
     std::uint8_t* direct_dst_{nullptr};  // page-locked output the assembly wrote the file into
     std::uint8_t* input_{nullptr};   // device NV12 frame for host input (allocated on first use)
     int profiling_{0};
+    std::uint32_t mode_{0};
     std::array<hipEvent_t, static_cast<int>(Stage::kCount) + 1> events_{};
     std::array<float, static_cast<int>(Stage::kCount)> stage_ms_{};
     std::uint32_t const_gs_{0}, const_qdc_{0};
@@ -191,6 +207,8 @@ private:
     std::uint64_t* dc_base_{nullptr};
     std::uint32_t* out_{nullptr};
     std::size_t out_cap_{0};
+    float* features_{nullptr};  // per-region features and labels (export on)
+    bool export_{false};
     std::uint64_t* total_host_{nullptr};  // pinned; the layout kernel writes the file size here
     std::uint64_t* total_dev_{nullptr};   // the same word as a device address
     // Readback: the file is copied in chunks through two pinned slots, the
@@ -203,6 +221,9 @@ private:
 
 bool device_exists(int ordinal);
 bool device_has_matrix_cores(int ordinal);
+// The constant tables (and the default network weights) on `device`, once;
+// the device must be current.
+bool ensure_device_tables(int device);
 // Test hook: the n-th device allocation from now fails (0 cancels).
 void fail_allocation_after(int n);
 

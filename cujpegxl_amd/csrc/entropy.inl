@@ -336,7 +336,7 @@ __device__ __forceinline__ void store_item_coefs(std::int16_t* coef, const AcIte
     }
 }
 
-__global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
+__device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(const Wg& wg, Geometry geo, const std::int16_t* __restrict__ ac,
                                                         const std::int8_t* __restrict__ acs,
                                                         const std::uint8_t* __restrict__ nz_grid,
                                                         const std::uint16_t* __restrict__ nz_count,
@@ -356,8 +356,8 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
     const int lane{static_cast<int>(threadIdx.x) & 63};
     const int half{lane >> 5}, sub{lane & 31};
     std::int16_t* const coef{stage[threadIdx.x >> 6][half]};
-    const std::uint32_t wave{blockIdx.x * 4 + (threadIdx.x >> 6)};
-    const std::uint32_t stride{2 * gridDim.x * 4};
+    const std::uint32_t wave{wg.x * 4 + (threadIdx.x >> 6)};
+    const std::uint32_t stride{2 * wg.nx * 4};
     // Walk the quantizer's list of items with non-zeros when there is one
     // (the others follow below; covered blocks' items already have 0
     // tokens), else every item.
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
     if (list != nullptr) {
         const std::uint32_t zeros{list_count[1]};
         const std::uint32_t last{3u * geo.nblocks - 1u};
-        for (std::uint32_t i{blockIdx.x * 256u + threadIdx.x}; i < zeros; i += gridDim.x * 256u) {
+        for (std::uint32_t i{wg.x * 256u + threadIdx.x}; i < zeros; i += wg.nx * 256u) {
             const AcItem it{ac_item(geo, acs, nz_grid, nz_count, list[last - i])};
             std::uint32_t sym{0}, nb{0}, bits{0};
             hybrid420(0u, sym, nb, bits);
@@ -430,19 +430,31 @@ __global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std:
         }
     }
     __syncthreads();
-    std::uint32_t* mine{partial + static_cast<std::size_t>(blockIdx.x) * kNumClusters * kAcSymbols};
+    std::uint32_t* mine{partial + static_cast<std::size_t>(wg.x) * kNumClusters * kAcSymbols};
     for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
         mine[i] = h[i];
     }
 }
 
+__global__ __launch_bounds__(256) void ac_tokens_kernel(Geometry geo, const std::int16_t* __restrict__ ac,
+                                                        const std::int8_t* __restrict__ acs,
+                                                        const std::uint8_t* __restrict__ nz_grid,
+                                                        const std::uint16_t* __restrict__ nz_count,
+                                                        std::uint32_t* __restrict__ tokens,
+                                                        std::uint16_t* __restrict__ item_tokens,
+                                                        std::uint32_t* __restrict__ partial,
+                                                        const std::uint32_t* __restrict__ list,
+                                                        const std::uint32_t* __restrict__ list_count) {
+    ac_tokens_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, ac, acs, nz_grid, nz_count, tokens, item_tokens, partial, list, list_count);
+}
+
 // Sums `parts` partial histograms of 32 x 64 bins into the 32 x 256 cluster
 // histograms: grid (bins / 256, slices); each thread adds its slice.
-__global__ __launch_bounds__(256) void ac_hist_reduce_kernel(const std::uint32_t* __restrict__ partial, int parts,
+__device__ __attribute__((always_inline)) inline void ac_hist_reduce_kernel_body(const Wg& wg, const std::uint32_t* __restrict__ partial, int parts,
                                                              std::uint32_t* __restrict__ hist) {
-    const int bin{static_cast<int>(blockIdx.x) * 256 + static_cast<int>(threadIdx.x)};
-    const int per{(parts + static_cast<int>(gridDim.y) - 1) / static_cast<int>(gridDim.y)};
-    const int p0{static_cast<int>(blockIdx.y) * per};
+    const int bin{static_cast<int>(wg.x) * 256 + static_cast<int>(threadIdx.x)};
+    const int per{(parts + static_cast<int>(wg.ny) - 1) / static_cast<int>(wg.ny)};
+    const int p0{static_cast<int>(wg.y) * per};
     const int p1{min(parts, p0 + per)};
     std::uint32_t sum{0};
     for (int p{p0}; p < p1; ++p) {
@@ -451,6 +463,11 @@ __global__ __launch_bounds__(256) void ac_hist_reduce_kernel(const std::uint32_t
     if (sum != 0) {
         atomicAdd(&hist[(bin / kAcSymbols) * kHistStride + bin % kAcSymbols], sum);
     }
+}
+
+__global__ __launch_bounds__(256) void ac_hist_reduce_kernel(const std::uint32_t* __restrict__ partial, int parts,
+                                                             std::uint32_t* __restrict__ hist) {
+    ac_hist_reduce_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, partial, parts, hist);
 }
 
 // Block-wide exclusive scan of one value per thread (blockDim.x multiple of 64).
@@ -517,28 +534,28 @@ struct GroupShared {
     std::uint32_t overflow;
 };
 
-__global__ __launch_bounds__(kPartThreads) void ac_group_emit_kernel(
+__device__ __attribute__((always_inline)) inline void ac_group_emit_kernel_body(const Wg& wg, 
     Geometry geo, const std::int8_t* __restrict__ acs, const std::uint16_t* __restrict__ item_tokens,
     const std::uint32_t* __restrict__ tokens, const std::uint8_t* __restrict__ depth,
     const std::uint16_t* __restrict__ codes, std::uint32_t* __restrict__ staging,
     std::uint64_t* __restrict__ part_bits) {
     __shared__ GroupShared S;
     const int tid{static_cast<int>(threadIdx.x)};
-    const int g{static_cast<int>(blockIdx.x) / kGroupParts};
-    const int part{static_cast<int>(blockIdx.x) % kGroupParts};
+    const int g{static_cast<int>(wg.x) / kGroupParts};
+    const int part{static_cast<int>(wg.x) % kGroupParts};
     const int bx0{(g % static_cast<int>(geo.xg)) * kAcGroupBlocks};
     const int by0{(g / static_cast<int>(geo.xg)) * kAcGroupBlocks + part * kPartRows};
     const int gbw{min(kAcGroupBlocks, static_cast<int>(geo.bw) - bx0)};
     const int gbh{min(kPartRows, static_cast<int>(geo.bh) - by0)};
     if (gbh <= 0) {
         if (tid == 0) {
-            part_bits[blockIdx.x] = 0;
+            part_bits[wg.x] = 0;
         }
         return;
     }
     const int row_items{3 * gbw};
     const int items{row_items * gbh};
-    std::uint32_t* const slot{staging + static_cast<std::size_t>(blockIdx.x) * kPartSlotWords};
+    std::uint32_t* const slot{staging + static_cast<std::size_t>(wg.x) * kPartSlotWords};
 
     // 1. Token prefix over the part's items.
     std::uint32_t carry{0};
@@ -673,8 +690,16 @@ __global__ __launch_bounds__(kPartThreads) void ac_group_emit_kernel(
         }
     }
     if (tid == 0) {
-        part_bits[blockIdx.x] = run;
+        part_bits[wg.x] = run;
     }
+}
+
+__global__ __launch_bounds__(kPartThreads) void ac_group_emit_kernel(
+    Geometry geo, const std::int8_t* __restrict__ acs, const std::uint16_t* __restrict__ item_tokens,
+    const std::uint32_t* __restrict__ tokens, const std::uint8_t* __restrict__ depth,
+    const std::uint16_t* __restrict__ codes, std::uint32_t* __restrict__ staging,
+    std::uint64_t* __restrict__ part_bits) {
+    ac_group_emit_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, acs, item_tokens, tokens, depth, codes, staging, part_bits);
 }
 
 // ---- DC groups -------------------------------------------------------------
@@ -846,7 +871,7 @@ constexpr int kDcChunk = 1024;               // items per workgroup in the DC pa
 constexpr int kDcPerThread = kDcChunk / 256;  // consecutive items per thread
 
 // DC and AcMetadata token histograms per group, plus the first-block count.
-__global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::int32_t* __restrict__ dc,
+__device__ __attribute__((always_inline)) inline void dc_hist_kernel_body(const Wg& wg, Geometry geo, const std::int32_t* __restrict__ dc,
                                                       const std::int8_t* __restrict__ acs,
                                                       const std::int32_t* __restrict__ qf,
                                                       const std::int8_t* __restrict__ ytox,
@@ -856,7 +881,7 @@ __global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::i
                                                       std::uint32_t* __restrict__ fb_count) {
     __shared__ std::uint32_t h[2 * kHistStride];
     __shared__ std::uint32_t cnt;
-    const int g{static_cast<int>(blockIdx.y)};
+    const int g{static_cast<int>(wg.y)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
     for (int i{static_cast<int>(threadIdx.x)}; i < 2 * kHistStride; i += 256) {
         h[i] = 0;
@@ -867,7 +892,7 @@ __global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::i
     __syncthreads();
     // Lane-strided items (neighbouring lanes read neighbouring blocks); the
     // histogram does not depend on the order.
-    const std::uint32_t k0{blockIdx.x * kDcChunk};
+    const std::uint32_t k0{wg.x * kDcChunk};
     const std::uint32_t k1{min(k0 + static_cast<std::uint32_t>(kDcChunk), d.items)};
     std::uint32_t fb{0};
     for (std::uint32_t k{k0 + threadIdx.x}; k < k1; k += 256) {
@@ -897,6 +922,17 @@ __global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::i
     if (threadIdx.x == 0 && cnt) {
         atomicAdd(&fb_count[g], cnt);
     }
+}
+
+__global__ __launch_bounds__(256) void dc_hist_kernel(Geometry geo, const std::int32_t* __restrict__ dc,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const std::int32_t* __restrict__ qf,
+                                                      const std::int8_t* __restrict__ ytox,
+                                                      const std::int8_t* __restrict__ ytob,
+                                                      std::uint32_t* __restrict__ dc_hist,
+                                                      std::uint32_t* __restrict__ meta_hist,
+                                                      std::uint32_t* __restrict__ fb_count) {
+    dc_hist_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, dc, acs, qf, ytox, ytob, dc_hist, meta_hist, fb_count);
 }
 
 struct DcBlobs {
@@ -962,7 +998,7 @@ __device__ void dc_chunk_bits_body(
 // Exclusive scan of chunk totals per group (one workgroup per group).
 // Also clears the words of the group's staging slot that its emission will
 // OR into.
-__global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::uint64_t* __restrict__ chunk_bits,
+__device__ __attribute__((always_inline)) inline void dc_chunk_scan_kernel_body(const Wg& wg, Geometry geo, std::uint64_t* __restrict__ chunk_bits,
                                                              std::uint32_t chunks_per_group,
                                                              const DcBlobs* __restrict__ blobs,
                                                              std::uint32_t* __restrict__ group_bytes,
@@ -970,7 +1006,7 @@ __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::
                                                              std::uint32_t* __restrict__ staging,
                                                              std::uint64_t slot_words) {
     __shared__ std::uint64_t sh[17];
-    const int g{static_cast<int>(blockIdx.x)};
+    const int g{static_cast<int>(wg.x)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
     const std::uint32_t used{(d.items + kDcChunk - 1) / kDcChunk};
     const std::uint32_t t{threadIdx.x};
@@ -997,6 +1033,16 @@ __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::
     }
 }
 
+__global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::uint64_t* __restrict__ chunk_bits,
+                                                             std::uint32_t chunks_per_group,
+                                                             const DcBlobs* __restrict__ blobs,
+                                                             std::uint32_t* __restrict__ group_bytes,
+                                                             std::uint64_t* __restrict__ group_bits,
+                                                             std::uint32_t* __restrict__ staging,
+                                                             std::uint64_t slot_words) {
+    dc_chunk_scan_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, chunk_bits, chunks_per_group, blobs, group_bytes, group_bits, staging, slot_words);
+}
+
 // Emission of one chunk of a DC group into the group's staging slot, at
 // group-relative bit positions: the chunk's items occupy one contiguous bit
 // range, assembled in LDS (interior words plain-stored, the two boundary
@@ -1004,7 +1050,7 @@ __global__ __launch_bounds__(1024) void dc_chunk_scan_kernel(Geometry geo, std::
 // emitted with global atomics instead.
 constexpr int kDcEmitWords = 2048;
 
-__global__ __launch_bounds__(256) void dc_emit_kernel(
+__device__ __attribute__((always_inline)) inline void dc_emit_kernel_body(const Wg& wg, 
     Geometry geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
     const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
     const std::int8_t* __restrict__ ytob, const std::uint8_t* __restrict__ dc_depth,
@@ -1014,13 +1060,13 @@ __global__ __launch_bounds__(256) void dc_emit_kernel(
     std::uint32_t chunks_per_group, std::uint32_t* __restrict__ staging, std::uint64_t slot_words) {
     __shared__ std::uint64_t sh[5];
     __shared__ std::uint32_t buf[kDcEmitWords];
-    const int g{static_cast<int>(blockIdx.y)};
+    const int g{static_cast<int>(wg.y)};
     const int tid{static_cast<int>(threadIdx.x)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
-    if (blockIdx.x * kDcChunk >= d.items) {
+    if (wg.x * kDcChunk >= d.items) {
         return;
     }
-    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * kDcPerThread};
+    const std::uint32_t k0{wg.x * kDcChunk + threadIdx.x * kDcPerThread};
     const std::uint32_t k1{min(k0 + kDcPerThread, d.items)};
     std::uint64_t run{0};
     if (k0 < k1) {
@@ -1039,7 +1085,7 @@ __global__ __launch_bounds__(256) void dc_emit_kernel(
     const std::uint64_t rel0{block_excl_scan(run, sh, total)};
     std::uint32_t* const out_words{staging + static_cast<std::size_t>(g) * slot_words};
     const std::uint64_t cap_words{slot_words};
-    const std::uint64_t start{chunk_off[static_cast<std::size_t>(g) * chunks_per_group + blockIdx.x]};
+    const std::uint64_t start{chunk_off[static_cast<std::size_t>(g) * chunks_per_group + wg.x]};
     const std::uint64_t w0{start >> 5};
     const std::uint64_t nw{((start + total + 31) >> 5) - w0};
     const bool staged{nw <= static_cast<std::uint64_t>(kDcEmitWords)};
@@ -1095,6 +1141,17 @@ __global__ __launch_bounds__(256) void dc_emit_kernel(
             out_words[gi] = v;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void dc_emit_kernel(
+    Geometry geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
+    const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
+    const std::int8_t* __restrict__ ytob, const std::uint8_t* __restrict__ dc_depth,
+    const std::uint16_t* __restrict__ dc_codes, const std::uint8_t* __restrict__ meta_depth,
+    const std::uint16_t* __restrict__ meta_codes, const DcBlobs* __restrict__ blobs,
+    const std::uint32_t* __restrict__ blob_words, const std::uint64_t* __restrict__ chunk_off,
+    std::uint32_t chunks_per_group, std::uint32_t* __restrict__ staging, std::uint64_t slot_words) {
+    dc_emit_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, dc, acs, qf, ytox, ytob, dc_depth, dc_codes, meta_depth, meta_codes, blobs, blob_words, chunk_off, chunks_per_group, staging, slot_words);
 }
 
 }  // namespace cjx

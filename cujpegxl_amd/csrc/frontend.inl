@@ -28,10 +28,11 @@ constexpr int kTileW = 32 + 2 * kHalo;  // 36
 // each side.
 constexpr int kChromaW = kTileW / 2 + 2;  // 20
 
-// Grid-strided zeroing of the Clears buffers by every thread of a launch.
-__device__ __forceinline__ void clear_words(const Clears& c) {
-    const std::uint32_t stride{gridDim.x * blockDim.x};
-    const std::uint32_t t{blockIdx.x * blockDim.x + threadIdx.x};
+// Grid-strided zeroing of the Clears buffers by every thread of a launch
+// (of the frame's workgroups, in a batched launch).
+__device__ __forceinline__ void clear_words(const Clears& c, const Wg& wg) {
+    const std::uint32_t stride{wg.nx * wg.ny * blockDim.x};
+    const std::uint32_t t{(wg.y * wg.nx + wg.x) * blockDim.x + threadIdx.x};
 #pragma unroll
     for (int k{0}; k < 4; ++k) {
         for (std::uint32_t i{t}; i < c.words[k]; i += stride) {
@@ -52,47 +53,29 @@ __device__ __forceinline__ float clamp01(float v) {
     return fminf(fmaxf(v, 0.0f), 1.0f);
 }
 
-// x^2.4 as pow24_det (exp2_det(2.4 log2_det x)) for x in [0.05, 1], the sRGB
-// curve's upper branch: 2.4 log2 x lies in [-10.4, 0], where exp2_det's
-// clamps to [-126, 127] change nothing, so they are left out here; every
-// other operation is exp2_det's, in the same order.
-__device__ __forceinline__ float pow24_unit(float x) {
-    const float t{2.4f * log2_det(x)};
-    const float n{__builtin_rintf(t)};
-    const float g{(t - n) * 0.693147181f};
-    float p{1.98412698e-4f};
-    p = __builtin_fmaf(p, g, 1.38888889e-3f);
-    p = __builtin_fmaf(p, g, 8.33333333e-3f);
-    p = __builtin_fmaf(p, g, 4.16666667e-2f);
-    p = __builtin_fmaf(p, g, 0.166666667f);
-    p = __builtin_fmaf(p, g, 0.5f);
-    p = __builtin_fmaf(p, g, 1.0f);
-    p = __builtin_fmaf(p, g, 1.0f);
-    return p * u2f(static_cast<std::uint32_t>(static_cast<int>(n) + 127) << 23);
+// xyb_impl.cuh:37-70 on the shared numeric primitives (the powers from the
+// table at `tab`, staged in LDS).
+__device__ __forceinline__ float srgb_to_linear(float e, const float* tab) {
+    return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_tab((e + 0.055f) * (1.0f / 1.055f), tab);
 }
 
-// xyb_impl.cuh:37-70 on the shared numeric primitives.
-__device__ __forceinline__ float srgb_to_linear(float e) {
-    return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_unit((e + 0.055f) * (1.0f / 1.055f));
-}
-
-__device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float& ox, float& oy,
-                                             float& ob) {
+__device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float& ox, float& oy, float& ob,
+                                             const float* tab) {
     constexpr float m00{0.30f}, m02{0.078f}, m01{1.0f - 0.078f - 0.30f};
     constexpr float m10{0.23f}, m12{0.078f}, m11{1.0f - 0.078f - 0.23f};
     constexpr float m20{0.24342268924547819f}, m21{0.20476744424496821f};
     constexpr float m22{1.0f - 0.24342268924547819f - 0.20476744424496821f};
     constexpr float bias{0.0037930732552754493f};
-    const float r{clamp01(srgb_to_linear(clamp01(yv + 1.5748f * cr)))};
-    const float g{clamp01(srgb_to_linear(clamp01(yv - 0.1873242729f * cb - 0.4681242729f * cr)))};
-    const float b{clamp01(srgb_to_linear(clamp01(yv + 1.8556f * cb)))};
+    const float r{clamp01(srgb_to_linear(clamp01(yv + 1.5748f * cr), tab))};
+    const float g{clamp01(srgb_to_linear(clamp01(yv - 0.1873242729f * cb - 0.4681242729f * cr), tab))};
+    const float b{clamp01(srgb_to_linear(clamp01(yv + 1.8556f * cb), tab))};
     float q0{fmaxf(m00 * r + m01 * g + m02 * b + bias, 0.0f)};
     float q1{fmaxf(m10 * r + m11 * g + m12 * b + bias, 0.0f)};
     float q2{fmaxf(m20 * r + m21 * g + m22 * b + bias, 0.0f)};
-    const float neg_cbrt_bias{-cbrt_det(bias)};
-    q0 = cbrt_det(q0) + neg_cbrt_bias;
-    q1 = cbrt_det(q1) + neg_cbrt_bias;
-    q2 = cbrt_det(q2) + neg_cbrt_bias;
+    const float neg_cbrt_bias{-cbrt_tab(bias, tab)};
+    q0 = cbrt_tab(q0, tab) + neg_cbrt_bias;
+    q1 = cbrt_tab(q1, tab) + neg_cbrt_bias;
+    q2 = cbrt_tab(q2, tab) + neg_cbrt_bias;
     ox = 0.5f * (q0 - q1);
     oy = 0.5f * (q0 + q1);
     ob = q2;
@@ -341,8 +324,13 @@ struct RegionShared {
     float cost8[16];       // quad-major
     float cost16[4];
     float cost32;
+    float cost32p[4];      // matrix-core form: the DCT32 cost's part from each wave
     float cfl[4][3];
     int acs[16];           // region-local raster
+    float feat[112];       // network features (block sums first; learned mode and export)
+    int blkqf[16];         // quant field of the region's blocks (export)
+    int choice[4];         // the heuristic's choice per quad (export)
+    int learned_ok;        // the network's decision was confident
 };
 // LDS is allocated in 1,280-byte blocks: five workgroups per CU (160 KB)
 // need at most 25 blocks each.
@@ -432,6 +420,11 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
                               2u * static_cast<std::uint32_t>(x)};
         S.u.f.t.chroma[i] = static_cast<std::uint16_t>(p[0] | (p[1] << 8));
     }
+    // The powers' table, in the sharpened planes' space (free until step 2).
+    float* const tab{&S.p.sharp[0][0]};
+    for (int i{tid}; i < kXybTabSize; i += 256) {
+        tab[i] = g_tables.xyb_tab[i];
+    }
     __syncthreads();
     for (int i{tid}; i < kTileW * kTileW; i += 256) {
         const int lx{i % kTileW}, ly{i / kTileW};
@@ -455,7 +448,7 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         const float yv{static_cast<float>(lrow[static_cast<std::uint32_t>(gx)]) * (1.0f / 255.0f)};
         const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
         const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
-        pixel_to_xyb(yv, cb, cr, S.u.f.xyb[0][i], S.u.f.xyb[1][i], S.u.f.xyb[2][i]);
+        pixel_to_xyb(yv, cb, cr, S.u.f.xyb[0][i], S.u.f.xyb[1][i], S.u.f.xyb[2][i], tab);
     }
 }
 
@@ -483,16 +476,333 @@ __device__ __forceinline__ void gaborish_tile(RegionShared& S, int tid) {
     }
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void region_kernel(
+// ---------------------------------------------------------------------------
+// Matrix-core DCT form with FP16 operands (kModeMfma16).  Every transform is
+// built from v_mfma_f32_16x16x16_f16 steps: D[i][j] += sum_k A[i][k] B[k][j]
+// over a 16x16x16 tile, lane l supplying A[l % 16][4 (l / 16) + 0..3] and
+// B[4 (l / 16) + 0..3][l % 16] and holding D[4 (l / 16) + u][l % 16], u =
+// 0..3.  A row pass T = P B^T takes the pixels as A and the basis as B; the
+// column pass C = B T takes the basis as A and T, exactly as the row pass
+// leaves it in the lanes, as B: no data moves between the passes.  A 16x16
+// quad is one tile per pass (basis A16, or blockdiag(A8, A8) for its four
+// DCT8s); a DCT32 output tile is two K = 16 steps per pass.  Every operand is
+// split into FP16 hi + lo (x = hi + lo to about 2^-22 relative) and each
+// product is lo*hi + hi*lo + hi*hi in FP32 (lo*lo dropped), so the DC and
+// the low frequencies keep about FP32 accuracy.  Pixels are scaled by 1/4
+// (exact) into |x| <= 1 (sharpened XYB stays within 2.97 * 0.9) and the
+// results by 4.
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+constexpr float kMfScale = 0.25f;
+
+struct MfOp {
+    h4v hi, lo;
+};
+
+__device__ __forceinline__ MfOp mf_split(float a, float b, float c, float d) {
+    MfOp o;
+    const float v[4]{a, b, c, d};
+#pragma unroll
+    for (int i{0}; i < 4; ++i) {
+        const _Float16 h{static_cast<_Float16>(v[i])};
+        o.hi[i] = h;
+        o.lo[i] = static_cast<_Float16>(v[i] - static_cast<float>(h));
+    }
+    return o;
+}
+
+// Row r of an FP16 basis table pair (n columns), columns c0 .. c0 + 3.
+__device__ __forceinline__ MfOp mf_basis(const std::uint16_t* hi, const std::uint16_t* lo, int n, int r, int c0) {
+    MfOp o;
+    o.hi = *reinterpret_cast<const h4v*>(hi + r * n + c0);
+    o.lo = *reinterpret_cast<const h4v*>(lo + r * n + c0);
+    return o;
+}
+
+// acc + A B with A = a.hi + a.lo, B = b.hi + b.lo; the small products first.
+__device__ __forceinline__ f4v mf_mma(const MfOp& a, const MfOp& b, f4v acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, acc, 0, 0, 0);
+}
+
+// Pixels (row, c0 .. c0 + 3) of a sharpened plane, scaled and split.
+__device__ __forceinline__ MfOp mf_pixels(const float* plane, int row, int c0) {
+    const float4 p{*reinterpret_cast<const float4*>(plane + sp_index(row, c0))};
+    return mf_split(p.x * kMfScale, p.y * kMfScale, p.z * kMfScale, p.w * kMfScale);
+}
+
+// The 16x16 quad at (row0, col0) of `plane` through basis tables (hi, lo):
+// lane l returns C[4 (l / 16) + u][l % 16] (row = vertical frequency).
+__device__ __forceinline__ f4v mf_quad(const float* plane, int row0, int col0, const std::uint16_t* hi,
+                                       const std::uint16_t* lo, int lane) {
+    const int m{lane & 15}, g{lane >> 4};
+    const MfOp b{mf_basis(hi, lo, 16, m, 4 * g)};
+    const f4v zero{0.0f, 0.0f, 0.0f, 0.0f};
+    const f4v t{mf_mma(mf_pixels(plane, row0 + m, col0 + 4 * g), b, zero)};
+    const f4v c{mf_mma(b, mf_split(t[0], t[1], t[2], t[3]), zero)};
+    return c * (1.0f / kMfScale);
+}
+
+// DCT32 of the region's plane: output tiles (0, fx0) and (16, fx0) (vertical,
+// horizontal frequency origins) into c[0], c[1]; with `both` false only
+// (fy0, fx0) into c[0].
+__device__ __forceinline__ void mf_dct32(const float* plane, int fx0, int fy0, bool both, int lane, f4v (&c)[2]) {
+    const int m{lane & 15}, g{lane >> 4};
+    const std::uint16_t* hi{g_tables.mf32[0]};
+    const std::uint16_t* lo{g_tables.mf32[1]};
+    const f4v zero{0.0f, 0.0f, 0.0f, 0.0f};
+    MfOp t[2];
+#pragma unroll
+    for (int h{0}; h < 2; ++h) {  // row-pass tiles at rows 16 h, frequencies fx0..
+        f4v acc{zero};
+#pragma unroll
+        for (int k{0}; k < 2; ++k) {
+            acc = mf_mma(mf_pixels(plane, 16 * h + m, 16 * k + 4 * g), mf_basis(hi, lo, 32, fx0 + m, 16 * k + 4 * g),
+                         acc);
+        }
+        t[h] = mf_split(acc[0], acc[1], acc[2], acc[3]);
+    }
+#pragma unroll
+    for (int o{0}; o < 2; ++o) {
+        if (o == 1 && !both) {
+            break;
+        }
+        const int f0{both ? 16 * o : fy0};
+        f4v acc{zero};
+#pragma unroll
+        for (int h{0}; h < 2; ++h) {
+            acc = mf_mma(mf_basis(hi, lo, 32, f0 + m, 16 * h + 4 * g), t[h], acc);
+        }
+        c[o] = acc * (1.0f / kMfScale);
+    }
+}
+
+// Lane l's four elements of a quad result, as (block j of the quad, element
+// e of the block's transform): DCT16 (n = 16) e = fx * 16 + fy, j = 0; four
+// DCT8s (n = 8) j = 2 (fy / 8) + fx / 8, e = (fx % 8) * 8 + fy % 8.
+__device__ __forceinline__ void mf_quad_element(int lane, int u, int n, int& j, int& e) {
+    const int fx{lane & 15}, fy{4 * (lane >> 4) + u};
+    if (n == 16) {
+        j = 0;
+        e = fx * 16 + fy;
+    } else {
+        j = 2 * (fy >> 3) + (fx >> 3);
+        e = (fx & 7) * 8 + (fy & 7);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The learned transform selection (kModeLearned) and the feature export of
+// the plain path that trains it.  Features of a region, 6 per 8x8 block b
+// (region raster, 16 blocks), all from the block's DCT8 of the sharpened Y
+// (the selection's DCT8 candidate, C[e] with e = fx * 8 + fy) and its
+// selection terms:
+//   f[6b + 0] = log2(1 + 4096 var)     var  = sum over AC of C^2
+//   f[6b + 1] = log2(1 + 1024 grad)    grad = (64 / 112) sum (l[fx] + l[fy]) C^2
+//   f[6b + 2] = log2(1 + 64 cost8)     the block's DCT8 RD cost
+//   f[6b + 3] = log2(1 + 262144 dist)  dist = sum of squared quantization errors
+//   f[6b + 4] = log2(1 + rate)         rate = sum of the rate terms
+//   f[6b + 5] = 4 C[0]                 the block's mean
+// With the basis' normalization (C = O P O^T / 8, O orthonormal) var is the
+// pixels' variance and grad the mean squared difference of horizontally and
+// vertically adjacent pixels inside the block (112 pairs): the DCT-II
+// diagonalizes the path's difference operator with eigenvalues
+// l[k] = 4 sin^2(pi k / 16).  Blocks past the frame's edge have all six at 0.
+struct MlpWeights {
+    std::uint16_t w1[64 * 96];  // FP16 bit patterns, row-major [hidden][input]
+    std::uint16_t w2[7 * 64];   // [output][hidden]
+    float b1[64];
+    float b2[8];
+    float threshold;  // |logit| below which a decision is not confident
+};
+
+__device__ __align__(16) MlpWeights g_mlp;
+
+__device__ __forceinline__ float h2f(std::uint16_t h) {
+    return static_cast<float>(__builtin_bit_cast(_Float16, h));
+}
+
+// (64 / 112) l[k] of the gradient feature, k = 0..7.
+__device__ __forceinline__ float grad_weight(int k) {
+    constexpr float w[8]{0.0f, 0.08699481934309006f, 0.33473509550094604f, 0.7055046558380127f,
+                         1.1428571939468384f, 1.5802096128463745f, 1.950979232788086f, 2.1987195014953613f};
+    return w[k];
+}
+
+// Block b's raw sums (var, grad, dist, rate, mean) into S.feat[6 b ..]; the
+// cost sits in S.cost8.
+__device__ __forceinline__ void feature_store(RegionShared& S, int b, float var, float grad, float dist, float rate,
+                                              float mean) {
+    S.feat[6 * b + 0] = var;
+    S.feat[6 * b + 1] = grad;
+    S.feat[6 * b + 3] = dist;
+    S.feat[6 * b + 4] = rate;
+    S.feat[6 * b + 5] = mean;
+}
+
+// The raw sums -> the 96 features in S.feat (every thread calls this;
+// barriers inside).
+__device__ __forceinline__ void feature_finish(RegionShared& S, int tid, int bx0, int by0, int bw, int bh) {
+    float f[6]{};
+    if (tid < 16) {
+        const int bx{tid & 3}, by{tid >> 2};
+        if (bx0 + bx < bw && by0 + by < bh) {
+            const float* s{&S.feat[6 * tid]};
+            const float c8{S.cost8[((by >> 1) * 2 + (bx >> 1)) * 4 + (by & 1) * 2 + (bx & 1)]};
+            f[0] = log2_det(1.0f + 4096.0f * fmaxf(s[0], 0.0f));
+            f[1] = log2_det(1.0f + 1024.0f * fmaxf(s[1], 0.0f));
+            f[2] = log2_det(1.0f + 64.0f * fmaxf(c8, 0.0f));
+            f[3] = log2_det(1.0f + 262144.0f * fmaxf(s[3], 0.0f));
+            f[4] = log2_det(1.0f + fmaxf(s[4], 0.0f));
+            f[5] = 4.0f * s[5];
+        }
+    }
+    __syncthreads();
+    if (tid < 16) {
+#pragma unroll
+        for (int k{0}; k < 6; ++k) {
+            S.feat[6 * tid + k] = f[k];
+        }
+    }
+    __syncthreads();
+}
+
+// The network on the region's features and, when every decision it takes is
+// confident (|logit| >= threshold for the root where a DCT32 fits and for
+// each quad where a DCT16 fits), the transform map in S.acs.  Returns that
+// confidence (the same value in every thread).  `hid` is LDS scratch of 72
+// floats.  Every thread calls this; barriers inside.
+__device__ __forceinline__ bool mlp_decide(RegionShared& S, const FrameParams& prm, float* hid, int tid, int bx0,
+                                           int by0, int bw, int bh) {
+    {
+        const int j{tid >> 2}, k{tid & 3};
+        const std::uint16_t* const w{g_mlp.w1 + j * 96 + 24 * k};
+        const float* const x{&S.feat[24 * k]};
+        float s{0.0f};
+#pragma unroll
+        for (int i{0}; i < 24; i += 8) {
+            const uint4 pw{*reinterpret_cast<const uint4*>(w + i)};
+            const std::uint32_t word[4]{pw.x, pw.y, pw.z, pw.w};
+#pragma unroll
+            for (int h{0}; h < 4; ++h) {
+                s = __builtin_fmaf(h2f(static_cast<std::uint16_t>(word[h] & 0xffffu)), x[i + 2 * h], s);
+                s = __builtin_fmaf(h2f(static_cast<std::uint16_t>(word[h] >> 16)), x[i + 2 * h + 1], s);
+            }
+        }
+        s += __shfl_down(s, 2, 4);
+        s += __shfl_down(s, 1, 4);
+        if (k == 0) {
+            hid[j] = fmaxf(s + g_mlp.b1[j], 0.0f);
+        }
+    }
+    __syncthreads();
+    if (tid < 56) {
+        const int o{tid >> 3}, p{tid & 7};
+        float s{0.0f};
+#pragma unroll
+        for (int q{0}; q < 8; ++q) {
+            s = __builtin_fmaf(h2f(g_mlp.w2[o * 64 + 8 * p + q]), hid[8 * p + q], s);
+        }
+#pragma unroll
+        for (int off{4}; off > 0; off >>= 1) {
+            s += __shfl_down(s, off, 8);
+        }
+        if (p == 0) {
+            hid[64 + o] = s + g_mlp.b2[o];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const float* z{hid + 64};
+        const float th{g_mlp.threshold};
+        const std::uint32_t force{(prm.mode & kModeForceMask) >> kModeForceShift};
+        const bool fit32{bx0 + 4 <= bw && by0 + 4 <= bh};
+        bool ok{force == 0};
+        const float root{z[1] - z[0]};
+        ok = ok && (!fit32 || fabsf(root) >= th);
+        const bool use32{fit32 && root > 0.0f};
+        int choice[4];
+        for (int q{0}; q < 4; ++q) {
+            const int qx{q & 1}, qy{q >> 1};
+            const bool fit16{bx0 + 2 * qx + 2 <= bw && by0 + 2 * qy + 2 <= bh};
+            choice[q] = fit16 && z[2 + q] > 0.0f ? 16 : 8;
+            if (!use32) {
+                ok = ok && (!fit16 || fabsf(z[2 + q]) >= th);
+            }
+        }
+        if (ok) {
+            if (use32) {
+                for (int b{0}; b < 16; ++b) {
+                    S.acs[b] = b == 0 ? 32 : 0;
+                }
+            } else {
+                for (int q{0}; q < 4; ++q) {
+                    const int ox{2 * (q & 1)}, oy{2 * (q >> 1)};
+                    if (choice[q] == 16) {
+                        S.acs[oy * 4 + ox] = 16;
+                        S.acs[oy * 4 + ox + 1] = 0;
+                        S.acs[(oy + 1) * 4 + ox] = 0;
+                        S.acs[(oy + 1) * 4 + ox + 1] = 0;
+                    }
+                }
+            }
+        }
+        S.learned_ok = ok ? 1 : 0;
+    }
+    __syncthreads();
+    return S.learned_ok != 0;
+}
+
+// Training rows of the plain path: the region's 96 features, then its labels
+// (12 floats: DCT32 at the root, DCT16 per quad as the heuristic chose it,
+// the mean log2 quant field of the region's blocks, the root mask (a DCT32
+// fits), the quad masks (a DCT16 fits), 0).
+constexpr int kFeatRow = 96 + 12;
+
+__device__ __forceinline__ void export_features(const RegionShared& S, float* out, int region, int tid, int bx0,
+                                                int by0, int bw, int bh) {
+    float* const row{out + static_cast<std::size_t>(region) * kFeatRow};
+    if (tid < 96) {
+        row[tid] = S.feat[tid];
+    } else if (tid < 108) {
+        const int k{tid - 96};
+        float v{0.0f};
+        if (k == 0) {
+            v = S.acs[0] == 32 ? 1.0f : 0.0f;
+        } else if (k <= 4) {
+            v = S.choice[k - 1] == 16 ? 1.0f : 0.0f;
+        } else if (k == 5) {
+            float s{0.0f};
+            int n{0};
+            for (int b{0}; b < 16; ++b) {
+                if (bx0 + (b & 3) < bw && by0 + (b >> 2) < bh) {
+                    s += log2_det(static_cast<float>(S.blkqf[b]));
+                    ++n;
+                }
+            }
+            v = s / static_cast<float>(n);
+        } else if (k == 6) {
+            v = bx0 + 4 <= bw && by0 + 4 <= bh ? 1.0f : 0.0f;
+        } else if (k <= 10) {
+            const int q{k - 7};
+            v = bx0 + 2 * (q & 1) + 2 <= bw && by0 + 2 * (q >> 1) + 2 <= bh ? 1.0f : 0.0f;
+        }
+        row[96 + k] = v;
+    }
+}
+
+template <bool kMf16, bool kLearned>
+__device__ __attribute__((always_inline)) inline void region_kernel_body(const Wg& wg, 
     Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
     const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
     std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
     std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part, std::uint32_t* __restrict__ list_count,
-    Clears clr) {
+    Clears clr, float* __restrict__ feat_out) {
     __shared__ RegionShared S;
-    clear_words(clr);  // every workgroup of the grid takes part
+    clear_words(clr, wg);  // every workgroup of the grid takes part
     const int tid{static_cast<int>(threadIdx.x)};
-    if (blockIdx.x == 0 && tid < 2 && list_count != nullptr) {
+    if (wg.x == 0 && tid < 2 && list_count != nullptr) {
         list_count[tid] = 0;  // the quantizer's item lists, appended after this kernel
     }
     const int w{tid >> 6};
@@ -502,7 +812,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     // regions so that rows shared by neighbouring regions stay in one L2.
     const int nreg{static_cast<int>(geo.rw * geo.rh)};
     const int per_xcd{(nreg + 7) / 8};
-    const int region{static_cast<int>(blockIdx.x & 7u) * per_xcd + static_cast<int>(blockIdx.x >> 3)};
+    const int region{static_cast<int>(wg.x & 7u) * per_xcd + static_cast<int>(wg.x >> 3)};
     if (region >= nreg) {
         return;
     }
@@ -618,43 +928,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
             int qi{static_cast<int>(qval * prm.inv_global_scale + 0.5f)};
             qi = qi < 1 ? 1 : (qi > 256 ? 256 : qi);
             qf_out[static_cast<std::size_t>(by0 + lby) * bw + bx0 + lbx] = qi;
+            S.blkqf[lb] = qi;
         }
     }
 
-    // 4. RD transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
-    //    The three sizes run side by side: wave 0 the DCT32 and wave 1 the four
-    //    DCT16s on the matrix cores, waves 2-3 the sixteen DCT8s on the VALU.
-    //    Each wave keeps its coefficients' selection terms in registers and
-    //    sums them there (per lane in register order, then a lane tree).
+    // 4. Transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
+    //    Plain: the three sizes run side by side (wave 0 the DCT32 and wave 1
+    //    the four DCT16s on the matrix cores, waves 2-3 the sixteen DCT8s on
+    //    the VALU); each wave keeps its coefficients' selection terms in
+    //    registers and sums them there (per lane in register order, then a
+    //    lane tree).  Learned: the DCT8s and their costs, the region's
+    //    features and the network first; the DCT16 and DCT32 candidates only
+    //    where the network is not confident (the heuristic decides there).
     __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
     float* const lines{S.u.scratch};
     const int grp{lane >> 4}, t16{lane & 15};
     const float* const yp{plane_of(S, 1)};
-    if (wu == 0) {
-        // DCT32 as four 16x16 quadrants: block (fh, rh) = (grp >> 1, grp & 1).
-        const int row{16 * (grp & 1) + t16};
-        const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32, yp + row * kSP, 0,
-                                      sp_swz(row))};
+    const bool want_features{kLearned || feat_out != nullptr};
+    float td[16], tr[16];
+    // Plain pieces: row passes of the DCT32 / DCT16 candidates (waves 0-1)
+    // and of the DCT8s (waves 2-3), then the column passes with the costs.
+    const auto rows_big = [&]() {
+        if (wu == 0) {
+            // DCT32 as four 16x16 quadrants: block (fh, rh) = (grp >> 1, grp & 1).
+            const int row{16 * (grp & 1) + t16};
+            const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32, yp + row * kSP, 0,
+                                          sp_swz(row))};
 #pragma unroll
-        for (int b{0}; b < 4; ++b) {
+            for (int b{0}; b < 4; ++b) {
 #pragma unroll
-            for (int u{0}; u < 4; ++u) {
-                const int f{16 * (b >> 1) + 4 * grp + u}, r{16 * (b & 1) + t16};
-                lines[kLines32 + f * RowBuf<32>::RS + r] = acc[4 * b + u];
+                for (int u{0}; u < 4; ++u) {
+                    const int f{16 * (b >> 1) + 4 * grp + u}, r{16 * (b & 1) + t16};
+                    lines[kLines32 + f * RowBuf<32>::RS + r] = acc[4 * b + u];
+                }
+            }
+        } else {
+            // Block q = quad q.
+            const int row{(grp >> 1) * 16 + t16};
+            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, yp + row * kSP, (grp & 1) * 16, sp_swz(row))};
+#pragma unroll
+            for (int q{0}; q < 4; ++q) {
+#pragma unroll
+                for (int u{0}; u < 4; ++u) {
+                    lines[kLines16 + q * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * q + u];
+                }
             }
         }
-    } else if (wu == 1) {
-        // Block q = quad q.
-        const int row{(grp >> 1) * 16 + t16};
-        const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, yp + row * kSP, (grp & 1) * 16, sp_swz(row))};
-#pragma unroll
-        for (int q{0}; q < 4; ++q) {
-#pragma unroll
-            for (int u{0}; u < 4; ++u) {
-                lines[kLines16 + q * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * q + u];
-            }
-        }
-    } else {
+    };
+    const auto rows8 = [&]() {
         // 16 blocks x 8 rows over 128 lanes, all 8 frequencies per lane.
         const int job{((wu - 2) << 6) | lane};
         const int b{job >> 3}, r{job & 7};
@@ -665,39 +986,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
         for (int f{0}; f < 8; ++f) {
             lines[kLines8 + b * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
         }
-    }
-    __syncthreads();
-    // Column passes: 16 terms per lane on waves 0-1, 8 on waves 2-3.
-    float td[16], tr[16];
-    if (wu == 0) {
-        const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
-                                      &lines[kLines32 + (16 * (grp & 1) + t16) * RowBuf<32>::RS])};
+    };
+    // Column passes: 16 terms per lane on waves 0-1, 8 on waves 2-3; costs
+    // straight from the registers: per lane the sum of its terms in register
+    // order, then the wave tree (DCT32; DCT16 per block) or an 8-lane tree
+    // (DCT8: a block's 64 terms sit in 8 consecutive lanes, one column each).
+    const auto cols_big = [&]() {
+        if (wu == 0) {
+            const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                          &lines[kLines32 + (16 * (grp & 1) + t16) * RowBuf<32>::RS])};
 #pragma unroll
-        for (int b{0}; b < 4; ++b) {
+            for (int b{0}; b < 4; ++b) {
 #pragma unroll
-            for (int u{0}; u < 4; ++u) {
-                const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
-                const int e{fx * 32 + fy};
-                const float v{acc[4 * b + u]};
-                selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, g_tables.rate, prm, td[4 * b + u], tr[4 * b + u]);
-                coefy_of(S, 2)[cpad(e)] = static_cast<_Float16>(v);
+                for (int u{0}; u < 4; ++u) {
+                    const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
+                    const int e{fx * 32 + fy};
+                    const float v{acc[4 * b + u]};
+                    selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, g_tables.rate, prm, td[4 * b + u],
+                                    tr[4 * b + u]);
+                    coefy_of(S, 2)[cpad(e)] = static_cast<_Float16>(v);
+                }
+            }
+            float ad{td[0]}, ar{tr[0]};
+#pragma unroll
+            for (int i{1}; i < 16; ++i) {
+                ad += td[i];
+                ar += tr[i];
+            }
+            const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
+            if (lane == 0) {
+                S.cost32 = 1024.0f * ds + 6.0e-4f * rs;
+            }
+        } else {
+            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                          &lines[kLines16 + grp * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
+#pragma unroll
+            for (int q{0}; q < 4; ++q) {
+#pragma unroll
+                for (int u{0}; u < 4; ++u) {
+                    const int fy{4 * grp + u}, fx{t16};
+                    const int e{fx * 16 + fy};
+                    const float v{acc[4 * q + u]};
+                    selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, g_tables.rate, prm, td[4 * q + u],
+                                    tr[4 * q + u]);
+                    coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(v);
+                }
+            }
+#pragma unroll
+            for (int q{0}; q < 4; ++q) {
+                float ad{td[4 * q]}, ar{tr[4 * q]};
+#pragma unroll
+                for (int u{1}; u < 4; ++u) {
+                    ad += td[4 * q + u];
+                    ar += tr[4 * q + u];
+                }
+                const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
+                if (lane == 0) {
+                    S.cost16[q] = 256.0f * ds + 6.0e-4f * rs;
+                }
             }
         }
-    } else if (wu == 1) {
-        const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
-                                      &lines[kLines16 + grp * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
-#pragma unroll
-        for (int q{0}; q < 4; ++q) {
-#pragma unroll
-            for (int u{0}; u < 4; ++u) {
-                const int fy{4 * grp + u}, fx{t16};
-                const int e{fx * 16 + fy};
-                const float v{acc[4 * q + u]};
-                selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, g_tables.rate, prm, td[4 * q + u], tr[4 * q + u]);
-                coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(v);
-            }
-        }
-    } else {
+    };
+    const auto cols8 = [&]() {
         const int job{((wu - 2) << 6) | lane};
         const int b{job >> 3}, fx{job & 7};
         const int bx{b & 3}, by{b >> 2};
@@ -710,58 +1060,122 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
             selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, g_tables.rate, prm, td[f], tr[f]);
             coefy_of(S, 0)[cpad(base + e)] = static_cast<_Float16>(o[f]);
         }
-    }
-    // Costs straight from the registers: per lane the sum of its terms in
-    // register order, then the wave tree (DCT32; DCT16 per block) or an
-    // 8-lane tree (DCT8: a block's 64 terms sit in 8 consecutive lanes, one
-    // column each).
-    if (wu == 0) {
-        float ad{td[0]}, ar{tr[0]};
-#pragma unroll
-        for (int i{1}; i < 16; ++i) {
-            ad += td[i];
-            ar += tr[i];
-        }
-        const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
-        if (lane == 0) {
-            S.cost32 = 1024.0f * ds + 6.0e-4f * rs;
-        }
-    } else if (wu == 1) {
-#pragma unroll
-        for (int q{0}; q < 4; ++q) {
-            float ad{td[4 * q]}, ar{tr[4 * q]};
-#pragma unroll
-            for (int u{1}; u < 4; ++u) {
-                ad += td[4 * q + u];
-                ar += tr[4 * q + u];
-            }
-            const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
-            if (lane == 0) {
-                S.cost16[q] = 256.0f * ds + 6.0e-4f * rs;
-            }
-        }
-    } else {
         float ad{td[0]}, ar{tr[0]};
 #pragma unroll
         for (int f{1}; f < 8; ++f) {
             ad += td[f];
             ar += tr[f];
         }
+        float fv{0.0f}, fg{0.0f};
+        if (want_features) {
+#pragma unroll
+            for (int f{0}; f < 8; ++f) {
+                const float c2{o[f] * o[f]};
+                fv += (fx | f) != 0 ? c2 : 0.0f;
+                fg = __builtin_fmaf(grad_weight(fx) + grad_weight(f), c2, fg);
+            }
+        }
 #pragma unroll
         for (int off{4}; off > 0; off >>= 1) {
             ad += __shfl_down(ad, off, 8);
             ar += __shfl_down(ar, off, 8);
+            if (want_features) {
+                fv += __shfl_down(fv, off, 8);
+                fg += __shfl_down(fg, off, 8);
+            }
         }
         if ((lane & 7) == 0) {
-            const int b{(((wu - 2) << 6) | lane) >> 3};
-            const int bx{b & 3}, by{b >> 2};
             S.cost8[((by >> 1) * 2 + (bx >> 1)) * 4 + (by & 1) * 2 + (bx & 1)] = 64.0f * ad + 6.0e-4f * ar;
+            if (want_features) {
+                feature_store(S, b, fv, fg, ad, ar, o[0]);
+            }
         }
-    }
-    __syncthreads();
-
-    if (tid == 0) {
+    };
+    // Matrix-core FP16 pieces: wave w transforms quad w as four DCT8s (with
+    // their costs, from registers) and as one DCT16, and output tile w of the
+    // DCT32 (vertical origin 16 (w / 2), horizontal 16 (w % 2)); the DCT16
+    // and DCT32 results stay in registers until every wave has read the Y
+    // plane, whose space then takes their coefficients.
+    const auto mf_dct8_costs = [&]() {
+        const int q{wu};
+        const f4v c8{mf_quad(yp, (q >> 1) * 16, (q & 1) * 16, g_tables.mf8[0], g_tables.mf8[1], lane)};
+        float d8{0.0f}, r8{0.0f}, fv{0.0f}, fg{0.0f};
+#pragma unroll
+        for (int u{0}; u < 4; ++u) {
+            int j, e;
+            float d, rt;
+            mf_quad_element(lane, u, 8, j, e);
+            selection_terms(c8[u], e, 8, g_tables.wsel8, g_tables.wsel_inv8, g_tables.rate, prm, d, rt);
+            coefy_of(S, 0)[cpad(256 * q + 64 * j + e)] = static_cast<_Float16>(c8[u]);
+            d8 += d;
+            r8 += rt;
+            if (want_features) {
+                const float c2{c8[u] * c8[u]};
+                fv += e != 0 ? c2 : 0.0f;
+                fg = __builtin_fmaf(grad_weight(e >> 3) + grad_weight(e & 7), c2, fg);
+            }
+        }
+        // The DC of block (grp / 2, column / 8) sits in lane 16 grp + 8 k,
+        // element u = 0, for even grp.
+        const float dc{c8[0]};
+        // A DCT8 block's 64 terms sit in 16 lanes: bits 0-2 of the column and
+        // bit 0 of the lane group vary within it.
+#pragma unroll
+        for (int off : {1, 2, 4, 16}) {
+            d8 += __shfl_xor(d8, off, 64);
+            r8 += __shfl_xor(r8, off, 64);
+            if (want_features) {
+                fv += __shfl_xor(fv, off, 64);
+                fg += __shfl_xor(fg, off, 64);
+            }
+        }
+        if ((lane & 7) == 0 && (grp & 1) == 0) {
+            const int j{2 * (grp >> 1) + ((lane & 15) >> 3)};
+            S.cost8[q * 4 + j] = 64.0f * d8 + 6.0e-4f * r8;
+            if (want_features) {
+                const int b{(2 * (q >> 1) + (j >> 1)) * 4 + 2 * (q & 1) + (j & 1)};
+                feature_store(S, b, fv, fg, d8, r8, dc);
+            }
+        }
+    };
+    const auto mf_big = [&]() {  // every wave; barriers inside
+        const int q{wu};
+        const f4v c16{mf_quad(yp, (q >> 1) * 16, (q & 1) * 16, g_tables.mf16[0], g_tables.mf16[1], lane)};
+        f4v c32[2];
+        mf_dct32(yp, 16 * (wu & 1), 16 * (wu >> 1), false, lane, c32);
+        __syncthreads();
+        float d16{0.0f}, r16{0.0f}, d32{0.0f}, r32{0.0f};
+#pragma unroll
+        for (int u{0}; u < 4; ++u) {
+            int j, e;
+            float d, rt;
+            mf_quad_element(lane, u, 16, j, e);
+            selection_terms(c16[u], e, 16, g_tables.wsel16, g_tables.wsel_inv16, g_tables.rate, prm, d, rt);
+            coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(c16[u]);
+            d16 += d;
+            r16 += rt;
+            const int fy{16 * (wu >> 1) + 4 * grp + u}, fx{16 * (wu & 1) + t16};
+            selection_terms(c32[0][u], fx * 32 + fy, 32, g_tables.wsel32, g_tables.wsel_inv32, g_tables.rate, prm,
+                            d, rt);
+            coefy_of(S, 2)[cpad(fx * 32 + fy)] = static_cast<_Float16>(c32[0][u]);
+            d32 += d;
+            r32 += rt;
+        }
+        const float ds16{wave_sum(d16)}, rs16{wave_sum(r16)}, ds32{wave_sum(d32)}, rs32{wave_sum(r32)};
+        if (lane == 0) {
+            S.cost16[q] = 256.0f * ds16 + 6.0e-4f * rs16;
+            S.cost32p[q] = 1024.0f * ds32 + 6.0e-4f * rs32;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            S.cost32 = (S.cost32p[0] + S.cost32p[1]) + (S.cost32p[2] + S.cost32p[3]);
+        }
+    };
+    // The heuristic's decision from the costs (thread 0).
+    const auto decide = [&]() {
         constexpr float inf{1.0e30f}, eps{1.0e-6f};
+        // A forced size (test modes): 1 DCT8, 2 DCT16, 3 DCT32 wherever it fits.
+        const std::uint32_t force{(prm.mode & kModeForceMask) >> kModeForceShift};
         const bool fit32{bx0 + 4 <= bw && by0 + 4 <= bh};
         const float c32{fit32 ? S.cost32 : inf};
         float split{0.0f};
@@ -784,7 +1198,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
             }
             const bool fit16{bx0 + 2 * qx + 2 <= bw && by0 + 2 * qy + 2 <= bh};
             const float c16{fit16 ? S.cost16[q] : inf};
-            if (fit16 && c16 <= sum8 + eps) {
+            if (force != 0 ? (force >= 2 && fit16) : (fit16 && c16 <= sum8 + eps)) {
                 choice[q] = 16;
                 split += c16;
             } else {
@@ -792,7 +1206,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
                 split += sum8;
             }
         }
-        if (fit32 && c32 <= split + eps) {
+        const bool root32{force != 0 ? (force == 3 && fit32) : (fit32 && c32 <= split + eps)};
+        for (int q{0}; q < 4; ++q) {
+            S.choice[q] = choice[q];
+        }
+        if (root32) {
             for (int b{0}; b < 16; ++b) {
                 S.acs[b] = b == 0 ? 32 : 0;
             }
@@ -807,8 +1225,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
                 }
             }
         }
+    };
+
+    if constexpr (!kMf16) {
+        if (wu < 2) {
+            if constexpr (!kLearned) {
+                rows_big();
+            }
+        } else {
+            rows8();
+        }
+        __syncthreads();
+        if (wu >= 2) {
+            cols8();
+        } else if constexpr (!kLearned) {
+            cols_big();
+        }
+    } else {
+        mf_dct8_costs();
+        if constexpr (!kLearned) {
+            mf_big();
+        }
     }
     __syncthreads();
+    if (want_features) {
+        feature_finish(S, tid, bx0, by0, bw, bh);
+    }
+    bool fallback{!kLearned};
+    if constexpr (kLearned) {
+        fallback = !mlp_decide(S, prm, lines, tid, bx0, by0, bw, bh);
+        if (fallback) {
+            // The heuristic for this region: the larger candidates and the
+            // decision from the costs.
+            if constexpr (!kMf16) {
+                if (wu < 2) {
+                    rows_big();
+                }
+                __syncthreads();
+                if (wu < 2) {
+                    cols_big();
+                }
+            } else {
+                mf_big();
+            }
+            __syncthreads();
+        }
+    }
+    if (fallback && tid == 0) {
+        decide();
+    }
+    __syncthreads();
+    if (feat_out != nullptr) {
+        export_features(S, feat_out, region, tid, bx0, by0, bw, bh);
+    }
     if (tid < 16) {
         const int bx{bx0 + (tid & 3)}, by{by0 + (tid >> 2)};
         if (bx < bw && by < bh) {
@@ -816,94 +1285,164 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
         }
     }
 
-    // 5. Chosen DCTs of X and B (Y reuses the selection coefficients) into
-    //    the X and B coefficient arrays in element order, then FP16 out and the CfL partials
-    //    (frontend_quantize.cu:49-117 on this region).
+    // 5. Chosen DCTs of X and B (Y reuses the selection coefficients; a region
+    //    the network decided transforms Y here too when it chose a DCT16 or
+    //    DCT32) into the coefficient arrays in element order, then FP16 out
+    //    and the CfL partials (frontend_quantize.cu:49-117 on this region).
+    const bool need_y{kLearned && !fallback};
     const auto cxb = [&](int ch) { return ch == 0 ? S.p.coef.x : S.p.coef.b; };
-    const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
-    if (acs0 == 32) {
-        // Waves 0 and 1 transform X and B on the matrix cores (DCT32 as four
-        // 16x16 quadrants); waves 2-3 wait.
-        const int ch{wu};
-        float* const cl{lines + ch * RowBuf<32>::BS};
-        if (wu < 2) {
-            const int row{16 * (grp & 1) + t16};
-            const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
-                                          plane_of(S, 2 * ch) + row * kSP, 0, sp_swz(row))};
+    if constexpr (!kMf16) {
+        const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
+        if (acs0 == 32) {
+            // Waves 0 and 1 transform X and B on the matrix cores (DCT32 as four
+            // 16x16 quadrants), wave 2 Y when it has no coefficients yet.
+            const int ch{wu};
+            float* const cl{lines + ch * RowBuf<32>::BS};
+            const bool active{wu < 2 || (need_y && wu == 2)};
+            if (active) {
+                const int row{16 * (grp & 1) + t16};
+                const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                              plane_of(S, ch == 0 ? 0 : (ch == 1 ? 2 : 1)) + row * kSP, 0,
+                                              sp_swz(row))};
 #pragma unroll
-            for (int b{0}; b < 4; ++b) {
+                for (int b{0}; b < 4; ++b) {
 #pragma unroll
-                for (int u{0}; u < 4; ++u) {
-                    cl[(16 * (b >> 1) + 4 * grp + u) * RowBuf<32>::RS + 16 * (b & 1) + t16] = acc[4 * b + u];
+                    for (int u{0}; u < 4; ++u) {
+                        cl[(16 * (b >> 1) + 4 * grp + u) * RowBuf<32>::RS + 16 * (b & 1) + t16] = acc[4 * b + u];
+                    }
+                }
+            }
+            __syncthreads();
+            if (active) {
+                const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
+                                              &cl[(16 * (grp & 1) + t16) * RowBuf<32>::RS])};
+                _Float16* const dst{ch < 2 ? cxb(ch) : coefy_of(S, 2)};
+#pragma unroll
+                for (int b{0}; b < 4; ++b) {
+#pragma unroll
+                    for (int u{0}; u < 4; ++u) {
+                        const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
+                        dst[cpad(fx * 32 + fy)] = static_cast<_Float16>(acc[4 * b + u]);
+                    }
+                }
+            }
+        } else {
+            // Wave w transforms quad w: one DCT16 on the matrix cores (block 0 =
+            // X, block 1 = B, block 2 = Y when it has no coefficients yet, else
+            // unused; block 3 unused) or four DCT8s on the VALU (lanes:
+            // channel, block, row).
+            constexpr int kQuadLines{3 * RowBuf<16>::BS};
+            static_assert(kQuadLines >= 2 * 4 * RowBuf<8>::BS && 4 * kQuadLines <= kScratch, "quad lines");
+            float* const ql{lines + wu * kQuadLines};
+            const int q{wu};
+            const int n16{__builtin_amdgcn_readfirstlane(S.acs[(q >> 1) * 8 + (q & 1) * 2])};
+            // Block grp's plane: X, B, then Y (or X again), B again; the
+            // results of repeated blocks are dropped.
+            const int ch16{grp == 2 ? (need_y ? 1 : 0) : (grp & 1) * 2};
+            if (n16 == 16) {
+                const int row{(q >> 1) * 16 + t16};
+                const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, plane_of(S, ch16) + row * kSP,
+                                              (q & 1) * 16, sp_swz(row))};
+#pragma unroll
+                for (int b{0}; b < 3; ++b) {
+#pragma unroll
+                    for (int u{0}; u < 4; ++u) {
+                        ql[b * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * b + u];
+                    }
+                }
+            } else {
+                const int ch{lane >> 5}, j{(lane >> 3) & 3}, r{lane & 7};
+                const int bx{2 * (q & 1) + (j & 1)}, by{2 * (q >> 1) + (j >> 1)};
+                float o[8];
+                const int row{by * 8 + r};
+                dct_line<8, 8>(plane_of(S, 2 * ch) + row * kSP, 0, o, bx * 8, sp_swz(row));
+#pragma unroll
+                for (int f{0}; f < 8; ++f) {
+                    ql[(ch * 4 + j) * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
+                }
+            }
+            __syncthreads();
+            if (n16 == 16) {
+                const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                              &ql[(grp < 3 ? grp : 1) * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
+#pragma unroll
+                for (int b{0}; b < 3; ++b) {
+                    if (b == 2 && !need_y) {
+                        break;
+                    }
+                    _Float16* const dst{b < 2 ? cxb(b) : coefy_of(S, 1)};
+#pragma unroll
+                    for (int u{0}; u < 4; ++u) {
+                        dst[cpad(256 * q + t16 * 16 + 4 * grp + u)] = static_cast<_Float16>(acc[4 * b + u]);
+                    }
+                }
+            } else {
+                const int ch{lane >> 5}, j{(lane >> 3) & 3}, fx{lane & 7};
+                float o[8];
+                dct_line<8, 8>(&ql[(ch * 4 + j) * RowBuf<8>::BS + fx * RowBuf<8>::RS], 0, o);
+#pragma unroll
+                for (int f{0}; f < 8; ++f) {
+                    cxb(ch)[cpad(256 * q + 64 * j + fx * 8 + f)] = static_cast<_Float16>(o[f]);
                 }
             }
         }
         __syncthreads();
-        if (wu < 2) {
-            const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
-                                          &cl[(16 * (grp & 1) + t16) * RowBuf<32>::RS])};
-#pragma unroll
-            for (int b{0}; b < 4; ++b) {
-#pragma unroll
-                for (int u{0}; u < 4; ++u) {
-                    const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
-                    cxb(ch)[cpad(fx * 32 + fy)] = static_cast<_Float16>(acc[4 * b + u]);
-                }
-            }
-        }
     } else {
-        // Wave w transforms quad w: one DCT16 on the matrix cores (block 0 = X,
-        // block 1 = B, blocks 2-3 unused) or four DCT8s on the VALU (lanes:
-        // channel, block, row).
-        constexpr int kQuadLines{2 * 4 * RowBuf<8>::BS};
-        float* const ql{lines + wu * kQuadLines};
-        const int q{wu};
-        const int n16{__builtin_amdgcn_readfirstlane(S.acs[(q >> 1) * 8 + (q & 1) * 2])};
-        const int ch16{grp & 1};  // lanes 32-63 repeat blocks 0-1; their results are dropped
-        if (n16 == 16) {
-            const int row{(q >> 1) * 16 + t16};
-            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, plane_of(S, 2 * ch16) + row * kSP,
-                                          (q & 1) * 16, sp_swz(row))};
+        // Matrix-core FP16 form; results in registers until every wave has
+        // read the planes, whose space then takes their coefficients.
+        const int acs0{__builtin_amdgcn_readfirstlane(S.acs[0])};
+        if (acs0 == 32) {
+            // Wave w: channel w % 2 (X, B), horizontal frequencies 16 (w / 2)..;
+            // waves 0-1 also Y's half w when it has no coefficients yet.
+            const int ch{wu & 1}, fx0{16 * (wu >> 1)};
+            f4v c[2], cy[2];
+            mf_dct32(plane_of(S, 2 * ch), fx0, 0, true, lane, c);
+            if (need_y && wu < 2) {
+                mf_dct32(plane_of(S, 1), 16 * wu, 0, true, lane, cy);
+            }
+            __syncthreads();
 #pragma unroll
-            for (int b{0}; b < 2; ++b) {
+            for (int o{0}; o < 2; ++o) {
 #pragma unroll
                 for (int u{0}; u < 4; ++u) {
-                    ql[b * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * b + u];
+                    const int fy{16 * o + 4 * grp + u}, fx{fx0 + t16};
+                    cxb(ch)[cpad(fx * 32 + fy)] = static_cast<_Float16>(c[o][u]);
+                    if (need_y && wu < 2) {
+                        const int fxy{16 * wu + t16};
+                        coefy_of(S, 2)[cpad(fxy * 32 + fy)] = static_cast<_Float16>(cy[o][u]);
+                    }
                 }
             }
         } else {
-            const int ch{lane >> 5}, j{(lane >> 3) & 3}, r{lane & 7};
-            const int bx{2 * (q & 1) + (j & 1)}, by{2 * (q >> 1) + (j >> 1)};
-            float o[8];
-            const int row{by * 8 + r};
-            dct_line<8, 8>(plane_of(S, 2 * ch) + row * kSP, 0, o, bx * 8, sp_swz(row));
+            // Wave w transforms quad w of X and B (and of Y, when it has no
+            // coefficients yet and takes a DCT16): one DCT16 or four DCT8s.
+            const int q{wu};
+            const int n16{__builtin_amdgcn_readfirstlane(S.acs[(q >> 1) * 8 + (q & 1) * 2])};
+            const int n{n16 == 16 ? 16 : 8};
+            const std::uint16_t* hi{n == 16 ? g_tables.mf16[0] : g_tables.mf8[0]};
+            const std::uint16_t* lo{n == 16 ? g_tables.mf16[1] : g_tables.mf8[1]};
+            const int qr{(q >> 1) * 16}, qc{(q & 1) * 16};
+            const f4v cx{mf_quad(plane_of(S, 0), qr, qc, hi, lo, lane)};
+            const f4v cb{mf_quad(plane_of(S, 2), qr, qc, hi, lo, lane)};
+            const bool y16{need_y && n == 16};
+            f4v cy{};
+            if (y16) {
+                cy = mf_quad(plane_of(S, 1), qr, qc, hi, lo, lane);
+            }
+            __syncthreads();
 #pragma unroll
-            for (int f{0}; f < 8; ++f) {
-                ql[(ch * 4 + j) * RowBuf<8>::BS + f * RowBuf<8>::RS + r] = o[f];
+            for (int u{0}; u < 4; ++u) {
+                int j, e;
+                mf_quad_element(lane, u, n, j, e);
+                cxb(0)[cpad(256 * q + 64 * j + e)] = static_cast<_Float16>(cx[u]);
+                cxb(1)[cpad(256 * q + 64 * j + e)] = static_cast<_Float16>(cb[u]);
+                if (y16) {
+                    coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(cy[u]);
+                }
             }
         }
         __syncthreads();
-        if (n16 == 16) {
-            const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
-                                          &ql[ch16 * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
-#pragma unroll
-            for (int b{0}; b < 2; ++b) {
-#pragma unroll
-                for (int u{0}; u < 4; ++u) {
-                    cxb(b)[cpad(256 * q + t16 * 16 + 4 * grp + u)] = static_cast<_Float16>(acc[4 * b + u]);
-                }
-            }
-        } else {
-            const int ch{lane >> 5}, j{(lane >> 3) & 3}, fx{lane & 7};
-            float o[8];
-            dct_line<8, 8>(&ql[(ch * 4 + j) * RowBuf<8>::BS + fx * RowBuf<8>::RS], 0, o);
-#pragma unroll
-            for (int f{0}; f < 8; ++f) {
-                cxb(ch)[cpad(256 * q + 64 * j + fx * 8 + f)] = static_cast<_Float16>(o[f]);
-            }
-        }
     }
-    __syncthreads();
 
     const std::size_t cplane{static_cast<std::size_t>(geo.nblocks) * 64};
     float acc[3]{0.0f, 0.0f, 0.0f};
@@ -960,6 +1499,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     }
 }
 
+template <bool kMf16, bool kLearned>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void region_kernel(
+    Geometry geo, FrameParams prm, const std::uint8_t* __restrict__ luma, std::size_t luma_pitch,
+    const std::uint8_t* __restrict__ chroma, std::size_t chroma_pitch,
+    std::uint16_t* __restrict__ coeffs, std::int8_t* __restrict__ acs_out,
+    std::int32_t* __restrict__ qf_out, float* __restrict__ cfl_part, std::uint32_t* __restrict__ list_count,
+    Clears clr, float* __restrict__ feat_out) {
+    region_kernel_body<kMf16, kLearned>(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, prm, luma, luma_pitch, chroma, chroma_pitch, coeffs, acs_out, qf_out, cfl_part, list_count, clr, feat_out);
+}
+
 // ---------------------------------------------------------------------------
 
 struct TileShared {
@@ -993,12 +1542,12 @@ __device__ __forceinline__ int owner_in_tile(const std::int8_t* acs, int bw, int
 // coefficients; then the DC quantization, B against Y's round trip) and,
 // for i below the tile count, the colour maps of tile i for the AcMetadata
 // (the quantizer derives its tile's maps with the same function).
-__global__ __launch_bounds__(256) void dc_maps_kernel(Geometry geo, FrameParams prm,
+__device__ __attribute__((always_inline)) inline void dc_maps_kernel_body(const Wg& wg, Geometry geo, FrameParams prm,
                                                       const std::uint16_t* __restrict__ coeffs,
                                                       const std::int8_t* __restrict__ acs,
                                                       const float* __restrict__ cfl_part, std::int32_t* __restrict__ dc,
                                                       std::int8_t* __restrict__ ytox, std::int8_t* __restrict__ ytob) {
-    const int i{static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x)};
+    const int i{static_cast<int>(wg.x * blockDim.x + threadIdx.x)};
     if (i < static_cast<int>(geo.cmw * geo.cmh)) {
         float r[4][3];
         tile_partials(geo, cfl_part, i % static_cast<int>(geo.cmw), i / static_cast<int>(geo.cmw), r);
@@ -1048,7 +1597,15 @@ __global__ __launch_bounds__(256) void dc_maps_kernel(Geometry geo, FrameParams 
     dc[2 * nblk + i] = static_cast<int>(__builtin_rintf((dcv[2] - 1.0f * y_dc_round) * dcq_b));
 }
 
-__global__ __launch_bounds__(256) void tile_quantize_kernel(
+__global__ __launch_bounds__(256) void dc_maps_kernel(Geometry geo, FrameParams prm,
+                                                      const std::uint16_t* __restrict__ coeffs,
+                                                      const std::int8_t* __restrict__ acs,
+                                                      const float* __restrict__ cfl_part, std::int32_t* __restrict__ dc,
+                                                      std::int8_t* __restrict__ ytox, std::int8_t* __restrict__ ytob) {
+    dc_maps_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, prm, coeffs, acs, cfl_part, dc, ytox, ytob);
+}
+
+__device__ __attribute__((always_inline)) inline void tile_quantize_kernel_body(const Wg& wg, 
     Geometry geo, FrameParams prm, const std::uint16_t* __restrict__ coeffs,
     const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
     const float* __restrict__ cfl_part, std::int16_t* __restrict__ ac,
@@ -1057,7 +1614,7 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
     const int tid{static_cast<int>(threadIdx.x)};
     const int w{tid >> 6};
     const int lane{tid & 63};
-    const int tile{static_cast<int>(blockIdx.x)};
+    const int tile{static_cast<int>(wg.x)};
     const int tx{tile % static_cast<int>(geo.cmw)}, ty{tile / static_cast<int>(geo.cmw)};
     const int bw{static_cast<int>(geo.bw)}, bh{static_cast<int>(geo.bh)};
     const std::size_t nblk{geo.nblocks};
@@ -1191,6 +1748,14 @@ __global__ __launch_bounds__(256) void tile_quantize_kernel(
             }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void tile_quantize_kernel(
+    Geometry geo, FrameParams prm, const std::uint16_t* __restrict__ coeffs,
+    const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
+    const float* __restrict__ cfl_part, std::int16_t* __restrict__ ac,
+    std::uint8_t* __restrict__ nz_grid, std::uint16_t* __restrict__ nz_count, ItemList list) {
+    tile_quantize_kernel_body(Wg{blockIdx.x, blockIdx.y, gridDim.x, gridDim.y}, geo, prm, coeffs, acs, qf, cfl_part, ac, nz_grid, nz_count, list);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,7 +37,22 @@ void launch_region(hipStream_t s, const Geometry& geo, const FrameParams& prm, c
                    std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,
                    std::uint16_t* coeffs, std::int8_t* acs, std::int32_t* qf, float* cfl_part,
                    std::uint32_t* list_count, const Clears& clr, hipEvent_t start = nullptr,
-                   hipEvent_t stop = nullptr);
+                   hipEvent_t stop = nullptr, float* features = nullptr);
+// The selection network's weights for the region kernel (FP16 W1 [64][96]
+// and W2 [7][64] bit patterns, FP32 biases, the confidence threshold as a
+// logit magnitude).
+bool set_mlp_weights(const std::uint16_t* w1, const float* b1, const std::uint16_t* w2, const float* b2,
+                     float threshold);
+// The weights the library was built with (csrc/mlp_weights.inc).
+bool set_default_mlp_weights();
+// Training: partial-gradient floats for a batch of n regions; one step
+// (gradient of the batch's mean loss into `grad`, the loss into `loss[0]`,
+// and with lr > 0 an Adam step on params / m / v); forward outputs [n][7].
+std::size_t mlp_partial_floats(std::uint32_t n);
+void launch_mlp_step(hipStream_t s, const float* x, const float* labels, std::uint32_t n, float lambda,
+                     float* partial, float* grad, float* loss, float* params, float* m, float* v, std::uint32_t step,
+                     float lr);
+void launch_mlp_forward(hipStream_t s, const float* params, const float* x, std::uint32_t n, float* z);
 // Config-1 semantics: DCT8 everywhere, uniform quant field, no CfL maps.
 void launch_dct8_uniform(hipStream_t s, const Geometry& geo, const FrameParams& prm, const std::uint8_t* luma,
                          std::size_t luma_pitch, const std::uint8_t* chroma, std::size_t chroma_pitch,

@@ -66,27 +66,6 @@ CJX_HD float exp2_det(float x) {
     return p * u2f(static_cast<std::uint32_t>(static_cast<int>(n) + 127) << 23);
 }
 
-// Cube root of x >= 0 without division: y ~ x^(-1/3) from an exponent-
-// thirding seed and three Newton steps y <- y (4 - x y^3) / 3, then c = x y^2
-// and one Newton correction c <- c - (c^3 - x) y^2 / 3 (<= 0.75 ulp on
-// [1e-6, 64]).
-CJX_HD float cbrt_det(float x) {
-    if (!(x > 0.0f)) {
-        return 0.0f;
-    }
-    float y{u2f(0x54a23300u - f2u(x) / 3u)};
-    const float x3{x * (1.0f / 3.0f)};
-#pragma unroll
-    for (int i{0}; i < 3; ++i) {
-        const float y3{(y * y) * y};
-        y = y * __builtin_fmaf(-x3, y3, 4.0f / 3.0f);
-    }
-    float c{(x * y) * y};
-    const float r{__builtin_fmaf(c * c, c, -x)};
-    c = __builtin_fmaf(-r, (y * y) * (1.0f / 3.0f), c);
-    return c;
-}
-
 // a / b correctly rounded, given r = RN(1 / b): two Markstein correction
 // steps (the first makes the quotient faithful, the second rounds it
 // correctly).  Valid when a / b and the remainders stay normal; the sign of
@@ -99,8 +78,45 @@ CJX_HD float div_by_recip(float a, float b, float r) {
     return __builtin_fmaf(e, r, q);
 }
 
-CJX_HD float pow24_det(float x) {
-    return exp2_det(2.4f * log2_det(x));
+// Table-driven powers of the XYB conversion, on one table of 272 floats
+// (kXybTab*): the mantissa m in [1, 2) of x = 2^k m falls in segment
+// j = its top 4 bits, with centre c_j = 1 + (j + 1/2) / 16 (exact) and
+// r_j = RN(1 / c_j); t = (m - c_j) r_j (the difference is exact), |t| <=
+// 1/32, and x^a = T[k][j] (1 + t)^a with T[k][j] = RN((2^k c_j)^a) and the
+// binomial series of (1 + t)^a to t^5 (truncation below 3e-11).
+constexpr int kXybTabSeg = 0;      // [16][2]: c_j, r_j
+constexpr int kXybTabPow = 32;     // [5][16]: x^2.4 for k = -4 .. 0
+constexpr int kXybTabCbrt = 112;   // [10][16]: x^(1/3) for k = -9 .. 0
+constexpr int kXybTabSize = 272;
+
+// x^2.4 for x in [2^-4, 2) (the sRGB curve's upper branch: x in [0.0905, 1]).
+CJX_HD float pow24_tab(float x, const float* tab) {
+    const std::uint32_t u{f2u(x)};
+    const std::uint32_t j{(u >> 19) & 15u};
+    const float m{u2f((u & 0x007fffffu) | 0x3f800000u)};
+    const float t{(m - tab[kXybTabSeg + 2 * j]) * tab[kXybTabSeg + 2 * j + 1]};
+    float p{0.010752f};
+    p = __builtin_fmaf(p, t, -0.0336f);
+    p = __builtin_fmaf(p, t, 0.224f);
+    p = __builtin_fmaf(p, t, 1.68f);
+    p = __builtin_fmaf(p, t, 2.4f);
+    p = __builtin_fmaf(p, t, 1.0f);
+    return tab[kXybTabPow + ((u >> 19) - (123u << 4))] * p;
+}
+
+// x^(1/3) for x in [2^-9, 2) (the opsin values: x in [0.0038, 1.004]).
+CJX_HD float cbrt_tab(float x, const float* tab) {
+    const std::uint32_t u{f2u(x)};
+    const std::uint32_t j{(u >> 19) & 15u};
+    const float m{u2f((u & 0x007fffffu) | 0x3f800000u)};
+    const float t{(m - tab[kXybTabSeg + 2 * j]) * tab[kXybTabSeg + 2 * j + 1]};
+    float p{0.0301783264f};
+    p = __builtin_fmaf(p, t, -0.0411522634f);
+    p = __builtin_fmaf(p, t, 0.0617283951f);
+    p = __builtin_fmaf(p, t, -0.111111111f);
+    p = __builtin_fmaf(p, t, 0.333333333f);
+    p = __builtin_fmaf(p, t, 1.0f);
+    return tab[kXybTabCbrt + ((u >> 19) - (118u << 4))] * p;
 }
 
 CJX_HD float exp_det(float x) {

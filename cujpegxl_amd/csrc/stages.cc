@@ -5,8 +5,11 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "common.h"
+#include "batch_encoder.h"
 #include "encoder.h"
 #include "handle.h"
 #include "kernels.h"
@@ -133,6 +136,138 @@ int cjx_stage_set_profiling(cujpegxl_encoder* e, int on) {
     }
     e->engine.set_profiling(on);
     return CUJPEGXL_OK;
+}
+
+int cjx_stage_set_mode(cujpegxl_encoder* e, uint32_t mode) {
+    if (e == nullptr || (mode & ~cjx::kModeMask) != 0) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    e->engine.set_mode(mode);
+    return CUJPEGXL_OK;
+}
+
+int cjx_stage_export_features(cujpegxl_encoder* e, int on) {
+    if (e == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return e->engine.export_features(on != 0) ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_mlp_set_weights(int32_t device, const uint16_t* w1, const float* b1, const uint16_t* w2, const float* b2,
+                        float threshold) {
+    if (w1 == nullptr || b1 == nullptr || w2 == nullptr || b2 == nullptr || !cjx::device_exists(device) ||
+        hipSetDevice(device) != hipSuccess) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    if (!cjx::ensure_device_tables(device)) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    return cjx::set_mlp_weights(w1, b1, w2, b2, threshold) ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_mlp_default_weights(int32_t device) {
+    if (!cjx::device_exists(device) || hipSetDevice(device) != hipSuccess) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return cjx::ensure_device_tables(device) && cjx::set_default_mlp_weights() ? CUJPEGXL_OK
+                                                                                : CUJPEGXL_INTERNAL_ERROR;
+}
+
+size_t cjx_mlp_partial_floats(uint32_t n) {
+    return cjx::mlp_partial_floats(n);
+}
+
+int cjx_mlp_step(const float* x, const float* labels, uint32_t n, float lambda, float* partial, float* grad,
+                 float* loss, float* params, float* m, float* v, uint32_t step, float lr, uintptr_t stream) {
+    if (x == nullptr || labels == nullptr || n == 0 || partial == nullptr || grad == nullptr || loss == nullptr ||
+        params == nullptr || (lr > 0.0f && (m == nullptr || v == nullptr || step == 0))) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    cjx::launch_mlp_step(reinterpret_cast<hipStream_t>(stream), x, labels, n, lambda, partial, grad, loss, params, m,
+                         v, step, lr);
+    return hipGetLastError() == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_mlp_forward(const float* params, const float* x, uint32_t n, float* z, uintptr_t stream) {
+    if (params == nullptr || x == nullptr || z == nullptr || n == 0) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    cjx::launch_mlp_forward(reinterpret_cast<hipStream_t>(stream), params, x, n, z);
+    return hipGetLastError() == hipSuccess ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+struct cjx_batch {
+    cjx::BatchEncoder engine;
+};
+
+int cjx_batch_create(int32_t device, uint32_t count, const uint32_t* width, const uint32_t* height,
+                     const float* distance, uint32_t mode, cjx_batch** out) {
+    if (out == nullptr || width == nullptr || height == nullptr || distance == nullptr || count == 0 ||
+        !cjx::device_exists(device)) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    auto* b{new (std::nothrow) cjx_batch{}};
+    if (b == nullptr) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    if (!b->engine.configure(static_cast<int>(count), width, height, distance, device, mode)) {
+        delete b;
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    *out = b;
+    return CUJPEGXL_OK;
+}
+
+int cjx_batch_set_inputs(cjx_batch* b, const cujpegxl_nv12_input* inputs) {
+    if (b == nullptr || inputs == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    const int n{b->engine.frames()};
+    std::vector<const std::uint8_t*> luma(n), chroma(n);
+    std::vector<std::size_t> lp(n), cp(n);
+    for (int i{0}; i < n; ++i) {
+        luma[i] = reinterpret_cast<const std::uint8_t*>(inputs[i].luma);
+        chroma[i] = reinterpret_cast<const std::uint8_t*>(inputs[i].chroma);
+        lp[i] = inputs[i].luma_pitch;
+        cp[i] = inputs[i].chroma_pitch;
+    }
+    return b->engine.set_inputs(luma.data(), lp.data(), chroma.data(), cp.data()) ? CUJPEGXL_OK
+                                                                                  : CUJPEGXL_INVALID_ARGUMENT;
+}
+
+int cjx_batch_run(cjx_batch* b, int32_t count, double* seconds) {
+    if (b == nullptr || count <= 0) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return b->engine.run(count, seconds) ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_batch_result(const cjx_batch* b, uint32_t i, const uint8_t** data, uint64_t* size) {
+    if (b == nullptr || data == nullptr || size == nullptr || i >= static_cast<uint32_t>(b->engine.frames())) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    *data = b->engine.file(static_cast<int>(i));
+    *size = b->engine.size(static_cast<int>(i));
+    return CUJPEGXL_OK;
+}
+
+int cjx_batch_tune(cjx_batch* b, int32_t reps, char* report, size_t cap) {
+    if (b == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    std::string text;
+    const bool ok{b->engine.tune(reps, &text)};
+    if (report != nullptr && cap > 0) {
+        const std::size_t n{text.size() < cap - 1 ? text.size() : cap - 1};
+        std::memcpy(report, text.data(), n);
+        report[n] = 0;
+    }
+    return ok ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+void cjx_batch_destroy(cjx_batch* b) {
+    delete b;
 }
 
 int cjx_stage_timings(cujpegxl_encoder* e, float* ms, int count) {

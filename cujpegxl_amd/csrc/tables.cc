@@ -21,6 +21,64 @@ void dct_basis(int n, float* out) {
     }
 }
 
+// IEEE binary16 bit pattern of a float, round to nearest even (normal and
+// subnormal results; no infinities arise from the basis entries).
+std::uint16_t f32_to_f16(float f) {
+    const std::uint32_t u{f2u(f)};
+    const std::uint32_t sign{(u >> 16) & 0x8000u};
+    const std::uint32_t a{u & 0x7fffffffu};
+    if (a < 0x33000001u) {  // below half the smallest subnormal: +-0
+        return static_cast<std::uint16_t>(sign);
+    }
+    const int e{static_cast<int>(a >> 23) - 127};
+    std::uint32_t m{(a & 0x007fffffu) | 0x00800000u};
+    // Shift so that the kept bits are the half's significand.
+    const int shift{e < -14 ? 13 + (-14 - e) : 13};
+    const std::uint32_t kept{m >> shift};
+    const std::uint32_t rest{m & ((1u << shift) - 1u)};
+    const std::uint32_t half{1u << (shift - 1)};
+    std::uint32_t r{kept + ((rest > half || (rest == half && (kept & 1u))) ? 1u : 0u)};
+    if (e < -14) {
+        return static_cast<std::uint16_t>(sign | r);  // subnormal (a carry makes it the smallest normal)
+    }
+    r += static_cast<std::uint32_t>(e + 15 - 1) << 10;  // the implicit bit adds the last exponent step
+    return static_cast<std::uint16_t>(sign | r);
+}
+
+float f16_to_f32(std::uint16_t h) {
+    const std::uint32_t sign{static_cast<std::uint32_t>(h & 0x8000u) << 16};
+    const std::uint32_t e{(h >> 10) & 31u}, m{h & 1023u};
+    if (e == 0) {
+        const float v{static_cast<float>(m) * (1.0f / 16777216.0f)};
+        return sign ? -v : v;
+    }
+    return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+// Hi and lo FP16 parts of a float matrix (n x n, row-major).
+void split_f16(const float* a, int count, std::uint16_t* hi, std::uint16_t* lo) {
+    for (int i{0}; i < count; ++i) {
+        hi[i] = f32_to_f16(a[i]);
+        lo[i] = f32_to_f16(a[i] - f16_to_f32(hi[i]));
+    }
+}
+
+// The table of pow24_tab / cbrt_tab (numeric.h): segment centres and
+// reciprocals, then RN((2^k c_j)^2.4) and RN((2^k c_j)^(1/3)) in double.
+void xyb_power_table(float* t) {
+    for (int j{0}; j < 16; ++j) {
+        const double c{1.0 + (j + 0.5) / 16.0};
+        t[kXybTabSeg + 2 * j] = static_cast<float>(c);
+        t[kXybTabSeg + 2 * j + 1] = static_cast<float>(1.0 / c);
+        for (int k{-4}; k <= 0; ++k) {
+            t[kXybTabPow + (k + 4) * 16 + j] = static_cast<float>(std::pow(std::ldexp(c, k), 2.4));
+        }
+        for (int k{-9}; k <= 0; ++k) {
+            t[kXybTabCbrt + (k + 9) * 16 + j] = static_cast<float>(std::cbrt(std::ldexp(c, k)));
+        }
+    }
+}
+
 // transform_select.cu:112-117: the DCT8 Y weight at the normalized frequency.
 void selection_weights(int n, const float* dct8_y, float* out) {
     for (int e{0}; e < n * n; ++e) {
@@ -102,6 +160,18 @@ DeviceTables build_tables() {
     recip(&t.dequant8[0][0], &t.dequant_inv8[0][0], 3 * 64);
     recip(&t.dequant16[0][0], &t.dequant_inv16[0][0], 3 * 256);
     recip(&t.dequant32[0][0], &t.dequant_inv32[0][0], 3 * 1024);
+    {
+        float bd[256]{};
+        for (int r{0}; r < 16; ++r) {
+            for (int c{0}; c < 16; ++c) {
+                bd[r * 16 + c] = (r >> 3) == (c >> 3) ? t.basis8[(r & 7) * 8 + (c & 7)] : 0.0f;
+            }
+        }
+        split_f16(bd, 256, t.mf8[0], t.mf8[1]);
+        split_f16(t.basis16, 256, t.mf16[0], t.mf16[1]);
+        split_f16(t.basis32, 1024, t.mf32[0], t.mf32[1]);
+    }
+    xyb_power_table(t.xyb_tab);
     t.rate[0] = 0.0f;
     for (int k{1}; k < 256; ++k) {
         t.rate[k] = 1.0f + log2_det(1.0f + static_cast<float>(k));

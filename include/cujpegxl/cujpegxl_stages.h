@@ -64,6 +64,69 @@ int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* encoder, const cujpegxl_nv12
  * 1 every stage, 2 the region kernel only (one event pair; the others read
  * 0).  Each timed event adds a few microseconds of idle device time. */
 int cjx_stage_set_profiling(cujpegxl_encoder* encoder, int on);
+
+/* Encoder mode bits (no counterpart in the reference), from the next encode:
+ * CJX_MODE_MFMA16 runs the region kernel's DCT8/16/32 as FP16-operand
+ * matrix-core products with FP32 accumulation (hi + lo split operands); 0 is
+ * the plain path, whose files equal the oracle's. */
+#define CJX_MODE_MFMA16 1u
+/* Test modes: every region takes DCT8, DCT16 or DCT32 (wherever that size
+ * fits) instead of the rate-distortion choice. */
+#define CJX_MODE_FORCE8 0x10u
+#define CJX_MODE_FORCE16 0x20u
+#define CJX_MODE_FORCE32 0x30u
+/* The transform map from the selection network wherever it is confident,
+ * the rate-distortion heuristic elsewhere. */
+#define CJX_MODE_LEARNED 2u
+
+/* From the next encode, the region kernel also writes each region's 96
+ * network features and the heuristic's 12 labels (108 floats per region,
+ * regions in raster order; buffer 15 of cjx_stage_fetch). */
+int cjx_stage_export_features(cujpegxl_encoder* encoder, int on);
+
+/* The selection network's weights on `device` for every later encode there:
+ * FP16 bit patterns of W1 [64][96] and W2 [7][64], FP32 biases b1 [64] and
+ * b2 [7], the confidence threshold as a logit magnitude. */
+int cjx_mlp_set_weights(int32_t device, const uint16_t* w1, const float* b1, const uint16_t* w2, const float* b2,
+                        float threshold);
+/* Back to the weights the library was built with. */
+int cjx_mlp_default_weights(int32_t device);
+
+/* Batches of frames of uneven sizes and distances (no counterpart in the
+ * reference, whose encoder takes one frame at a time): every stage runs as
+ * one launch over all the batch's frames, the batch's 14 launches are
+ * captured once as a hipGraph and replayed, and the launches' shapes come
+ * from an autotuner (cjx_batch_tune).  Files land in page-locked host memory
+ * the library owns (cjx_batch_result gives each file's address and size,
+ * valid until the next run).  `mode`: 0 or CJX_MODE_MFMA16 / CJX_MODE_LEARNED
+ * bits, the same for every frame. */
+typedef struct cjx_batch cjx_batch;
+int cjx_batch_create(int32_t device, uint32_t count, const uint32_t* width, const uint32_t* height,
+                     const float* distance, uint32_t mode, cjx_batch** out);
+/* Each frame's NV12 device planes (an array of `count` inputs). */
+int cjx_batch_set_inputs(cjx_batch* batch, const cujpegxl_nv12_input* inputs);
+/* `count` replays of the batch's graph back to back, then a wait; the wall
+ * seconds of the whole into *seconds. */
+int cjx_batch_run(cjx_batch* batch, int32_t count, double* seconds);
+int cjx_batch_result(const cjx_batch* batch, uint32_t index, const uint8_t** data, uint64_t* size);
+/* Times every candidate launch shape (tokenizer workgroups per frame x
+ * output words per assembly thread) over `reps` replays and keeps the
+ * fastest; one line per candidate, then the choice, into `report`. */
+int cjx_batch_tune(cjx_batch* batch, int32_t reps, char* report, size_t cap);
+void cjx_batch_destroy(cjx_batch* batch);
+
+/* Training on the device (all pointers device memory, `stream` a hipStream_t
+ * as an integer): cjx_mlp_step computes the mean loss (loss[0]) and its
+ * gradient (grad, 6,663 floats in the parameter layout W1, b1, W2, b2) over
+ * n regions (features x [n][96], labels [n][12]) into a partial buffer of
+ * cjx_mlp_partial_floats(n) floats, and with lr > 0 takes one Adam step
+ * (moments m, v; step counts from 1) on params.  cjx_mlp_forward writes the
+ * outputs z [n][7]. */
+size_t cjx_mlp_partial_floats(uint32_t n);
+int cjx_mlp_step(const float* x, const float* labels, uint32_t n, float lambda, float* partial, float* grad,
+                 float* loss, float* params, float* m, float* v, uint32_t step, float lr, uintptr_t stream);
+int cjx_mlp_forward(const float* params, const float* x, uint32_t n, float* z, uintptr_t stream);
+int cjx_stage_set_mode(cujpegxl_encoder* encoder, uint32_t mode);
 int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
 
 /* `count` synchronous cujpegxl_encoder_encode calls on the same input and

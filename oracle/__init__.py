@@ -114,6 +114,10 @@ def lib() -> C.CDLL:
         L.jo_select_transforms.argtypes = [_P, _U32, _U32, _F, _P]
         L.jo_forward_dct.argtypes = [_P, _U32, _U32, _P, _P]
         L.jo_forward_dct_f64.argtypes = [_P, _U32, _U32, _P, _P]
+        L.jo_mlp_forward.argtypes = [_P, _P, _U32, _P]
+        L.jo_mlp_loss_grad.restype = C.c_double
+        L.jo_mlp_loss_grad.argtypes = [_P, _P, _P, _U32, _F, _P]
+        L.jo_block_costs8.argtypes = [_P, _U32, _U32, _F, _P, _P, _P]
         L.jo_cfl_region_partials.argtypes = [_P, _P, _U32, _U32, _P]
         L.jo_cfl_maps.argtypes = [_P, _U32, _U32, _P, _P]
         L.jo_quantize.argtypes = [_P, _P, _P, _P, _P, _U32, _U32, _F, _P, _P, _P]
@@ -353,3 +357,37 @@ def decode(data: bytes) -> dict:
         raise DecodeError(lib().jd_error().decode())
     out.update(width=w, height=h, global_scale=gs.value, quant_dc=qdc.value)
     return out
+
+
+MLP_IN, MLP_HID, MLP_OUT, MLP_LABELS = 96, 64, 7, 12
+MLP_PARAMS = MLP_HID * MLP_IN + MLP_HID + MLP_OUT * MLP_HID + MLP_OUT
+
+
+def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Network outputs [n][7] in float64 (jxl_mlp.c)."""
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+    z = np.zeros((x.shape[0], MLP_OUT), np.float64)
+    lib().jo_mlp_forward(ptr(params), ptr(x), x.shape[0], ptr(z))
+    return z
+
+
+def mlp_loss_grad(params: np.ndarray, x: np.ndarray, labels: np.ndarray, lam: float,
+                  want_grad: bool = True) -> tuple[float, np.ndarray | None]:
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+    labels = np.ascontiguousarray(labels, np.float32).reshape(-1, MLP_LABELS)
+    grad = np.zeros(MLP_PARAMS, np.float64) if want_grad else None
+    loss = lib().jo_mlp_loss_grad(ptr(params), ptr(x), ptr(labels), x.shape[0], lam, ptr(grad))
+    return float(loss), grad
+
+
+def block_costs8(y: np.ndarray, distance: float, parts: bool = False):
+    """Per-block DCT8 RD costs of the selection; with `parts`, also the
+    squared-error and rate sums they are made of."""
+    h, w = y.shape
+    n = (h // 8) * (w // 8)
+    out, dist, rate = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(n, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    lib().jo_block_costs8(ptr(y), w, h, distance, ptr(out), ptr(dist), ptr(rate))
+    return (out, dist, rate) if parts else out

@@ -4,7 +4,8 @@
  * the numeric contract).  Citations are file:line in m-pilia/cujpegxl.
  *
  * Pinning (what checks each part; every row is exercised under tests/):
- *   numerics      jo_cbrt/jo_log2/jo_exp2 against libm double, error bounds
+ *   numerics      jo_cbrt/jo_pow24/jo_log2/jo_exp2 against libm double over
+ *                 their domains (dense samples + every segment edge)
  *   calibration   oracle/_ref (quant_calibration.h) on a distance sweep
  *   XYB           FP64 closed form jo_nv12_to_xyb_f64, flat-tile exactness
  *   gaborish      brute-force 5x5 convolution with clamp in double
@@ -95,28 +96,69 @@ float jo_exp2(float x) {
     return p * u2f((uint32_t)((int)n + 127) << 23);
 }
 
-/* Cube root of x >= 0 without division: y ~ x^(-1/3) from an exponent-
- * thirding seed and three Newton steps y <- y (4 - x y^3) / 3, then
- * c = x y^2 and one Newton correction c <- c - (c^3 - x) y^2 / 3
- * (<= 0.75 ulp on [1e-6, 64]). */
-float jo_cbrt(float x) {
-    if (!(x > 0.0f)) {
-        return 0.0f;
+/* The powers of the XYB conversion from one table of 272 floats: x = 2^k m,
+ * m in [1, 2), segment j = the top 4 mantissa bits with centre
+ * c_j = 1 + (j + 1/2)/16 and r_j = RN(1/c_j); t = (m - c_j) r_j (|t| <= 1/32,
+ * the difference exact), x^a = RN((2^k c_j)^a) * (1 + t)^a with the binomial
+ * series of (1 + t)^a to t^5.  A project restatement of the reference's powf
+ * and cbrtf (xyb_impl.cuh:37-40, 62-65) with a fixed evaluation order, so
+ * every platform computes identical bits; relative error <= 1e-6. */
+static float g_xyb_tab[272];
+static int g_xyb_tab_ready = 0;
+
+static void xyb_tab_init(void) {
+    if (g_xyb_tab_ready) {
+        return;
     }
-    float y = u2f(0x54a23300u - f2u(x) / 3u);
-    const float x3 = x * (1.0f / 3.0f);
-    for (int i = 0; i < 3; ++i) {
-        const float y3 = (y * y) * y;
-        y = y * fmaf(-x3, y3, 4.0f / 3.0f);
+    for (int j = 0; j < 16; ++j) {
+        const double c = 1.0 + (j + 0.5) / 16.0;
+        g_xyb_tab[2 * j] = (float)c;
+        g_xyb_tab[2 * j + 1] = (float)(1.0 / c);
+        for (int k = -4; k <= 0; ++k) {
+            g_xyb_tab[32 + (k + 4) * 16 + j] = (float)pow(ldexp(c, k), 2.4);
+        }
+        for (int k = -9; k <= 0; ++k) {
+            g_xyb_tab[112 + (k + 9) * 16 + j] = (float)cbrt(ldexp(c, k));
+        }
     }
-    float c = (x * y) * y;
-    const float r = fmaf(c * c, c, -x);
-    c = fmaf(-r, (y * y) * (1.0f / 3.0f), c);
-    return c;
+    g_xyb_tab_ready = 1;
 }
 
+void jo_xyb_table(float* out) {
+    xyb_tab_init();
+    memcpy(out, g_xyb_tab, sizeof(g_xyb_tab));
+}
+
+/* x^2.4 for x in [2^-4, 2). */
 float jo_pow24(float x) {
-    return jo_exp2(2.4f * jo_log2(x));
+    xyb_tab_init();
+    const uint32_t u = f2u(x);
+    const uint32_t j = (u >> 19) & 15u;
+    const float m = u2f((u & 0x007fffffu) | 0x3f800000u);
+    const float t = (m - g_xyb_tab[2 * j]) * g_xyb_tab[2 * j + 1];
+    float p = 0.010752f;
+    p = fmaf(p, t, -0.0336f);
+    p = fmaf(p, t, 0.224f);
+    p = fmaf(p, t, 1.68f);
+    p = fmaf(p, t, 2.4f);
+    p = fmaf(p, t, 1.0f);
+    return g_xyb_tab[32 + ((u >> 19) - (123u << 4))] * p;
+}
+
+/* x^(1/3) for x in [2^-9, 2). */
+float jo_cbrt(float x) {
+    xyb_tab_init();
+    const uint32_t u = f2u(x);
+    const uint32_t j = (u >> 19) & 15u;
+    const float m = u2f((u & 0x007fffffu) | 0x3f800000u);
+    const float t = (m - g_xyb_tab[2 * j]) * g_xyb_tab[2 * j + 1];
+    float p = 0.0301783264f;
+    p = fmaf(p, t, -0.0411522634f);
+    p = fmaf(p, t, 0.0617283951f);
+    p = fmaf(p, t, -0.111111111f);
+    p = fmaf(p, t, 0.333333333f);
+    p = fmaf(p, t, 1.0f);
+    return g_xyb_tab[112 + ((u >> 19) - (118u << 4))] * p;
 }
 
 float jo_exp(float x) {
@@ -777,7 +819,15 @@ static float tree8(const float* s) {
     return ((s[0] + s[4]) + (s[2] + s[6])) + ((s[1] + s[5]) + (s[3] + s[7]));
 }
 
+static float block_cost_parts(const float* y, size_t pitch, int n, float qgsf, float* dist_out,
+                              float* rate_out);
+
 static float block_cost(const float* y, size_t pitch, int n, float qgsf) {
+    return block_cost_parts(y, pitch, n, qgsf, NULL, NULL);
+}
+
+static float block_cost_parts(const float* y, size_t pitch, int n, float qgsf, float* dist_out,
+                              float* rate_out) {
     const float* wsel = wsel_for(n);
     float coef[1024], d[1024], r[1024];
     dct_block(y, pitch, n, coef);
@@ -820,6 +870,12 @@ static float block_cost(const float* y, size_t pitch, int n, float qgsf) {
         dist = tree64(ld);
         rate = tree64(lr);
     }
+    if (dist_out) {
+        *dist_out = dist;
+    }
+    if (rate_out) {
+        *rate_out = rate;
+    }
     return (float)(n * n) * dist + 6.0e-4f * rate;
 }
 
@@ -828,6 +884,29 @@ static void write_first_block(int8_t* acs, uint32_t bw, uint32_t bx, uint32_t by
     for (int dy = 0; dy < s; ++dy) {
         for (int dx = 0; dx < s; ++dx) {
             acs[(size_t)(by + dy) * bw + bx + dx] = (dy == 0 && dx == 0) ? (int8_t)n : 0;
+        }
+    }
+}
+
+void jo_block_costs8(const float* y, uint32_t w, uint32_t h, float distance, float* cost8, float* dist8,
+                     float* rate8) {
+    const uint32_t bw = w / 8, bh = h / 8;
+    jo_cal cal;
+    jo_calibrate(distance, &cal);
+    const float qgsf = (float)cal.raw_quant_field * cal.global_scale_float;
+    ensure_basis();
+#pragma omp parallel for num_threads(g_threads) schedule(dynamic, 1)
+    for (uint32_t by = 0; by < bh; ++by) {
+        for (uint32_t bx = 0; bx < bw; ++bx) {
+            const size_t i = (size_t)by * bw + bx;
+            float d, r;
+            cost8[i] = block_cost_parts(y + (size_t)by * 8 * w + bx * 8, w, 8, qgsf, &d, &r);
+            if (dist8) {
+                dist8[i] = d;
+            }
+            if (rate8) {
+                rate8[i] = r;
+            }
         }
     }
 }

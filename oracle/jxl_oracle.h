@@ -7,8 +7,9 @@
  *
  * Numeric contract shared with the HIP kernels: float arithmetic with
  * contraction off, explicit fmaf where a fused multiply-add is meant, the
- * transcendental functions jo_cbrt / jo_log2 / jo_exp2 below (built from IEEE
- * basic operations only), DCT bases computed in double and narrowed, and
+ * transcendental functions jo_cbrt / jo_pow24 (table-driven on the XYB
+ * conversion's domains) and jo_log2 / jo_exp2 below (built from IEEE basic
+ * operations only), DCT bases computed in double and narrowed, and
  * reductions in the stated tree orders.  Integer outputs (quant field,
  * transform map, quantized coefficients, histograms, codes, bytes) are then
  * bit-exact between this file and the device path.
@@ -32,10 +33,11 @@ typedef struct jo_cal {
 } jo_cal;
 
 /* ---- numeric primitives ------------------------------------------------ */
-float jo_cbrt(float x);
+float jo_cbrt(float x);  /* x in [2^-9, 2) */
 float jo_log2(float x);
 float jo_exp2(float x);
-float jo_pow24(float x);
+float jo_pow24(float x); /* x in [2^-4, 2) */
+void jo_xyb_table(float* out); /* the 272-float table of jo_cbrt / jo_pow24 */
 float jo_exp(float x);
 uint16_t jo_f32_to_f16(float x);
 float jo_f16_to_f32(uint16_t h);
@@ -127,6 +129,24 @@ int jd_decode(const uint8_t* file, size_t n, uint32_t width, uint32_t height, ui
               uint32_t* quant_dc, int8_t* acs, int32_t* qf, int32_t* dc, int16_t* ac, int8_t* ytox,
               int8_t* ytob);
 const char* jd_error(void);
+
+/* --- the learned selection / AQ network (jxl_mlp.c) ---------------------- */
+#define JO_MLP_IN 96
+#define JO_MLP_HID 64
+#define JO_MLP_OUT 7
+#define JO_MLP_PARAMS (JO_MLP_HID * JO_MLP_IN + JO_MLP_HID + JO_MLP_OUT * JO_MLP_HID + JO_MLP_OUT)
+#define JO_MLP_LABELS 12
+/* Outputs z[n][7] of n feature rows, in float64. */
+void jo_mlp_forward(const float* params, const float* x, uint32_t n, double* z);
+/* Mean loss over n tiles and, when `grad` is not null, its gradient with
+ * respect to the parameters (JO_MLP_PARAMS doubles). */
+double jo_mlp_loss_grad(const float* params, const float* x, const float* labels, uint32_t n, float lambda,
+                        double* grad);
+/* The selection's RD cost of every 8x8 block's DCT8 (the sharpened Y plane;
+ * the same function and summation order as jo_select_transforms), and when
+ * not null its two sums: squared quantization errors and rate terms. */
+void jo_block_costs8(const float* y, uint32_t w, uint32_t h, float distance, float* cost8, float* dist8,
+                     float* rate8);
 
 #ifdef __cplusplus
 }

@@ -12,20 +12,52 @@ import pytest
 from cujpegxl_amd.synth import synthetic_nv12
 
 
+def _edges(lo_exp: int, hi_exp: int) -> np.ndarray:
+    """Both sides of every table segment edge (16 per binade) in
+    [2^lo_exp, 2^hi_exp)."""
+    out = []
+    for e in range(lo_exp, hi_exp):
+        for j in range(16):
+            v = np.float32(np.ldexp(1.0 + j / 16.0, e))
+            out += [v, np.nextafter(v, np.float32(np.inf)), np.nextafter(v, np.float32(0))]
+    out = np.array(out, dtype=np.float32)
+    return out[(out >= np.float32(2.0 ** lo_exp)) & (out < np.float32(2.0 ** hi_exp))]
+
+
 def test_numeric_primitives_error_bounds(jo):
     L = jo.lib()
     rng = np.random.default_rng(1)
     xs = np.concatenate([rng.uniform(0.0037, 1.2, 4000), rng.uniform(1e-3, 50, 2000)])
     for x in xs.astype(np.float32):
-        x = float(x)
-        assert abs(L.jo_cbrt(x) - np.cbrt(x)) <= 2.5e-7 * np.cbrt(x)
-        assert abs(L.jo_log2(x) - np.log2(x)) <= 2e-7 * max(1.0, abs(np.log2(x)))
+        assert abs(L.jo_log2(float(x)) - np.log2(float(x))) <= 2e-7 * max(1.0, abs(np.log2(float(x))))
     for x in rng.uniform(-20, 20, 4000).astype(np.float32):
         assert abs(L.jo_exp2(float(x)) - 2.0 ** float(x)) <= 3e-7 * 2.0 ** float(x)
-    for x in rng.uniform(0.05, 1.0, 4000).astype(np.float32):
+    # The table-driven powers over their whole domains: the XYB conversion
+    # calls x^2.4 on [0.0905, 1] and x^(1/3) on [0.0038, 1.0038].
+    cb = np.concatenate([np.exp2(rng.uniform(-9, 1, 6000)).astype(np.float32), _edges(-9, 1),
+                         np.float32([0.0037930732552754493, 1.0037930732552754, 1.0])])
+    for x in cb:
+        want = np.cbrt(float(x))
+        assert abs(L.jo_cbrt(float(x)) - want) <= 5e-7 * want, x
+    pw = np.concatenate([np.exp2(rng.uniform(-4, 1, 6000)).astype(np.float32), _edges(-4, 1),
+                         np.float32([(0.04045 + 0.055) / 1.055, 1.0])])
+    for x in pw:
         want = float(x) ** 2.4
-        assert abs(L.jo_pow24(float(x)) - want) <= 1.5e-6 * want
-    assert L.jo_cbrt(0.0) == 0.0 and L.jo_cbrt(8.0) == pytest.approx(2.0, rel=1e-7)
+        assert abs(L.jo_pow24(float(x)) - want) <= 8e-7 * want, x
+
+
+def test_power_table_is_rounded_powers_of_segment_centres(jo):
+    """The 272-float table: centres c_j = 1 + (j + 1/2)/16, RN(1/c_j), then
+    RN((2^k c_j)^2.4) for k = -4..0 and RN((2^k c_j)^(1/3)) for k = -9..0."""
+    t = np.zeros(272, np.float32)
+    jo.lib().jo_xyb_table(t.ctypes.data_as(C.POINTER(C.c_float)))
+    c = 1.0 + (np.arange(16) + 0.5) / 16.0
+    assert np.array_equal(t[0:32:2], c.astype(np.float32))
+    assert np.array_equal(t[1:32:2], (1.0 / c).astype(np.float32))
+    pw = np.array([[(c[j] * 2.0 ** k) ** 2.4 for j in range(16)] for k in range(-4, 1)])
+    cb = np.array([[np.cbrt(c[j] * 2.0 ** k) for j in range(16)] for k in range(-9, 1)])
+    assert np.allclose(t[32:112], pw.ravel().astype(np.float32), rtol=6e-8, atol=0)
+    assert np.allclose(t[112:272], cb.ravel().astype(np.float32), rtol=6e-8, atol=0)
 
 
 def test_f16_conversion_matches_ieee_rne(jo):
