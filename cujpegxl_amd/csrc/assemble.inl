@@ -231,7 +231,7 @@ __device__ __attribute__((always_inline)) inline void dc_blobs_bits_kernel_body(
     if (wg.x == chunks_per_group) {
         dc_blobs_body(geo, k, cs, fb_count, blobs, blob_words, static_cast<int>(wg.y));
     } else {
-        dc_chunk_bits_body(geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
+        dc_chunk_bits_body(wg, geo, dc, acs, qf, ytox, ytob, &cs->depth[kNumClusters][0],
                            &cs->depth[kNumClusters + geo.num_dc][0], chunk_bits, chunks_per_group);
     }
 }

@@ -966,15 +966,15 @@ __device__ __forceinline__ std::uint32_t dc_cursor_code(const Geometry& geo, con
 // Bits of one chunk of a DC group's items, the two header blobs not
 // counted (the scan adds them: their sizes are built beside this pass).
 __device__ void dc_chunk_bits_body(
-    const Geometry& geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
+    const Wg& wg, const Geometry& geo, const std::int32_t* __restrict__ dc, const std::int8_t* __restrict__ acs,
     const std::int32_t* __restrict__ qf, const std::int8_t* __restrict__ ytox,
     const std::int8_t* __restrict__ ytob, const std::uint8_t* __restrict__ dc_depth,
     const std::uint8_t* __restrict__ meta_depth, std::uint64_t* __restrict__ chunk_bits,
     std::uint32_t chunks_per_group) {
     __shared__ std::uint64_t sh[5];
-    const int g{static_cast<int>(blockIdx.y)};
+    const int g{static_cast<int>(wg.y)};
     const DcGroupGeo d{dc_group_geo(geo, g)};
-    const std::uint32_t k0{blockIdx.x * kDcChunk + threadIdx.x * kDcPerThread};
+    const std::uint32_t k0{wg.x * kDcChunk + threadIdx.x * kDcPerThread};
     const std::uint32_t k1{min(k0 + kDcPerThread, d.items)};
     std::uint64_t run{0};
     if (k0 < k1) {
@@ -991,7 +991,7 @@ __device__ void dc_chunk_bits_body(
     std::uint64_t total{0};
     block_excl_scan(run, sh, total);
     if (threadIdx.x == 0) {
-        chunk_bits[static_cast<std::size_t>(g) * chunks_per_group + blockIdx.x] = total;
+        chunk_bits[static_cast<std::size_t>(g) * chunks_per_group + wg.x] = total;
     }
 }
 
