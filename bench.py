@@ -48,6 +48,7 @@ COUNTER_FILE = os.path.join(HERE, "profiles", "sq_counters_4k_d1.txt")
 CUS, SCLK_GHZ = 256, 2.4  # MI355X compute units and peak engine clock
 # Forms of the path timed beside the plain one (encoder mode bits).
 FORMS = (("mfma16", 1), ("learned", 2), ("learned+mfma16", 3))
+BATCH_FORMS = (("batch8", 0), ("batch8 learned+mfma16", 3))
 
 
 def region_valu_instructions() -> int | None:
@@ -100,10 +101,11 @@ MIXED_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)
 
 def mixed_batch(cj, synthetic_nv12, frames: int = 64, seed: int = 4) -> dict:
     """64 frames with sizes drawn uniformly from MIXED_SIZES and distances
-    uniform in [0.8, 3.0] (seeded), encoded one after another into a
-    page-locked buffer: one encoder per size, reconfigured for each frame's
-    distance; one seeded synthetic frame per size, resident in HBM.  Reports
-    the batch rate and the per-frame synchronous latency."""
+    uniform in [0.8, 3.0] (seeded); one seeded synthetic frame per size,
+    resident in HBM.  Sequential: one encoder per size, reconfigured for each
+    frame's distance, encoded one after another into a page-locked buffer
+    (rate and per-frame synchronous latency).  Batched: the same frames as
+    one Batch (rate over replays of the whole batch)."""
     rng = np.random.default_rng(seed)
     picks = rng.integers(0, len(MIXED_SIZES), frames)
     dists = rng.uniform(0.8, 3.0, frames)
@@ -131,16 +133,38 @@ def mixed_batch(cj, synthetic_nv12, frames: int = 64, seed: int = 4) -> dict:
         lat.append(time.perf_counter() - t)
     total = time.perf_counter() - t0
     pinned.close()
-    for x in encs + bufs:
+    for x in encs:
         x.close()
     lat_ms = np.array(lat) * 1e3
     pixels = sum(MIXED_SIZES[p][0] * MIXED_SIZES[p][1] for p in picks)
-    return {"frames": frames, "fps": round(frames / total, 2),
-            "megapixels_per_s": round(pixels / total / 1e6, 1),
-            "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
-            "p99_ms": round(float(np.percentile(lat_ms, 99)), 4),
-            "bytes": nbytes, "sizes": "640x360 .. 3840x2160 uniform, d ~ U[0.8, 3.0], seed 4",
-            "launches": "one synchronous encode per frame (no graph, no batching across frames)"}
+    sequential = {"fps": round(frames / total, 2), "megapixels_per_s": round(pixels / total / 1e6, 1),
+                  "p50_ms": round(float(np.percentile(lat_ms, 50)), 4),
+                  "p99_ms": round(float(np.percentile(lat_ms, 99)), 4), "bytes": nbytes,
+                  "launches": "one synchronous encode per frame (14 launches each)"}
+    # The same 64 frames as one batch: every stage one launch over all
+    # frames, the 14 launches replayed from a captured hipGraph, launch
+    # shapes chosen by the autotuner first.
+    batch = cj.Batch([MIXED_SIZES[p] for p in picks], [float(d) for d in dists])
+    batch.set_inputs([(bufs[p].ptr, bufs[p].ptr + MIXED_SIZES[p][0] * MIXED_SIZES[p][1], MIXED_SIZES[p][0],
+                       MIXED_SIZES[p][0]) for p in picks])
+    batch.run(2)
+    tune = batch.tune(3)
+    batch.run(2)
+    reps = 20
+    sec = batch.run(reps)
+    batch_bytes = sum(len(f) for f in batch.files())
+    batch.close()
+    for x in bufs:
+        x.close()
+    batched = {"fps": round(frames * reps / sec, 2),
+               "megapixels_per_s": round(pixels * reps / sec / 1e6, 1),
+               "ms_per_batch": round(1e3 * sec / reps, 4), "bytes": batch_bytes,
+               "bytes_equal_sequential": batch_bytes == nbytes,
+               "tuned": tune.strip().splitlines()[-1],
+               "launches": "one launch per stage over the 64 frames (14 per batch), replayed from a hipGraph"}
+    return {"frames": frames, "sizes": "640x360 .. 3840x2160 uniform, d ~ U[0.8, 3.0], seed 4",
+            "sequential": sequential, "batched": batched,
+            "batched_vs_sequential": round(batched["fps"] / sequential["fps"], 3)}
 
 
 def host_info() -> dict:
@@ -276,6 +300,25 @@ def main() -> int:
         forms[name] = {"fps": round(f_fps, 3), "bpp": round(8.0 * float(np.mean(f_sizes)) / (WIDTH * HEIGHT), 4),
                        "region_ms": round(rsum / max(nreg, 1), 4)}
     enc.set_mode(0)
+
+    # Batched forms: the eight distinct frames as one batch (every stage one
+    # launch over the eight frames, replayed from a captured hipGraph, launch
+    # shapes from the autotuner), outputs into mapped page-locked memory.
+    for name, mode in BATCH_FORMS:
+        batch = cj.Batch([(WIDTH, HEIGHT)] * SEEDS, [DISTANCE] * SEEDS, mode=mode)
+        batch.set_inputs(inputs[:SEEDS])
+        batch.run(2)
+        tuned = batch.tune(2).strip().splitlines()[-1]
+        batch.run(1)
+        reps = max(1, args.steps // SEEDS)
+        sec = batch.run(reps)
+        b_sizes = [len(f) for f in batch.files()]
+        batch.close()
+        forms[name] = {"fps": round(SEEDS * reps / sec, 3),
+                       "bpp": round(8.0 * float(np.mean(b_sizes)) / (WIDTH * HEIGHT), 4),
+                       "frames_per_batch": SEEDS, "batches_timed": reps, "tuned": tuned}
+        if mode == 0:
+            forms[name]["sizes_equal_single_frame"] = b_sizes == sizes
 
     # The region kernel's duration (the roofline figure), from HIP events
     # bound to the kernel's own start and end on the encoder's stream, over

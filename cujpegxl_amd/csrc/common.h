@@ -135,7 +135,7 @@ struct DeviceTables {
     double llf2[4];        // T[f][o] = ortho_2(f, o) * resample_scale_2[f]
     double llf4[16];
     float rate[256];       // selection rate term 1 + log2(1 + k) for |q| = k >= 1; 0 for k = 0
-    float xyb_tab[kXybTabSize];  // segments and powers of pow24_tab / cbrt_tab
+    float2 xyb_tab[kXybTabPairs];  // {power, segment reciprocal} pairs of pow24_tab / cbrt_tab
     // FP16 operands of the matrix-core DCT form: [0] = RN16(a), [1] =
     // RN16(a - [0]) of every basis entry a (FP16 bit patterns, row-major):
     // blockdiag(A8, A8) (a 16x16 quad of four DCT8s), A16 and A32.

@@ -55,12 +55,12 @@ __device__ __forceinline__ float clamp01(float v) {
 
 // xyb_impl.cuh:37-70 on the shared numeric primitives (the powers from the
 // table at `tab`, staged in LDS).
-__device__ __forceinline__ float srgb_to_linear(float e, const float* tab) {
+__device__ __forceinline__ float srgb_to_linear(float e, const float2* tab) {
     return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_tab((e + 0.055f) * (1.0f / 1.055f), tab);
 }
 
 __device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float& ox, float& oy, float& ob,
-                                             const float* tab) {
+                                             const float2* tab) {
     constexpr float m00{0.30f}, m02{0.078f}, m01{1.0f - 0.078f - 0.30f};
     constexpr float m10{0.23f}, m12{0.078f}, m11{1.0f - 0.078f - 0.23f};
     constexpr float m20{0.24342268924547819f}, m21{0.20476744424496821f};
@@ -421,9 +421,9 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         S.u.f.t.chroma[i] = static_cast<std::uint16_t>(p[0] | (p[1] << 8));
     }
     // The powers' table, in the sharpened planes' space (free until step 2).
-    float* const tab{&S.p.sharp[0][0]};
-    for (int i{tid}; i < kXybTabSize; i += 256) {
-        tab[i] = g_tables.xyb_tab[i];
+    float2* const tab{reinterpret_cast<float2*>(&S.p.sharp[0][0])};
+    if (tid < kXybTabPairs) {
+        tab[tid] = g_tables.xyb_tab[tid];
     }
     __syncthreads();
     for (int i{tid}; i < kTileW * kTileW; i += 256) {

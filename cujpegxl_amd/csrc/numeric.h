@@ -78,45 +78,49 @@ CJX_HD float div_by_recip(float a, float b, float r) {
     return __builtin_fmaf(e, r, q);
 }
 
-// Table-driven powers of the XYB conversion, on one table of 272 floats
-// (kXybTab*): the mantissa m in [1, 2) of x = 2^k m falls in segment
-// j = its top 4 bits, with centre c_j = 1 + (j + 1/2) / 16 (exact) and
-// r_j = RN(1 / c_j); t = (m - c_j) r_j (the difference is exact), |t| <=
-// 1/32, and x^a = T[k][j] (1 + t)^a with T[k][j] = RN((2^k c_j)^a) and the
-// binomial series of (1 + t)^a to t^5 (truncation below 3e-11).
-constexpr int kXybTabSeg = 0;      // [16][2]: c_j, r_j
-constexpr int kXybTabPow = 32;     // [5][16]: x^2.4 for k = -4 .. 0
-constexpr int kXybTabCbrt = 112;   // [10][16]: x^(1/3) for k = -9 .. 0
-constexpr int kXybTabSize = 272;
+// Table-driven powers of the XYB conversion.  The mantissa m in [1, 2) of
+// x = 2^k m falls in segment j = its top 4 bits, with centre
+// c_j = 1 + (j + 1/2) / 16 (built from the bits of x) and r_j = RN(1 / c_j);
+// t = (m - c_j) r_j (the difference is exact), |t| <= 1/32, and
+// x^a = T[k][j] (1 + t)^a with T[k][j] = RN((2^k c_j)^a) and the binomial
+// series of (1 + t)^a to t^5 (truncation below 3e-11).  The table holds one
+// pair {T[k][j], r_j} per (k, j), so each power reads one 8-byte entry.
+constexpr int kXybPow = 0;    // [5][16]: x^2.4 for k = -4 .. 0
+constexpr int kXybCbrt = 80;  // [10][16]: x^(1/3) for k = -9 .. 0
+constexpr int kXybTabPairs = 240;
+
+CJX_HD float xyb_series_arg(std::uint32_t u, float r) {
+    const float m{u2f((u & 0x007fffffu) | 0x3f800000u)};
+    const float c{u2f((u & 0x00780000u) | 0x3f840000u)};
+    return (m - c) * r;
+}
 
 // x^2.4 for x in [2^-4, 2) (the sRGB curve's upper branch: x in [0.0905, 1]).
-CJX_HD float pow24_tab(float x, const float* tab) {
+CJX_HD float pow24_tab(float x, const float2* tab) {
     const std::uint32_t u{f2u(x)};
-    const std::uint32_t j{(u >> 19) & 15u};
-    const float m{u2f((u & 0x007fffffu) | 0x3f800000u)};
-    const float t{(m - tab[kXybTabSeg + 2 * j]) * tab[kXybTabSeg + 2 * j + 1]};
+    const float2 e{tab[kXybPow + ((u >> 19) - (123u << 4))]};
+    const float t{xyb_series_arg(u, e.y)};
     float p{0.010752f};
     p = __builtin_fmaf(p, t, -0.0336f);
     p = __builtin_fmaf(p, t, 0.224f);
     p = __builtin_fmaf(p, t, 1.68f);
     p = __builtin_fmaf(p, t, 2.4f);
     p = __builtin_fmaf(p, t, 1.0f);
-    return tab[kXybTabPow + ((u >> 19) - (123u << 4))] * p;
+    return e.x * p;
 }
 
 // x^(1/3) for x in [2^-9, 2) (the opsin values: x in [0.0038, 1.004]).
-CJX_HD float cbrt_tab(float x, const float* tab) {
+CJX_HD float cbrt_tab(float x, const float2* tab) {
     const std::uint32_t u{f2u(x)};
-    const std::uint32_t j{(u >> 19) & 15u};
-    const float m{u2f((u & 0x007fffffu) | 0x3f800000u)};
-    const float t{(m - tab[kXybTabSeg + 2 * j]) * tab[kXybTabSeg + 2 * j + 1]};
+    const float2 e{tab[kXybCbrt + ((u >> 19) - (118u << 4))]};
+    const float t{xyb_series_arg(u, e.y)};
     float p{0.0301783264f};
     p = __builtin_fmaf(p, t, -0.0411522634f);
     p = __builtin_fmaf(p, t, 0.0617283951f);
     p = __builtin_fmaf(p, t, -0.111111111f);
     p = __builtin_fmaf(p, t, 0.333333333f);
     p = __builtin_fmaf(p, t, 1.0f);
-    return tab[kXybTabCbrt + ((u >> 19) - (118u << 4))] * p;
+    return e.x * p;
 }
 
 CJX_HD float exp_det(float x) {

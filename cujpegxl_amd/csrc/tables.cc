@@ -63,18 +63,17 @@ void split_f16(const float* a, int count, std::uint16_t* hi, std::uint16_t* lo) 
     }
 }
 
-// The table of pow24_tab / cbrt_tab (numeric.h): segment centres and
-// reciprocals, then RN((2^k c_j)^2.4) and RN((2^k c_j)^(1/3)) in double.
-void xyb_power_table(float* t) {
+// The table of pow24_tab / cbrt_tab (numeric.h): per exponent k and segment j,
+// {RN((2^k c_j)^a), RN(1 / c_j)} with c_j = 1 + (j + 1/2) / 16, in double.
+void xyb_power_table(float2* t) {
     for (int j{0}; j < 16; ++j) {
         const double c{1.0 + (j + 0.5) / 16.0};
-        t[kXybTabSeg + 2 * j] = static_cast<float>(c);
-        t[kXybTabSeg + 2 * j + 1] = static_cast<float>(1.0 / c);
+        const float r{static_cast<float>(1.0 / c)};
         for (int k{-4}; k <= 0; ++k) {
-            t[kXybTabPow + (k + 4) * 16 + j] = static_cast<float>(std::pow(std::ldexp(c, k), 2.4));
+            t[kXybPow + (k + 4) * 16 + j] = float2{static_cast<float>(std::pow(std::ldexp(c, k), 2.4)), r};
         }
         for (int k{-9}; k <= 0; ++k) {
-            t[kXybTabCbrt + (k + 9) * 16 + j] = static_cast<float>(std::cbrt(std::ldexp(c, k)));
+            t[kXybCbrt + (k + 9) * 16 + j] = float2{static_cast<float>(std::cbrt(std::ldexp(c, k))), r};
         }
     }
 }
