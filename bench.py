@@ -93,7 +93,7 @@ def quality_summary() -> dict | None:
     return {"source": "profiles/quality_4k.json (tests/test_gpu_quality.py, CJX_QUALITY_FULL=1)",
             "metric": q.get("metric"), "size": q.get("size"),
             "bpp_increase_at_plain_score": q.get("mean_bpp_increase"),
-            "region_ms": q.get("mean_region_ms")}
+            "region_ms_3840x2160": q.get("region_ms_4k")}
 
 
 MIXED_SIZES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
@@ -220,6 +220,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the matrix-core, learned and batched forms (plain path only)")
     ap.add_argument("--no-mixed-batch", action="store_true",
                     help="skip the mixed-resolution batch (4K frames only, e.g. under a profiler)")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -280,7 +282,7 @@ def main() -> int:
     # The other forms of the path on the same frames and loop: FP16-operand
     # matrix-core DCTs, the learned transform selection, and both.
     forms = {}
-    for name, mode in FORMS:
+    for name, mode in () if args.no_forms else FORMS:
         enc.set_mode(mode)
         for k in range(len(inputs)):
             enc.encode_into(*inputs[k], pinned.ptr, enc.capacity)
@@ -304,7 +306,7 @@ def main() -> int:
     # Batched forms: the eight distinct frames as one batch (every stage one
     # launch over the eight frames, replayed from a captured hipGraph, launch
     # shapes from the autotuner), outputs into mapped page-locked memory.
-    for name, mode in BATCH_FORMS:
+    for name, mode in () if args.no_forms else BATCH_FORMS:
         batch = cj.Batch([(WIDTH, HEIGHT)] * SEEDS, [DISTANCE] * SEEDS, mode=mode)
         batch.set_inputs(inputs[:SEEDS])
         batch.run(2)

@@ -275,6 +275,9 @@ __device__ __forceinline__ std::size_t token_slot(const Geometry& geo, int c, in
 // each pair's coefficient blocks are read as 16-byte vectors while the
 // previous pair is tokenized from its LDS copy.
 constexpr int kAcSymbols = 64;
+// LDS row stride of a cluster's histogram: one word more than the symbols,
+// so equal symbols of different clusters fall in different banks.
+constexpr int kHistRow = kAcSymbols + 1;
 constexpr int kTokenGrid = 2048;
 constexpr int kTokenBatch = 32;
 
@@ -345,10 +348,10 @@ __device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(cons
                                                         std::uint32_t* __restrict__ partial,
                                                         const std::uint32_t* __restrict__ list,
                                                         const std::uint32_t* __restrict__ list_count) {
-    __shared__ std::uint32_t h[kNumClusters * kAcSymbols];
+    __shared__ std::uint32_t h[kNumClusters * kHistRow];
     __shared__ TokenTables T;
     __shared__ __align__(16) std::int16_t stage[4][2][1024];
-    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
+    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kHistRow; i += 256) {
         h[i] = 0;
     }
     load_token_tables(T);
@@ -395,7 +398,7 @@ __device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(cons
                                    [&](bool valid, std::uint32_t k, int cl, std::uint32_t sym, std::uint32_t nb,
                                        std::uint32_t extra) {
                                        if (valid) {
-                                           atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
+                                           atomicAdd(&h[cl * kHistRow + (sym & (kAcSymbols - 1))], 1u);
                                            const std::uint32_t q{k >> 6};
                                            tokens[slot0 + (q >> l2m) * row + ((q & (m - 1)) << 6) + (k & 63u)] =
                                                token_record(cl, sym, nb, extra);
@@ -423,7 +426,7 @@ __device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(cons
             std::uint32_t sym{0}, nb{0}, bits{0};
             hybrid420(0u, sym, nb, bits);
             const int cl{ac_cluster(it.nz_ctx)};
-            atomicAdd(&h[cl * kAcSymbols + (sym & (kAcSymbols - 1))], 1u);
+            atomicAdd(&h[cl * kHistRow + (sym & (kAcSymbols - 1))], 1u);
             tokens[token_slot(geo, it.c, it.bx, it.by, 1, 0)] = token_record(cl, sym, nb, bits);
             item_tokens[(static_cast<std::uint32_t>(it.by) * geo.bw + static_cast<std::uint32_t>(it.bx)) * 3u +
                         (it.c == 1 ? 0u : (it.c == 0 ? 1u : 2u))] = 1;
@@ -432,7 +435,7 @@ __device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(cons
     __syncthreads();
     std::uint32_t* mine{partial + static_cast<std::size_t>(wg.x) * kNumClusters * kAcSymbols};
     for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
-        mine[i] = h[i];
+        mine[i] = h[(i >> 6) * kHistRow + (i & (kAcSymbols - 1))];
     }
 }
 
