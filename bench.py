@@ -93,6 +93,7 @@ def quality_summary() -> dict | None:
     return {"source": "profiles/quality_4k.json (tests/test_gpu_quality.py, CJX_QUALITY_FULL=1)",
             "metric": q.get("metric"), "size": q.get("size"),
             "bpp_increase_at_plain_score": q.get("mean_bpp_increase"),
+            "network_decided": [f.get("network_decided") for f in q.get("frames", [])],
             "region_ms_3840x2160": q.get("region_ms_4k")}
 
 
@@ -152,13 +153,17 @@ def mixed_batch(cj, synthetic_nv12, frames: int = 64, seed: int = 4) -> dict:
     batch.run(2)
     reps = 20
     sec = batch.run(reps)
+    # Per-batch latency: every frame of a batch completes with the batch.
+    lat_b = np.array([batch.run(1) for _ in range(reps)]) * 1e3
     batch_bytes = sum(len(f) for f in batch.files())
     batch.close()
     for x in bufs:
         x.close()
     batched = {"fps": round(frames * reps / sec, 2),
                "megapixels_per_s": round(pixels * reps / sec / 1e6, 1),
-               "ms_per_batch": round(1e3 * sec / reps, 4), "bytes": batch_bytes,
+               "ms_per_batch": round(1e3 * sec / reps, 4),
+               "frame_latency_p50_ms": round(float(np.percentile(lat_b, 50)), 4),
+               "frame_latency_p99_ms": round(float(np.percentile(lat_b, 99)), 4), "bytes": batch_bytes,
                "bytes_equal_sequential": batch_bytes == nbytes,
                "tuned": tune.strip().splitlines()[-1],
                "launches": "one launch per stage over the 64 frames (14 per batch), replayed from a hipGraph"}
@@ -306,19 +311,36 @@ def main() -> int:
     # Batched forms: the eight distinct frames as one batch (every stage one
     # launch over the eight frames, replayed from a captured hipGraph, launch
     # shapes from the autotuner), outputs into mapped page-locked memory.
+    # One batch at a time, then two batches (over the frames' first and
+    # second device copies) in flight on their own streams.
     for name, mode in () if args.no_forms else BATCH_FORMS:
-        batch = cj.Batch([(WIDTH, HEIGHT)] * SEEDS, [DISTANCE] * SEEDS, mode=mode)
-        batch.set_inputs(inputs[:SEEDS])
-        batch.run(2)
-        tuned = batch.tune(2).strip().splitlines()[-1]
-        batch.run(1)
+        pair = [cj.Batch([(WIDTH, HEIGHT)] * SEEDS, [DISTANCE] * SEEDS, mode=mode) for _ in range(2)]
+        tuned = []
+        for j, batch in enumerate(pair):
+            batch.set_inputs(inputs[j * SEEDS:(j + 1) * SEEDS])
+            batch.run(2)
+            tuned.append(batch.tune(2).strip().splitlines()[-1])
+            batch.run(1)
         reps = max(1, args.steps // SEEDS)
-        sec = batch.run(reps)
-        b_sizes = [len(f) for f in batch.files()]
-        batch.close()
+        sec = pair[0].run(reps)
+        b_sizes = [len(f) for f in pair[0].files()]
+        reps2 = max(1, reps // 2)
+        lib.cjx_device_synchronize()
+        t1 = time.perf_counter()
+        for _ in range(reps2):
+            for batch in pair:
+                batch.launch(1)
+        for batch in pair:
+            batch.wait()
+        sec2 = time.perf_counter() - t1
+        b2_sizes = [len(f) for f in pair[1].files()]
+        for batch in pair:
+            batch.close()
         forms[name] = {"fps": round(SEEDS * reps / sec, 3),
                        "bpp": round(8.0 * float(np.mean(b_sizes)) / (WIDTH * HEIGHT), 4),
-                       "frames_per_batch": SEEDS, "batches_timed": reps, "tuned": tuned}
+                       "frames_per_batch": SEEDS, "batches_timed": reps, "tuned": tuned[0],
+                       "two_in_flight_fps": round(2 * SEEDS * reps2 / sec2, 3),
+                       "two_in_flight_sizes_equal": b2_sizes == b_sizes}
         if mode == 0:
             forms[name]["sizes_equal_single_frame"] = b_sizes == sizes
 
@@ -418,6 +440,17 @@ def main() -> int:
     quality = quality_summary()
     if quality is not None:
         out["quality"] = quality
+    if "learned+mfma16" in forms and "batch8 learned+mfma16" in forms:
+        # The comparison the metric names: the learned + matrix-core form
+        # against the plain path, in speed and in rate at equal quality.
+        lm, blm, bp = forms["learned+mfma16"], forms["batch8 learned+mfma16"], forms["batch8"]
+        out["learned_mfma_vs_plain"] = {
+            "single_frame_speedup": lm["vs_plain_fps"],
+            "batched_speedup_vs_plain_single_frame": blm["vs_plain_fps"],
+            "batched_speedup_vs_plain_batched": round(blm["fps"] / bp["fps"], 4),
+            "bpp_vs_plain_same_distance": lm["bpp_vs_plain"],
+            "bpp_increase_at_equal_score": (quality or {}).get("bpp_increase_at_plain_score", {}).get("learned+mfma16"),
+            "target": ">= 1.3x the plain GPU form at <= 2 % bpp increase at equal distance-1.0 score"}
     if not args.no_mixed_batch:
         out["mixed_batch"] = mixed_batch(cj, synthetic_nv12)
     if not args.no_cpu_baseline:

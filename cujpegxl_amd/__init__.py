@@ -123,6 +123,8 @@ def stages_lib() -> C.CDLL:
         L.cjx_batch_create.argtypes = [C.c_int32, C.c_uint32, P, P, P, C.c_uint32, C.POINTER(C.c_void_p)]
         L.cjx_batch_set_inputs.argtypes = [P, P]
         L.cjx_batch_run.argtypes = [P, C.c_int32, C.POINTER(C.c_double)]
+        L.cjx_batch_launch.argtypes = [P, C.c_int32]
+        L.cjx_batch_wait.argtypes = [P]
         L.cjx_batch_result.argtypes = [P, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
         L.cjx_batch_tune.argtypes = [P, C.c_int32, C.c_char_p, C.c_size_t]
         L.cjx_batch_destroy.argtypes = [P]
@@ -533,6 +535,15 @@ class Batch:
         if st != OK:
             raise RuntimeError(f"batch run failed: {status_string(st)}")
         return sec.value
+
+    def launch(self, count: int = 1) -> None:
+        """Queues `count` replays without waiting (see wait())."""
+        if self.lib.cjx_batch_launch(self.handle, count) != OK:
+            raise RuntimeError("batch launch failed")
+
+    def wait(self) -> None:
+        if self.lib.cjx_batch_wait(self.handle) != OK:
+            raise RuntimeError("batch wait failed")
 
     def files(self) -> list[bytes]:
         out = []

@@ -187,6 +187,10 @@ bool BatchEncoder::launch(int count) {
     return true;
 }
 
+bool BatchEncoder::wait() {
+    return stream_ != nullptr && hipSetDevice(device_) == hipSuccess && hipStreamSynchronize(stream_) == hipSuccess;
+}
+
 bool BatchEncoder::run(int count, double* seconds) {
     const auto t0{std::chrono::steady_clock::now()};
     const bool ok{launch(count) && hipStreamSynchronize(stream_) == hipSuccess};

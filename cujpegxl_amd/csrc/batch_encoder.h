@@ -35,6 +35,11 @@ public:
                     const std::uint8_t* const* chroma, const std::size_t* chroma_pitch);
     // `count` replays of the batch's graph, then a wait; wall seconds.
     bool run(int count, double* seconds);
+    // `count` replays queued on the batch's own stream, no wait: another
+    // batch's replays on its own stream may run beside them.
+    bool launch(int count);
+    // Waits for the queued replays (files and sizes are then valid).
+    bool wait();
     std::uint64_t size(int i) const;
     const std::uint8_t* file(int i) const;
     // Times every candidate launch shape over `reps` replays and keeps the
@@ -46,7 +51,6 @@ public:
 private:
     void release();
     bool apply_shape(const BatchShape& shape);
-    bool launch(int count);
 
     int device_{-1};
     std::uint32_t mode_{0};

@@ -243,6 +243,20 @@ int cjx_batch_run(cjx_batch* b, int32_t count, double* seconds) {
     return b->engine.run(count, seconds) ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
 }
 
+int cjx_batch_launch(cjx_batch* b, int32_t count) {
+    if (b == nullptr || count <= 0) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return b->engine.launch(count) ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
+int cjx_batch_wait(cjx_batch* b) {
+    if (b == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return b->engine.wait() ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
+}
+
 int cjx_batch_result(const cjx_batch* b, uint32_t i, const uint8_t** data, uint64_t* size) {
     if (b == nullptr || data == nullptr || size == nullptr || i >= static_cast<uint32_t>(b->engine.frames())) {
         return CUJPEGXL_INVALID_ARGUMENT;

@@ -108,6 +108,11 @@ int cjx_batch_set_inputs(cjx_batch* batch, const cujpegxl_nv12_input* inputs);
 /* `count` replays of the batch's graph back to back, then a wait; the wall
  * seconds of the whole into *seconds. */
 int cjx_batch_run(cjx_batch* batch, int32_t count, double* seconds);
+/* `count` replays queued on the batch's own stream without a wait (another
+ * batch's replays may then run beside them), and the wait for them; results
+ * are valid after cjx_batch_wait. */
+int cjx_batch_launch(cjx_batch* batch, int32_t count);
+int cjx_batch_wait(cjx_batch* batch);
 int cjx_batch_result(const cjx_batch* batch, uint32_t index, const uint8_t** data, uint64_t* size);
 /* Times every candidate launch shape (tokenizer workgroups per frame x
  * output words per assembly thread) over `reps` replays and keeps the

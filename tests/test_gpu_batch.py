@@ -77,3 +77,30 @@ def test_batch_modes_match_single_frames(mode):
     b.close()
     for d in devs:
         d.close()
+
+
+def test_two_batches_in_flight(jo):
+    """Two batches queued on their own streams, alternately, without waits
+    between their replays: each one's files still equal the oracle's."""
+    say("two batches in flight")
+    sizes = [(w, h) for w, h, _ in FRAMES[:3]]
+    dists = [d for _, _, d in FRAMES[:3]]
+    pair, devs_all, want = [], [], []
+    for seed0 in (100, 120):
+        host = [synthetic_nv12(w, h, seed0 + i) for i, (w, h) in enumerate(sizes)]
+        devs = [cj.DeviceBuffer.from_array(np.concatenate([l.ravel(), c.ravel()])) for l, c in host]
+        b = cj.Batch(sizes, dists)
+        b.set_inputs([(d.ptr, d.ptr + w * h, w, w) for d, (w, h) in zip(devs, sizes)])
+        pair.append(b)
+        devs_all += devs
+        want.append([jo.encode_nv12(l, c, w, h, d) for (l, c), (w, h), d in zip(host, sizes, dists)])
+    for _ in range(3):
+        for b in pair:
+            b.launch(1)
+    for b in pair:
+        b.wait()
+    for b, w in zip(pair, want):
+        assert b.files() == w
+        b.close()
+    for d in devs_all:
+        d.close()
