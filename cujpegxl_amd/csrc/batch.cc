@@ -53,6 +53,16 @@ void BatchEncoder::release() {
         (void)hipStreamDestroy(stream_);
         stream_ = nullptr;
     }
+    if (dc_stream_ != nullptr) {
+        (void)hipStreamDestroy(dc_stream_);
+        dc_stream_ = nullptr;
+    }
+    for (hipEvent_t* e : {&fork_, &join_}) {
+        if (*e != nullptr) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
+    }
     jobs_.clear();
     out_off_.clear();
     caps_.clear();
@@ -81,6 +91,9 @@ bool BatchEncoder::configure(int n, const std::uint32_t* w, const std::uint32_t*
         outs += cap[i];
     }
     if (hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&dc_stream_, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&fork_, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&arena_, arena) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&jobs_dev_), sizeof(FrameJob) * n) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&out_host_), outs, hipHostMallocMapped) != hipSuccess ||
@@ -144,7 +157,7 @@ bool BatchEncoder::apply_shape(const BatchShape& shape) {
     if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         return false;
     }
-    launch_batch(stream_, jobs_dev_, static_cast<int>(jobs_.size()), total_, mode_);
+    launch_batch(stream_, dc_stream_, fork_, join_, jobs_dev_, static_cast<int>(jobs_.size()), total_, mode_);
     if (hipStreamEndCapture(stream_, &graph_) != hipSuccess ||
         hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) != hipSuccess) {
         return false;

@@ -91,8 +91,11 @@ struct BatchShape {
 // Stage grids of one frame for a shape (nx, ny only; `first` is the caller's).
 void batch_stage_grids(const Geometry& geo, const BatchShape& shape, StageGrid (&grid)[kBStages]);
 
-// Queues the 14 batched launches of a batch on `s` (all frames in `mode`).
-void launch_batch(hipStream_t s, const FrameJob* jobs, int count, const std::uint32_t (&total)[kBStages],
-                  std::uint32_t mode);
+// Queues the 14 batched launches of a batch (all frames in `mode`): the
+// region kernel, the AC chain, the layout and the assembly on `s`, the DC
+// chain on `s_dc`, forked after the region kernel (event `fork`) and joined
+// before the layout (event `join`).
+void launch_batch(hipStream_t s, hipStream_t s_dc, hipEvent_t fork, hipEvent_t join, const FrameJob* jobs,
+                  int count, const std::uint32_t (&total)[kBStages], std::uint32_t mode);
 
 }  // namespace cjx

@@ -153,23 +153,30 @@ void batch_stage_grids(const Geometry& geo, const BatchShape& shape, StageGrid (
     set(kBAssemble, static_cast<std::uint32_t>(ag < 2048 ? ag : 2048), 1);
 }
 
-void launch_batch(hipStream_t s, const FrameJob* jobs, int count, const std::uint32_t (&total)[kBStages],
-                  std::uint32_t mode) {
+void launch_batch(hipStream_t s, hipStream_t s_dc, hipEvent_t fork, hipEvent_t join, const FrameJob* jobs,
+                  int count, const std::uint32_t (&total)[kBStages], std::uint32_t mode) {
     const bool mf{(mode & kModeMfma16) != 0}, learned{(mode & kModeLearned) != 0};
     auto* const region{mf ? (learned ? region_batch_kernel<true, true> : region_batch_kernel<true, false>)
                           : (learned ? region_batch_kernel<false, true> : region_batch_kernel<false, false>)};
     region<<<total[kBRegion], 256, 0, s>>>(jobs, count);
+    // The DC chain (DC values and colour maps, DC / AcMetadata histograms and
+    // codes, header blobs, sizes, emission) reads only what the region kernel
+    // wrote: it forks here and joins before the layout, beside the AC chain.
+    (void)hipEventRecord(fork, s);
+    (void)hipStreamWaitEvent(s_dc, fork, 0);
+    dc_maps_batch_kernel<<<total[kBDcMaps], 256, 0, s_dc>>>(jobs, count);
+    dc_hist_batch_kernel<<<total[kBDcHist], 256, 0, s_dc>>>(jobs, count);
+    huffman_batch_kernel<kBHuffDc><<<total[kBHuffDc], 64 * kTrials, 0, s_dc>>>(jobs, count);
+    dc_blobs_batch_kernel<<<total[kBDcBlobs], 256, 0, s_dc>>>(jobs, count);
+    dc_scan_batch_kernel<<<total[kBDcScan], 1024, 0, s_dc>>>(jobs, count);
+    dc_emit_batch_kernel<<<total[kBDcEmit], 256, 0, s_dc>>>(jobs, count);
+    (void)hipEventRecord(join, s_dc);
     quantize_batch_kernel<<<total[kBQuantize], 256, 0, s>>>(jobs, count);
-    dc_maps_batch_kernel<<<total[kBDcMaps], 256, 0, s>>>(jobs, count);
-    dc_hist_batch_kernel<<<total[kBDcHist], 256, 0, s>>>(jobs, count);
-    huffman_batch_kernel<kBHuffDc><<<total[kBHuffDc], 64 * kTrials, 0, s>>>(jobs, count);
-    dc_blobs_batch_kernel<<<total[kBDcBlobs], 256, 0, s>>>(jobs, count);
-    dc_scan_batch_kernel<<<total[kBDcScan], 1024, 0, s>>>(jobs, count);
-    dc_emit_batch_kernel<<<total[kBDcEmit], 256, 0, s>>>(jobs, count);
     tokens_batch_kernel<<<total[kBTokens], 256, 0, s>>>(jobs, count);
     reduce_batch_kernel<<<total[kBReduce], 256, 0, s>>>(jobs, count);
     huffman_batch_kernel<kBHuffAc><<<total[kBHuffAc], 64 * kTrials, 0, s>>>(jobs, count);
     ac_emit_batch_kernel<<<total[kBAcEmit], kPartThreads, 0, s>>>(jobs, count);
+    (void)hipStreamWaitEvent(s, join, 0);
     layout_batch_kernel<<<total[kBLayout], 256, 0, s>>>(jobs, count);
     assemble_batch_kernel<<<total[kBAssemble], 256, 0, s>>>(jobs, count);
 }

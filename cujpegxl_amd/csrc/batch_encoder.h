@@ -64,6 +64,8 @@ private:
     std::uint8_t* out_host_{nullptr};
     std::uint64_t* sizes_host_{nullptr};
     hipStream_t stream_{nullptr};
+    hipStream_t dc_stream_{nullptr};  // the DC chain's branch of the captured graph
+    hipEvent_t fork_{nullptr}, join_{nullptr};
     hipGraph_t graph_{nullptr};
     hipGraphExec_t exec_{nullptr};
     bool inputs_set_{false};
