@@ -41,12 +41,44 @@ __device__ __forceinline__ void clear_words(const Clears& c, const Wg& wg) {
     }
 }
 
+// Cross-lane sums without the LDS crossbar.  add_down_row<n>: lane i adds
+// lane i + n of its row of 16 lanes (a DPP row shift; n = 1, 2, 4, 8);
+// add_down_16: lanes of rows 0 and 2 add rows 1 and 3 (v_permlane16_swap);
+// add_down_32: lanes 0-31 add lanes 32-63 (v_permlane32_swap).  Callers read
+// only lanes whose partners lie in the same row (or the upper row / half),
+// where the result equals a shfl_down (or shfl_xor) tree's term for term.
+template <int kN>
+__device__ __forceinline__ float add_down_row(float v) {
+    static_assert(kN == 1 || kN == 2 || kN == 4 || kN == 8, "row shift");
+    return v + u2f(static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(f2u(v)), 0x100 + kN,
+                                                                          0xf, 0xf, true)));
+}
+
+__device__ __forceinline__ float add_down_16(float v) {
+    const auto r{__builtin_amdgcn_permlane16_swap(f2u(v), f2u(v), false, false)};
+    return v + u2f(r[1]);
+}
+
+__device__ __forceinline__ float add_down_32(float v) {
+    const auto r{__builtin_amdgcn_permlane32_swap(f2u(v), f2u(v), false, false)};
+    return v + u2f(r[1]);
+}
+
+// Lane i - 1 of the same row of 16 lanes (DPP row shift right).
+__device__ __forceinline__ float from_lane_below(float v) {
+    return u2f(static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(f2u(v)), 0x111, 0xf, 0xf,
+                                                                      true)));
+}
+
+// Sum of the wave's 64 values in lane 0 (offsets 32, 16, ..., 1 of the
+// shfl_down tree; the other lanes hold partial sums).
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-    for (int off{32}; off > 0; off >>= 1) {
-        v += __shfl_down(v, off, 64);
-    }
-    return v;
+    v = add_down_32(v);
+    v = add_down_16(v);
+    v = add_down_row<8>(v);
+    v = add_down_row<4>(v);
+    v = add_down_row<2>(v);
+    return add_down_row<1>(v);
 }
 
 __device__ __forceinline__ float clamp01(float v) {
@@ -690,8 +722,8 @@ __device__ __forceinline__ bool mlp_decide(RegionShared& S, const FrameParams& p
                 s = __builtin_fmaf(h2f(static_cast<std::uint16_t>(word[h] >> 16)), x[i + 2 * h + 1], s);
             }
         }
-        s += __shfl_down(s, 2, 4);
-        s += __shfl_down(s, 1, 4);
+        s = add_down_row<2>(s);
+        s = add_down_row<1>(s);
         if (k == 0) {
             hid[j] = fmaxf(s + g_mlp.b1[j], 0.0f);
         }
@@ -704,10 +736,9 @@ __device__ __forceinline__ bool mlp_decide(RegionShared& S, const FrameParams& p
         for (int q{0}; q < 8; ++q) {
             s = __builtin_fmaf(h2f(g_mlp.w2[o * 64 + 8 * p + q]), hid[8 * p + q], s);
         }
-#pragma unroll
-        for (int off{4}; off > 0; off >>= 1) {
-            s += __shfl_down(s, off, 8);
-        }
+        s = add_down_row<4>(s);
+        s = add_down_row<2>(s);
+        s = add_down_row<1>(s);
         if (p == 0) {
             hid[64 + o] = s + g_mlp.b2[o];
         }
@@ -852,7 +883,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         float sum{0.0f};
 #pragma unroll
         for (int k{0}; k < 4; ++k) {
-            const float prev{__shfl_up(sum, 1, 4)};
+            const float prev{from_lane_below(sum)};
             if (dy == k) {
                 float acc{k == 0 ? 0.0f : prev};
                 acc += m[0];
@@ -1075,14 +1106,12 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
                 fg = __builtin_fmaf(grad_weight(fx) + grad_weight(f), c2, fg);
             }
         }
-#pragma unroll
-        for (int off{4}; off > 0; off >>= 1) {
-            ad += __shfl_down(ad, off, 8);
-            ar += __shfl_down(ar, off, 8);
-            if (want_features) {
-                fv += __shfl_down(fv, off, 8);
-                fg += __shfl_down(fg, off, 8);
-            }
+        const auto sum8 = [](float x) { return add_down_row<1>(add_down_row<2>(add_down_row<4>(x))); };
+        ad = sum8(ad);
+        ar = sum8(ar);
+        if (want_features) {
+            fv = sum8(fv);
+            fg = sum8(fg);
         }
         if ((lane & 7) == 0) {
             S.cost8[((by >> 1) * 2 + (bx >> 1)) * 4 + (by & 1) * 2 + (bx & 1)] = 64.0f * ad + 6.0e-4f * ar;
@@ -1120,14 +1149,12 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         const float dc{c8[0]};
         // A DCT8 block's 64 terms sit in 16 lanes: bits 0-2 of the column and
         // bit 0 of the lane group vary within it.
-#pragma unroll
-        for (int off : {1, 2, 4, 16}) {
-            d8 += __shfl_xor(d8, off, 64);
-            r8 += __shfl_xor(r8, off, 64);
-            if (want_features) {
-                fv += __shfl_xor(fv, off, 64);
-                fg += __shfl_xor(fg, off, 64);
-            }
+        const auto sum16 = [](float x) { return add_down_16(add_down_row<4>(add_down_row<2>(add_down_row<1>(x)))); };
+        d8 = sum16(d8);
+        r8 = sum16(r8);
+        if (want_features) {
+            fv = sum16(fv);
+            fg = sum16(fg);
         }
         if ((lane & 7) == 0 && (grp & 1) == 0) {
             const int j{2 * (grp >> 1) + ((lane & 15) >> 3)};
