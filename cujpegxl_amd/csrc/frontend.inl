@@ -492,19 +492,42 @@ __device__ __forceinline__ void gaborish_tile(RegionShared& S, int tid) {
     constexpr float k4{-0.0014789063378272242f};
     constexpr float nm{1.0f / (1.0f + 4.0f * (k0 + k1 + k2 + k4 + 2.0f * k3))};
     constexpr float wc{nm}, wr{nm * k0}, wr2{nm * k2}, wd{nm * k1}, wl{nm * k3}, wd2{nm * k4};
-#pragma unroll 4
-    for (int i{tid}; i < 3 * 1024; i += 256) {
-        const int c{i >> 10}, p{i & 1023};
-        const float* s{&S.u.f.xyb[c][((p >> 5) + kHalo) * kTileW + (p & 31) + kHalo]};
-#define AT(dx, dy) s[(dy) * kTileW + (dx)]
-        const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
-        const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
-        const float dg{AT(-1, -1) + AT(1, -1) + AT(-1, 1) + AT(1, 1)};
-        const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
-                       AT(-1, 2) + AT(1, 2)};
-        const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
-        plane_of(S, c)[sp_index(p >> 5, p & 31)] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
+    // Thread (gy, gx) of each channel: the four outputs of row gy, columns
+    // 4 gx .. 4 gx + 3, from the five input rows gy - 2 .. gy + 2, columns
+    // 4 gx - 2 .. 4 gx + 5 (two aligned 16-byte reads per row; the tile's
+    // row pitch is 36 floats).
+    const int gx{tid & 7}, gy{tid >> 3};
+#pragma unroll 1
+    for (int c{0}; c < 3; ++c) {
+        const float* const base{&S.u.f.xyb[c][gy * kTileW + 4 * gx]};
+        float r[5][8];
+#pragma unroll
+        for (int dy{0}; dy < 5; ++dy) {
+            const float4 a{*reinterpret_cast<const float4*>(base + dy * kTileW)};
+            const float4 b{*reinterpret_cast<const float4*>(base + dy * kTileW + 4)};
+            r[dy][0] = a.x;
+            r[dy][1] = a.y;
+            r[dy][2] = a.z;
+            r[dy][3] = a.w;
+            r[dy][4] = b.x;
+            r[dy][5] = b.y;
+            r[dy][6] = b.z;
+            r[dy][7] = b.w;
+        }
+        float o[4];
+#pragma unroll
+        for (int k{0}; k < 4; ++k) {
+#define AT(dx, dy) r[(dy) + 2][k + 2 + (dx)]
+            const float e{AT(-1, 0) + AT(1, 0) + AT(0, -1) + AT(0, 1)};
+            const float r2{AT(-2, 0) + AT(2, 0) + AT(0, -2) + AT(0, 2)};
+            const float dg{AT(-1, -1) + AT(1, -1) + AT(-1, 1) + AT(1, 1)};
+            const float l8{AT(-2, -1) + AT(2, -1) + AT(-2, 1) + AT(2, 1) + AT(-1, -2) + AT(1, -2) +
+                           AT(-1, 2) + AT(1, 2)};
+            const float d4{AT(-2, -2) + AT(2, -2) + AT(-2, 2) + AT(2, 2)};
+            o[k] = wc * AT(0, 0) + wr * e + wr2 * r2 + wd * dg + wl * l8 + wd2 * d4;
 #undef AT
+        }
+        *reinterpret_cast<float4*>(&plane_of(S, c)[sp_index(gy, 4 * gx)]) = float4{o[0], o[1], o[2], o[3]};
     }
 }
 
