@@ -55,25 +55,52 @@ __device__ __forceinline__ int ac_cluster(std::uint32_t ctx) {
     return 8 + group * 12 + band;
 }
 
+
+// Inclusive prefix sums over the wave by DPP moves (no LDS crossbar): within
+// each row of 16 lanes by shifts of 1, 2, 4 and 8 (lanes shifted in from
+// outside the row read 0), then the last lane of rows 0 and 2 into rows 1
+// and 3 and the last lane of row 1 into rows 2 and 3 (row broadcasts; rows
+// the mask leaves out add 0).
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ std::uint32_t dpp_u32(std::uint32_t x) {
+    return static_cast<std::uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(x), kCtrl, kRowMask, 0xf, kCtrl < 0x140));
+}
+
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ std::uint64_t dpp_u64(std::uint64_t x) {
+    const std::uint32_t lo{dpp_u32<kCtrl, kRowMask>(static_cast<std::uint32_t>(x))};
+    const std::uint32_t hi{dpp_u32<kCtrl, kRowMask>(static_cast<std::uint32_t>(x >> 32))};
+    return (static_cast<std::uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ std::uint32_t wave_incl_scan_u32(std::uint32_t x) {
+    x += dpp_u32<0x111, 0xf>(x);
+    x += dpp_u32<0x112, 0xf>(x);
+    x += dpp_u32<0x114, 0xf>(x);
+    x += dpp_u32<0x118, 0xf>(x);
+    x += dpp_u32<0x142, 0xa>(x);
+    return x + dpp_u32<0x143, 0xc>(x);
+}
+
+__device__ __forceinline__ std::uint64_t wave_incl_scan_u64(std::uint64_t x) {
+    x += dpp_u64<0x111, 0xf>(x);
+    x += dpp_u64<0x112, 0xf>(x);
+    x += dpp_u64<0x114, 0xf>(x);
+    x += dpp_u64<0x118, 0xf>(x);
+    x += dpp_u64<0x142, 0xa>(x);
+    return x + dpp_u64<0x143, 0xc>(x);
+}
+
+// Sum over the wave, in every lane (the scan's last lane, read as a scalar).
 __device__ __forceinline__ int wave_sum_int(int v) {
-#pragma unroll
-    for (int off{32}; off > 0; off >>= 1) {
-        v += __shfl_down(v, off, 64);
-    }
-    return __shfl(v, 0, 64);
+    return __builtin_amdgcn_readlane(static_cast<int>(wave_incl_scan_u32(static_cast<std::uint32_t>(v))), 63);
 }
 
 // Exclusive prefix over the wave.
 __device__ __forceinline__ int wave_excl_scan(int v, int lane) {
-    int x{v};
-#pragma unroll
-    for (int off{1}; off < 64; off <<= 1) {
-        const int y{__shfl_up(x, off, 64)};
-        if (lane >= off) {
-            x += y;
-        }
-    }
-    return x - v;
+    (void)lane;
+    return static_cast<int>(wave_incl_scan_u32(static_cast<std::uint32_t>(v)) - static_cast<std::uint32_t>(v));
 }
 
 // ORs `len` (<= 32) bits of `value` at absolute bit position `pos`; words at
@@ -477,14 +504,7 @@ __global__ __launch_bounds__(256) void ac_hist_reduce_kernel(const std::uint32_t
 __device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std::uint64_t& total) {
     const int tid{static_cast<int>(threadIdx.x)};
     const int lane{tid & 63}, wv{tid >> 6}, nw{static_cast<int>(blockDim.x) >> 6};
-    std::uint64_t x{v};
-#pragma unroll
-    for (int off{1}; off < 64; off <<= 1) {
-        const std::uint64_t y{__shfl_up(x, off, 64)};
-        if (lane >= off) {
-            x += y;
-        }
-    }
+    const std::uint64_t x{wave_incl_scan_u64(v)};
     if (lane == 63) {
         sh[wv] = x;
     }
