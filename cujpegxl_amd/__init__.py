@@ -22,7 +22,7 @@ PUBLIC_LIB = LIB_DIR / "libcujpegxl.so.1"
 STAGES_LIB = LIB_DIR / "libcujpegxl_stages.so"
 
 API_VERSION = 3  # cujpegxl.h's CUJPEGXL_API_VERSION, the reference's value
-EXT_VERSION = 1  # CUJPEGXL_EXT_VERSION: host-input and asynchronous entry points
+EXT_VERSION = 2  # CUJPEGXL_EXT_VERSION: host-input, asynchronous and encoder-form entry points
 
 OK, NOT_IMPLEMENTED, INVALID_ARGUMENT, UNSUPPORTED_RESOLUTION, NO_DEVICE, INTERNAL_ERROR, \
     BUFFER_TOO_SMALL = range(7)
@@ -73,6 +73,7 @@ def _bind_public(L: C.CDLL) -> C.CDLL:
     L.cujpegxl_max_output_size.argtypes = [C.POINTER(Config), C.POINTER(C.c_size_t)]
     L.cujpegxl_encoder_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
     L.cujpegxl_encoder_configure.argtypes = [C.c_void_p, C.POINTER(Config)]
+    L.cujpegxl_encoder_set_mode.argtypes = [C.c_void_p, C.c_uint32]
     L.cujpegxl_encoder_encode.argtypes = [C.c_void_p, C.POINTER(Nv12Input), C.POINTER(Output)]
     L.cujpegxl_encoder_destroy.argtypes = [C.c_void_p]
     L.cujpegxl_encoder_destroy.restype = None
@@ -131,6 +132,8 @@ def stages_lib() -> C.CDLL:
         L.cjx_batch_destroy.restype = None
         L.cjx_stage_encode_repeat.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output), C.c_int,
                                               C.POINTER(C.c_double)]
+        L.cjx_stage_encode_pipelined.argtypes = [P, C.c_int, C.POINTER(Nv12Input), C.c_int, C.POINTER(Output),
+                                                 C.c_int, C.POINTER(C.c_double)]
         L.cjx_stage_encode_cycle.argtypes = [P, C.POINTER(Nv12Input), C.c_int, C.POINTER(Output), C.c_int,
                                              C.POINTER(C.c_double)]
         L.cjx_stage_encode_dct8_uniform.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
@@ -376,6 +379,21 @@ class Encoder:
             raise RuntimeError(f"encode failed: {status_string(s)}")
         return self.out[: out.size].tobytes()
 
+    @staticmethod
+    def encode_pipelined(encoders: list["Encoder"], inputs: list[tuple[int, int, int, int]], outputs: list[int],
+                         capacity: int, count: int) -> float:
+        """`count` frames through encode_async / finish of `encoders` in turn,
+        up to len(encoders) frames in flight, finished into the page-locked
+        buffers `outputs` (one per encoder), in native code; wall seconds."""
+        encs = (C.c_void_p * len(encoders))(*(e.handle for e in encoders))
+        arr = (Nv12Input * len(inputs))(*(Nv12Input(*i) for i in inputs))
+        outs = (Output * len(outputs))(*(Output(o, 0, capacity) for o in outputs))
+        sec = C.c_double(0.0)
+        s = stages_lib().cjx_stage_encode_pipelined(encs, len(encoders), arr, len(inputs), outs, count, C.byref(sec))
+        if s != OK:
+            raise RuntimeError(f"encode_pipelined failed: {status_string(s)}")
+        return sec.value
+
     def encode_async(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
                      stream: int = 0) -> int:
         """Queues a frame after the work on `stream`; returns the status."""
@@ -424,9 +442,13 @@ class Encoder:
         self.lib.cjx_stage_set_profiling(self.handle, (2 if region_only else 1) if on else 0)
 
     def set_mode(self, mode: int) -> None:
-        """Encoder mode bits (MODE_MFMA16; 0 for the plain path), from the
-        next encode."""
-        if self.lib.cjx_stage_set_mode(self.handle, mode) != OK:
+        """Encoder mode bits (MODE_MFMA16 | MODE_LEARNED; 0 for the plain
+        path), from the next encode: the public library's
+        cujpegxl_encoder_set_mode, or the stage library's switch, which also
+        takes the forced-size test modes."""
+        st = (self.lib.cjx_stage_set_mode(self.handle, mode) if self.lib is stages_lib()
+              else self.lib.cujpegxl_encoder_set_mode(self.handle, mode))
+        if st != OK:
             raise ValueError(f"bad mode {mode}")
 
     def export_features(self, on: bool = True) -> None:

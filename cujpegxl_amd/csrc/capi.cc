@@ -213,4 +213,15 @@ cujpegxl_status cujpegxl_encoder_finish(cujpegxl_encoder* encoder, cujpegxl_outp
     return size > output->capacity ? CUJPEGXL_BUFFER_TOO_SMALL : CUJPEGXL_OK;
 }
 
+cujpegxl_status cujpegxl_encoder_set_mode(cujpegxl_encoder* encoder, uint32_t mode) {
+    static_assert(CUJPEGXL_MODE_MFMA16 == cjx::kModeMfma16 && CUJPEGXL_MODE_LEARNED == cjx::kModeLearned,
+                  "public mode bits");
+    if (encoder == nullptr || (mode & ~(CUJPEGXL_MODE_MFMA16 | CUJPEGXL_MODE_LEARNED)) != 0 ||
+        encoder->engine.pending()) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    encoder->engine.set_mode(mode);
+    return CUJPEGXL_OK;
+}
+
 }  // extern "C"

@@ -26,3 +26,8 @@ CUJPEGXL_3 {
         cujpegxl_noalloc_workspace_bytes;
         cujpegxl_encode_noalloc;
 } CUJPEGXL_2;
+
+CUJPEGXL_4 {
+    global:
+        cujpegxl_encoder_set_mode;
+} CUJPEGXL_3;

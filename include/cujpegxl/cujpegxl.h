@@ -16,11 +16,12 @@
 extern "C" {
 #endif
 
-/* cujpegxl.h:24: the reference's ABI version, unchanged.  The host-input
- * and asynchronous entry points at the end of this header are extensions,
- * versioned on their own (CUJPEGXL_EXT_VERSION, cujpegxl_ext_version). */
+/* cujpegxl.h:24: the reference's ABI version, unchanged.  The host-input,
+ * asynchronous, allocation-free and encoder-form entry points at the end of
+ * this header are extensions, versioned on their own (CUJPEGXL_EXT_VERSION,
+ * cujpegxl_ext_version). */
 #define CUJPEGXL_API_VERSION 3u
-#define CUJPEGXL_EXT_VERSION 1u
+#define CUJPEGXL_EXT_VERSION 2u
 
 /* cujpegxl.h:26-34; values are ABI. */
 typedef enum cujpegxl_status {
@@ -90,7 +91,8 @@ void cujpegxl_encoder_destroy(cujpegxl_encoder* encoder);
 
 /* ---- Extensions, version 1 (no counterpart in the reference) ------------ */
 
-/* CUJPEGXL_EXT_VERSION of the library: 1 for the three entry points below.
+/* CUJPEGXL_EXT_VERSION of the library: 1 for the three entry points below,
+ * 2 with cujpegxl_encoder_set_mode.
  * Exported under the version node CUJPEGXL_2; a caller that finds this
  * symbol may use them. */
 uint32_t cujpegxl_ext_version(void);
@@ -165,6 +167,20 @@ size_t cujpegxl_noalloc_workspace_bytes(uint32_t width, uint32_t height);
 cujpegxl_status cujpegxl_encode_noalloc(const cujpegxl_config* config, const cujpegxl_nv12_input* input,
                                         uintptr_t output, size_t capacity, uintptr_t size_out,
                                         uintptr_t stream);
+
+/* ---- Encoder forms (no counterpart in the reference) --------------------
+ *
+ * Exported under the version node CUJPEGXL_4 (extension version 2).  The
+ * plain path (mode 0, every encoder's start) writes the reference's files;
+ * CUJPEGXL_MODE_MFMA16 computes the DCTs as FP16-operand matrix-core
+ * products (coefficients within the FP16 storage error of the plain ones);
+ * CUJPEGXL_MODE_LEARNED lets the compiled-in selection network choose the
+ * transform sizes where it is confident (the heuristic elsewhere).  Only
+ * encoder decisions change: files stay conformant JPEG XL.  Applies from the
+ * next encode; not while a frame is pending. */
+#define CUJPEGXL_MODE_MFMA16 1u
+#define CUJPEGXL_MODE_LEARNED 2u
+cujpegxl_status cujpegxl_encoder_set_mode(cujpegxl_encoder* encoder, uint32_t mode);
 
 #ifdef __cplusplus
 } /* extern "C" */

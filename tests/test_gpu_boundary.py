@@ -198,3 +198,25 @@ def test_sync_encode_is_ordered_after_the_null_stream(jo, frame):
     for b in srcs + [planes]:
         b.close()
     enc.close()
+
+
+def test_public_form_switch_matches_stage_library(frame):
+    """cujpegxl_encoder_set_mode (public library) gives the files of the stage
+    library's switch in the same mode; mode 0 is the reference's file again;
+    bits outside the public forms are refused."""
+    luma, chroma, nv12, want = frame
+    say(f"public encoder forms {W}x{H}")
+    buf = cj.DeviceBuffer.from_array(nv12)
+    pub = cj.Encoder(W, H, D)
+    stg = cj.Encoder(W, H, D, stages=True)
+    for mode in (cj.MODE_MFMA16, cj.MODE_LEARNED, cj.MODE_MFMA16 | cj.MODE_LEARNED):
+        pub.set_mode(mode)
+        stg.set_mode(mode)
+        assert pub.encode_device(buf.ptr, buf.ptr + W * H, W, W) == stg.encode_device(buf.ptr, buf.ptr + W * H, W, W)
+    pub.set_mode(0)
+    assert pub.encode_device(buf.ptr, buf.ptr + W * H, W, W) == want
+    with pytest.raises(ValueError):
+        pub.set_mode(cj.MODE_FORCE16)
+    pub.close()
+    stg.close()
+    buf.close()

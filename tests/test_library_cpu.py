@@ -30,9 +30,9 @@ def exported(lib: pathlib.Path) -> set[str]:
 def test_public_library_exports_exactly_the_abi():
     want = declared("cujpegxl.h", "cujpegxl_")
     # cujpegxl.h:66-82, plus the extension version query, the host-input
-    # and asynchronous entry points, and the allocation-free encode with its
-    # workspace query.
-    assert len(want) == 15
+    # and asynchronous entry points, the allocation-free encode with its
+    # workspace query, and the encoder-form switch.
+    assert len(want) == 16
     assert exported(cj.PUBLIC_LIB) == want
     lib = cj.public_lib()
     for name in want:
@@ -52,7 +52,7 @@ def test_status_and_backend_strings():
     lib = cj.public_lib()
     assert [lib.cujpegxl_backend_string(i).decode() for i in range(3)] == ["unknown", "fp32-simt",
                                                                             "fp16-tensor"]
-    assert cj.api_version() == 3 and cj.ext_version() == 1
+    assert cj.api_version() == 3 and cj.ext_version() == 2
 
 
 def test_max_output_size_and_validation():
@@ -101,7 +101,8 @@ def test_python_binding_argument_errors():
 def test_symbol_versions_keep_the_reference_node():
     """The reference's nine entry points stay under CUJPEGXL_1
     (cujpegxl.lds:4-9); the extensions sit under CUJPEGXL_2, the
-    allocation-free encode under CUJPEGXL_3."""
+    allocation-free encode under CUJPEGXL_3, the encoder-form switch under
+    CUJPEGXL_4."""
     out = subprocess.run(["objdump", "-T", str(cj.PUBLIC_LIB)], capture_output=True, text=True,
                          check=True).stdout
     nodes = {}
@@ -112,9 +113,11 @@ def test_symbol_versions_keep_the_reference_node():
     ext = {"cujpegxl_ext_version", "cujpegxl_encoder_encode_host", "cujpegxl_encoder_encode_async",
            "cujpegxl_encoder_finish"}
     noalloc = {"cujpegxl_noalloc_workspace_bytes", "cujpegxl_encode_noalloc"}
-    assert len(nodes) == 15
+    assert len(nodes) == 16
     for name, node in nodes.items():
         want = "CUJPEGXL_2" if name in ext else ("CUJPEGXL_3" if name in noalloc else "CUJPEGXL_1")
+        if name == "cujpegxl_encoder_set_mode":
+            want = "CUJPEGXL_4"
         assert node == want, name
 
 
