@@ -132,8 +132,6 @@ def stages_lib() -> C.CDLL:
         L.cjx_batch_destroy.restype = None
         L.cjx_stage_encode_repeat.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output), C.c_int,
                                               C.POINTER(C.c_double)]
-        L.cjx_stage_encode_pipelined.argtypes = [P, C.c_int, C.POINTER(Nv12Input), C.c_int, C.POINTER(Output),
-                                                 C.c_int, C.POINTER(C.c_double)]
         L.cjx_stage_encode_cycle.argtypes = [P, C.POINTER(Nv12Input), C.c_int, C.POINTER(Output), C.c_int,
                                              C.POINTER(C.c_double)]
         L.cjx_stage_encode_dct8_uniform.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output)]
@@ -378,21 +376,6 @@ class Encoder:
         if s != OK:
             raise RuntimeError(f"encode failed: {status_string(s)}")
         return self.out[: out.size].tobytes()
-
-    @staticmethod
-    def encode_pipelined(encoders: list["Encoder"], inputs: list[tuple[int, int, int, int]], outputs: list[int],
-                         capacity: int, count: int) -> float:
-        """`count` frames through encode_async / finish of `encoders` in turn,
-        up to len(encoders) frames in flight, finished into the page-locked
-        buffers `outputs` (one per encoder), in native code; wall seconds."""
-        encs = (C.c_void_p * len(encoders))(*(e.handle for e in encoders))
-        arr = (Nv12Input * len(inputs))(*(Nv12Input(*i) for i in inputs))
-        outs = (Output * len(outputs))(*(Output(o, 0, capacity) for o in outputs))
-        sec = C.c_double(0.0)
-        s = stages_lib().cjx_stage_encode_pipelined(encs, len(encoders), arr, len(inputs), outs, count, C.byref(sec))
-        if s != OK:
-            raise RuntimeError(f"encode_pipelined failed: {status_string(s)}")
-        return sec.value
 
     def encode_async(self, luma_ptr: int, chroma_ptr: int, luma_pitch: int, chroma_pitch: int,
                      stream: int = 0) -> int:

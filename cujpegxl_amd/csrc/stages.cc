@@ -115,34 +115,6 @@ int cjx_stage_encode_cycle(cujpegxl_encoder* e, const cujpegxl_nv12_input* input
     return CUJPEGXL_OK;
 }
 
-int cjx_stage_encode_pipelined(cujpegxl_encoder* const* encoders, int n_encoders, const cujpegxl_nv12_input* inputs,
-                               int n_inputs, cujpegxl_output* outputs, int count, double* seconds) {
-    if (encoders == nullptr || n_encoders <= 0 || inputs == nullptr || n_inputs <= 0 || outputs == nullptr ||
-        count <= 0 || seconds == nullptr) {
-        return CUJPEGXL_INVALID_ARGUMENT;
-    }
-    const auto t0{std::chrono::steady_clock::now()};
-    for (int i{0}; i < count + n_encoders - 1; ++i) {
-        // Frame i - (n - 1) is finished after frame i is queued, so that up
-        // to n frames are in flight on the encoders' own streams.
-        if (i < count) {
-            const int s{cujpegxl_encoder_encode_async(encoders[i % n_encoders], &inputs[i % n_inputs], 0)};
-            if (s != CUJPEGXL_OK) {
-                return s;
-            }
-        }
-        const int j{i - (n_encoders - 1)};
-        if (j >= 0) {
-            const int s{cujpegxl_encoder_finish(encoders[j % n_encoders], &outputs[j % n_encoders])};
-            if (s != CUJPEGXL_OK) {
-                return s;
-            }
-        }
-    }
-    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    return CUJPEGXL_OK;
-}
-
 int cjx_stage_encode_dct8_uniform(cujpegxl_encoder* e, const cujpegxl_nv12_input* in, cujpegxl_output* out) {
     if (e == nullptr || in == nullptr || out == nullptr || out->data == nullptr || in->luma == 0 ||
         in->chroma == 0 || in->luma_pitch < e->config.width || in->chroma_pitch < e->config.width) {

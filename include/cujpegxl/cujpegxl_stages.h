@@ -153,14 +153,6 @@ int cjx_stage_encode_repeat(cujpegxl_encoder* encoder, const cujpegxl_nv12_input
 int cjx_stage_encode_cycle(cujpegxl_encoder* encoder, const cujpegxl_nv12_input* inputs, int n_inputs,
                            cujpegxl_output* output, int count, double* seconds);
 
-/* `count` frames through the asynchronous entry points of `n_encoders`
- * encoders in turn (frame i on encoder i mod n, cycling through `inputs`),
- * each frame finished into outputs[i mod n] once the next n - 1 frames are
- * queued: up to n frames in flight on the encoders' own streams (no
- * counterpart in the reference, whose frames run one at a time).  Wall
- * seconds of the whole into *seconds. */
-int cjx_stage_encode_pipelined(cujpegxl_encoder* const* encoders, int n_encoders, const cujpegxl_nv12_input* inputs,
-                               int n_inputs, cujpegxl_output* outputs, int count, double* seconds);
 
 /* Prefix codes of `count` (<= 96) histograms of 256 uint32 bins, built on
  * device `device_ordinal` by the encoder's code kernel (prefix_code.cc:36-397
