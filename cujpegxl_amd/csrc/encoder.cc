@@ -857,8 +857,12 @@ bool Encoder::submit_nv12(const std::uint8_t* luma, std::size_t luma_pitch, cons
     // The frame starts after the work already queued on `caller` (the null
     // stream for the synchronous entry points: a caller that filled the
     // planes there, as callers of the reference did, is ordered before the
-    // region kernel reads them).
-    if (hipEventRecord(ev_caller_, caller) != hipSuccess || hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess) {
+    // region kernel reads them).  A null stream with nothing left to run
+    // needs no event: that saves a marker and a barrier packet ahead of the
+    // region kernel on every frame of a synchronous loop.
+    const bool settled{caller == nullptr && hipStreamQuery(nullptr) == hipSuccess};
+    if (!settled &&
+        (hipEventRecord(ev_caller_, caller) != hipSuccess || hipStreamWaitEvent(stream_, ev_caller_, 0) != hipSuccess)) {
         return false;
     }
     // With profiling on, the region kernel's own start and end times open the
