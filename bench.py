@@ -315,11 +315,14 @@ def main() -> int:
     # second device copies) in flight on their own streams.
     for name, mode in () if args.no_forms else BATCH_FORMS:
         pair = [cj.Batch([(WIDTH, HEIGHT)] * SEEDS, [DISTANCE] * SEEDS, mode=mode) for _ in range(2)]
-        tuned = []
+        tuned, report = [], []
         for j, batch in enumerate(pair):
             batch.set_inputs(inputs[j * SEEDS:(j + 1) * SEEDS])
             batch.run(2)
-            tuned.append(batch.tune(2).strip().splitlines()[-1])
+            lines = batch.tune(2).strip().splitlines()
+            tuned.append(lines[-1])
+            if j == 0:
+                report = lines[:-1]
             batch.run(1)
         reps = max(1, args.steps // SEEDS)
         sec = pair[0].run(reps)
@@ -339,6 +342,7 @@ def main() -> int:
         forms[name] = {"fps": round(SEEDS * reps / sec, 3),
                        "bpp": round(8.0 * float(np.mean(b_sizes)) / (WIDTH * HEIGHT), 4),
                        "frames_per_batch": SEEDS, "batches_timed": reps, "tuned": tuned[0],
+                       "tune_report": report,
                        "two_in_flight_fps": round(2 * SEEDS * reps2 / sec2, 3),
                        "two_in_flight_sizes_equal": b2_sizes == b_sizes}
         if mode == 0:

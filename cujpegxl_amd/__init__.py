@@ -128,6 +128,7 @@ def stages_lib() -> C.CDLL:
         L.cjx_batch_wait.argtypes = [P]
         L.cjx_batch_result.argtypes = [P, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
         L.cjx_batch_tune.argtypes = [P, C.c_int32, C.c_char_p, C.c_size_t]
+        L.cjx_batch_set_shape.argtypes = [P, C.c_uint32, C.c_uint32, C.c_uint32]
         L.cjx_batch_destroy.argtypes = [P]
         L.cjx_batch_destroy.restype = None
         L.cjx_stage_encode_repeat.argtypes = [P, C.POINTER(Nv12Input), C.POINTER(Output), C.c_int,
@@ -564,6 +565,13 @@ class Batch:
         if self.lib.cjx_batch_tune(self.handle, reps, buf, len(buf)) != OK:
             raise RuntimeError("batch tune failed")
         return buf.value.decode()
+
+    def set_shape(self, token_cap: int, assemble_words: int, region_wgs: int = 0) -> None:
+        """Launch shape: tokenizer workgroups per frame, output words per
+        assembly thread, and 0 (one workgroup per region) or the size of a
+        persistent region grid."""
+        if self.lib.cjx_batch_set_shape(self.handle, token_cap, assemble_words, region_wgs) != OK:
+            raise ValueError("bad batch shape")
 
     def close(self) -> None:
         if self.handle:

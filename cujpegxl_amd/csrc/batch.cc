@@ -41,6 +41,10 @@ void BatchEncoder::release() {
         (void)hipFree(jobs_dev_);
         jobs_dev_ = nullptr;
     }
+    if (counters_ != nullptr) {
+        (void)hipFree(counters_);
+        counters_ = nullptr;
+    }
     if (out_host_ != nullptr) {
         (void)hipHostFree(out_host_);
         out_host_ = nullptr;
@@ -96,6 +100,7 @@ bool BatchEncoder::configure(int n, const std::uint32_t* w, const std::uint32_t*
         hipEventCreateWithFlags(&join_, hipEventDisableTiming) != hipSuccess ||
         hipMalloc(&arena_, arena) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&jobs_dev_), sizeof(FrameJob) * n) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&counters_), 8 * sizeof(std::uint32_t)) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&out_host_), outs, hipHostMallocMapped) != hipSuccess ||
         hipHostMalloc(reinterpret_cast<void**>(&sizes_host_), sizeof(std::uint64_t) * n, hipHostMallocMapped) !=
             hipSuccess) {
@@ -157,7 +162,8 @@ bool BatchEncoder::apply_shape(const BatchShape& shape) {
     if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
         return false;
     }
-    launch_batch(stream_, dc_stream_, fork_, join_, jobs_dev_, static_cast<int>(jobs_.size()), total_, mode_);
+    launch_batch(stream_, dc_stream_, fork_, join_, jobs_dev_, static_cast<int>(jobs_.size()), total_, mode_, shape_,
+                 counters_);
     if (hipStreamEndCapture(stream_, &graph_) != hipSuccess ||
         hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0) != hipSuccess) {
         return false;
@@ -221,37 +227,67 @@ const std::uint8_t* BatchEncoder::file(int i) const {
     return i >= 0 && i < static_cast<int>(jobs_.size()) ? out_host_ + out_off_[i] : nullptr;
 }
 
+bool BatchEncoder::set_shape(const BatchShape& shape) {
+    if (shape.token_cap == 0 || shape.assemble_words == 0 || shape.region_wgs % 8 != 0 ||
+        shape.region_wgs > 65536 || hipSetDevice(device_) != hipSuccess) {
+        return false;
+    }
+    return apply_shape(shape);
+}
+
+// Coordinate search: the tokenizer and assembly shapes with one workgroup per
+// region first, then the region stage's persistent grids with the best of
+// those (one and two resident rounds of five workgroups per CU).
 bool BatchEncoder::tune(int reps, std::string* report) {
     if (!inputs_set_ || reps <= 0) {
         return false;
     }
     static constexpr std::uint32_t kTokenCaps[]{256, 512, 1024, 2048};
     static constexpr std::uint32_t kAssembleWords[]{4, 16, 64};
+    int cus{0};
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess || cus <= 0) {
+        cus = 256;
+    }
+    const std::uint32_t resident{static_cast<std::uint32_t>(5 * cus) & ~7u};
+    const std::uint32_t kRegionGrids[]{resident, 2 * resident};
     BatchShape best{shape_};
     double best_s{1e30};
-    char line[160];
+    char line[192];
+    const auto trial = [&](const BatchShape& c) {
+        if (!apply_shape(c)) {
+            return false;
+        }
+        double s{0.0};
+        if (!run(2, nullptr) || !run(reps, &s)) {
+            return false;
+        }
+        s /= reps;
+        std::snprintf(line, sizeof(line), "token_cap=%u assemble_words=%u region_wgs=%u batch_ms=%.4f\n",
+                      c.token_cap, c.assemble_words, c.region_wgs, 1e3 * s);
+        if (report != nullptr) {
+            *report += line;
+        }
+        if (s < best_s) {
+            best_s = s;
+            best = c;
+        }
+        return true;
+    };
     for (std::uint32_t tc : kTokenCaps) {
         for (std::uint32_t aw : kAssembleWords) {
-            if (!apply_shape(BatchShape{tc, aw})) {
+            if (!trial(BatchShape{tc, aw, 0})) {
                 return false;
-            }
-            double s{0.0};
-            if (!run(2, nullptr) || !run(reps, &s)) {
-                return false;
-            }
-            s /= reps;
-            std::snprintf(line, sizeof(line), "token_cap=%u assemble_words=%u batch_ms=%.4f\n", tc, aw, 1e3 * s);
-            if (report != nullptr) {
-                *report += line;
-            }
-            if (s < best_s) {
-                best_s = s;
-                best = BatchShape{tc, aw};
             }
         }
     }
-    std::snprintf(line, sizeof(line), "chosen token_cap=%u assemble_words=%u batch_ms=%.4f\n", best.token_cap,
-                  best.assemble_words, 1e3 * best_s);
+    const BatchShape base{best};
+    for (std::uint32_t rg : kRegionGrids) {
+        if (!trial(BatchShape{base.token_cap, base.assemble_words, rg})) {
+            return false;
+        }
+    }
+    std::snprintf(line, sizeof(line), "chosen token_cap=%u assemble_words=%u region_wgs=%u batch_ms=%.4f\n",
+                  best.token_cap, best.assemble_words, best.region_wgs, 1e3 * best_s);
     if (report != nullptr) {
         *report += line;
     }

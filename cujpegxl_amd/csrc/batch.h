@@ -86,6 +86,9 @@ struct FrameJob {
 struct BatchShape {
     std::uint32_t token_cap;         // tokenizer workgroups per frame, at most
     std::uint32_t assemble_words;    // output words per assembly thread (sets its grid)
+    // Region stage: 0 for one workgroup per region, else a persistent grid of
+    // this many workgroups (a multiple of 8) that claim the regions in turn.
+    std::uint32_t region_wgs;
 };
 
 // Stage grids of one frame for a shape (nx, ny only; `first` is the caller's).
@@ -94,8 +97,11 @@ void batch_stage_grids(const Geometry& geo, const BatchShape& shape, StageGrid (
 // Queues the 14 batched launches of a batch (all frames in `mode`): the
 // region kernel, the AC chain, the layout and the assembly on `s`, the DC
 // chain on `s_dc`, forked after the region kernel (event `fork`) and joined
-// before the layout (event `join`).
+// before the layout (event `join`).  A persistent region stage
+// (shape.region_wgs != 0) claims its regions through the eight device
+// counters at `counters`, which a one-workgroup launch clears first.
 void launch_batch(hipStream_t s, hipStream_t s_dc, hipEvent_t fork, hipEvent_t join, const FrameJob* jobs,
-                  int count, const std::uint32_t (&total)[kBStages], std::uint32_t mode);
+                  int count, const std::uint32_t (&total)[kBStages], std::uint32_t mode, const BatchShape& shape,
+                  std::uint32_t* counters);
 
 }  // namespace cjx

@@ -6,9 +6,10 @@
 
 namespace cjx {
 
-__device__ __forceinline__ const FrameJob& batch_job(const FrameJob* __restrict__ jobs, int count, int stage,
-                                                     Wg& wg) {
-    const std::uint32_t b{blockIdx.x};
+// The frame of batched workgroup `b` of a stage and the workgroup's
+// coordinates in that frame's own grid.
+__device__ __forceinline__ const FrameJob& batch_job_at(const FrameJob* __restrict__ jobs, int count, int stage,
+                                                        std::uint32_t b, Wg& wg) {
     int lo{0}, hi{count - 1};
     while (lo < hi) {
         const int mid{(lo + hi + 1) >> 1};
@@ -25,6 +26,11 @@ __device__ __forceinline__ const FrameJob& batch_job(const FrameJob* __restrict_
     return j;
 }
 
+__device__ __forceinline__ const FrameJob& batch_job(const FrameJob* __restrict__ jobs, int count, int stage,
+                                                     Wg& wg) {
+    return batch_job_at(jobs, count, stage, blockIdx.x, wg);
+}
+
 // The DC passes' chunk count per group and staging slot size (the values
 // the per-frame launchers pass).
 constexpr std::uint32_t kBatchDcChunks{(2 + 6 * 65536 + 2 * 1024 + kDcChunk - 1) / kDcChunk};
@@ -37,6 +43,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void r
     const FrameJob& j{batch_job(jobs, count, kBRegion, wg)};
     region_kernel_body<kMf16, kLearned>(wg, j.geo, j.prm, j.luma, j.luma_pitch, j.chroma, j.chroma_pitch, j.coeffs,
                                         j.acs, j.qf, j.cfl_part, j.item_count, j.clr, nullptr);
+}
+
+// The region stage as a persistent grid: workgroup i stays on XCD i mod 8
+// and claims, through that XCD's counter, the batched region workgroups
+// b = 8 t + (i mod 8) in turn (t = 0, 1, ...), which are the ones the
+// one-workgroup-per-region launch would place on the same XCD: the same
+// contiguous runs of regions per XCD.  Every workgroup leaves once its
+// XCD's share is claimed; no workgroup waits for another.
+template <bool kMf16, bool kLearned>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void region_persistent_batch_kernel(
+    const FrameJob* __restrict__ jobs, int count, std::uint32_t total, std::uint32_t* __restrict__ counters) {
+    __shared__ std::uint32_t claimed;
+    const std::uint32_t x{blockIdx.x & 7u};
+    for (;;) {
+        __syncthreads();  // every thread is done with the previous region and with `claimed`
+        if (threadIdx.x == 0) {
+            claimed = atomicAdd(&counters[x], 1u);
+        }
+        __syncthreads();
+        const std::uint32_t b{8u * claimed + x};
+        if (b >= total) {
+            break;
+        }
+        Wg wg;
+        const FrameJob& j{batch_job_at(jobs, count, kBRegion, b, wg)};
+        region_kernel_body<kMf16, kLearned>(wg, j.geo, j.prm, j.luma, j.luma_pitch, j.chroma, j.chroma_pitch,
+                                            j.coeffs, j.acs, j.qf, j.cfl_part, j.item_count, j.clr, nullptr);
+    }
+}
+
+__global__ void clear_counters_kernel(std::uint32_t* __restrict__ counters) {
+    if (threadIdx.x < 8) {
+        counters[threadIdx.x] = 0;
+    }
 }
 
 __global__ __launch_bounds__(256) void quantize_batch_kernel(const FrameJob* __restrict__ jobs, int count) {
@@ -154,11 +194,21 @@ void batch_stage_grids(const Geometry& geo, const BatchShape& shape, StageGrid (
 }
 
 void launch_batch(hipStream_t s, hipStream_t s_dc, hipEvent_t fork, hipEvent_t join, const FrameJob* jobs,
-                  int count, const std::uint32_t (&total)[kBStages], std::uint32_t mode) {
+                  int count, const std::uint32_t (&total)[kBStages], std::uint32_t mode, const BatchShape& shape,
+                  std::uint32_t* counters) {
     const bool mf{(mode & kModeMfma16) != 0}, learned{(mode & kModeLearned) != 0};
-    auto* const region{mf ? (learned ? region_batch_kernel<true, true> : region_batch_kernel<true, false>)
-                          : (learned ? region_batch_kernel<false, true> : region_batch_kernel<false, false>)};
-    region<<<total[kBRegion], 256, 0, s>>>(jobs, count);
+    if (shape.region_wgs == 0) {
+        auto* const region{mf ? (learned ? region_batch_kernel<true, true> : region_batch_kernel<true, false>)
+                              : (learned ? region_batch_kernel<false, true> : region_batch_kernel<false, false>)};
+        region<<<total[kBRegion], 256, 0, s>>>(jobs, count);
+    } else {
+        auto* const region{mf ? (learned ? region_persistent_batch_kernel<true, true>
+                                         : region_persistent_batch_kernel<true, false>)
+                              : (learned ? region_persistent_batch_kernel<false, true>
+                                         : region_persistent_batch_kernel<false, false>)};
+        clear_counters_kernel<<<1, 64, 0, s>>>(counters);
+        region<<<shape.region_wgs, 256, 0, s>>>(jobs, count, total[kBRegion], counters);
+    }
     // The DC chain (DC values and colour maps, DC / AcMetadata histograms and
     // codes, header blobs, sizes, emission) reads only what the region kernel
     // wrote: it forks here and joins before the layout, beside the AC chain.

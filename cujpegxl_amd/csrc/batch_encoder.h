@@ -45,6 +45,9 @@ public:
     // Times every candidate launch shape over `reps` replays and keeps the
     // fastest; one line per candidate into `report`.
     bool tune(int reps, std::string* report);
+    // A launch shape of the caller's (region_wgs: 0, or a multiple of 8 up to
+    // 65,536); re-captures the batch's graph.
+    bool set_shape(const BatchShape& shape);
     const BatchShape& shape() const { return shape_; }
     int frames() const { return static_cast<int>(jobs_.size()); }
 
@@ -54,13 +57,14 @@ private:
 
     int device_{-1};
     std::uint32_t mode_{0};
-    BatchShape shape_{1024, 16};
+    BatchShape shape_{1024, 16, 0};
     std::vector<std::unique_ptr<Encoder>> frames_;
     std::vector<FrameJob> jobs_;
     std::vector<std::size_t> out_off_, caps_;
     std::uint32_t total_[kBStages]{};
     void* arena_{nullptr};
     FrameJob* jobs_dev_{nullptr};
+    std::uint32_t* counters_{nullptr};  // the persistent region stage's claim counters (8)
     std::uint8_t* out_host_{nullptr};
     std::uint64_t* sizes_host_{nullptr};
     hipStream_t stream_{nullptr};

@@ -280,6 +280,14 @@ int cjx_batch_tune(cjx_batch* b, int32_t reps, char* report, size_t cap) {
     return ok ? CUJPEGXL_OK : CUJPEGXL_INTERNAL_ERROR;
 }
 
+int cjx_batch_set_shape(cjx_batch* b, uint32_t token_cap, uint32_t assemble_words, uint32_t region_wgs) {
+    if (b == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return b->engine.set_shape(cjx::BatchShape{token_cap, assemble_words, region_wgs}) ? CUJPEGXL_OK
+                                                                                      : CUJPEGXL_INVALID_ARGUMENT;
+}
+
 void cjx_batch_destroy(cjx_batch* b) {
     delete b;
 }

@@ -114,10 +114,17 @@ int cjx_batch_run(cjx_batch* batch, int32_t count, double* seconds);
 int cjx_batch_launch(cjx_batch* batch, int32_t count);
 int cjx_batch_wait(cjx_batch* batch);
 int cjx_batch_result(const cjx_batch* batch, uint32_t index, const uint8_t** data, uint64_t* size);
-/* Times every candidate launch shape (tokenizer workgroups per frame x
- * output words per assembly thread) over `reps` replays and keeps the
- * fastest; one line per candidate, then the choice, into `report`. */
+/* Times candidate launch shapes over `reps` replays and keeps the fastest:
+ * tokenizer workgroups per frame x output words per assembly thread with one
+ * workgroup per region, then the region stage as a persistent grid (one and
+ * two resident rounds) with the best of those; one line per candidate, then
+ * the choice, into `report`. */
 int cjx_batch_tune(cjx_batch* batch, int32_t reps, char* report, size_t cap);
+/* Sets a launch shape directly: tokenizer workgroups per frame (>= 1),
+ * output words per assembly thread (>= 1), and region_wgs = 0 for one
+ * workgroup per region or the size of a persistent region grid (a multiple
+ * of 8, at most 65,536). */
+int cjx_batch_set_shape(cjx_batch* batch, uint32_t token_cap, uint32_t assemble_words, uint32_t region_wgs);
 void cjx_batch_destroy(cjx_batch* batch);
 
 /* Training on the device (all pointers device memory, `stream` a hipStream_t

@@ -104,3 +104,27 @@ def test_two_batches_in_flight(jo):
         b.close()
     for d in devs_all:
         d.close()
+
+
+@pytest.mark.parametrize("mode", [0, cj.MODE_MFMA16 | cj.MODE_LEARNED])
+def test_persistent_region_grid(jo, mode):
+    """The region stage as a persistent grid (workgroups claiming regions
+    through per-XCD counters) gives the files of one workgroup per region,
+    for grids smaller and larger than the batch's region count."""
+    say(f"persistent region grid, mode {mode}")
+    host, devs, inputs = frames_for(140)
+    b = cj.Batch([(w, h) for w, h, _ in FRAMES], [d for _, _, d in FRAMES], mode=mode)
+    b.set_inputs(inputs)
+    b.run(1)
+    want = b.files()
+    if mode == 0:
+        assert want == [jo.encode_nv12(l, c, w, h, d) for (l, c), (w, h, d) in zip(host, FRAMES)]
+    for grid in (8, 1280, 4096):
+        b.set_shape(512, 16, grid)
+        b.run(2)
+        assert b.files() == want, grid
+    with pytest.raises(ValueError):
+        b.set_shape(512, 16, 12)
+    b.close()
+    for d in devs:
+        d.close()
