@@ -91,37 +91,29 @@ __device__ __forceinline__ float srgb_to_linear(float e, const float2* tab) {
     return e <= 0.04045f ? e * (1.0f / 12.92f) : pow24_tab((e + 0.055f) * (1.0f / 1.055f), tab);
 }
 
+// Opsin absorbance bias of the XYB transform.
+constexpr float kOpsinBias{0.0037930732552754493f};
+
+// `neg_cbrt_bias` is -cbrt_tab(kOpsinBias, tab), evaluated once by the caller.
 __device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float& ox, float& oy, float& ob,
-                                             const float2* tab) {
+                                             const float2* tab, float neg_cbrt_bias) {
     constexpr float m00{0.30f}, m02{0.078f}, m01{1.0f - 0.078f - 0.30f};
     constexpr float m10{0.23f}, m12{0.078f}, m11{1.0f - 0.078f - 0.23f};
     constexpr float m20{0.24342268924547819f}, m21{0.20476744424496821f};
     constexpr float m22{1.0f - 0.24342268924547819f - 0.20476744424496821f};
-    constexpr float bias{0.0037930732552754493f};
+    constexpr float bias{kOpsinBias};
     const float r{clamp01(srgb_to_linear(clamp01(yv + 1.5748f * cr), tab))};
     const float g{clamp01(srgb_to_linear(clamp01(yv - 0.1873242729f * cb - 0.4681242729f * cr), tab))};
     const float b{clamp01(srgb_to_linear(clamp01(yv + 1.8556f * cb), tab))};
     float q0{fmaxf(m00 * r + m01 * g + m02 * b + bias, 0.0f)};
     float q1{fmaxf(m10 * r + m11 * g + m12 * b + bias, 0.0f)};
     float q2{fmaxf(m20 * r + m21 * g + m22 * b + bias, 0.0f)};
-    const float neg_cbrt_bias{-cbrt_tab(bias, tab)};
     q0 = cbrt_tab(q0, tab) + neg_cbrt_bias;
     q1 = cbrt_tab(q1, tab) + neg_cbrt_bias;
     q2 = cbrt_tab(q2, tab) + neg_cbrt_bias;
     ox = 0.5f * (q0 - q1);
     oy = 0.5f * (q0 + q1);
     ob = q2;
-}
-
-// Centered 4:2:0 bilinear, clamped taps (tests/reference.py:23-46): weights
-// 1/4 and 3/4; the result is exact in float.
-__device__ __forceinline__ void chroma_taps(int x, int cw, int& i0, int& i1, float& w0, float& w1) {
-    const int k{x >> 1};
-    const bool odd{(x & 1) != 0};
-    i0 = odd ? k : (k > 0 ? k - 1 : 0);
-    i1 = odd ? (k + 1 < cw ? k + 1 : cw - 1) : k;
-    w0 = odd ? 0.75f : 0.25f;
-    w1 = odd ? 0.25f : 0.75f;
 }
 
 // adaptive_quant_impl.cuh:29-61
@@ -319,9 +311,10 @@ __device__ __forceinline__ int cpad(int e) {
 
 // Arrays whose lifetimes do not overlap share storage, which keeps the
 // workgroup under 32 KB of LDS (five resident workgroups per CU):
-// - steps 1-3: the XYB tile, then the chroma window (step 1) or the AQ
-//   terms (step 3) past its end; steps 4-5: the row-pass lines;
-// - the sharpened planes, from step 2 until their last row pass (Y in
+// - steps 1-3: the XYB tile, then the AQ terms (step 3) past its end;
+//   steps 4-5: the row-pass lines;
+// - the powers' table and the chroma window in step 1, then the
+//   sharpened planes, from step 2 until their last row pass (Y in
 //   step 4, X and B in step 5); each plane's space then holds coefficients:
 //   X's and B's the chosen transforms (step 5 on), Y's (the last plane, and
 //   256 bytes past it) the DCT16 and DCT32 candidates (step 4 on).  The DCT8
@@ -333,14 +326,11 @@ struct RegionShared {
     union {
         struct {
             float xyb[3][kTileW * kTileW];  // NV12 -> XYB tile with halo (steps 1-3)
-            union {
-                std::uint16_t chroma[kChromaW * kChromaW];  // the tile's CbCr window (step 1)
-                struct {
-                    float cells[64];
-                    float ero[64];     // fuzzy erosion of each cell
-                    float blk[16][3];  // per block: HF sum, gamma, blue sum
-                } aq;                  // step 3
-            } t;
+            struct {
+                float cells[64];
+                float ero[64];     // fuzzy erosion of each cell
+                float blk[16][3];  // per block: HF sum, gamma, blue sum
+            } aq;                  // AQ terms (step 3)
         } f;
         float scratch[kScratch];  // row-pass lines (steps 4-5)
     } u;
@@ -435,13 +425,27 @@ __device__ __forceinline__ void tile_partials(const Geometry& geo, const float* 
 }
 
 // Step 1 of a region: NV12 -> XYB over the 36x36 tile (2-pixel halo,
-// clamped coordinates), exact 4:2:0 bilinear chroma.
+// clamped coordinates), exact 4:2:0 bilinear chroma (centred taps clamped
+// to the plane, tests/reference.py:23-46).
+//
+// The bilinear's weights are 3/4 on the sample of the pixel's own chroma
+// cell (coordinate x >> 1) and 1/4 on its neighbour towards the pixel
+// (clamped to the plane), in each direction.  Every partial sum of the
+// float form (products of samples and weights, then their sums) is exact,
+// so the interpolated value is exactly (9 s_kk + 3 s_kn + 3 s_nk + s_nn) / 16:
+// the tile computes that integer sum, for Cb and Cr at once in the two
+// halves of one word (at most 16 * 255 each), and scales it by 1/16, which
+// gives the float form's bits.
 __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* luma, std::size_t luma_pitch,
                                          const std::uint8_t* chroma, std::size_t chroma_pitch, int px0, int py0,
                                          int W, int H, int tid) {
-    // The tile's chroma window first: samples (cx0 + j, cy0 + i), clamped to
-    // the plane, Cb in the low byte.  Every tap of the tile's pixels (their
-    // coordinates clamped to the frame, the taps to the plane) lands in it.
+    // The powers' table, then the tile's chroma window: samples
+    // (cx0 + j, cy0 + i), clamped to the plane, Cb in the low half-word and
+    // Cr in the high one.  Every tap of the tile's pixels (their coordinates
+    // clamped to the frame, the taps to the plane) lands in the window.  Both
+    // live in the sharpened planes' space, which is free until step 2.
+    float2* const tab{reinterpret_cast<float2*>(&S.p.sharp[0][0])};
+    std::uint32_t* const win{reinterpret_cast<std::uint32_t*>(&S.p.sharp[0][0]) + 2 * kXybTabPairs};
     const int cw{W / 2}, ch{H / 2};
     const int cx0{(px0 >> 1) - 2}, cy0{(py0 >> 1) - 2};
     for (int i{tid}; i < kChromaW * kChromaW; i += 256) {
@@ -450,37 +454,53 @@ __device__ __forceinline__ void xyb_tile(RegionShared& S, const std::uint8_t* lu
         y = y < 0 ? 0 : (y >= ch ? ch - 1 : y);
         const std::uint8_t* p{chroma + static_cast<std::size_t>(static_cast<std::uint32_t>(y)) * chroma_pitch +
                               2u * static_cast<std::uint32_t>(x)};
-        S.u.f.t.chroma[i] = static_cast<std::uint16_t>(p[0] | (p[1] << 8));
+        win[i] = static_cast<std::uint32_t>(p[0]) | (static_cast<std::uint32_t>(p[1]) << 16);
     }
-    // The powers' table, in the sharpened planes' space (free until step 2).
-    float2* const tab{reinterpret_cast<float2*>(&S.p.sharp[0][0])};
     if (tid < kXybTabPairs) {
         tab[tid] = g_tables.xyb_tab[tid];
     }
     __syncthreads();
-    for (int i{tid}; i < kTileW * kTileW; i += 256) {
-        const int lx{i % kTileW}, ly{i / kTileW};
-        int gx{px0 + lx - kHalo}, gy{py0 + ly - kHalo};
+    // Thread t converts column t % 36 of the tile, rows t / 36 + 7 k: the
+    // column's luma address and horizontal chroma taps are computed once.
+    if (tid < 7 * kTileW) {
+        const int lx{tid % kTileW};
+        int gx{px0 + lx - kHalo};
         gx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
-        gy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
-        int x0, x1, y0, y1;
-        float wx0, wx1, wy0, wy1;
-        chroma_taps(gx, cw, x0, x1, wx0, wx1);
-        chroma_taps(gy, ch, y0, y1, wy0, wy1);
-        const int r0{(y0 - cy0) * kChromaW}, r1{(y1 - cy0) * kChromaW};
-        const std::uint32_t s00{S.u.f.t.chroma[r0 + x0 - cx0]}, s01{S.u.f.t.chroma[r0 + x1 - cx0]};
-        const std::uint32_t s10{S.u.f.t.chroma[r1 + x0 - cx0]}, s11{S.u.f.t.chroma[r1 + x1 - cx0]};
-        const float b0{static_cast<float>(s00 & 255u) * wx0 + static_cast<float>(s01 & 255u) * wx1};
-        const float b1{static_cast<float>(s10 & 255u) * wx0 + static_cast<float>(s11 & 255u) * wx1};
-        const float c0{static_cast<float>(s00 >> 8) * wx0 + static_cast<float>(s01 >> 8) * wx1};
-        const float c1{static_cast<float>(s10 >> 8) * wx0 + static_cast<float>(s11 >> 8) * wx1};
-        const float cbu{b0 * wy0 + b1 * wy1};
-        const float cru{c0 * wy0 + c1 * wy1};
-        const std::uint8_t* lrow{luma + static_cast<std::size_t>(static_cast<std::uint32_t>(gy)) * luma_pitch};
-        const float yv{static_cast<float>(lrow[static_cast<std::uint32_t>(gx)]) * (1.0f / 255.0f)};
-        const float cb{cbu * (1.0f / 255.0f) - 128.0f / 255.0f};
-        const float cr{cru * (1.0f / 255.0f) - 128.0f / 255.0f};
-        pixel_to_xyb(yv, cb, cr, S.u.f.xyb[0][i], S.u.f.xyb[1][i], S.u.f.xyb[2][i], tab);
+        const int kx{gx >> 1};
+        const int nx{(gx & 1) != 0 ? (kx + 1 < cw ? kx + 1 : cw - 1) : (kx > 0 ? kx - 1 : 0)};
+        const int ox{kx - cx0}, onx{nx - cx0};
+        const std::uint8_t* const lcol{luma + static_cast<std::uint32_t>(gx)};
+        const float neg_cbrt_bias{-cbrt_tab(kOpsinBias, tab)};
+        // (1/255) / 16, exact: the 1/16 of the sum folded into the scale.
+        constexpr float kScale{(1.0f / 255.0f) * 0.0625f};
+        const auto row_of = [&](int ly) {
+            const int gy{py0 + ly - kHalo};
+            return gy < 0 ? 0 : (gy >= H ? H - 1 : gy);
+        };
+        const auto luma_at = [&](int gy) {
+            return lcol[static_cast<std::size_t>(static_cast<std::uint32_t>(gy)) * luma_pitch];
+        };
+        // The next row's luma is loaded one iteration ahead (the row past
+        // the tile's last is clamped to the frame, so the load is in bounds).
+        int gy{row_of(tid / kTileW)};
+        std::uint8_t lnext{luma_at(gy)};
+        for (int ly{tid / kTileW}; ly < kTileW; ly += 7) {
+            const int i{ly * kTileW + lx};
+            const std::uint8_t lcur{lnext};
+            const int gy_next{row_of(ly + 7)};
+            lnext = luma_at(gy_next);
+            const int ky{gy >> 1};
+            const int ny{(gy & 1) != 0 ? (ky + 1 < ch ? ky + 1 : ch - 1) : (ky > 0 ? ky - 1 : 0)};
+            const int rk{(ky - cy0) * kChromaW}, rn{(ny - cy0) * kChromaW};
+            const std::uint32_t hk{__umul24(win[rk + ox], 3u) + win[rk + onx]};
+            const std::uint32_t hn{__umul24(win[rn + ox], 3u) + win[rn + onx]};
+            const std::uint32_t v{(hk << 1) + hk + hn};
+            const float yv{static_cast<float>(lcur) * (1.0f / 255.0f)};
+            const float cb{static_cast<float>(v & 0xffffu) * kScale - 128.0f / 255.0f};
+            const float cr{static_cast<float>(v >> 16) * kScale - 128.0f / 255.0f};
+            pixel_to_xyb(yv, cb, cr, S.u.f.xyb[0][i], S.u.f.xyb[1][i], S.u.f.xyb[2][i], tab, neg_cbrt_bias);
+            gy = gy_next;
+        }
     }
 }
 
@@ -917,7 +937,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             }
         }
         if (dy == 3) {
-            S.u.f.t.aq.cells[cell] = 0.25f * sum;
+            S.u.f.aq.cells[cell] = 0.25f * sum;
         }
     }
     __syncthreads();
@@ -928,7 +948,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     //     (a block's four erosion terms sum as (e0 + e2) + (e1 + e3), which is
     //     the wave tree over four non-zero lanes).
     if (tid < 64) {
-        S.u.f.t.aq.ero[tid] = erosion_cell(S.u.f.t.aq.cells, tid & 7, tid >> 3, prm);
+        S.u.f.aq.ero[tid] = erosion_cell(S.u.f.aq.cells, tid & 7, tid >> 3, prm);
     }
 #pragma unroll
     for (int j{0}; j < 4; ++j) {
@@ -952,9 +972,9 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         const float gamma_overall{wave_sum(gm) * (0.5f / 64.0f)};
         const float blue_sum{wave_sum(bl)};
         if (lane == 0) {
-            S.u.f.t.aq.blk[lb][0] = hf_sum;
-            S.u.f.t.aq.blk[lb][1] = gamma_overall;
-            S.u.f.t.aq.blk[lb][2] = blue_sum;
+            S.u.f.aq.blk[lb][0] = hf_sum;
+            S.u.f.aq.blk[lb][1] = gamma_overall;
+            S.u.f.aq.blk[lb][2] = blue_sum;
         }
     }
     __syncthreads();
@@ -962,10 +982,10 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         const int lb{tid};
         const int lbx{lb & 3}, lby{lb >> 2};
         if (bx0 + lbx < bw && by0 + lby < bh) {
-            const float e0{S.u.f.t.aq.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.u.f.t.aq.ero[(2 * lby) * 8 + 2 * lbx + 1]};
-            const float e2{S.u.f.t.aq.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.u.f.t.aq.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
+            const float e0{S.u.f.aq.ero[(2 * lby) * 8 + 2 * lbx]}, e1{S.u.f.aq.ero[(2 * lby) * 8 + 2 * lbx + 1]};
+            const float e2{S.u.f.aq.ero[(2 * lby + 1) * 8 + 2 * lbx]}, e3{S.u.f.aq.ero[(2 * lby + 1) * 8 + 2 * lbx + 1]};
             const float aq{(e0 + e2) + (e1 + e3)};
-            float blue_sum{S.u.f.t.aq.blk[lb][2]};
+            float blue_sum{S.u.f.aq.blk[lb][2]};
             constexpr float k_limit{0.027121074570634722f};
             if (blue_sum >= 32.0f * k_limit) {
                 blue_sum = 64.0f * k_limit - blue_sum;
@@ -975,8 +995,8 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
                 blue_sum = k_max_limit * k_limit;
             }
             float ov{compute_mask(aq)};
-            ov += S.u.f.t.aq.blk[lb][0] * -0.38f + 0.42f;
-            ov += 0.1005613337192697f * log2_det(S.u.f.t.aq.blk[lb][1]);
+            ov += S.u.f.aq.blk[lb][0] * -0.38f + 0.42f;
+            ov += 0.1005613337192697f * log2_det(S.u.f.aq.blk[lb][1]);
             ov += blue_sum * 0.14207000358439159f;
             const float qval{exp_det(ov) * prm.mul_pb + prm.add_pb};
             int qi{static_cast<int>(qval * prm.inv_global_scale + 0.5f)};

@@ -228,6 +228,40 @@ def test_chroma_upsample_is_the_exact_bilinear(jo):
     np.testing.assert_array_equal(cr, upsample(chroma[:, 1::2]))
 
 
+def test_chroma_integer_form_gives_the_float_bits(jo):
+    """The device's form of the bilinear: 9 s_kk + 3 s_kn + 3 s_nk + s_nn over
+    the pixel's own chroma cell k = x >> 1 and its clamped neighbour n towards
+    the pixel, in each direction, then the 8-bit scale with the 1/16 folded
+    in.  Checked against the oracle's float bilinear on odd and even sizes
+    and every clamped edge, and the scale over every possible sum."""
+    rng = np.random.default_rng(61)
+    for w, h in ((22, 14), (8, 8), (40, 18)):
+        chroma = rng.integers(0, 256, (h // 2, w), dtype=np.uint8)
+        chroma[0, :4] = 255
+        cb = np.zeros((h, w), np.float32)
+        cr = np.zeros((h, w), np.float32)
+        jo.lib().jo_chroma_upsample(jo.ptr(chroma), w, w, h, jo.ptr(cb), jo.ptr(cr))
+
+        def taps(n, size):
+            x = np.arange(n)
+            k = x >> 1
+            other = np.where(x & 1, np.minimum(k + 1, size - 1), np.maximum(k - 1, 0))
+            return k, other
+
+        kx, nx = taps(w, w // 2)
+        ky, ny = taps(h, h // 2)
+        for plane, out in ((chroma[:, 0::2], cb), (chroma[:, 1::2], cr)):
+            s = plane.astype(np.int64)
+            v = (9 * s[ky][:, kx] + 3 * s[ky][:, nx] + 3 * s[ny][:, kx] + s[ny][:, nx])
+            np.testing.assert_array_equal(v.astype(np.float32) * np.float32(0.0625), out)
+    k = np.float32(1.0) / np.float32(255.0)
+    c = np.float32(128.0) / np.float32(255.0)
+    v = np.arange(0, 16 * 255 + 1, dtype=np.float32)
+    folded = v * np.float32(k * np.float32(0.0625)) - c
+    direct = (v * np.float32(0.0625)) * k - c
+    np.testing.assert_array_equal(folded.view(np.uint32), direct.view(np.uint32))
+
+
 def test_gaborish_against_bruteforce(jo):
     rng = np.random.default_rng(5)
     x = rng.uniform(-0.1, 0.9, (3, 40, 56)).astype(np.float32)
