@@ -377,23 +377,55 @@ __device__ __forceinline__ bool is_llf(int e, int n) {
     return (e >> l2) < m && (e & (n - 1)) < m;
 }
 
-// Selection terms of one Y coefficient (transform_select_impl.cuh:49-108):
-// squared quantization error and rate proxy; LLF coefficients contribute 0.
-// The two quotients are the correctly rounded ones, by reciprocals.
-__device__ __forceinline__ void selection_terms(float s, int e, int n, const float* wsel, const float* wsel_inv,
-                                                const float* rate, const FrameParams& prm, float& d, float& rt) {
-    const int m{n / 8};
-    if (e / n < m && e % n < m) {
-        d = 0.0f;
-        rt = 0.0f;
-        return;
+// Selection terms of M Y coefficients of one lane, s[i] at frequency e[i] of
+// an n x n transform (transform_select_impl.cuh:49-108): squared quantization
+// error and rate proxy; LLF coefficients contribute 0.  The two quotients are
+// the correctly rounded ones, by reciprocals.  The weights come in as
+// arguments (read four at a time by the callers, the frequencies of a lane
+// run in fours); the rate entries of all M quotients are read together.
+template <int M>
+__device__ __forceinline__ void selection_terms(const float (&s)[M], const int (&e)[M], int n, const float (&wq)[M],
+                                                const float (&wi)[M], const FrameParams& prm, float (&d)[M],
+                                                float (&rt)[M]) {
+    float aq[M];
+    int k[M];
+#pragma unroll
+    for (int i{0}; i < M; ++i) {
+        const float q{__builtin_rintf(div_by_recip(s[i] * prm.qgsf_select, wq[i], wi[i]))};
+        const float err{div_by_recip(q * wq[i], prm.qgsf_select, prm.inv_qgsf_select) - s[i]};
+        d[i] = is_llf(e[i], n) ? 0.0f : err * err;
+        aq[i] = fabsf(q);
+        k[i] = aq[i] < 256.0f ? static_cast<int>(aq[i]) : 0;
     }
-    const float wq{wsel[e]};
-    const float q{__builtin_rintf(div_by_recip(s * prm.qgsf_select, wq, wsel_inv[e]))};
-    const float err{div_by_recip(q * wq, prm.qgsf_select, prm.inv_qgsf_select) - s};
-    d = err * err;
-    const float aq{fabsf(q)};
-    rt = aq < 256.0f ? rate[static_cast<int>(aq)] : 1.0f + log2_det(1.0f + aq);
+    float r[M];
+#pragma unroll
+    for (int i{0}; i < M; ++i) {
+        r[i] = g_tables.rate[k[i]];
+    }
+#pragma unroll
+    for (int i{0}; i < M; ++i) {
+        if (aq[i] >= 256.0f) {
+            r[i] = 1.0f + log2_det(1.0f + aq[i]);
+        }
+        rt[i] = is_llf(e[i], n) ? 0.0f : r[i];
+    }
+}
+
+// Weights and reciprocals of frequencies e0 .. e0 + 3 (e0 a multiple of 4)
+// into wq[i0 ..], wi[i0 ..].
+template <int M>
+__device__ __forceinline__ void selection_weights4(const float* w, const float* w_inv, int e0, int i0,
+                                                   float (&wq)[M], float (&wi)[M]) {
+    const float4 a{*reinterpret_cast<const float4*>(w + e0)};
+    const float4 b{*reinterpret_cast<const float4*>(w_inv + e0)};
+    wq[i0] = a.x;
+    wq[i0 + 1] = a.y;
+    wq[i0 + 2] = a.z;
+    wq[i0 + 3] = a.w;
+    wi[i0] = b.x;
+    wi[i0 + 1] = b.y;
+    wi[i0 + 2] = b.z;
+    wi[i0 + 3] = b.w;
 }
 
 // Colour maps of a 64x64 tile from its regions' partial sums (cfl.h:58-64 on
@@ -1069,17 +1101,22 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         if (wu == 0) {
             const v16f acc{mfma_lines<32>(g_tables.basis32 + (16 * (grp >> 1) + t16) * 32,
                                           &lines[kLines32 + (16 * (grp & 1) + t16) * RowBuf<32>::RS])};
+            float v[16], wq[16], wi[16];
+            int e[16];
 #pragma unroll
             for (int b{0}; b < 4; ++b) {
+                const int e0{(16 * (b & 1) + t16) * 32 + 16 * (b >> 1) + 4 * grp};
+                selection_weights4(g_tables.wsel32, g_tables.wsel_inv32, e0, 4 * b, wq, wi);
 #pragma unroll
                 for (int u{0}; u < 4; ++u) {
-                    const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
-                    const int e{fx * 32 + fy};
-                    const float v{acc[4 * b + u]};
-                    selection_terms(v, e, 32, g_tables.wsel32, g_tables.wsel_inv32, g_tables.rate, prm, td[4 * b + u],
-                                    tr[4 * b + u]);
-                    coefy_of(S, 2)[cpad(e)] = static_cast<_Float16>(v);
+                    e[4 * b + u] = e0 + u;
+                    v[4 * b + u] = acc[4 * b + u];
                 }
+            }
+            selection_terms(v, e, 32, wq, wi, prm, td, tr);
+#pragma unroll
+            for (int i{0}; i < 16; ++i) {
+                coefy_of(S, 2)[cpad(e[i])] = static_cast<_Float16>(v[i]);
             }
             float ad{td[0]}, ar{tr[0]};
 #pragma unroll
@@ -1094,17 +1131,23 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         } else {
             const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
                                           &lines[kLines16 + grp * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
+            // Frequency (t16, 4 grp + u) in each of the four blocks.
+            const int e0{t16 * 16 + 4 * grp};
+            float w4[4], wi4[4];
+            selection_weights4(g_tables.wsel16, g_tables.wsel_inv16, e0, 0, w4, wi4);
+            float v[16], wq[16], wi[16];
+            int e[16];
 #pragma unroll
-            for (int q{0}; q < 4; ++q) {
+            for (int i{0}; i < 16; ++i) {
+                e[i] = e0 + (i & 3);
+                v[i] = acc[i];
+                wq[i] = w4[i & 3];
+                wi[i] = wi4[i & 3];
+            }
+            selection_terms(v, e, 16, wq, wi, prm, td, tr);
 #pragma unroll
-                for (int u{0}; u < 4; ++u) {
-                    const int fy{4 * grp + u}, fx{t16};
-                    const int e{fx * 16 + fy};
-                    const float v{acc[4 * q + u]};
-                    selection_terms(v, e, 16, g_tables.wsel16, g_tables.wsel_inv16, g_tables.rate, prm, td[4 * q + u],
-                                    tr[4 * q + u]);
-                    coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(v);
-                }
+            for (int i{0}; i < 16; ++i) {
+                coefy_of(S, 1)[cpad(256 * (i >> 2) + e[i])] = static_cast<_Float16>(v[i]);
             }
 #pragma unroll
             for (int q{0}; q < 4; ++q) {
@@ -1128,11 +1171,22 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         const int base{256 * ((by >> 1) * 2 + (bx >> 1)) + 64 * ((by & 1) * 2 + (bx & 1))};
         float o[8];
         dct_line<8, 8>(&lines[kLines8 + b * RowBuf<8>::BS + fx * RowBuf<8>::RS], 0, o);
+        {
+            float wq[8], wi[8], d8[8], r8[8];
+            int e[8];
+            selection_weights4(g_tables.wsel8, g_tables.wsel_inv8, fx * 8, 0, wq, wi);
+            selection_weights4(g_tables.wsel8, g_tables.wsel_inv8, fx * 8 + 4, 4, wq, wi);
 #pragma unroll
-        for (int f{0}; f < 8; ++f) {
-            const int e{fx * 8 + f};
-            selection_terms(o[f], e, 8, g_tables.wsel8, g_tables.wsel_inv8, g_tables.rate, prm, td[f], tr[f]);
-            coefy_of(S, 0)[cpad(base + e)] = static_cast<_Float16>(o[f]);
+            for (int f{0}; f < 8; ++f) {
+                e[f] = fx * 8 + f;
+            }
+            selection_terms(o, e, 8, wq, wi, prm, d8, r8);
+#pragma unroll
+            for (int f{0}; f < 8; ++f) {
+                td[f] = d8[f];
+                tr[f] = r8[f];
+                coefy_of(S, 0)[cpad(base + e[f])] = static_cast<_Float16>(o[f]);
+            }
         }
         float ad{td[0]}, ar{tr[0]};
 #pragma unroll
@@ -1172,19 +1226,24 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         const int q{wu};
         const f4v c8{mf_quad(yp, (q >> 1) * 16, (q & 1) * 16, g_tables.mf8[0], g_tables.mf8[1], lane)};
         float d8{0.0f}, r8{0.0f}, fv{0.0f}, fg{0.0f};
+        float v[4], wq[4], wi[4], d[4], rt[4];
+        int e[4], j[4];
 #pragma unroll
         for (int u{0}; u < 4; ++u) {
-            int j, e;
-            float d, rt;
-            mf_quad_element(lane, u, 8, j, e);
-            selection_terms(c8[u], e, 8, g_tables.wsel8, g_tables.wsel_inv8, g_tables.rate, prm, d, rt);
-            coefy_of(S, 0)[cpad(256 * q + 64 * j + e)] = static_cast<_Float16>(c8[u]);
-            d8 += d;
-            r8 += rt;
+            mf_quad_element(lane, u, 8, j[u], e[u]);
+            v[u] = c8[u];
+        }
+        selection_weights4(g_tables.wsel8, g_tables.wsel_inv8, e[0], 0, wq, wi);
+        selection_terms(v, e, 8, wq, wi, prm, d, rt);
+#pragma unroll
+        for (int u{0}; u < 4; ++u) {
+            coefy_of(S, 0)[cpad(256 * q + 64 * j[u] + e[u])] = static_cast<_Float16>(c8[u]);
+            d8 += d[u];
+            r8 += rt[u];
             if (want_features) {
                 const float c2{c8[u] * c8[u]};
-                fv += e != 0 ? c2 : 0.0f;
-                fg = __builtin_fmaf(grad_weight(e >> 3) + grad_weight(e & 7), c2, fg);
+                fv += e[u] != 0 ? c2 : 0.0f;
+                fg = __builtin_fmaf(grad_weight(e[u] >> 3) + grad_weight(e[u] & 7), c2, fg);
             }
         }
         // The DC of block (grp / 2, column / 8) sits in lane 16 grp + 8 k,
@@ -1215,21 +1274,29 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         mf_dct32(yp, 16 * (wu & 1), 16 * (wu >> 1), false, lane, c32);
         __syncthreads();
         float d16{0.0f}, r16{0.0f}, d32{0.0f}, r32{0.0f};
+        float v16[4], w16[4], wi16[4], v32[4], w32[4], wi32[4], d[2][4], rt[2][4];
+        int e16[4], e32[4];
+        const int e32_0{(16 * (wu & 1) + t16) * 32 + 16 * (wu >> 1) + 4 * grp};
 #pragma unroll
         for (int u{0}; u < 4; ++u) {
-            int j, e;
-            float d, rt;
-            mf_quad_element(lane, u, 16, j, e);
-            selection_terms(c16[u], e, 16, g_tables.wsel16, g_tables.wsel_inv16, g_tables.rate, prm, d, rt);
-            coefy_of(S, 1)[cpad(256 * q + e)] = static_cast<_Float16>(c16[u]);
-            d16 += d;
-            r16 += rt;
-            const int fy{16 * (wu >> 1) + 4 * grp + u}, fx{16 * (wu & 1) + t16};
-            selection_terms(c32[0][u], fx * 32 + fy, 32, g_tables.wsel32, g_tables.wsel_inv32, g_tables.rate, prm,
-                            d, rt);
-            coefy_of(S, 2)[cpad(fx * 32 + fy)] = static_cast<_Float16>(c32[0][u]);
-            d32 += d;
-            r32 += rt;
+            int j;
+            mf_quad_element(lane, u, 16, j, e16[u]);
+            v16[u] = c16[u];
+            e32[u] = e32_0 + u;
+            v32[u] = c32[0][u];
+        }
+        selection_weights4(g_tables.wsel16, g_tables.wsel_inv16, e16[0], 0, w16, wi16);
+        selection_weights4(g_tables.wsel32, g_tables.wsel_inv32, e32_0, 0, w32, wi32);
+        selection_terms(v16, e16, 16, w16, wi16, prm, d[0], rt[0]);
+        selection_terms(v32, e32, 32, w32, wi32, prm, d[1], rt[1]);
+#pragma unroll
+        for (int u{0}; u < 4; ++u) {
+            coefy_of(S, 1)[cpad(256 * q + e16[u])] = static_cast<_Float16>(c16[u]);
+            d16 += d[0][u];
+            r16 += rt[0][u];
+            coefy_of(S, 2)[cpad(e32[u])] = static_cast<_Float16>(c32[0][u]);
+            d32 += d[1][u];
+            r32 += rt[1][u];
         }
         const float ds16{wave_sum(d16)}, rs16{wave_sum(r16)}, ds32{wave_sum(d32)}, rs32{wave_sum(r32)};
         if (lane == 0) {
