@@ -249,6 +249,14 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 template <int N>
 __device__ __forceinline__ v16f mfma_lines(const float* __restrict__ a_row, const float* __restrict__ b_row,
                                            int col0 = 0, int swz = 0) {
+    // The lane's whole basis row is read before the chain starts (one
+    // memory latency instead of one per four steps).
+    float4 ar[N / 4];
+#pragma unroll
+    for (int c{0}; c < N / 4; ++c) {
+        ar[c] = *reinterpret_cast<const float4*>(a_row + 4 * c);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the chain
     v16f acc;
 #pragma unroll
     for (int i{0}; i < 16; ++i) {
@@ -256,7 +264,7 @@ __device__ __forceinline__ v16f mfma_lines(const float* __restrict__ a_row, cons
     }
 #pragma unroll
     for (int c{0}; c < N; c += 4) {
-        const float4 a{*reinterpret_cast<const float4*>(a_row + c)};
+        const float4 a{ar[c / 4]};
         const float4 b{*reinterpret_cast<const float4*>(b_row + ((((col0 + c) >> 2) ^ swz) << 2))};
         acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.x, b.x, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x1f32(a.y, b.y, acc, 0, 0, 0);
@@ -1460,6 +1468,40 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
                     for (int u{0}; u < 4; ++u) {
                         const int fy{16 * (b >> 1) + 4 * grp + u}, fx{16 * (b & 1) + t16};
                         dst[cpad(fx * 32 + fy)] = static_cast<_Float16>(acc[4 * b + u]);
+                    }
+                }
+            }
+        } else if (__builtin_amdgcn_readfirstlane(S.acs[2] & S.acs[8] & S.acs[10]) == 16 && acs0 == 16) {
+            // Every quad a DCT16: wave c transforms channel c of the four quads
+            // as one chain (quad q = block q): X, B, then Y when it has no
+            // coefficients yet.
+            constexpr int kChanLines{4 * RowBuf<16>::BS};
+            static_assert(3 * kChanLines <= kScratch, "channel lines");
+            const int nch{need_y ? 3 : 2};
+            float* const cl{lines + wu * kChanLines};
+            const int plane{wu == 0 ? 0 : (wu == 1 ? 2 : 1)};
+            if (wu < nch) {
+                const int row{(grp >> 1) * 16 + t16};
+                const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16, plane_of(S, plane) + row * kSP,
+                                              (grp & 1) * 16, sp_swz(row))};
+#pragma unroll
+                for (int q{0}; q < 4; ++q) {
+#pragma unroll
+                    for (int u{0}; u < 4; ++u) {
+                        cl[q * RowBuf<16>::BS + (4 * grp + u) * RowBuf<16>::RS + t16] = acc[4 * q + u];
+                    }
+                }
+            }
+            __syncthreads();
+            if (wu < nch) {
+                const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
+                                              &cl[grp * RowBuf<16>::BS + t16 * RowBuf<16>::RS])};
+                _Float16* const dst{wu < 2 ? cxb(wu) : coefy_of(S, 1)};
+#pragma unroll
+                for (int q{0}; q < 4; ++q) {
+#pragma unroll
+                    for (int u{0}; u < 4; ++u) {
+                        dst[cpad(256 * q + t16 * 16 + 4 * grp + u)] = static_cast<_Float16>(acc[4 * q + u]);
                     }
                 }
             }
