@@ -1625,32 +1625,36 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
 
     const std::size_t cplane{static_cast<std::size_t>(geo.nblocks) * 64};
     float acc[3]{0.0f, 0.0f, 0.0f};
+    // Wave w writes blocks (2 (k % 2) + w % 2, 2 (k / 2) + w / 2), k = 0..3,
+    // one coefficient per lane; the block's transform and slot are
+    // wave-uniform (scalar registers).
+    const int acs_root{__builtin_amdgcn_readfirstlane(S.acs[0])};
 #pragma unroll
     for (int k{0}; k < 4; ++k) {
-        const int bxl{2 * (k & 1) + (w & 1)}, byl{2 * (k >> 1) + (w >> 1)};
+        const int bxl{2 * (k & 1) + (wu & 1)}, byl{2 * (k >> 1) + (wu >> 1)};
         int n{8}, ox{bxl}, oy{byl};
-        if (S.acs[0] == 32) {
+        if (acs_root == 32) {
             n = 32;
             ox = 0;
             oy = 0;
-        } else if (S.acs[(byl & ~1) * 4 + (bxl & ~1)] == 16) {
+        } else if (__builtin_amdgcn_readfirstlane(S.acs[(byl & ~1) * 4 + (bxl & ~1)]) == 16) {
             n = 16;
             ox = bxl & ~1;
             oy = byl & ~1;
         }
         const int qc{(byl - oy) * (n / 8) + (bxl - ox)};
         const int e{qc * 64 + lane};
-        const int idx{n == 32 ? e : 256 * k + 64 * w + lane};
+        const int idx{n == 32 ? e : 256 * k + 64 * wu + lane};
         const int sz{n == 32 ? 2 : (n == 16 ? 1 : 0)};
         const _Float16 hx{cxb(0)[cpad(idx)]};
         const _Float16 hy{coefy_of(S, sz)[cpad(idx)]};
         const _Float16 hb{cxb(1)[cpad(idx)]};
         float v[3]{0.0f, 0.0f, 0.0f};
         if (bx0 + bxl < bw && by0 + byl < bh) {
-            const std::size_t slot{(static_cast<std::size_t>(by0 + byl) * bw + bx0 + bxl) * 64 + lane};
-            coeffs[slot] = __builtin_bit_cast(std::uint16_t, hx);
-            coeffs[cplane + slot] = __builtin_bit_cast(std::uint16_t, hy);
-            coeffs[2 * cplane + slot] = __builtin_bit_cast(std::uint16_t, hb);
+            std::uint16_t* const out{coeffs + (static_cast<std::size_t>(by0 + byl) * bw + bx0 + bxl) * 64};
+            out[lane] = __builtin_bit_cast(std::uint16_t, hx);
+            out[cplane + lane] = __builtin_bit_cast(std::uint16_t, hy);
+            out[2 * cplane + lane] = __builtin_bit_cast(std::uint16_t, hb);
             if (!is_llf(e, n)) {
                 const float xr{static_cast<float>(hx)}, yr{static_cast<float>(hy)};
                 const float br{static_cast<float>(hb)};
@@ -1668,7 +1672,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     for (int c{0}; c < 3; ++c) {
         const float s{wave_sum(acc[c])};
         if (lane == 0) {
-            S.cfl[w][c] = s;
+            S.cfl[wu][c] = s;
         }
     }
     __syncthreads();
