@@ -919,8 +919,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     if (wg.x == 0 && tid < 2 && list_count != nullptr) {
         list_count[tid] = 0;  // the quantizer's item lists, appended after this kernel
     }
-    const int w{tid >> 6};
-    const int wu{__builtin_amdgcn_readfirstlane(w)};
+    const int wu{__builtin_amdgcn_readfirstlane(tid >> 6)};  // the wave (uniform)
     const int lane{tid & 63};
     // Workgroup i runs on XCD i mod 8: give each XCD a contiguous run of
     // regions so that rows shared by neighbouring regions stay in one L2.
@@ -992,7 +991,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     }
 #pragma unroll
     for (int j{0}; j < 4; ++j) {
-        const int lb{j * 4 + w};
+        const int lb{j * 4 + wu};
         const int lbx{lb & 3}, lby{lb >> 2};
         const int fx{lane & 7}, fy{lane >> 3};
         const int idx{(lby * 8 + fy + kHalo) * kTileW + lbx * 8 + fx + kHalo};
