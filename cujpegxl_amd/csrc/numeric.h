@@ -83,7 +83,8 @@ CJX_HD float div_by_recip(float a, float b, float r) {
 // c_j = 1 + (j + 1/2) / 16 (built from the bits of x) and r_j = RN(1 / c_j);
 // t = (m - c_j) r_j (the difference is exact), |t| <= 1/32, and
 // x^a = T[k][j] (1 + t)^a with T[k][j] = RN((2^k c_j)^a) and the binomial
-// series of (1 + t)^a to t^5 (truncation below 3e-11).  The table holds one
+// series of (1 + t)^a to t^3 (truncation below 4e-8: the t^4 terms are
+// 0.0336 t^4 and 0.0412 t^4).  The table holds one
 // pair {T[k][j], r_j} per (k, j), so each power reads one 8-byte entry.
 constexpr int kXybPow = 0;    // [5][16]: x^2.4 for k = -4 .. 0
 constexpr int kXybCbrt = 80;  // [10][16]: x^(1/3) for k = -9 .. 0
@@ -100,9 +101,7 @@ CJX_HD float pow24_tab(float x, const float2* tab) {
     const std::uint32_t u{f2u(x)};
     const float2 e{tab[kXybPow + ((u >> 19) - (123u << 4))]};
     const float t{xyb_series_arg(u, e.y)};
-    float p{0.010752f};
-    p = __builtin_fmaf(p, t, -0.0336f);
-    p = __builtin_fmaf(p, t, 0.224f);
+    float p{0.224f};
     p = __builtin_fmaf(p, t, 1.68f);
     p = __builtin_fmaf(p, t, 2.4f);
     p = __builtin_fmaf(p, t, 1.0f);
@@ -114,9 +113,7 @@ CJX_HD float cbrt_tab(float x, const float2* tab) {
     const std::uint32_t u{f2u(x)};
     const float2 e{tab[kXybCbrt + ((u >> 19) - (118u << 4))]};
     const float t{xyb_series_arg(u, e.y)};
-    float p{0.0301783264f};
-    p = __builtin_fmaf(p, t, -0.0411522634f);
-    p = __builtin_fmaf(p, t, 0.0617283951f);
+    float p{0.0617283951f};
     p = __builtin_fmaf(p, t, -0.111111111f);
     p = __builtin_fmaf(p, t, 0.333333333f);
     p = __builtin_fmaf(p, t, 1.0f);
