@@ -305,6 +305,10 @@ constexpr int kAcSymbols = 64;
 // LDS row stride of a cluster's histogram: one word more than the symbols,
 // so equal symbols of different clusters fall in different banks.
 constexpr int kHistRow = kAcSymbols + 1;
+// Two copies of the histogram, taken by even and odd lanes (neighbouring
+// lanes often carry the same cluster and symbol); the second starts 16
+// banks after the first, so one key's two words sit in different banks.
+constexpr int kHistCopy = kNumClusters * kHistRow + 16;
 constexpr int kTokenGrid = 2048;
 constexpr int kTokenBatch = 32;
 
@@ -375,16 +379,17 @@ __device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(cons
                                                         std::uint32_t* __restrict__ partial,
                                                         const std::uint32_t* __restrict__ list,
                                                         const std::uint32_t* __restrict__ list_count) {
-    __shared__ std::uint32_t h[kNumClusters * kHistRow];
+    __shared__ std::uint32_t h2[2 * kHistCopy];
     __shared__ TokenTables T;
     __shared__ __align__(16) std::int16_t stage[4][2][1024];
-    for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kHistRow; i += 256) {
-        h[i] = 0;
+    for (int i{static_cast<int>(threadIdx.x)}; i < 2 * kHistCopy; i += 256) {
+        h2[i] = 0;
     }
     load_token_tables(T);
     __syncthreads();
     const int lane{static_cast<int>(threadIdx.x) & 63};
     const int half{lane >> 5}, sub{lane & 31};
+    std::uint32_t* const h{h2 + (lane & 1) * kHistCopy};
     std::int16_t* const coef{stage[threadIdx.x >> 6][half]};
     const std::uint32_t wave{wg.x * 4 + (threadIdx.x >> 6)};
     const std::uint32_t stride{2 * wg.nx * 4};
@@ -462,7 +467,8 @@ __device__ __attribute__((always_inline)) inline void ac_tokens_kernel_body(cons
     __syncthreads();
     std::uint32_t* mine{partial + static_cast<std::size_t>(wg.x) * kNumClusters * kAcSymbols};
     for (int i{static_cast<int>(threadIdx.x)}; i < kNumClusters * kAcSymbols; i += 256) {
-        mine[i] = h[(i >> 6) * kHistRow + (i & (kAcSymbols - 1))];
+        const int r{(i >> 6) * kHistRow + (i & (kAcSymbols - 1))};
+        mine[i] = h2[r] + h2[kHistCopy + r];
     }
 }
 
