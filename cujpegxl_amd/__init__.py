@@ -12,12 +12,15 @@ framework is needed for device memory.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import pathlib
 
 import numpy as np
 
 PKG = pathlib.Path(__file__).resolve().parent
-LIB_DIR = PKG / "lib"
+# CJX_REGION_CUT=1 selects the profiling build (lib_cut/, made by
+# `python -m cujpegxl_amd.build --region-cut`) for cujpegxl_amd/region_phases.py.
+LIB_DIR = PKG / ("lib_cut" if os.environ.get("CJX_REGION_CUT") == "1" else "lib")
 PUBLIC_LIB = LIB_DIR / "libcujpegxl.so.1"
 STAGES_LIB = LIB_DIR / "libcujpegxl_stages.so"
 

@@ -21,6 +21,8 @@ ROOT = PKG.parent
 SRC = PKG / "csrc"
 OBJ = PKG / "build" / "obj"
 LIB = PKG / "lib"
+OBJ_CUT = PKG / "build" / "obj_cut"
+LIB_CUT = PKG / "lib_cut"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CJX_OFFLOAD_ARCH", "gfx950")
 
@@ -36,9 +38,9 @@ CFLAGS = [
 ]
 
 
-def _compile(src: str) -> pathlib.Path:
-    out = OBJ / (src.rsplit(".", 1)[0] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", str(SRC / src), "-o", str(out)]
+def _compile(src: str, obj_dir: pathlib.Path = OBJ, extra_flags: list[str] | None = None) -> pathlib.Path:
+    out = obj_dir / (src.rsplit(".", 1)[0] + ".o")
+    cmd = [HIPCC, *CFLAGS, *(extra_flags or []), "-c", str(SRC / src), "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
@@ -49,11 +51,17 @@ def _compile(src: str) -> pathlib.Path:
     return out
 
 
-def build(jobs: int = 8) -> list[pathlib.Path]:
-    OBJ.mkdir(parents=True, exist_ok=True)
-    LIB.mkdir(parents=True, exist_ok=True)
+def build(jobs: int = 8, region_cut: bool = False) -> list[pathlib.Path]:
+    """Builds both libraries into lib/.  region_cut=True instead builds the
+    profiling variant (CJX_REGION_CUT: the region kernel's cut switch and the
+    region-only stage entry) into lib_cut/, which only
+    cujpegxl_amd/region_phases.py loads."""
+    obj_dir, lib_dir = (OBJ_CUT, LIB_CUT) if region_cut else (OBJ, LIB)
+    obj_dir.mkdir(parents=True, exist_ok=True)
+    lib_dir.mkdir(parents=True, exist_ok=True)
+    extra_flags = ["-DCJX_REGION_CUT"] if region_cut else []
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda src: _compile(src, obj_dir, extra_flags), SOURCES))
     by_name = {o.stem: o for o in objs}
     core = [by_name[n] for n in ("device", "tables", "encoder")]
     outs = []
@@ -62,7 +70,7 @@ def build(jobs: int = 8) -> list[pathlib.Path]:
         ("libcujpegxl_stages.so", [by_name["capi"], by_name["noalloc"], by_name["batch"], by_name["stages"]], "libcujpegxl_stages.lds",
          "libcujpegxl_stages.so"),
     ):
-        target = LIB / name
+        target = lib_dir / name
         tmp = target.with_suffix(target.suffix + ".tmp")
         cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, core),
                *map(str, extra), f"-Wl,--version-script={SRC / script}", f"-Wl,-soname,{soname}"]
@@ -73,5 +81,5 @@ def build(jobs: int = 8) -> list[pathlib.Path]:
 
 
 if __name__ == "__main__":
-    for p in build():
+    for p in build(region_cut="--region-cut" in sys.argv[1:]):
         print(p)

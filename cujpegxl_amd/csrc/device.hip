@@ -331,3 +331,10 @@ void launch_dc_emit(hipStream_t s, const Geometry& geo, const std::int32_t* dc, 
 }
 
 }  // namespace cjx
+
+#ifdef CJX_REGION_CUT
+// Profiling builds: the region kernel's cut step (0 = none).
+extern "C" int cjx_set_region_cut(int k) {
+    return hipMemcpyToSymbol(HIP_SYMBOL(cjx::g_region_cut), &k, sizeof(int)) == hipSuccess ? 0 : 1;
+}
+#endif

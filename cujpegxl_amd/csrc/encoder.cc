@@ -560,6 +560,18 @@ bool Encoder::fill_job(FrameJob& j, hipStream_t s, std::uint8_t* out_dev, std::s
     return true;
 }
 
+#ifdef CJX_REGION_CUT
+bool Encoder::region_only(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                          std::size_t chroma_pitch) {
+    if (!configured_ || workspace_ || pending_ || hipSetDevice(device_) != hipSuccess) {
+        return false;
+    }
+    launch_region(stream_, geo_, prm_, luma, luma_pitch, chroma, chroma_pitch, coeffs_, acs_, qf_, cfl_part_,
+                  item_count_, entropy_clears());
+    return hipStreamSynchronize(stream_) == hipSuccess;
+}
+#endif
+
 // One frame queued on `s` in dependency order (the two entropy chains one
 // after the other); nothing waits on the host unless the constant bit
 // strings change (a new shape or distance), which copies them in order on

@@ -88,6 +88,12 @@ public:
                         std::size_t bytes);
     bool encode_on(hipStream_t s, const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
                    std::size_t chroma_pitch, std::uint8_t* out_dev, std::size_t cap, std::uint64_t* size_dev);
+#ifdef CJX_REGION_CUT
+    // Profiling builds: the region kernel alone on the encoder's stream, then
+    // a wait for it.
+    bool region_only(const std::uint8_t* luma, std::size_t luma_pitch, const std::uint8_t* chroma,
+                     std::size_t chroma_pitch);
+#endif
     // A workspace-bound encoder's frame as a batch job: its constant bit
     // strings uploaded (in order on `s`), its buffers, output and size word;
     // the stage grids are the caller's.

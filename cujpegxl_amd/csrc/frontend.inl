@@ -464,6 +464,21 @@ __device__ __forceinline__ void tile_partials(const Geometry& geo, const float* 
     }
 }
 
+// Profiling builds only (CJX_REGION_CUT, never the shipped libraries): the
+// region kernel returns, uniformly, at the start of step k + 1 when the
+// device word says k (k = 6, 7, 8 cut the transform selection after its
+// opening barrier, its row passes and its column passes), so that the time
+// and instruction counts of each step are differences of two cut builds.
+#ifdef CJX_REGION_CUT
+__device__ int g_region_cut;
+#define CJX_REGION_CUT_AT(k)                                        \
+    if (__builtin_amdgcn_readfirstlane(g_region_cut) == (k)) { \
+        return;                                                     \
+    }
+#else
+#define CJX_REGION_CUT_AT(k)
+#endif
+
 // Step 1 of a region: NV12 -> XYB over the 36x36 tile (2-pixel halo,
 // clamped coordinates), exact 4:2:0 bilinear chroma (centred taps clamped
 // to the plane, tests/reference.py:23-46).
@@ -943,6 +958,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     }
     __syncthreads();
 
+    CJX_REGION_CUT_AT(1)
     // 2. Inverse gaborish (gaborish.h:16-28; tap order frontend_transform.cu:30-48).
     {
         gaborish_tile(S, tid);
@@ -981,6 +997,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     }
     __syncthreads();
 
+    CJX_REGION_CUT_AT(2)
     // 3b. Per-block modulation and the quant integer (frontend_fused.cu:117-203).
     //     The 64 cells' fuzzy erosion and the 16 blocks' final step each run
     //     once, in parallel lanes; the per-pixel sums are wave sums as before
@@ -1045,6 +1062,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         }
     }
 
+    CJX_REGION_CUT_AT(3)
     // 4. Transform selection on the sharpened Y (transform_select_impl.cuh:49-201).
     //    Plain: the three sizes run side by side (wave 0 the DCT32 and wave 1
     //    the four DCT16s on the matrix cores, waves 2-3 the sixteen DCT8s on
@@ -1054,6 +1072,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     //    features and the network first; the DCT16 and DCT32 candidates only
     //    where the network is not confident (the heuristic decides there).
     __syncthreads();  // the XYB tile is dead from here on; its space holds row-pass lines
+    CJX_REGION_CUT_AT(6)
     float* const lines{S.u.scratch};
     const int grp{lane >> 4}, t16{lane & 15};
     const float* const yp{plane_of(S, 1)};
@@ -1380,6 +1399,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             rows8();
         }
         __syncthreads();
+        CJX_REGION_CUT_AT(7)
         if (wu >= 2) {
             cols8();
         } else if constexpr (!kLearned) {
@@ -1392,6 +1412,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         }
     }
     __syncthreads();
+    CJX_REGION_CUT_AT(8)
     if (want_features) {
         feature_finish(S, tid, bx0, by0, bw, bh);
     }
@@ -1429,6 +1450,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         }
     }
 
+    CJX_REGION_CUT_AT(4)
     // 5. Chosen DCTs of X and B (Y reuses the selection coefficients; a region
     //    the network decided transforms Y here too when it chose a DCT16 or
     //    DCT32) into the coefficient arrays in element order, then FP16 out
@@ -1622,6 +1644,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         __syncthreads();
     }
 
+    CJX_REGION_CUT_AT(5)
     const std::size_t cplane{static_cast<std::size_t>(geo.nblocks) * 64};
     float acc[3]{0.0f, 0.0f, 0.0f};
     // Wave w writes blocks (2 (k % 2) + w % 2, 2 (k / 2) + w / 2), k = 0..3,

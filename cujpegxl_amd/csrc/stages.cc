@@ -417,4 +417,17 @@ size_t cjx_host_table(int which, void* out, size_t cap) {
     return n;
 }
 
+#ifdef CJX_REGION_CUT
+// Profiling builds: the region kernel alone (see cjx_set_region_cut).
+int cjx_stage_region_only(cujpegxl_encoder* e, const cujpegxl_nv12_input* in) {
+    if (e == nullptr || in == nullptr) {
+        return CUJPEGXL_INVALID_ARGUMENT;
+    }
+    return e->engine.region_only(reinterpret_cast<const std::uint8_t*>(in->luma), in->luma_pitch,
+                                 reinterpret_cast<const std::uint8_t*>(in->chroma), in->chroma_pitch)
+               ? CUJPEGXL_OK
+               : CUJPEGXL_INTERNAL_ERROR;
+}
+#endif
+
 }  // extern "C"
