@@ -385,23 +385,26 @@ __device__ __forceinline__ bool is_llf(int e, int n) {
     return (e >> l2) < m && (e & (n - 1)) < m;
 }
 
-// Selection terms of M Y coefficients of one lane, s[i] at frequency e[i] of
-// an n x n transform (transform_select_impl.cuh:49-108): squared quantization
-// error and rate proxy; LLF coefficients contribute 0.  The two quotients are
-// the correctly rounded ones, by reciprocals.  The weights come in as
-// arguments (read four at a time by the callers, the frequencies of a lane
-// run in fours); the rate entries of all M quotients are read together.
+// Selection terms of M Y coefficients of one lane (transform_select_impl.cuh:
+// 49-108): squared quantization error and rate proxy; LLF coefficients
+// (llf[i], from the callers' closed forms of is_llf for their lane layouts)
+// contribute 0, which the terms of a zero coefficient are exactly (q = 0,
+// err = 0, rate[0] = 0).  The two quotients are the correctly rounded ones,
+// by reciprocals.  The weights come in as arguments (read four at a time by
+// the callers, the frequencies of a lane run in fours); the rate entries of
+// all M quotients are read together.
 template <int M>
-__device__ __forceinline__ void selection_terms(const float (&s)[M], const int (&e)[M], int n, const float (&wq)[M],
+__device__ __forceinline__ void selection_terms(const float (&s)[M], const bool (&llf)[M], const float (&wq)[M],
                                                 const float (&wi)[M], const FrameParams& prm, float (&d)[M],
                                                 float (&rt)[M]) {
     float aq[M];
     int k[M];
 #pragma unroll
     for (int i{0}; i < M; ++i) {
-        const float q{__builtin_rintf(div_by_recip(s[i] * prm.qgsf_select, wq[i], wi[i]))};
-        const float err{div_by_recip(q * wq[i], prm.qgsf_select, prm.inv_qgsf_select) - s[i]};
-        d[i] = is_llf(e[i], n) ? 0.0f : err * err;
+        const float sv{llf[i] ? 0.0f : s[i]};
+        const float q{__builtin_rintf(div_by_recip(sv * prm.qgsf_select, wq[i], wi[i]))};
+        const float err{div_by_recip(q * wq[i], prm.qgsf_select, prm.inv_qgsf_select) - sv};
+        d[i] = err * err;
         aq[i] = fabsf(q);
         k[i] = aq[i] < 256.0f ? static_cast<int>(aq[i]) : 0;
     }
@@ -415,7 +418,7 @@ __device__ __forceinline__ void selection_terms(const float (&s)[M], const int (
         if (aq[i] >= 256.0f) {
             r[i] = 1.0f + log2_det(1.0f + aq[i]);
         }
-        rt[i] = is_llf(e[i], n) ? 0.0f : r[i];
+        rt[i] = r[i];
     }
 }
 
@@ -1129,6 +1132,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
                                           &lines[kLines32 + (16 * (grp & 1) + t16) * RowBuf<32>::RS])};
             float v[16], wq[16], wi[16];
             int e[16];
+            bool llf[16];
 #pragma unroll
             for (int b{0}; b < 4; ++b) {
                 const int e0{(16 * (b & 1) + t16) * 32 + 16 * (b >> 1) + 4 * grp};
@@ -1137,9 +1141,10 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
                 for (int u{0}; u < 4; ++u) {
                     e[4 * b + u] = e0 + u;
                     v[4 * b + u] = acc[4 * b + u];
+                    llf[4 * b + u] = b == 0 && lane < 4;  // fx, fy < 4
                 }
             }
-            selection_terms(v, e, 32, wq, wi, prm, td, tr);
+            selection_terms(v, llf, wq, wi, prm, td, tr);
 #pragma unroll
             for (int i{0}; i < 16; ++i) {
                 coefy_of(S, 2)[cpad(e[i])] = static_cast<_Float16>(v[i]);
@@ -1163,14 +1168,16 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             selection_weights4(g_tables.wsel16, g_tables.wsel_inv16, e0, 0, w4, wi4);
             float v[16], wq[16], wi[16];
             int e[16];
+            bool llf[16];
 #pragma unroll
             for (int i{0}; i < 16; ++i) {
                 e[i] = e0 + (i & 3);
                 v[i] = acc[i];
                 wq[i] = w4[i & 3];
                 wi[i] = wi4[i & 3];
+                llf[i] = lane < 2 && (i & 3) < 2;  // fx, fy < 2
             }
-            selection_terms(v, e, 16, wq, wi, prm, td, tr);
+            selection_terms(v, llf, wq, wi, prm, td, tr);
 #pragma unroll
             for (int i{0}; i < 16; ++i) {
                 coefy_of(S, 1)[cpad(256 * (i >> 2) + e[i])] = static_cast<_Float16>(v[i]);
@@ -1200,13 +1207,15 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         {
             float wq[8], wi[8], d8[8], r8[8];
             int e[8];
+            bool llf[8];
             selection_weights4(g_tables.wsel8, g_tables.wsel_inv8, fx * 8, 0, wq, wi);
             selection_weights4(g_tables.wsel8, g_tables.wsel_inv8, fx * 8 + 4, 4, wq, wi);
 #pragma unroll
             for (int f{0}; f < 8; ++f) {
                 e[f] = fx * 8 + f;
+                llf[f] = f == 0 && fx == 0;  // the DC
             }
-            selection_terms(o, e, 8, wq, wi, prm, d8, r8);
+            selection_terms(o, llf, wq, wi, prm, d8, r8);
 #pragma unroll
             for (int f{0}; f < 8; ++f) {
                 td[f] = d8[f];
@@ -1254,13 +1263,15 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         float d8{0.0f}, r8{0.0f}, fv{0.0f}, fg{0.0f};
         float v[4], wq[4], wi[4], d[4], rt[4];
         int e[4], j[4];
+        bool llf[4];
 #pragma unroll
         for (int u{0}; u < 4; ++u) {
             mf_quad_element(lane, u, 8, j[u], e[u]);
             v[u] = c8[u];
+            llf[u] = u == 0 && (lane & 7) == 0 && (grp & 1) == 0;  // the DC
         }
         selection_weights4(g_tables.wsel8, g_tables.wsel_inv8, e[0], 0, wq, wi);
-        selection_terms(v, e, 8, wq, wi, prm, d, rt);
+        selection_terms(v, llf, wq, wi, prm, d, rt);
 #pragma unroll
         for (int u{0}; u < 4; ++u) {
             coefy_of(S, 0)[cpad(256 * q + 64 * j[u] + e[u])] = static_cast<_Float16>(c8[u]);
@@ -1302,6 +1313,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         float d16{0.0f}, r16{0.0f}, d32{0.0f}, r32{0.0f};
         float v16[4], w16[4], wi16[4], v32[4], w32[4], wi32[4], d[2][4], rt[2][4];
         int e16[4], e32[4];
+        bool llf16[4], llf32[4];
         const int e32_0{(16 * (wu & 1) + t16) * 32 + 16 * (wu >> 1) + 4 * grp};
 #pragma unroll
         for (int u{0}; u < 4; ++u) {
@@ -1310,11 +1322,13 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             v16[u] = c16[u];
             e32[u] = e32_0 + u;
             v32[u] = c32[0][u];
+            llf16[u] = u < 2 && lane < 2;   // fx, fy < 2
+            llf32[u] = wu == 0 && lane < 4;  // fx, fy < 4
         }
         selection_weights4(g_tables.wsel16, g_tables.wsel_inv16, e16[0], 0, w16, wi16);
         selection_weights4(g_tables.wsel32, g_tables.wsel_inv32, e32_0, 0, w32, wi32);
-        selection_terms(v16, e16, 16, w16, wi16, prm, d[0], rt[0]);
-        selection_terms(v32, e32, 32, w32, wi32, prm, d[1], rt[1]);
+        selection_terms(v16, llf16, w16, wi16, prm, d[0], rt[0]);
+        selection_terms(v32, llf32, w32, wi32, prm, d[1], rt[1]);
 #pragma unroll
         for (int u{0}; u < 4; ++u) {
             coefy_of(S, 1)[cpad(256 * q + e16[u])] = static_cast<_Float16>(c16[u]);
