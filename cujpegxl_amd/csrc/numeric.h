@@ -80,20 +80,20 @@ CJX_HD float div_by_recip(float a, float b, float r) {
 
 // Table-driven powers of the XYB conversion.  The mantissa m in [1, 2) of
 // x = 2^k m falls in segment j = its top 4 bits, with centre
-// c_j = 1 + (j + 1/2) / 16 (built from the bits of x) and r_j = RN(1 / c_j);
-// t = (m - c_j) r_j (the difference is exact), |t| <= 1/32, and
-// x^a = T[k][j] (1 + t)^a with T[k][j] = RN((2^k c_j)^a) and the binomial
-// series of (1 + t)^a to t^3 (truncation below 4e-8: the t^4 terms are
-// 0.0336 t^4 and 0.0412 t^4).  The table holds one
-// pair {T[k][j], r_j} per (k, j), so each power reads one 8-byte entry.
+// c_j = 1 + (j + 1/2) / 16 and r_j = RN(1 / c_j); t = (m - c_j) r_j,
+// |t| <= 1/32, and x^a = T[k][j] (1 + t)^a with T[k][j] = RN((2^k c_j)^a)
+// and the binomial series of (1 + t)^a to t^3 (truncation below 4e-8: the
+// t^4 terms are 0.0336 t^4 and 0.0412 t^4).  The table holds one pair
+// {T[k][j], r_j 2^-23} per (k, j), so each power reads one 8-byte entry.
+// m - c_j is exactly D 2^-23 with the integer D = (low 19 mantissa bits of
+// x) - 2^18, so t = D (r_j 2^-23) is the same real product as (m - c_j) r_j
+// and rounds to the same float.
 constexpr int kXybPow = 0;    // [5][16]: x^2.4 for k = -4 .. 0
 constexpr int kXybCbrt = 80;  // [10][16]: x^(1/3) for k = -9 .. 0
 constexpr int kXybTabPairs = 240;
 
-CJX_HD float xyb_series_arg(std::uint32_t u, float r) {
-    const float m{u2f((u & 0x007fffffu) | 0x3f800000u)};
-    const float c{u2f((u & 0x00780000u) | 0x3f840000u)};
-    return (m - c) * r;
+CJX_HD float xyb_series_arg(std::uint32_t u, float r_scaled) {
+    return (static_cast<float>(u & 0x0007ffffu) - 262144.0f) * r_scaled;
 }
 
 // x^2.4 for x in [2^-4, 2) (the sRGB curve's upper branch: x in [0.0905, 1]).

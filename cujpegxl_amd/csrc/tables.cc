@@ -68,7 +68,9 @@ void split_f16(const float* a, int count, std::uint16_t* hi, std::uint16_t* lo) 
 void xyb_power_table(float2* t) {
     for (int j{0}; j < 16; ++j) {
         const double c{1.0 + (j + 0.5) / 16.0};
-        const float r{static_cast<float>(1.0 / c)};
+        // RN(1 / c) scaled by 2^-23 (exact): the series argument takes the
+        // mantissa difference as an integer count of 2^-23 steps.
+        const float r{std::ldexp(static_cast<float>(1.0 / c), -23)};
         for (int k{-4}; k <= 0; ++k) {
             t[kXybPow + (k + 4) * 16 + j] = float2{static_cast<float>(std::pow(std::ldexp(c, k), 2.4)), r};
         }

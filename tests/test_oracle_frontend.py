@@ -60,6 +60,23 @@ def test_power_table_is_rounded_powers_of_segment_centres(jo):
     assert np.allclose(t[112:272], cb.ravel().astype(np.float32), rtol=6e-8, atol=0)
 
 
+def test_power_series_argument_from_the_integer_mantissa(jo):
+    """The device forms the series argument t = (m - c_j) r_j of the oracle's
+    powers as (x - 2^18) (r_j 2^-23), x the low 19 mantissa bits: the same
+    real product, so the same float for every one of the 2^23 mantissas."""
+    t = np.zeros(272, np.float32)
+    jo.lib().jo_xyb_table(t.ctypes.data_as(C.POINTER(C.c_float)))
+    centre, recip = t[0:32:2], t[1:32:2]
+    u = np.arange(1 << 23, dtype=np.uint32) | np.uint32(0x3F800000)
+    j = (u >> np.uint32(19)) & np.uint32(15)
+    m = u.view(np.float32)
+    oracle_t = (m - centre[j]) * recip[j]
+    scaled = np.ldexp(recip, -23).astype(np.float32)
+    assert np.array_equal(np.ldexp(scaled.astype(np.float64), 23).astype(np.float32), recip)
+    device_t = ((u & np.uint32(0x7FFFF)).astype(np.float32) - np.float32(262144.0)) * scaled[j]
+    assert np.array_equal(oracle_t.view(np.uint32), device_t.view(np.uint32))
+
+
 def test_f16_conversion_matches_ieee_rne(jo):
     L = jo.lib()
     rng = np.random.default_rng(2)
