@@ -108,6 +108,22 @@ __global__ void fill_u32_kernel(std::uint32_t* p, std::size_t n, std::uint32_t v
     }
 }
 
+bool check_aq_math(unsigned long long* mismatches) {
+    unsigned long long* dev{nullptr};
+    if (hipMalloc(reinterpret_cast<void**>(&dev), sizeof(unsigned long long)) != hipSuccess) {
+        return false;
+    }
+    bool ok{hipMemset(dev, 0, sizeof(unsigned long long)) == hipSuccess};
+    if (ok) {
+        // v in [0, 4): bit patterns 0 .. 0x40800000; masking_sqrt below 0.25.
+        check_aq_math_kernel<<<8192, 256>>>(0u, 0x40800000u, 0x3e800000u, dev);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpy(mismatches, dev, sizeof(unsigned long long), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    (void)hipFree(dev);
+    return ok;
+}
+
 std::size_t sizeof_constants() { return sizeof(Constants); }
 std::size_t sizeof_codeset() { return sizeof(CodeSet); }
 std::size_t sizeof_layout() { return sizeof(Layout); }

@@ -116,7 +116,31 @@ __device__ __forceinline__ void pixel_to_xyb(float yv, float cb, float cr, float
     ob = q2;
 }
 
+// a / b and sqrt(x), correctly rounded, for normal operands of moderate
+// size with normal results (the AQ terms: 0.01 <= a, b <= 2e4, 27 <= x <=
+// 5e5): the compiler's correctly rounded sequences (a reciprocal, one
+// Newton step and two Markstein corrections; a square root and its
+// one-ulp neighbour tests) without their operand scaling and special-case
+// steps, which are identities for such operands.  check_aq_math_kernel
+// compares both against the compiler's forms over every reachable input.
+__device__ __forceinline__ float div_moderate(float a, float b) {
+    float r{__builtin_amdgcn_rcpf(b)};
+    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    float q{a * r};
+    q = __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+    return __builtin_fmaf(__builtin_fmaf(-b, q, a), r, q);
+}
+
+__device__ __forceinline__ float sqrt_moderate(float x) {
+    const float s{__builtin_amdgcn_sqrtf(x)};
+    const float below{__builtin_bit_cast(float, __builtin_bit_cast(std::int32_t, s) - 1)};
+    const float above{__builtin_bit_cast(float, __builtin_bit_cast(std::int32_t, s) + 1)};
+    const float r{__builtin_fmaf(-below, s, x) <= 0.0f ? below : s};
+    return __builtin_fmaf(-above, s, x) > 0.0f ? above : r;
+}
+
 // adaptive_quant_impl.cuh:29-61
+template <bool kModerate = true>
 __device__ __forceinline__ float ratio_cbrt_gamma(float v, bool invert) {
     constexpr float k_sg_mul{226.77216153508914f};
     constexpr float k_sg_log2{0.693147181f};
@@ -130,12 +154,45 @@ __device__ __forceinline__ float ratio_cbrt_gamma(float v, bool invert) {
     const float v2{v * v};
     const float num{num_mul * v2 + eps};
     const float den{den_mul * v * v2 + v_offset};
-    return invert ? num / den : den / num;
+    if constexpr (kModerate) {
+        return invert ? div_moderate(num, den) : div_moderate(den, num);
+    } else {
+        return invert ? num / den : den / num;
+    }
 }
 
+template <bool kModerate = true>
 __device__ __forceinline__ float masking_sqrt(float v) {
     constexpr float mul{211.66567973503678f * 1e8f};
-    return 0.25f * __builtin_sqrtf(v * __builtin_sqrtf(mul) + 27.505837037000106f);
+    const float x{v * __builtin_sqrtf(mul) + 27.505837037000106f};
+    if constexpr (kModerate) {
+        return 0.25f * sqrt_moderate(x);
+    } else {
+        return 0.25f * __builtin_sqrtf(x);
+    }
+}
+
+// The AQ terms' division and square root in both forms over every float
+// input of their domains (ratio_cbrt_gamma: v in [0, 4), both directions;
+// masking_sqrt: v in [0, 0.25)): mismatching bit patterns are counted.
+__global__ void check_aq_math_kernel(std::uint32_t first, std::uint32_t count, std::uint32_t sqrt_limit,
+                                     unsigned long long* mismatches) {
+    unsigned int bad{0};
+    for (std::uint32_t i{blockIdx.x * blockDim.x + threadIdx.x}; i < count; i += gridDim.x * blockDim.x) {
+        const std::uint32_t bits{first + i};
+        const float v{__builtin_bit_cast(float, bits)};
+        bad += __builtin_bit_cast(std::uint32_t, ratio_cbrt_gamma<true>(v, true)) !=
+               __builtin_bit_cast(std::uint32_t, ratio_cbrt_gamma<false>(v, true));
+        bad += __builtin_bit_cast(std::uint32_t, ratio_cbrt_gamma<true>(v, false)) !=
+               __builtin_bit_cast(std::uint32_t, ratio_cbrt_gamma<false>(v, false));
+        if (bits < sqrt_limit) {
+            bad += __builtin_bit_cast(std::uint32_t, masking_sqrt<true>(v)) !=
+                   __builtin_bit_cast(std::uint32_t, masking_sqrt<false>(v));
+        }
+    }
+    if (bad != 0) {
+        atomicAdd(mismatches, static_cast<unsigned long long>(bad));
+    }
 }
 
 __device__ __forceinline__ float compute_mask(float out_val) {

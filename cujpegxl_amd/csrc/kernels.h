@@ -70,6 +70,10 @@ void launch_dc_maps(hipStream_t s, const Geometry& geo, const FrameParams& prm, 
 void launch_nz(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                std::uint8_t* nz_grid, std::uint16_t* nz_count, const Clears& clr);
 void launch_fill_u32(hipStream_t s, std::uint32_t* p, std::size_t n, std::uint32_t v);
+// Mismatching results of the AQ terms' moderate-operand division and square
+// root against the compiler's correctly rounded forms, over every float
+// input of their domains (check_aq_math_kernel); false on a HIP error.
+bool check_aq_math(unsigned long long* mismatches);
 void launch_ac_tokens(hipStream_t s, const Geometry& geo, const std::int16_t* ac, const std::int8_t* acs,
                       const std::uint8_t* nz_grid, const std::uint16_t* nz_count, std::uint32_t* tokens,
                       std::uint16_t* item_tokens, std::uint32_t* partial, std::uint32_t* hist,

@@ -292,6 +292,15 @@ void cjx_batch_destroy(cjx_batch* b) {
     delete b;
 }
 
+int cjx_stage_check_aq_math(int32_t device_ordinal, uint64_t* mismatches) {
+    unsigned long long n{0};
+    if (mismatches == nullptr || hipSetDevice(device_ordinal) != hipSuccess || !cjx::check_aq_math(&n)) {
+        return CUJPEGXL_INTERNAL_ERROR;
+    }
+    *mismatches = n;
+    return CUJPEGXL_OK;
+}
+
 int cjx_stage_timings(cujpegxl_encoder* e, float* ms, int count) {
     if (e == nullptr || ms == nullptr) {
         return CUJPEGXL_INVALID_ARGUMENT;

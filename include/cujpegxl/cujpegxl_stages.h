@@ -141,6 +141,14 @@ int cjx_mlp_forward(const float* params, const float* x, uint32_t n, float* z, u
 int cjx_stage_set_mode(cujpegxl_encoder* encoder, uint32_t mode);
 int cjx_stage_timings(cujpegxl_encoder* encoder, float* ms, int count);
 
+/* The region kernel's AQ terms divide and take square roots with the
+ * compiler's correctly rounded sequences stripped of their operand scaling
+ * and special-case steps (identities for the terms' moderate operands).
+ * Counts, on the device, the results of those forms that differ from the
+ * compiler's full forms over every float input of the terms' domains;
+ * *mismatches = 0 is the expected answer. */
+int cjx_stage_check_aq_math(int32_t device_ordinal, uint64_t* mismatches);
+
 /* `count` synchronous cujpegxl_encoder_encode calls on the same input and
  * output, back to back, timed on the host around the whole loop (wall
  * seconds into *seconds).  No counterpart in the reference's library: the
