@@ -70,15 +70,29 @@ __device__ __forceinline__ float from_lane_below(float v) {
                                                                       true)));
 }
 
-// Sum of the wave's 64 values in lane 0 (offsets 32, 16, ..., 1 of the
-// shfl_down tree; the other lanes hold partial sums).
-__device__ __forceinline__ float wave_sum(float v) {
-    v = add_down_32(v);
-    v = add_down_16(v);
+// Four wave sums at once, each the shfl_down tree of its 64 values (offsets
+// 32, 16, ..., 1; lane i adds lane i + offset) with the same operands in the
+// same order: one permlane32 swap and add carry out offset 32 for two
+// values (a in lanes 0-31, b in lanes 32-63; c and d likewise), one
+// permlane16 swap and add offset 16 for all four (rows: a, c, b, d), then
+// the row shifts 8..1 for all four.  The result holds sum(a) in lane 0,
+// sum(c) in lane 16, sum(b) in lane 32 and sum(d) in lane 48.
+__device__ __forceinline__ float wave_sum4(float a, float b, float c, float d) {
+    const auto ab{__builtin_amdgcn_permlane32_swap(f2u(a), f2u(b), false, false)};
+    const float x{u2f(ab[0]) + u2f(ab[1])};
+    const auto cd{__builtin_amdgcn_permlane32_swap(f2u(c), f2u(d), false, false)};
+    const float y{u2f(cd[0]) + u2f(cd[1])};
+    const auto xy{__builtin_amdgcn_permlane16_swap(f2u(x), f2u(y), false, false)};
+    float v{u2f(xy[0]) + u2f(xy[1])};
     v = add_down_row<8>(v);
     v = add_down_row<4>(v);
     v = add_down_row<2>(v);
     return add_down_row<1>(v);
+}
+
+// Lane l's value of v (a wave-uniform read).
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return u2f(static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(f2u(v)), l)));
 }
 
 __device__ __forceinline__ float clamp01(float v) {
@@ -1084,11 +1098,11 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         const float gm{ratio_cbrt_gamma(iny - xv, true) + ratio_cbrt_gamma(iny + xv, true)};
         const float pye{yv + 0.084381641171960495f + fabsf(xv)};
         const float bl{bv > pye ? fminf(bv - pye, 0.027121074570634722f) : 0.0f};
-        const float hf_sum{wave_sum(hf)};
-        const float gamma_overall{wave_sum(gm) * (0.5f / 64.0f)};
-        const float blue_sum{wave_sum(bl)};
+        const float sums{wave_sum4(hf, gm, bl, 0.0f)};  // lanes 0, 32, 16
+        const float gamma_overall{lane_value(sums, 32) * (0.5f / 64.0f)};
+        const float blue_sum{lane_value(sums, 16)};
         if (lane == 0) {
-            S.u.f.aq.blk[lb][0] = hf_sum;
+            S.u.f.aq.blk[lb][0] = sums;
             S.u.f.aq.blk[lb][1] = gamma_overall;
             S.u.f.aq.blk[lb][2] = blue_sum;
         }
@@ -1212,9 +1226,10 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
                 ad += td[i];
                 ar += tr[i];
             }
-            const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
+            const float sums{wave_sum4(ad, ar, 0.0f, 0.0f)};  // lanes 0, 32
+            const float rs{lane_value(sums, 32)};
             if (lane == 0) {
-                S.cost32 = 1024.0f * ds + 6.0e-4f * rs;
+                S.cost32 = 1024.0f * sums + 6.0e-4f * rs;
             }
         } else {
             const v16f acc{mfma_lines<16>(g_tables.basis16 + t16 * 16,
@@ -1239,17 +1254,25 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             for (int i{0}; i < 16; ++i) {
                 coefy_of(S, 1)[cpad(256 * (i >> 2) + e[i])] = static_cast<_Float16>(v[i]);
             }
+            float ad[4], ar[4];
 #pragma unroll
             for (int q{0}; q < 4; ++q) {
-                float ad{td[4 * q]}, ar{tr[4 * q]};
+                ad[q] = td[4 * q];
+                ar[q] = tr[4 * q];
 #pragma unroll
                 for (int u{1}; u < 4; ++u) {
-                    ad += td[4 * q + u];
-                    ar += tr[4 * q + u];
+                    ad[q] += td[4 * q + u];
+                    ar[q] += tr[4 * q + u];
                 }
-                const float ds{wave_sum(ad)}, rs{wave_sum(ar)};
+            }
+#pragma unroll
+            for (int h{0}; h < 2; ++h) {
+                // Quads 2h (lanes 0, 32) and 2h + 1 (lanes 16, 48).
+                const float sums{wave_sum4(ad[2 * h], ar[2 * h], ad[2 * h + 1], ar[2 * h + 1])};
+                const float d1{lane_value(sums, 16)}, r0{lane_value(sums, 32)}, r1{lane_value(sums, 48)};
                 if (lane == 0) {
-                    S.cost16[q] = 256.0f * ds + 6.0e-4f * rs;
+                    S.cost16[2 * h] = 256.0f * sums + 6.0e-4f * r0;
+                    S.cost16[2 * h + 1] = 256.0f * d1 + 6.0e-4f * r1;
                 }
             }
         }
@@ -1395,7 +1418,8 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             d32 += d[1][u];
             r32 += rt[1][u];
         }
-        const float ds16{wave_sum(d16)}, rs16{wave_sum(r16)}, ds32{wave_sum(d32)}, rs32{wave_sum(r32)};
+        const float sums{wave_sum4(d16, r16, d32, r32)};  // lanes 0, 32, 16, 48
+        const float ds16{sums}, rs16{lane_value(sums, 32)}, ds32{lane_value(sums, 16)}, rs32{lane_value(sums, 48)};
         if (lane == 0) {
             S.cost16[q] = 256.0f * ds16 + 6.0e-4f * rs16;
             S.cost32p[q] = 1024.0f * ds32 + 6.0e-4f * rs32;
@@ -1761,11 +1785,13 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
             acc[c] = k == 0 ? v[c] : acc[c] + v[c];
         }
     }
-#pragma unroll
-    for (int c{0}; c < 3; ++c) {
-        const float s{wave_sum(acc[c])};
+    {
+        const float sums{wave_sum4(acc[0], acc[1], acc[2], 0.0f)};  // lanes 0, 32, 16
+        const float s1{lane_value(sums, 32)}, s2{lane_value(sums, 16)};
         if (lane == 0) {
-            S.cfl[wu][c] = s;
+            S.cfl[wu][0] = sums;
+            S.cfl[wu][1] = s1;
+            S.cfl[wu][2] = s2;
         }
     }
     __syncthreads();
