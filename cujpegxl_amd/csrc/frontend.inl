@@ -827,6 +827,32 @@ __device__ __forceinline__ float h2f(std::uint16_t h) {
     return static_cast<float>(__builtin_bit_cast(_Float16, h));
 }
 
+// One thread's share of the network's weights, read at the kernel's start
+// (the learned forms), so that no read waits inside the decision: layer 1's
+// 24 weights of hidden unit tid / 4 (inputs 24 (tid % 4) ..) and its bias,
+// and layer 2's 8 weights of output tid / 8 (hidden 8 (tid % 8) ..; threads
+// below 56) and its bias.
+struct MlpRegs {
+    uint4 w1[3];
+    uint4 w2;
+    float b1, b2;
+};
+
+__device__ __forceinline__ MlpRegs mlp_load(int tid) {
+    MlpRegs r;
+    const int j{tid >> 2}, k{tid & 3};
+    const std::uint16_t* const w{g_mlp.w1 + j * 96 + 24 * k};
+#pragma unroll
+    for (int i{0}; i < 3; ++i) {
+        r.w1[i] = *reinterpret_cast<const uint4*>(w + 8 * i);
+    }
+    const int o{tid < 56 ? tid >> 3 : 0}, p{tid & 7};
+    r.w2 = *reinterpret_cast<const uint4*>(g_mlp.w2 + o * 64 + 8 * p);
+    r.b1 = g_mlp.b1[j];
+    r.b2 = g_mlp.b2[o];
+    return r;
+}
+
 // (64 / 112) l[k] of the gradient feature, k = 0..7.
 __device__ __forceinline__ float grad_weight(int k) {
     constexpr float w[8]{0.0f, 0.08699481934309006f, 0.33473509550094604f, 0.7055046558380127f,
@@ -877,16 +903,15 @@ __device__ __forceinline__ void feature_finish(RegionShared& S, int tid, int bx0
 // each quad where a DCT16 fits), the transform map in S.acs.  Returns that
 // confidence (the same value in every thread).  `hid` is LDS scratch of 72
 // floats.  Every thread calls this; barriers inside.
-__device__ __forceinline__ bool mlp_decide(RegionShared& S, const FrameParams& prm, float* hid, int tid, int bx0,
-                                           int by0, int bw, int bh) {
+__device__ __forceinline__ bool mlp_decide(RegionShared& S, const FrameParams& prm, const MlpRegs& mr, float* hid,
+                                           int tid, int bx0, int by0, int bw, int bh) {
     {
         const int j{tid >> 2}, k{tid & 3};
-        const std::uint16_t* const w{g_mlp.w1 + j * 96 + 24 * k};
         const float* const x{&S.feat[24 * k]};
         float s{0.0f};
 #pragma unroll
         for (int i{0}; i < 24; i += 8) {
-            const uint4 pw{*reinterpret_cast<const uint4*>(w + i)};
+            const uint4 pw{mr.w1[i / 8]};
             const std::uint32_t word[4]{pw.x, pw.y, pw.z, pw.w};
 #pragma unroll
             for (int h{0}; h < 4; ++h) {
@@ -897,22 +922,24 @@ __device__ __forceinline__ bool mlp_decide(RegionShared& S, const FrameParams& p
         s = add_down_row<2>(s);
         s = add_down_row<1>(s);
         if (k == 0) {
-            hid[j] = fmaxf(s + g_mlp.b1[j], 0.0f);
+            hid[j] = fmaxf(s + mr.b1, 0.0f);
         }
     }
     __syncthreads();
     if (tid < 56) {
         const int o{tid >> 3}, p{tid & 7};
+        const std::uint32_t word[4]{mr.w2.x, mr.w2.y, mr.w2.z, mr.w2.w};
         float s{0.0f};
 #pragma unroll
         for (int q{0}; q < 8; ++q) {
-            s = __builtin_fmaf(h2f(g_mlp.w2[o * 64 + 8 * p + q]), hid[8 * p + q], s);
+            const std::uint32_t w{word[q >> 1]};
+            s = __builtin_fmaf(h2f(static_cast<std::uint16_t>((q & 1) != 0 ? w >> 16 : w & 0xffffu)), hid[8 * p + q], s);
         }
         s = add_down_row<4>(s);
         s = add_down_row<2>(s);
         s = add_down_row<1>(s);
         if (p == 0) {
-            hid[64 + o] = s + g_mlp.b2[o];
+            hid[64 + o] = s + mr.b2;
         }
     }
     __syncthreads();
@@ -1009,6 +1036,10 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
         list_count[tid] = 0;  // the quantizer's item lists, appended after this kernel
     }
     const int wu{__builtin_amdgcn_readfirstlane(tid >> 6)};  // the wave (uniform)
+    [[maybe_unused]] MlpRegs mlp_regs;
+    if constexpr (kLearned) {
+        mlp_regs = mlp_load(tid);
+    }
     const int lane{tid & 63};
     // Workgroup i runs on XCD i mod 8: give each XCD a contiguous run of
     // regions so that rows shared by neighbouring regions stay in one L2.
@@ -1513,7 +1544,7 @@ __device__ __attribute__((always_inline)) inline void region_kernel_body(const W
     }
     bool fallback{!kLearned};
     if constexpr (kLearned) {
-        fallback = !mlp_decide(S, prm, lines, tid, bx0, by0, bw, bh);
+        fallback = !mlp_decide(S, prm, mlp_regs, lines, tid, bx0, by0, bw, bh);
         if (fallback) {
             // The heuristic for this region: the larger candidates and the
             // decision from the costs.
