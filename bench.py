@@ -233,8 +233,8 @@ def main() -> int:
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build encodes on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        print("bench.py measures one GPU: the encoder runs on a single MI355X "
+              "(sharding a frame's AC-group rows over several GPUs is not implemented)", file=sys.stderr)
         return 2
 
     import cujpegxl_amd as cj
