@@ -1962,7 +1962,7 @@ __device__ __attribute__((always_inline)) inline void tile_quantize_kernel_body(
         }
         S.own[tid] = ((oy - ty * 8) * 8 + (ox - tx * 8)) | (n << 8);
     }
-    if (tid == 0) {
+    if (tid == 64) {  // wave 1, beside wave 0's owners
         float r[4][3];
         tile_partials(geo, cfl_part, tx, ty, r);
         int mx, mb;
