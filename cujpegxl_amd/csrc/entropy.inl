@@ -586,20 +586,49 @@ __device__ __attribute__((always_inline)) inline void ac_group_emit_kernel_body(
     const int items{row_items * gbh};
     std::uint32_t* const slot{staging + static_cast<std::size_t>(wg.x) * kPartSlotWords};
 
-    // 1. Token prefix over the part's items.
+    // 1. Token prefix over the part's items (at most two per thread).  Every
+    //    read of this step (the items' transforms and token counts, the code
+    //    table) is issued before the first wait; a covered block's count is
+    //    read too and then taken as 0.
+    static_assert(kPartItems <= 2 * kPartThreads, "two items per thread");
+    static_assert((kNumClusters * kAcSymbols) % kPartThreads == 0, "code table entries per thread");
+    constexpr int kLutPerThread{kNumClusters * kAcSymbols / kPartThreads};
+    int item_n[2]{0, 0};
+    std::uint32_t item_cnt[2]{0u, 0u};
+#pragma unroll
+    for (int k{0}; k < 2; ++k) {
+        const int i{k * kPartThreads + tid};
+        if (i < items) {
+            const int row{i / row_items}, rem{i % row_items};
+            const std::size_t pos{static_cast<std::size_t>(by0 + row) * geo.bw + bx0 + rem / 3};
+            item_n[k] = acs[pos];
+            item_cnt[k] = item_tokens[pos * 3 + rem % 3];
+        }
+    }
+    std::uint32_t lut[kLutPerThread];
+#pragma unroll
+    for (int k{0}; k < kLutPerThread; ++k) {
+        const int i{k * kPartThreads + tid};
+        const int at{(i / kAcSymbols) * kHistStride + i % kAcSymbols};
+        lut[k] = static_cast<std::uint32_t>(depth[at]) | (static_cast<std::uint32_t>(codes[at]) << 16);
+    }
     std::uint32_t carry{0};
-    for (int base{0}; base < items; base += kPartThreads) {
+#pragma unroll
+    for (int k{0}; k < 2; ++k) {
+        const int base{k * kPartThreads};
+        if (base >= items) {
+            break;
+        }
         const int i{base + tid};
         std::uint32_t cnt{0};
         if (i < items) {
             const int row{i / row_items}, rem{i % row_items};
-            const std::size_t pos{static_cast<std::size_t>(by0 + row) * geo.bw + bx0 + rem / 3};
-            const int n{acs[pos]};
+            const int n{item_n[k]};
             const int ci{rem % 3};
             S.m[i] = static_cast<std::int8_t>(n / 8);
             S.slot0[i] = static_cast<std::uint32_t>(
                 token_slot(geo, ci == 0 ? 1 : (ci == 1 ? 0 : 2), bx0 + rem / 3, by0 + row, 1, 0));
-            cnt = n != 0 ? item_tokens[pos * 3 + ci] : 0u;
+            cnt = n != 0 ? item_cnt[k] : 0u;
         }
         std::uint64_t total{0};
         const std::uint64_t p{block_excl_scan(cnt, S.sh, total)};
@@ -612,9 +641,9 @@ __device__ __attribute__((always_inline)) inline void ac_group_emit_kernel_body(
         S.prefix[items] = carry;
         S.overflow = 0;
     }
-    for (int i{tid}; i < kNumClusters * kAcSymbols; i += kPartThreads) {
-        const int at{(i / kAcSymbols) * kHistStride + i % kAcSymbols};
-        S.lut[i] = static_cast<std::uint32_t>(depth[at]) | (static_cast<std::uint32_t>(codes[at]) << 16);
+#pragma unroll
+    for (int k{0}; k < kLutPerThread; ++k) {
+        S.lut[k * kPartThreads + tid] = lut[k];
     }
     for (int i{tid}; i < kPartWords; i += kPartThreads) {
         S.buf[i] = 0;
