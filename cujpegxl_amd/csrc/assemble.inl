@@ -302,6 +302,23 @@ __device__ __attribute__((always_inline)) inline void layout_kernel_body(const W
     __shared__ std::uint64_t agbits[kMaxAcGroups];
     __shared__ std::uint64_t sh[17];
     const int t{static_cast<int>(threadIdx.x)};
+    // The reads that depend on nothing computed here, issued together before
+    // the first wait: each thread's first word of the AcGlobal prefix and of
+    // the head (the head's constant words are one per thread), its code
+    // header's alphabet size or bit count, its DC group sizes for the TOC
+    // (entry t: group t - 1) and for the section positions (group t).
+    static_assert(kHeadConstWords == 256, "one head word per thread");
+    const std::uint32_t overflow0{blobs->overflow};
+    const std::uint32_t pre_bits{k->ac_prefix_bits};
+    const std::uint32_t hb{k->head_bits};
+    const std::uint32_t dc_global_bits{k->dc_global_bits};
+    const std::uint32_t prefix_w0{k->ac_prefix_words[t]};
+    const std::uint32_t head_w0{k->head_words[t]};
+    const std::uint32_t alpha_t{t < kNumClusters ? cs->alpha[t] : 0u};
+    const std::uint32_t hdr_bits_t{t >= kNumClusters && t < 2 * kNumClusters ? cs->hdr_bits[t - kNumClusters] : 0u};
+    const std::uint32_t dc_bytes_t{static_cast<std::uint32_t>(t) < geo.num_dc ? dc_group_bytes[t] : 0u};
+    const std::uint32_t dc_bytes_prev{t >= 1 && static_cast<std::uint32_t>(t - 1) < geo.num_dc ? dc_group_bytes[t - 1]
+                                                                                                  : 0u};
     // AC group sizes: the sums of their parts' bits.
     for (int g{t}; g < static_cast<int>(geo.num_ac); g += 256) {
         std::uint64_t sum{0};
@@ -319,18 +336,17 @@ __device__ __attribute__((always_inline)) inline void layout_kernel_body(const W
         hw[i] = 0;
     }
     __syncthreads();
-    std::uint32_t overflow{blobs->overflow};
+    std::uint32_t overflow{overflow0};
 
     // AcGlobal.
-    const std::uint32_t pre_bits{k->ac_prefix_bits};
     for (std::uint32_t i{static_cast<std::uint32_t>(t)}; i < (pre_bits + 31) / 32; i += 256) {
         const std::uint32_t rem{pre_bits - 32 * i};
-        const std::uint32_t w{k->ac_prefix_words[i]};
+        const std::uint32_t w{i == static_cast<std::uint32_t>(t) ? prefix_w0 : k->ac_prefix_words[i]};
         agw[i] = rem >= 32 ? w : (w & ((1u << rem) - 1u));
     }
     std::uint32_t len{0}, val{0};
     if (t < kNumClusters) {
-        const std::uint32_t n{cs->alpha[t] - 1};
+        const std::uint32_t n{alpha_t - 1};
         if (n == 0) {
             len = 1;
         } else {
@@ -339,7 +355,7 @@ __device__ __attribute__((always_inline)) inline void layout_kernel_body(const W
             val = 1u | (nb << 1) | ((n - (1u << nb)) << 5);
         }
     } else if (t < 2 * kNumClusters) {
-        len = cs->hdr_bits[t - kNumClusters];
+        len = hdr_bits_t;
     }
     std::uint64_t pieces{0};
     const std::uint64_t off{pre_bits + block_excl_scan(len, sh, pieces)};
@@ -357,15 +373,14 @@ __device__ __attribute__((always_inline)) inline void layout_kernel_body(const W
     }
 
     // Head: constant bits, no permutation (one 0 bit), byte-aligned TOC.
-    const std::uint32_t hb{k->head_bits};
     for (std::uint32_t i{static_cast<std::uint32_t>(t)}; i < (hb + 31) / 32; i += 256) {
         const std::uint32_t rem{hb - 32 * i};
-        const std::uint32_t w{k->head_words[i]};
+        const std::uint32_t w{head_w0};  // i == t: the head has at most 256 words
         hw[i] = rem >= 32 ? w : (w & ((1u << rem) - 1u));
     }
     const Distr toc[4]{{false, 0, 10}, {false, 1024, 14}, {false, 17408, 22}, {false, 4211712, 30}};
     const bool single{geo.num_ac == 1};
-    const std::uint32_t dcg{(k->dc_global_bits + 7) / 8};
+    const std::uint32_t dcg{(dc_global_bits + 7) / 8};
     const std::uint32_t acg{static_cast<std::uint32_t>((abits + 7) / 8)};
     const std::uint32_t entries{single ? 1u : 2u + geo.num_dc + geo.num_ac};
     std::uint64_t at{(hb + 1u + 7u) & ~7u};
@@ -375,12 +390,12 @@ __device__ __attribute__((always_inline)) inline void layout_kernel_body(const W
         std::uint32_t size{0}, code{0}, clen{0};
         if (e < entries) {
             if (single) {
-                const std::uint64_t bits{k->dc_global_bits + dc_single_bits[0] + abits + agbits[0]};
+                const std::uint64_t bits{dc_global_bits + dc_single_bits[0] + abits + agbits[0]};
                 size = static_cast<std::uint32_t>((bits + 7) / 8);
             } else if (e == 0) {
                 size = dcg;
             } else if (e <= geo.num_dc) {
-                size = dc_group_bytes[e - 1];
+                size = dc_bytes_prev;  // e = t: the TOC's DC entries all lie in its first 256
             } else if (e == geo.num_dc + 1) {
                 size = acg;
             } else {
@@ -410,15 +425,15 @@ __device__ __attribute__((always_inline)) inline void layout_kernel_body(const W
     std::uint64_t ac_global_bit{0};
     if (single) {
         if (t == 0) {
-            dc_base[0] = dcg_bit + k->dc_global_bits;
-            ac_global_bit = dcg_bit + k->dc_global_bits + dc_single_bits[0];
+            dc_base[0] = dcg_bit + dc_global_bits;
+            ac_global_bit = dcg_bit + dc_global_bits + dc_single_bits[0];
             ac_base[0] = ac_global_bit + abits;
         }
     } else {
         const std::uint64_t first{dcg_bit + static_cast<std::uint64_t>(dcg) * 8};
         std::uint64_t dc_total{0};
         const std::uint32_t g{static_cast<std::uint32_t>(t)};
-        const std::uint64_t dsz{g < geo.num_dc ? static_cast<std::uint64_t>(dc_group_bytes[g]) * 8 : 0ull};
+        const std::uint64_t dsz{g < geo.num_dc ? static_cast<std::uint64_t>(dc_bytes_t) * 8 : 0ull};
         const std::uint64_t dpre{block_excl_scan(dsz, sh, dc_total)};
         if (g < geo.num_dc) {
             dc_base[g] = first + dpre;
