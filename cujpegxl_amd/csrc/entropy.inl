@@ -534,7 +534,7 @@ __device__ std::uint64_t block_excl_scan(std::uint64_t v, std::uint64_t* sh, std
 // ---- AC groups emitted in parts -----------------------------------------------
 //
 // Each AC group is cut into kGroupParts parts of kPartRows block rows (each a
-// contiguous run of the group's stream), and one 512-thread workgroup writes
+// contiguous run of the group's stream), and one 256-thread workgroup writes
 // a part's token stream at part-relative bit positions into the part's
 // staging slot: the part's items in decoder order (block rows, blocks left to
 // right, channels Y, X, B) and their token counts give a flat token numbering
