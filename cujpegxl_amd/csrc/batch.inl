@@ -79,7 +79,7 @@ __global__ void clear_counters_kernel(std::uint32_t* __restrict__ counters) {
     }
 }
 
-__global__ __launch_bounds__(256) void quantize_batch_kernel(const FrameJob* __restrict__ jobs, int count) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void quantize_batch_kernel(const FrameJob* __restrict__ jobs, int count) {
     Wg wg;
     const FrameJob& j{batch_job(jobs, count, kBQuantize, wg)};
     tile_quantize_kernel_body(wg, j.geo, j.prm, j.coeffs, j.acs, j.qf, j.cfl_part, j.ac, j.nz_grid, j.nz_count,

@@ -1973,49 +1973,89 @@ __device__ __attribute__((always_inline)) inline void tile_quantize_kernel_body(
     __syncthreads();
 
     const float yx{S.maps[0]}, yb{S.maps[1]};
-#pragma unroll 4
-    for (int j{0}; j < 16; ++j) {
-        const int p{j * 4 + w};
+    // Lane l quantizes coefficients 4 (l % 16) .. 4 (l % 16) + 3 of block
+    // p = 16 j + 4 (l / 16) + w: a wave takes four blocks per step, one
+    // 8-byte read per plane and lane, the weights and their reciprocals as
+    // 16-byte reads.  y_round = qy wy / qgsf is the correctly rounded
+    // quotient, by the block's RN(1 / qgsf).
+    const int grp{lane >> 4}, e4{4 * (lane & 15)};
+    static_assert(offsetof(DeviceTables, dequant8) % 16 == 0 && offsetof(DeviceTables, dequant16) % 16 == 0 &&
+                      offsetof(DeviceTables, dequant32) % 16 == 0 && offsetof(DeviceTables, dequant_inv8) % 16 == 0 &&
+                      offsetof(DeviceTables, dequant_inv16) % 16 == 0 &&
+                      offsetof(DeviceTables, dequant_inv32) % 16 == 0,
+                  "16-byte weight reads");
+    const auto f16_at = [](uint2 h, int i) {
+        const std::uint32_t wd{i < 2 ? h.x : h.y};
+        return static_cast<float>(
+            __builtin_bit_cast(_Float16, static_cast<std::uint16_t>((i & 1) != 0 ? wd >> 16 : wd & 0xffffu)));
+    };
+    const auto f4_at = [](const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); };
+#pragma unroll 1
+    for (int j{0}; j < 4; ++j) {
+        const int p{16 * j + 4 * grp + w};
         const int bx{tx * 8 + (p & 7)}, by{ty * 8 + (p >> 3)};
-        if (bx >= bw || by >= bh) {
-            continue;
-        }
+        const bool in{bx < bw && by < bh};
         const int own{S.own[p]};
         const int n{own >> 8}, op{own & 255};
-        const int ox{tx * 8 + (op & 7)}, oy{ty * 8 + (op >> 3)};
-        const int m{n / 8};
-        const int e{((by - oy) * m + (bx - ox)) * 64 + lane};
-        const std::size_t slot{(static_cast<std::size_t>(by) * bw + bx) * 64 + lane};
-        const float xr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[slot]))};
-        const float yr{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[cplane + slot]))};
-        const float br{static_cast<float>(__builtin_bit_cast(_Float16, coeffs[2 * cplane + slot]))};
-        std::int16_t q0{0}, q1{0}, q2{0};
-        if (!is_llf(e, n)) {
+        // Non-zeros among the lane's four: X | Y << 8 | B << 16.
+        std::uint32_t cnt{0};
+        if (in) {
+            const int ox{tx * 8 + (op & 7)}, oy{ty * 8 + (op >> 3)};
+            const int m{n >> 3};
+            const int e0{((by - oy) * m + (bx - ox)) * 64 + e4};
+            const std::size_t slot{(static_cast<std::size_t>(by) * bw + bx) * 64 + e4};
+            const uint2 hx{*reinterpret_cast<const uint2*>(coeffs + slot)};
+            const uint2 hy{*reinterpret_cast<const uint2*>(coeffs + cplane + slot)};
+            const uint2 hb{*reinterpret_cast<const uint2*>(coeffs + 2 * cplane + slot)};
             const float qgsf{static_cast<float>(qf[static_cast<std::size_t>(oy) * bw + ox]) *
                              prm.global_scale_float};
-            // Quotients by the table weights are the correctly rounded ones, by
-            // reciprocals; y_round divides by the block's own qgsf.
-            const float wy{dequant_of(n, 1, e)};
-            const int qy{static_cast<int>(__builtin_rintf(div_by_recip(yr * qgsf, wy, dequant_inv_of(n, 1, e))))};
-            const float y_round{static_cast<float>(qy) * wy / qgsf};
-            const float wx{dequant_of(n, 0, e)};
-            const float wb{dequant_of(n, 2, e)};
-            q0 = static_cast<std::int16_t>(static_cast<int>(
-                __builtin_rintf(div_by_recip((xr - yx * y_round) * qgsf, wx, dequant_inv_of(n, 0, e)))));
-            q1 = static_cast<std::int16_t>(qy);
-            q2 = static_cast<std::int16_t>(static_cast<int>(
-                __builtin_rintf(div_by_recip((br - yb * y_round) * qgsf, wb, dequant_inv_of(n, 2, e)))));
+            const float rq{1.0f / qgsf};
+            const int cs{n * n};
+            const float* const wt{n == 32 ? &g_tables.dequant32[0][0]
+                                          : (n == 16 ? &g_tables.dequant16[0][0] : &g_tables.dequant8[0][0])};
+            const float* const wit{n == 32 ? &g_tables.dequant_inv32[0][0]
+                                           : (n == 16 ? &g_tables.dequant_inv16[0][0] : &g_tables.dequant_inv8[0][0])};
+            const float4 wx4{*reinterpret_cast<const float4*>(wt + e0)};
+            const float4 wy4{*reinterpret_cast<const float4*>(wt + cs + e0)};
+            const float4 wb4{*reinterpret_cast<const float4*>(wt + 2 * cs + e0)};
+            const float4 ix4{*reinterpret_cast<const float4*>(wit + e0)};
+            const float4 iy4{*reinterpret_cast<const float4*>(wit + cs + e0)};
+            const float4 ib4{*reinterpret_cast<const float4*>(wit + 2 * cs + e0)};
+            std::uint32_t q0[2]{0u, 0u}, q1[2]{0u, 0u}, q2[2]{0u, 0u};
+#pragma unroll
+            for (int i{0}; i < 4; ++i) {
+                std::int16_t a0{0}, a1{0}, a2{0};
+                if (!is_llf(e0 + i, n)) {
+                    const float wy{f4_at(wy4, i)};
+                    const int qy{static_cast<int>(
+                        __builtin_rintf(div_by_recip(f16_at(hy, i) * qgsf, wy, f4_at(iy4, i))))};
+                    const float y_round{div_by_recip(static_cast<float>(qy) * wy, qgsf, rq)};
+                    a0 = static_cast<std::int16_t>(static_cast<int>(__builtin_rintf(div_by_recip(
+                        (f16_at(hx, i) - yx * y_round) * qgsf, f4_at(wx4, i), f4_at(ix4, i)))));
+                    a1 = static_cast<std::int16_t>(qy);
+                    a2 = static_cast<std::int16_t>(static_cast<int>(__builtin_rintf(div_by_recip(
+                        (f16_at(hb, i) - yb * y_round) * qgsf, f4_at(wb4, i), f4_at(ib4, i)))));
+                }
+                const int sh{16 * (i & 1)};
+                q0[i >> 1] |= static_cast<std::uint32_t>(static_cast<std::uint16_t>(a0)) << sh;
+                q1[i >> 1] |= static_cast<std::uint32_t>(static_cast<std::uint16_t>(a1)) << sh;
+                q2[i >> 1] |= static_cast<std::uint32_t>(static_cast<std::uint16_t>(a2)) << sh;
+                cnt += (a0 != 0 ? 1u : 0u) | (a1 != 0 ? 0x100u : 0u) | (a2 != 0 ? 0x10000u : 0u);
+            }
+            *reinterpret_cast<uint2*>(ac + slot) = uint2{q0[0], q0[1]};
+            *reinterpret_cast<uint2*>(ac + cplane + slot) = uint2{q1[0], q1[1]};
+            *reinterpret_cast<uint2*>(ac + 2 * cplane + slot) = uint2{q2[0], q2[1]};
         }
-        ac[slot] = q0;
-        ac[cplane + slot] = q1;
-        ac[2 * cplane + slot] = q2;
-        const unsigned int c0{static_cast<unsigned int>(__popcll(__ballot(q0 != 0)))};
-        const unsigned int c1{static_cast<unsigned int>(__popcll(__ballot(q1 != 0)))};
-        const unsigned int c2{static_cast<unsigned int>(__popcll(__ballot(q2 != 0)))};
-        if (lane == 0) {
-            atomicAdd(&S.nz[op][0], c0);
-            atomicAdd(&S.nz[op][1], c1);
-            atomicAdd(&S.nz[op][2], c2);
+        // The block's counts into lane 0 of its row of 16 lanes (row
+        // shifts; lanes shifted in from outside the row read 0).
+        cnt += static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(cnt), 0x101, 0xf, 0xf, true));
+        cnt += static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(cnt), 0x102, 0xf, 0xf, true));
+        cnt += static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(cnt), 0x104, 0xf, 0xf, true));
+        cnt += static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(cnt), 0x108, 0xf, 0xf, true));
+        if (in && (lane & 15) == 0) {
+            atomicAdd(&S.nz[op][0], cnt & 0xffu);
+            atomicAdd(&S.nz[op][1], (cnt >> 8) & 0xffu);
+            atomicAdd(&S.nz[op][2], cnt >> 16);
         }
     }
     __syncthreads();
@@ -2083,7 +2123,7 @@ __device__ __attribute__((always_inline)) inline void tile_quantize_kernel_body(
     }
 }
 
-__global__ __launch_bounds__(256) void tile_quantize_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void tile_quantize_kernel(
     Geometry geo, FrameParams prm, const std::uint16_t* __restrict__ coeffs,
     const std::int8_t* __restrict__ acs, const std::int32_t* __restrict__ qf,
     const float* __restrict__ cfl_part, std::int16_t* __restrict__ ac,
